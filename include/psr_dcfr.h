@@ -1,0 +1,180 @@
+/* psr_dcfr.h -- C ABI of the MI355X range-solver DCFR library (libpsr_dcfr.so).
+ *
+ * Drop-in boundary for the reference's range-solver / gpu-range-solver API
+ * (tjennings/poker_solver_rust).  Every entry point names the reference
+ * interface it replaces.  Conventions:
+ *   - return value 0 = success, negative = error; psr_last_error() then holds
+ *     a thread-local message (in place of the reference's Result / panic),
+ *   - buffers are caller-owned, with explicit element counts,
+ *   - hands of player p are indexed 0..H_p-1 in ascending (card1, card2)
+ *     order (crates/range-solver/src/range.rs:256-275),
+ *   - a card is 4*rank + suit (2..A -> 0..12, c,d,h,s -> 0..3), card.rs:7-16,
+ *   - per-node A x H arrays are action-major: element [a*H + h].
+ * One batch handle owns one HIP stream; a handle is not thread-safe.
+ */
+#ifndef PSR_DCFR_H
+#define PSR_DCFR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Thread-local text of the last error (never NULL). */
+const char* psr_last_error(void);
+
+/* ---------------------------------------------------------------- cards */
+
+/* card.rs:50-56 card_pair_to_index: index 0..1325 of an unordered pair. */
+int psr_card_pair_to_index(uint8_t card1, uint8_t card2);
+/* card.rs:61-65 index_to_card_pair. */
+int psr_index_to_card_pair(int index, uint8_t* card1, uint8_t* card2);
+/* card.rs:420-483 card_from_str ("As" -> 51); returns the card or < 0. */
+int psr_card_from_str(const char* s);
+/* range.rs:721-760 Range::from_str: PioSOLVER-style range string into 1326
+ * weights in card_pair_to_index order. */
+int psr_range_parse(const char* s, float weights_out[1326]);
+/* hand.rs:68-87 Hand::evaluate: strength index of a 5..7 card hand. */
+int psr_hand_evaluate(const uint8_t* cards, int n);
+
+/* --------------------------------------------------------- tree config */
+
+/* action_tree.rs:130-192 TreeConfig.  Bet-size strings use the grammar of
+ * bet_size.rs:38-240 ("50%", "2.5x", "100c3r", "2e", "a"); NULL = "". */
+typedef struct psr_tree_config {
+    int32_t initial_state;       /* 0 flop, 1 turn, 2 river (BoardState) */
+    int32_t starting_pot;
+    int32_t effective_stack;
+    double rake_rate;
+    double rake_cap;
+    const char* bet[3][2];       /* [street][player] bet sizes   */
+    const char* raise[3][2];     /* [street][player] raise sizes */
+    const char* donk[2];         /* turn, river donk sizes; NULL = None */
+    double add_allin_threshold;
+    double force_allin_threshold;
+    double merging_threshold;
+    int32_t depth_limit;         /* -1 = None */
+    /* Extension, not a reference field: maximum raises per street
+     * (0 = unlimited = reference behaviour). */
+    int32_t max_raises;
+} psr_tree_config;
+
+/* action_tree.rs:37-67 Action; kinds in the reference's Ord order. */
+enum psr_action_kind {
+    PSR_ACT_NONE = 0, PSR_ACT_FOLD = 1, PSR_ACT_CHECK = 2, PSR_ACT_CALL = 3,
+    PSR_ACT_BET = 4, PSR_ACT_RAISE = 5, PSR_ACT_ALLIN = 6, PSR_ACT_CHANCE = 7
+};
+typedef struct psr_action {
+    int32_t kind;
+    int32_t amount; /* chips for bets/raises/all-in, card for chance */
+} psr_action;
+
+typedef struct psr_action_tree psr_action_tree;
+
+/* action_tree.rs:326-334 ActionTree::new. */
+int psr_action_tree_build(const psr_tree_config* cfg, psr_action_tree** out);
+void psr_action_tree_free(psr_action_tree* tree);
+/* action_tree.rs:433-480 apply_history + available_actions; returns the
+ * number of actions (written up to `cap`), or < 0. */
+int psr_action_tree_actions(const psr_action_tree* tree, const psr_action* history, int n_history,
+                            psr_action* out, int cap);
+/* action_tree.rs:1171-1195 count_num_action_nodes: nodes per street. */
+int psr_action_tree_count_nodes(const psr_action_tree* tree, uint64_t out[3]);
+
+/* ---------------------------------------------------------------- game */
+
+/* card.rs:20-32 CardConfig: ranges as 1326 weights; NOT_DEALT = 0xFF. */
+typedef struct psr_card_config {
+    const float* range[2];
+    uint8_t flop[3];
+    uint8_t turn;
+    uint8_t river;
+} psr_card_config;
+
+typedef struct psr_game psr_game;
+
+/* game/interpreter.rs:128-166 PostFlopGame::with_config (+ allocate_memory:
+ * storage is device-side and allocated by a batch). */
+int psr_game_build(const psr_card_config* cards, const psr_action_tree* tree, psr_game** out);
+void psr_game_free(psr_game* game);
+/* interface.rs num_private_hands / private_cards / initial_weights. */
+int psr_game_num_hands(const psr_game* game, int player);
+int psr_game_private_cards(const psr_game* game, int player, uint8_t* cards_out /* 2*H */);
+int psr_game_initial_weights(const psr_game* game, int player, float* out /* H */);
+/* game/mod.rs:282-322 num_nodes / node accessors, as flat arrays. */
+int64_t psr_game_num_nodes(const psr_game* game);
+double psr_game_num_combinations(const psr_game* game);
+/* sum over decision nodes of actions * hands: the size of the regret and of
+ * the strategy-sum arrays (game/interpreter.rs:212-249 num_storage). */
+int64_t psr_game_num_storage(const psr_game* game);
+/* Flat description of the built game (format in game.cpp) -- the input of the
+ * CPU oracle and of the device layout.  Returns the byte size; writes it when
+ * cap is large enough.  Replaces extract_topology/extract_terminal_data
+ * (gpu-range-solver extract.rs:55-360). */
+int64_t psr_game_serialize(const psr_game* game, uint8_t* buf, int64_t cap);
+
+/* ------------------------------------------------------------- solving */
+
+/* gpu-range-solver lib.rs:14-28 GpuSolverConfig / GpuSolveResult. */
+typedef struct psr_solver_config {
+    uint32_t max_iterations;
+    float target_exploitability;   /* chips; stop when exploitability <= target */
+    int32_t print_progress;        /* accepted for parity, ignored */
+    int32_t check_interval;        /* exploitability every k iterations (reference: 5) */
+} psr_solver_config;
+
+typedef struct psr_solve_result {
+    float exploitability;
+    uint32_t iterations_run;
+} psr_solve_result;
+
+/* lib.rs:71-87 gpu_solve_hand_parallel (and gpu_solve_game / gpu_solve_mega,
+ * lib.rs:31-69): one game on the GPU with the reference solve() schedule.
+ * root_strategy (A_root x H_oop average strategy) may be NULL. */
+int psr_gpu_solve(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result,
+                  float* root_strategy, int64_t cap);
+
+typedef struct psr_batch psr_batch;
+
+/* batch.rs:146-253 GpuBatchSolver::new: device, stream and work queue. */
+int psr_batch_create(int device, psr_batch** out);
+void psr_batch_destroy(psr_batch* batch);
+/* batch.rs:301-409 prepare_batch, generalised: every game may have its own
+ * tree, board and ranges.  Uploads topology, hands and showdown tables, and
+ * zeroes the regrets and strategy sums.  max_iterations sizes the discount
+ * table.  targets (chips, per game) may be NULL = 0. */
+int psr_batch_prepare(psr_batch* batch, const psr_game* const* games, int n, uint32_t max_iterations,
+                      const float* targets);
+/* Zero regrets and strategy sums of every game (restart the solve). */
+int psr_batch_reset(psr_batch* batch);
+/* batch.rs:414-534 run_iterations(start, end): DCFR iterations [start, end)
+ * of every game, no exploitability checks. */
+int psr_batch_run_iterations(psr_batch* batch, uint32_t start, uint32_t end);
+/* solver.rs:108-176 solve(): initial check, iterations with a check every
+ * check_interval and at max_iterations, early stop at the per-game target.
+ * expl_out / iters_out: one entry per game (may be NULL). */
+int psr_batch_solve(psr_batch* batch, uint32_t max_iterations, int32_t check_interval, float* expl_out,
+                    uint32_t* iters_out);
+/* utility.rs:380-392 compute_exploitability of every game (chips). */
+int psr_batch_exploitability(psr_batch* batch, float* expl_out);
+/* utility.rs:407-443 root_cfvalues: per-hand root CFVs of every game under
+ * the average strategy; after this call psr_batch_root_cfvalues reads them. */
+int psr_batch_compute_root_cfvalues(psr_batch* batch);
+int psr_batch_root_cfvalues(psr_batch* batch, int game, int player, float* out /* H */);
+/* batch.rs:587-609 extract_results: regret / strategy-sum arrays of one game,
+ * psr_game_num_storage() floats each, decision nodes in game arena order. */
+int psr_batch_strategy_sum(psr_batch* batch, int game, float* out, int64_t cap);
+int psr_batch_regrets(psr_batch* batch, int game, float* out, int64_t cap);
+/* Wall time of the last device job in milliseconds (HIP events on the batch
+ * stream) -- used for roofline accounting. */
+float psr_batch_last_kernel_ms(psr_batch* batch);
+/* Device properties: number of CUs and the workgroups launched. */
+int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSR_DCFR_H */

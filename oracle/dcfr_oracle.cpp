@@ -1,0 +1,757 @@
+// CPU oracle for the range-solver DCFR path.  TEST INFRASTRUCTURE ONLY: it is
+// loaded by tests/, by __graft_entry__.smoke() as the checker and by
+// bench.py's cpu_baseline leg, never by the product library.
+//
+// A restatement of crates/range-solver's solver on a game serialized by the
+// product's game build (PostFlopGame::serialize).  Each function follows the
+// reference function cited beside it, with the same operation order so that
+// the f32 instantiation reproduces the reference's rounding:
+//   DiscountParams          solver.rs:19-45
+//   solve / solve_step      solver.rs:108-209
+//   solve_recursive         solver.rs:221-479
+//   regret matching         solver.rs:600-626, sliceop.rs:35-39
+//   strategy/regret update  solver.rs:556-589
+//   evaluate_internal       game/evaluation.rs:57-538 (fold, showdown, raked)
+//   exploitability          utility.rs:380-392,501-543
+//   best response           utility.rs:804-988
+//   cfvalue recursion       utility.rs:552-795, finalize utility.rs:348-372
+// The double instantiation (every T = f64) is the FP64 oracle used to measure
+// the reduced-precision solves.
+//
+// Parity pins: tests/test_oracle_*.py (golden vectors, brute force,
+// invariants).  Built by oracle/Makefile into oracle/liboracle.so.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr uint8_t TERMINAL = 8, CHANCE = 4, FOLD = 24, BOUNDARY = 40, MASK = 3;
+constexpr uint16_t NONE16 = 0xFFFF;
+
+struct Node {
+    uint8_t player, turn, river;
+    int32_t amount;
+    uint32_t first_child, num_children;
+    int32_t runout;
+    int64_t storage = -1;   // offset of the A*H block of a decision node
+    int iso = -1;           // index into Game::iso
+};
+
+struct Runout {
+    uint8_t turn, river;
+    std::vector<uint16_t> valid[2];
+    std::vector<std::pair<uint16_t, uint16_t>> strength[2];   // (strength, index), sentinels included
+};
+
+struct Iso {
+    uint32_t ref;
+    std::vector<std::pair<uint16_t, uint16_t>> swap[2];
+};
+
+struct Game {
+    int32_t starting_pot, effective_stack;
+    double rake_rate, rake_cap, num_combinations;
+    uint8_t flop[3], turn, river, initial_state;
+    std::vector<std::pair<uint8_t, uint8_t>> cards[2];
+    std::vector<float> weights[2];
+    std::vector<uint16_t> same[2];
+    std::vector<Node> nodes;
+    std::vector<Runout> runouts;
+    std::vector<std::vector<Iso>> iso;
+    int64_t storage_size = 0;
+    // leaf values injected at depth boundaries: [ordinal][player] -> H values
+    std::vector<int> boundary_ordinal;   // per node, -1 if not a boundary
+    std::vector<std::vector<float>> boundary_cfv[2];
+    int H(int p) const { return (int)cards[p].size(); }
+    bool raked() const { return rake_rate > 0.0 && rake_cap > 0.0; }
+};
+
+struct Reader {
+    const uint8_t* p;
+    const uint8_t* end;
+    bool ok = true;
+    template <class T> T get() {
+        T v{};
+        if (p + sizeof(T) > end) { ok = false; return v; }
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return v;
+    }
+};
+
+bool load(const uint8_t* blob, size_t len, Game& g) {
+    Reader r{blob, blob + len};
+    if (r.get<uint32_t>() != 0x47525350u || r.get<uint32_t>() != 1) return false;
+    g.starting_pot = r.get<int32_t>();
+    g.effective_stack = r.get<int32_t>();
+    g.rake_rate = r.get<double>();
+    g.rake_cap = r.get<double>();
+    g.num_combinations = r.get<double>();
+    for (int i = 0; i < 3; ++i) g.flop[i] = r.get<uint8_t>();
+    g.turn = r.get<uint8_t>();
+    g.river = r.get<uint8_t>();
+    g.initial_state = r.get<uint8_t>();
+    r.get<uint8_t>();
+    r.get<uint8_t>();
+    for (int p = 0; p < 2; ++p) {
+        uint32_t h = r.get<uint32_t>();
+        g.cards[p].resize(h);
+        g.weights[p].resize(h);
+        g.same[p].resize(h);
+        for (auto& c : g.cards[p]) { c.first = r.get<uint8_t>(); c.second = r.get<uint8_t>(); }
+        for (auto& w : g.weights[p]) w = r.get<float>();
+        for (auto& s : g.same[p]) s = r.get<uint16_t>();
+    }
+    uint32_t n = r.get<uint32_t>();
+    g.nodes.resize(n);
+    for (auto& nd : g.nodes) {
+        nd.player = r.get<uint8_t>();
+        nd.turn = r.get<uint8_t>();
+        nd.river = r.get<uint8_t>();
+        r.get<uint8_t>();
+        nd.amount = r.get<int32_t>();
+        nd.first_child = r.get<uint32_t>();
+        nd.num_children = r.get<uint32_t>();
+        nd.runout = r.get<int32_t>();
+    }
+    uint32_t nr = r.get<uint32_t>();
+    g.runouts.resize(nr);
+    for (auto& ro : g.runouts) {
+        ro.turn = r.get<uint8_t>();
+        ro.river = r.get<uint8_t>();
+        r.get<uint16_t>();
+        for (int p = 0; p < 2; ++p) {
+            uint32_t k = r.get<uint32_t>();
+            ro.valid[p].resize(k);
+            for (auto& v : ro.valid[p]) v = r.get<uint16_t>();
+        }
+        for (int p = 0; p < 2; ++p) {
+            uint32_t k = r.get<uint32_t>();
+            ro.strength[p].resize(k);
+            for (auto& s : ro.strength[p]) { s.first = r.get<uint16_t>(); s.second = r.get<uint16_t>(); }
+        }
+    }
+    uint32_t nc = r.get<uint32_t>();
+    for (uint32_t i = 0; i < nc; ++i) {
+        uint32_t node = r.get<uint32_t>(), cnt = r.get<uint32_t>();
+        if (node >= n) return false;
+        std::vector<Iso> v(cnt);
+        for (auto& is : v) {
+            is.ref = r.get<uint32_t>();
+            for (int p = 0; p < 2; ++p) {
+                uint32_t k = r.get<uint32_t>();
+                is.swap[p].resize(k);
+                for (auto& s : is.swap[p]) { s.first = r.get<uint16_t>(); s.second = r.get<uint16_t>(); }
+            }
+        }
+        g.nodes[node].iso = (int)g.iso.size();
+        g.iso.push_back(std::move(v));
+    }
+    if (!r.ok) return false;
+    // storage offsets in arena order (A*H of the acting player per decision node)
+    int64_t off = 0;
+    int bcount = 0;
+    g.boundary_ordinal.assign(n, -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        Node& nd = g.nodes[i];
+        if (nd.player & TERMINAL) {
+            if ((nd.player & BOUNDARY) == BOUNDARY) g.boundary_ordinal[i] = bcount++;
+            continue;
+        }
+        if (nd.player & CHANCE) continue;
+        nd.storage = off;
+        off += (int64_t)nd.num_children * g.H(nd.player);
+    }
+    g.storage_size = off;
+    for (int p = 0; p < 2; ++p) g.boundary_cfv[p].assign(bcount, std::vector<float>());
+    return true;
+}
+
+// ------------------------------------------------------------- scalar ops --
+// sliceop.rs:5-11: max without NaN propagation
+template <class T> inline T max_(T x, T y) { return x > y ? x : y; }
+
+struct DiscountParams {
+    float alpha, beta, gamma;
+};
+
+// solver.rs:26-44
+DiscountParams discount(uint32_t t) {
+    uint32_t p4 = t == 0 ? 0 : 1u << ((31 - __builtin_clz(t)) & ~1u);
+    double ta = (double)std::max((int64_t)t - 1, (int64_t)0);
+    double tg = (double)(t - p4);
+    double pa = ta * std::sqrt(ta);
+    double x = tg / (tg + 1.0);
+    double pg = x * x * x;
+    return {(float)(pa / (pa + 1.0)), 0.5f, (float)pg};
+}
+
+template <class T>
+struct Solver {
+    Game& g;
+    std::vector<T> regret, strategy;   // storage2 / storage1 of every decision node
+
+    explicit Solver(Game& game) : g(game), regret(game.storage_size, T(0)), strategy(game.storage_size, T(0)) {}
+
+    // ------------------------------------------------ terminal evaluation --
+    // game/evaluation.rs:57-538
+    void evaluate(T* result, const Node& node, uint32_t node_index, int player, const T* cfreach) const {
+        const int H = g.H(player);
+        double pot = (double)(g.starting_pot + 2 * node.amount);
+        double half_pot = 0.5 * pot;
+        double rake = std::min(pot * g.rake_rate, g.rake_cap);
+        double amount_win = (half_pot - rake) / g.num_combinations;
+        double amount_lose = -half_pot / g.num_combinations;
+        const auto& pc = g.cards[player];
+        const auto& oc = g.cards[player ^ 1];
+        for (int i = 0; i < H; ++i) result[i] = T(0);
+        double sum = 0.0, minus[52] = {0};
+
+        if ((node.player & BOUNDARY) == BOUNDARY) {
+            evaluate_boundary(result, node, node_index, player, cfreach, half_pot);
+            return;
+        }
+        const Runout& ro = g.runouts[node.runout];
+        if ((node.player & FOLD) == FOLD) {
+            int folded = node.player & MASK;
+            double payoff = folded != player ? amount_win : amount_lose;
+            for (uint16_t i : ro.valid[player ^ 1]) {
+                T r = cfreach[i];
+                if (r != T(0)) {
+                    double rd = (double)r;
+                    sum += rd;
+                    minus[oc[i].first] += rd;
+                    minus[oc[i].second] += rd;
+                }
+            }
+            if (sum == 0.0) return;
+            for (uint16_t i : ro.valid[player]) {
+                uint16_t s = g.same[player][i];
+                double same = s == NONE16 ? 0.0 : (double)cfreach[s];
+                double c = sum + same - minus[pc[i].first] - minus[pc[i].second];
+                result[i] = (T)(payoff * c);
+            }
+            return;
+        }
+        const auto& ps = ro.strength[player];
+        const auto& os = ro.strength[player ^ 1];
+        if (rake == 0.0) {
+            size_t j = 1;
+            for (size_t k = 1; k + 1 < ps.size(); ++k) {
+                uint16_t st = ps[k].first, idx = ps[k].second;
+                while (os[j].first < st) {
+                    uint16_t oi = os[j].second;
+                    T r = cfreach[oi];
+                    if (r != T(0)) {
+                        double rd = (double)r;
+                        sum += rd;
+                        minus[oc[oi].first] += rd;
+                        minus[oc[oi].second] += rd;
+                    }
+                    ++j;
+                }
+                double c = sum - minus[pc[idx].first] - minus[pc[idx].second];
+                result[idx] = (T)(amount_win * c);
+            }
+            sum = 0.0;
+            std::fill(minus, minus + 52, 0.0);
+            j = os.size() - 2;
+            for (size_t k = ps.size() - 2; k >= 1; --k) {
+                uint16_t st = ps[k].first, idx = ps[k].second;
+                while (os[j].first > st) {
+                    uint16_t oi = os[j].second;
+                    T r = cfreach[oi];
+                    if (r != T(0)) {
+                        double rd = (double)r;
+                        sum += rd;
+                        minus[oc[oi].first] += rd;
+                        minus[oc[oi].second] += rd;
+                    }
+                    --j;
+                }
+                double c = sum - minus[pc[idx].first] - minus[pc[idx].second];
+                result[idx] += (T)(amount_lose * c);
+            }
+            return;
+        }
+        // raked showdown, evaluation.rs:429-537
+        double amount_tie = -0.5 * rake / g.num_combinations;
+        for (size_t k = 1; k + 1 < os.size(); ++k) {
+            uint16_t oi = os[k].second;
+            T r = cfreach[oi];
+            if (r != T(0)) {
+                double rd = (double)r;
+                sum += rd;
+                minus[oc[oi].first] += rd;
+                minus[oc[oi].second] += rd;
+            }
+        }
+        if (sum == 0.0) return;
+        double sum_win = 0.0, sum_tie = 0.0, minus_win[52] = {0}, minus_tie[52] = {0};
+        size_t i = 1, j = 1;
+        uint16_t prev = 0;
+        for (size_t k = 1; k + 1 < ps.size(); ++k) {
+            uint16_t st = ps[k].first, idx = ps[k].second;
+            if (st > prev) {
+                prev = st;
+                if (i < j) {
+                    sum_win = sum_tie;
+                    std::copy(minus_tie, minus_tie + 52, minus_win);
+                    i = j;
+                }
+                while (os[i].first < st) {
+                    uint16_t oi = os[i].second;
+                    double rd = (double)cfreach[oi];
+                    sum_win += rd;
+                    minus_win[oc[oi].first] += rd;
+                    minus_win[oc[oi].second] += rd;
+                    ++i;
+                }
+                if (j < i) {
+                    sum_tie = sum_win;
+                    std::copy(minus_win, minus_win + 52, minus_tie);
+                    j = i;
+                }
+                while (os[j].first == st) {
+                    uint16_t oi = os[j].second;
+                    double rd = (double)cfreach[oi];
+                    sum_tie += rd;
+                    minus_tie[oc[oi].first] += rd;
+                    minus_tie[oc[oi].second] += rd;
+                    ++j;
+                }
+            }
+            uint8_t c1 = pc[idx].first, c2 = pc[idx].second;
+            double total = sum - minus[c1] - minus[c2];
+            double win = sum_win - minus_win[c1] - minus_win[c2];
+            double tie = sum_tie - minus_tie[c1] - minus_tie[c2];
+            uint16_t s = g.same[player][idx];
+            double same = s == NONE16 ? 0.0 : (double)cfreach[s];
+            double v = amount_win * win + amount_tie * (tie - win + same) + amount_lose * (total - tie);
+            result[idx] = (T)v;
+        }
+    }
+
+    // Depth-boundary leaf with injected per-hand values (single continuation,
+    // evaluation.rs:544-604): bcfv * (half_pot / num_comb) * blocked reach.
+    void evaluate_boundary(T* result, const Node& node, uint32_t node_index, int player, const T* cfreach,
+                           double half_pot) const {
+        int ord = g.boundary_ordinal[node_index];
+        const auto& bcfv = g.boundary_cfv[player][ord];
+        if (bcfv.empty()) return;
+        const Runout& ro = g.runouts[node.runout];
+        const auto& pc = g.cards[player];
+        const auto& oc = g.cards[player ^ 1];
+        double scale = half_pot / g.num_combinations;
+        double sum = 0.0, minus[52] = {0};
+        for (uint16_t i : ro.valid[player ^ 1]) {
+            T r = cfreach[i];
+            if (r != T(0)) {
+                double rd = (double)r;
+                sum += rd;
+                minus[oc[i].first] += rd;
+                minus[oc[i].second] += rd;
+            }
+        }
+        if (sum == 0.0) return;
+        for (uint16_t i : ro.valid[player]) {
+            uint16_t s = g.same[player][i];
+            double same = s == NONE16 ? 0.0 : (double)cfreach[s];
+            double adj = sum + same - minus[pc[i].first] - minus[pc[i].second];
+            result[i] = (T)((double)bcfv[i] * scale * adj);
+        }
+    }
+
+    // ------------------------------------------------------------ helpers --
+    // solver.rs:600-626
+    void regret_match(const T* R, int A, int H, std::vector<T>& s) const {
+        s.resize((size_t)A * H);
+        for (int k = 0; k < A * H; ++k) s[k] = max_<T>(R[k], T(0));
+        std::vector<T> denom(s.begin(), s.begin() + H);
+        for (int a = 1; a < A; ++a)
+            for (int h = 0; h < H; ++h) denom[h] += s[(size_t)a * H + h];
+        T def = T(1) / (T)A;
+        for (int a = 0; a < A; ++a)
+            for (int h = 0; h < H; ++h) {
+                T& x = s[(size_t)a * H + h];
+                x = denom[h] == T(0) ? def : x / denom[h];
+            }
+    }
+    // utility.rs:229-245 (normalized_strategy)
+    void normalized(const T* S, int A, int H, std::vector<T>& s) const {
+        s.resize((size_t)A * H);
+        std::vector<T> denom(S, S + H);
+        for (int a = 1; a < A; ++a)
+            for (int h = 0; h < H; ++h) denom[h] += S[(size_t)a * H + h];
+        T def = T(1) / (T)A;
+        for (int a = 0; a < A; ++a)
+            for (int h = 0; h < H; ++h)
+                s[(size_t)a * H + h] = denom[h] == T(0) ? def : S[(size_t)a * H + h] / denom[h];
+    }
+    void chance_sum(T* result, const Node& node, int player, std::vector<T>& cfv, int H) const {
+        int A = (int)node.num_children;
+        std::vector<double> acc(H);
+        for (int h = 0; h < H; ++h) acc[h] = (double)cfv[h];
+        for (int a = 1; a < A; ++a)
+            for (int h = 0; h < H; ++h) acc[h] += (double)cfv[(size_t)a * H + h];
+        if (node.iso >= 0) {
+            for (const Iso& is : g.iso[node.iso]) {
+                T* row = &cfv[(size_t)is.ref * H];
+                for (auto& sw : is.swap[player]) std::swap(row[sw.first], row[sw.second]);
+                for (int h = 0; h < H; ++h) acc[h] += (double)row[h];
+                for (auto& sw : is.swap[player]) std::swap(row[sw.first], row[sw.second]);
+            }
+        }
+        for (int h = 0; h < H; ++h) result[h] = (T)acc[h];
+    }
+    static T chance_scale(const Node& node) {   // 1 / chance_factor, computed in T
+        return T(1) / (T)(node.turn == 0xFF ? 45 : 44);
+    }
+
+    // --------------------------------------------------- solve_recursive --
+    void solve_rec(T* result, uint32_t ni, int player, const T* cfreach, const DiscountParams& dp) {
+        const Node& node = g.nodes[ni];
+        if (node.player & TERMINAL) { evaluate(result, node, ni, player, cfreach); return; }
+        const int A = (int)node.num_children;
+        const int H = g.H(player);
+        const bool chance = node.player & CHANCE;
+        if (A == 1 && !chance) { solve_rec(result, node.first_child, player, cfreach, dp); return; }
+        std::vector<T> cfv((size_t)A * H);
+        if (chance) {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> upd(Ho);
+            T sc = chance_scale(node);
+            for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
+            for (int a = 0; a < A; ++a) solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data(), dp);
+            chance_sum(result, node, player, cfv, H);
+        } else if (node.player == player) {
+            for (int a = 0; a < A; ++a) solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach, dp);
+            T* R = &regret[node.storage];
+            T* S = &strategy[node.storage];
+            std::vector<T> sig;
+            regret_match(R, A, H, sig);
+            for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
+            const T gamma = (T)dp.gamma, alpha = (T)dp.alpha, beta = (T)dp.beta;
+            for (int k = 0; k < A * H; ++k) S[k] = S[k] * gamma + sig[k];
+            for (int k = 0; k < A * H; ++k) {
+                T coef = std::signbit(R[k]) ? beta : alpha;
+                R[k] = R[k] * coef + cfv[k];
+            }
+            for (int a = 0; a < A; ++a)
+                for (int h = 0; h < H; ++h) R[(size_t)a * H + h] -= result[h];
+        } else {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> sig;
+            regret_match(&regret[node.storage], A, Ho, sig);
+            for (int a = 0; a < A; ++a)
+                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
+            for (int a = 0; a < A; ++a)
+                solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho], dp);
+            for (int h = 0; h < H; ++h) result[h] = cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+        }
+    }
+
+    // --------------------------------------------------- best response --
+    void best_cfv_rec(T* result, uint32_t ni, int player, const T* cfreach) const {
+        const Node& node = g.nodes[ni];
+        if (node.player & TERMINAL) { evaluate(result, node, ni, player, cfreach); return; }
+        const int A = (int)node.num_children;
+        const int H = g.H(player);
+        const bool chance = node.player & CHANCE;
+        if (A == 1 && !chance) { best_cfv_rec(result, node.first_child, player, cfreach); return; }
+        std::vector<T> cfv((size_t)A * H);
+        if (chance) {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> upd(Ho);
+            T sc = chance_scale(node);
+            for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
+            for (int a = 0; a < A; ++a) best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data());
+            std::vector<T> tmp(cfv);
+            chance_sum(result, node, player, tmp, H);
+        } else if (node.player == player) {
+            for (int a = 0; a < A; ++a) best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach);
+            for (int h = 0; h < H; ++h) result[h] = cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] = max_<T>(result[h], cfv[(size_t)a * H + h]);
+        } else {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> sig;
+            normalized(&strategy[node.storage], A, Ho, sig);
+            for (int a = 0; a < A; ++a)
+                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
+            for (int a = 0; a < A; ++a)
+                best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho]);
+            for (int h = 0; h < H; ++h) result[h] = cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+        }
+    }
+
+    // ---------------------------------------------- cfvalue (avg strategy) --
+    void cfvalue_rec(T* result, uint32_t ni, int player, const T* cfreach) const {
+        const Node& node = g.nodes[ni];
+        if (node.player & TERMINAL) { evaluate(result, node, ni, player, cfreach); return; }
+        const int A = (int)node.num_children;
+        const int H = g.H(player);
+        const bool chance = node.player & CHANCE;
+        if (!chance && node.player != player && A == 1) {
+            cfvalue_rec(result, node.first_child, player, cfreach);
+            return;
+        }
+        std::vector<T> cfv((size_t)A * H);
+        if (chance) {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> upd(Ho);
+            T sc = chance_scale(node);
+            for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
+            for (int a = 0; a < A; ++a) cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data());
+            chance_sum(result, node, player, cfv, H);
+        } else if (node.player == player) {
+            for (int a = 0; a < A; ++a) cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach);
+            std::vector<T> sig;
+            normalized(&strategy[node.storage], A, H, sig);
+            for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
+        } else {
+            const int Ho = g.H(player ^ 1);
+            std::vector<T> sig;
+            normalized(&strategy[node.storage], A, Ho, sig);
+            for (int a = 0; a < A; ++a)
+                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
+            for (int a = 0; a < A; ++a)
+                cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho]);
+            for (int h = 0; h < H; ++h) result[h] = cfv[h];
+            for (int a = 1; a < A; ++a)
+                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+        }
+    }
+
+    std::vector<T> weights_T(int p) const { return std::vector<T>(g.weights[p].begin(), g.weights[p].end()); }
+
+    // utility.rs:99-103 (weighted_sum): f64 accumulate, cast to the value type
+    T weighted_sum(const std::vector<T>& v, int p) const {
+        double s = 0.0;
+        for (size_t i = 0; i < v.size(); ++i) s += (double)v[i] * (double)g.weights[p][i];
+        return (T)s;
+    }
+
+    void mes_ev(T out[2]) const {
+        for (int p = 0; p < 2; ++p) {
+            std::vector<T> cfv(g.H(p)), reach = weights_T(p ^ 1);
+            best_cfv_rec(cfv.data(), 0, p, reach.data());
+            out[p] = weighted_sum(cfv, p);
+        }
+    }
+    void current_ev(T out[2]) const {
+        for (int p = 0; p < 2; ++p) {
+            std::vector<T> cfv(g.H(p)), reach = weights_T(p ^ 1);
+            cfvalue_rec(cfv.data(), 0, p, reach.data());
+            out[p] = weighted_sum(cfv, p);
+        }
+    }
+    // utility.rs:380-392
+    T exploitability() const {
+        T m[2];
+        mes_ev(m);
+        if (!g.raked()) return (m[0] + m[1]) * T(0.5);
+        T c[2];
+        current_ev(c);
+        return ((m[0] - c[0]) + (m[1] - c[1])) * T(0.5);
+    }
+
+    void step(uint32_t t) {
+        DiscountParams dp = discount(t);
+        for (int p = 0; p < 2; ++p) {
+            std::vector<T> res(g.H(p)), reach = weights_T(p ^ 1);
+            solve_rec(res.data(), 0, p, reach.data(), dp);
+        }
+    }
+
+    // solver.rs:108-176 (finalize's cfvalue pass is read back on demand via
+    // root_cfvalues, which is the same recursion)
+    T solve(uint32_t max_iter, T target, uint32_t* iters_run) {
+        T expl = exploitability();
+        uint32_t t = 0;
+        for (; t < max_iter; ++t) {
+            if (expl <= target) break;
+            step(t);
+            if ((t + 1) % 5 == 0 || t + 1 == max_iter) expl = exploitability();
+        }
+        if (iters_run) *iters_run = t;
+        return expl;
+    }
+
+    void root_cfvalues(int p, T* out) const {
+        std::vector<T> reach = weights_T(p ^ 1);
+        cfvalue_rec(out, 0, p, reach.data());
+    }
+};
+
+struct Handle {
+    Game game;
+    std::unique_ptr<Solver<float>> f32;
+    std::unique_ptr<Solver<double>> f64;
+};
+
+thread_local std::string g_err;
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+// precision: 32 = the reference's f32 arithmetic, 64 = all-f64 oracle
+void* oracle_game_load(const uint8_t* blob, size_t len, int precision) {
+    auto h = std::make_unique<Handle>();
+    if (!load(blob, len, h->game)) {
+        g_err = "malformed game blob";
+        return nullptr;
+    }
+    if (precision == 64) h->f64 = std::make_unique<Solver<double>>(h->game);
+    else h->f32 = std::make_unique<Solver<float>>(h->game);
+    return h.release();
+}
+
+void oracle_game_free(void* h) { delete static_cast<Handle*>(h); }
+
+int64_t oracle_storage_size(void* hv) { return static_cast<Handle*>(hv)->game.storage_size; }
+
+int oracle_num_hands(void* hv, int p) { return static_cast<Handle*>(hv)->game.H(p); }
+
+void oracle_discount(uint32_t t, float out[3]) {
+    DiscountParams d = discount(t);
+    out[0] = d.alpha;
+    out[1] = d.beta;
+    out[2] = d.gamma;
+}
+
+#define DISPATCH(h, ...)                                   \
+    do {                                                   \
+        Handle* H_ = static_cast<Handle*>(h);              \
+        if (H_->f32) { auto& S = *H_->f32; __VA_ARGS__; }  \
+        else { auto& S = *H_->f64; __VA_ARGS__; }          \
+    } while (0)
+
+void oracle_solve_step(void* h, uint32_t t) { DISPATCH(h, S.step(t)); }
+
+double oracle_exploitability(void* h) {
+    double r = 0;
+    DISPATCH(h, r = (double)S.exploitability());
+    return r;
+}
+
+double oracle_solve(void* h, uint32_t max_iter, double target, uint32_t* iters_run) {
+    double r = 0;
+    DISPATCH(h, r = (double)S.solve(max_iter, (decltype(S.exploitability()))target, iters_run));
+    return r;
+}
+
+void oracle_mes_ev(void* h, double out[2]) {
+    DISPATCH(h, {
+        decltype(S.exploitability()) m[2];
+        S.mes_ev(m);
+        out[0] = m[0];
+        out[1] = m[1];
+    });
+}
+
+void oracle_current_ev(void* h, double out[2]) {
+    DISPATCH(h, {
+        decltype(S.exploitability()) m[2];
+        S.current_ev(m);
+        out[0] = m[0];
+        out[1] = m[1];
+    });
+}
+
+// per-hand CFVs at the root under the average strategy (utility.rs:407-443)
+void oracle_root_cfvalues(void* h, int p, double* out) {
+    DISPATCH(h, {
+        std::vector<decltype(S.exploitability())> v(S.g.H(p));
+        S.root_cfvalues(p, v.data());
+        for (size_t i = 0; i < v.size(); ++i) out[i] = (double)v[i];
+    });
+}
+
+// regrets / strategy sums of every decision node in arena order (A*H blocks)
+void oracle_state(void* h, double* regret, double* strategy) {
+    DISPATCH(h, {
+        for (size_t i = 0; i < S.regret.size(); ++i) {
+            if (regret) regret[i] = (double)S.regret[i];
+            if (strategy) strategy[i] = (double)S.strategy[i];
+        }
+    });
+}
+
+void oracle_set_state(void* h, const double* regret, const double* strategy) {
+    DISPATCH(h, {
+        for (size_t i = 0; i < S.regret.size(); ++i) {
+            S.regret[i] = (decltype(S.exploitability()))regret[i];
+            S.strategy[i] = (decltype(S.exploitability()))strategy[i];
+        }
+    });
+}
+
+// one terminal evaluation, for unit tests of the fold/showdown formulas
+int oracle_evaluate(void* h, uint32_t node, int player, const double* cfreach, double* out) {
+    int rc = 0;
+    DISPATCH(h, {
+        auto& g = S.g;
+        if (node >= g.nodes.size() || !(g.nodes[node].player & TERMINAL)) { rc = -1; break; }
+        using T = decltype(S.exploitability());
+        std::vector<T> r(cfreach, cfreach + g.H(player ^ 1)), o(g.H(player));
+        S.evaluate(o.data(), g.nodes[node], node, player, r.data());
+        for (size_t i = 0; i < o.size(); ++i) out[i] = (double)o[i];
+    });
+    return rc;
+}
+
+// leaf values for depth-boundary ordinal `ord` (evaluation.rs:544-604)
+int oracle_set_boundary_cfvs(void* hv, int ord, int player, const float* v, int n) {
+    Handle* h = static_cast<Handle*>(hv);
+    auto& b = h->game.boundary_cfv[player];
+    if (ord < 0 || ord >= (int)b.size() || n != h->game.H(player)) return -1;
+    b[ord].assign(v, v + n);
+    return 0;
+}
+
+// CPU baseline: solve `n` serialized games with `threads` worker threads, one
+// solve per thread at a time (the reference runs river subgames single-
+// threaded, node.rs:347-349).  Returns wall seconds; writes exploitability.
+double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n, uint32_t iters, int threads,
+                         double* expl_out) {
+    std::atomic<int> next{0};
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&]() {
+        for (;;) {
+            int i = next.fetch_add(1);
+            if (i >= n) break;
+            Game g;
+            if (!load(blobs[i], lens[i], g)) { expl_out[i] = NAN; continue; }
+            Solver<float> s(g);
+            uint32_t run = 0;
+            expl_out[i] = (double)s.solve(iters, 0.0f, &run);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,560 @@
+// Device batch: builds the HBM layout of a set of games (dcfr_layout.hpp),
+// owns the device buffers and HIP stream, and exposes the batch C ABI.
+// Replaces gpu-range-solver's extract.rs (topology / terminal extraction) and
+// batch.rs (GpuBatchSolver), with a per-game tree instead of one shared
+// topology and O(H) showdown tables instead of H x H outcome matrices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/psr_dcfr.h"
+#include "capi_internal.hpp"
+#include "dcfr_launch.hpp"
+#include "dcfr_layout.hpp"
+#include "game.hpp"
+
+using namespace psr;
+
+namespace {
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) return fail(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HostLayout {
+    std::vector<GameDesc> games;
+    std::vector<NodeRec> nodes;
+    std::vector<int32_t> levels;
+    std::vector<double> payoff;
+    std::vector<uint16_t> cards, same;
+    std::vector<float> weights;
+    std::vector<int32_t> terminals;
+    std::vector<RunoutTab> runouts;
+    std::vector<ChanceTab> chance;
+    std::vector<int32_t> iso_ref, iso_perm;
+    std::vector<uint16_t> u16;
+    std::vector<uint32_t> u32;
+    std::vector<int64_t> storage_size;   // per game
+    int64_t state_total = 0, scratch_total = 0;
+    int max_hands = 0;
+    std::vector<int> order;   // work-queue order, most expensive first
+};
+
+int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, size_t align = 4) {
+    while (pool.size() % align) pool.push_back(0);
+    int32_t off = (int32_t)pool.size();
+    pool.insert(pool.end(), v.begin(), v.end());
+    return off;
+}
+
+// Showdown / fold tables of one runout for traversing player p.
+RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayout& L) {
+    const int o = p ^ 1;
+    const auto& oc = g.private_cards(o);
+    const auto& pcards = g.private_cards(p);
+    const bool showdown = ro.river != NOT_DEALT;
+    // opponent valid hands in ascending strength (valid order for fold-only runouts)
+    std::vector<uint16_t> sorted;
+    std::vector<uint16_t> opp_strength(oc.size(), 0);
+    if (showdown) {
+        for (size_t k = 1; k + 1 < ro.strength[o].size(); ++k) {
+            sorted.push_back(ro.strength[o][k].index);
+            opp_strength[ro.strength[o][k].index] = ro.strength[o][k].strength;
+        }
+    } else {
+        sorted = ro.valid[o];
+    }
+    // group by card, keeping the strength order inside each card
+    std::vector<uint16_t> card_off(53, 0), card_hands;
+    std::vector<std::vector<uint16_t>> by_card(52);
+    for (uint16_t oi : sorted) {
+        by_card[oc[oi].first].push_back(oi);
+        by_card[oc[oi].second].push_back(oi);
+    }
+    for (int c = 0; c < 52; ++c) {
+        card_off[c] = (uint16_t)card_hands.size();
+        card_hands.insert(card_hands.end(), by_card[c].begin(), by_card[c].end());
+    }
+    card_off[52] = (uint16_t)card_hands.size();
+
+    const int Hp = (int)pcards.size();
+    std::vector<uint32_t> lohi(Hp, 0xFFFFFFFFu);
+    std::vector<uint16_t> clohi((size_t)4 * Hp, 0);
+    std::vector<int> pstrength(Hp, -1);
+    if (showdown) {
+        for (size_t k = 1; k + 1 < ro.strength[p].size(); ++k) pstrength[ro.strength[p][k].index] = ro.strength[p][k].strength;
+    } else {
+        for (uint16_t i : ro.valid[p]) pstrength[i] = 0;
+    }
+    auto count_below = [&](const std::vector<uint16_t>& list, int s, bool inclusive) {
+        int n = 0;
+        for (uint16_t oi : list) {
+            int os = opp_strength[oi];
+            if (os < s || (inclusive && os == s)) ++n;
+            else break;   // lists are strength-ascending
+        }
+        return n;
+    };
+    for (int i = 0; i < Hp; ++i) {
+        if (pstrength[i] < 0) continue;
+        if (!showdown) {
+            lohi[i] = 0;
+            continue;
+        }
+        const int s = pstrength[i];
+        // sorted is ascending: binary search for the bounds
+        int lo = (int)(std::lower_bound(sorted.begin(), sorted.end(), s,
+                                        [&](uint16_t oi, int v) { return (int)opp_strength[oi] < v; }) -
+                       sorted.begin());
+        int hi = (int)(std::upper_bound(sorted.begin(), sorted.end(), s,
+                                        [&](int v, uint16_t oi) { return v < (int)opp_strength[oi]; }) -
+                       sorted.begin());
+        lohi[i] = (uint32_t)lo | ((uint32_t)hi << 16);
+        const int c1 = pcards[i].first, c2 = pcards[i].second;
+        clohi[4 * i + 0] = (uint16_t)count_below(by_card[c1], s, false);
+        clohi[4 * i + 1] = (uint16_t)count_below(by_card[c1], s, true);
+        clohi[4 * i + 2] = (uint16_t)count_below(by_card[c2], s, false);
+        clohi[4 * i + 3] = (uint16_t)count_below(by_card[c2], s, true);
+    }
+    RunoutTab t{};
+    t.sorted = push_u16(L.u16, sorted);
+    t.card_off = push_u16(L.u16, card_off);
+    t.card_hands = push_u16(L.u16, card_hands);
+    t.card_lohi = push_u16(L.u16, clohi);
+    t.lohi = (int32_t)L.u32.size();
+    L.u32.insert(L.u32.end(), lohi.begin(), lohi.end());
+    t.num_valid_opp = (int32_t)sorted.size();
+    return t;
+}
+
+int build_layout(const psr_game* const* games, int n, const float* targets, HostLayout& L) {
+    const int maxa = dcfr_max_actions();
+    std::vector<double> cost(n);
+    for (int gi = 0; gi < n; ++gi) {
+        const PostFlopGame& g = game_of(games[gi]);
+        const TreeConfig& tc = g.tree_config();
+        if (tc.rake_rate > 0.0 && tc.rake_cap > 0.0) return fail("raked games are not supported by the device path");
+        const auto& nodes = g.nodes();
+        GameDesc d{};
+        d.node_base = (int32_t)L.nodes.size();
+        d.num_nodes = (int32_t)nodes.size();
+        d.H[0] = g.num_private_hands(0);
+        d.H[1] = g.num_private_hands(1);
+        L.max_hands = std::max({L.max_hands, d.H[0], d.H[1]});
+        if (d.H[0] > 1326 || d.H[1] > 1326) return fail("too many hands");
+        d.row_stride = (std::max(d.H[0], d.H[1]) + 63) / 64 * 64;
+        d.state_base = L.state_total;
+        d.scratch_base = L.scratch_total;
+        d.target = targets ? targets[gi] : 0.0f;
+
+        // hands
+        for (int p = 0; p < 2; ++p) {
+            d.hand_base[p] = (int32_t)L.cards.size();
+            const auto& hc = g.private_cards(p);
+            for (size_t i = 0; i < hc.size(); ++i) {
+                L.cards.push_back((uint16_t)(hc[i].first | (hc[i].second << 8)));
+                L.same.push_back(g.same_hand_index(p)[i]);
+                L.weights.push_back(g.initial_weights(p)[i]);
+            }
+        }
+        // runout tables (global ids)
+        const int runout_base = (int)L.runouts.size() / 2;
+        for (const Runout& ro : g.runouts())
+            for (int p = 0; p < 2; ++p) L.runouts.push_back(build_runout(g, ro, p, L));
+
+        // levels (arena is breadth-first)
+        std::vector<int> depth(nodes.size(), 0);
+        for (size_t i = 0; i < nodes.size(); ++i)
+            for (uint32_t a = 0; a < nodes[i].num_children; ++a) depth[nodes[i].first_child + a] = depth[i] + 1;
+        d.level_base = (int32_t)L.levels.size();
+        int nlev = depth.empty() ? 0 : depth.back() + 1;
+        for (int lv = 0, i = 0; lv <= nlev; ++lv) {
+            while (i < (int)nodes.size() && depth[i] < lv) ++i;
+            L.levels.push_back(i);
+        }
+        d.num_levels = nlev;
+
+        // nodes
+        int64_t off = 0;
+        d.term_base = (int32_t)L.terminals.size();
+        const double half_div = g.num_combinations();
+        for (size_t i = 0; i < nodes.size(); ++i) {
+            const GameNode& gn = nodes[i];
+            NodeRec r{};
+            r.first_child = (int32_t)gn.first_child;
+            r.num_children = gn.num_children;
+            r.state_off = -1;
+            r.aux = -1;
+            double win = 0, lose = 0, tie = 0;
+            if (gn.is_terminal()) {
+                if (gn.is_boundary()) return fail("depth-boundary leaves are not supported by the device path yet");
+                r.kind = gn.is_fold() ? KIND_FOLD : KIND_SHOWDOWN;
+                r.player = gn.player & PLAYER_MASK;
+                int ro = g.runout_index(gn.turn, gn.river);
+                if (ro < 0) return fail("terminal without runout");
+                r.aux = runout_base + ro;
+                L.terminals.push_back((int32_t)i);
+                double pot = (double)(tc.starting_pot + 2 * gn.amount);
+                double half = 0.5 * pot;
+                double rake = std::min(pot * tc.rake_rate, tc.rake_cap);
+                win = (half - rake) / half_div;
+                lose = -half / half_div;
+                tie = -0.5 * rake / half_div;
+            } else if (gn.is_chance()) {
+                r.kind = KIND_CHANCE;
+                ChanceTab ct{};
+                ct.scale = 1.0f / (float)g.chance_factor(gn);
+                const auto& iso = g.iso_chances((uint32_t)i);
+                ct.iso_count = (int32_t)iso.size();
+                ct.iso_base = (int32_t)L.iso_ref.size();
+                for (const IsoChance& ic : iso) {
+                    L.iso_ref.push_back(ic.ref);
+                    for (int p = 0; p < 2; ++p) {
+                        std::vector<uint16_t> perm(g.num_private_hands(p));
+                        std::iota(perm.begin(), perm.end(), 0);
+                        for (auto& s : ic.swap[p]) std::swap(perm[s.first], perm[s.second]);
+                        L.iso_perm.push_back(push_u16(L.u16, perm, 1));
+                    }
+                }
+                r.aux = (int32_t)L.chance.size();
+                L.chance.push_back(ct);
+            } else {
+                r.kind = KIND_DECISION;
+                r.player = gn.player;
+                if (gn.num_children > maxa) return fail("node has more actions than the device supports");
+                r.state_off = (int32_t)off;
+                off += (int64_t)gn.num_children * d.H[gn.player];
+            }
+            L.nodes.push_back(r);
+            L.payoff.push_back(win);
+            L.payoff.push_back(lose);
+            L.payoff.push_back(tie);
+        }
+        d.num_terminals = (int32_t)L.terminals.size() - d.term_base;
+        L.storage_size.push_back(off);
+        L.state_total += (off + 63) / 64 * 64;
+        L.scratch_total += (int64_t)nodes.size() * d.row_stride;
+        cost[gi] = (double)off + (double)nodes.size() * d.row_stride;
+        L.games.push_back(d);
+    }
+    L.order.resize(n);
+    std::iota(L.order.begin(), L.order.end(), 0);
+    std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    return 0;
+}
+
+template <class T>
+int upload(T** dptr, const std::vector<T>& v) {
+    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
+    HIP_OK(hipMalloc((void**)dptr, bytes));
+    if (!v.empty()) HIP_OK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+struct psr_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    HostLayout L;
+    bool prepared = false;
+    uint32_t max_iterations = 0;
+    int grid = 0, num_cus = 0, blocks_per_cu = 0;
+    float last_ms = 0.0f;
+    // device buffers
+    GameDesc* d_games = nullptr;
+    NodeRec* d_nodes = nullptr;
+    int32_t* d_levels = nullptr;
+    double* d_payoff = nullptr;
+    uint16_t *d_cards = nullptr, *d_same = nullptr;
+    float* d_weights = nullptr;
+    int32_t* d_terms = nullptr;
+    RunoutTab* d_runouts = nullptr;
+    ChanceTab* d_chance = nullptr;
+    int32_t *d_iso_ref = nullptr, *d_iso_perm = nullptr;
+    uint16_t* d_u16 = nullptr;
+    uint32_t* d_u32 = nullptr;
+    float *d_regret = nullptr, *d_strategy = nullptr, *d_reach = nullptr, *d_cfv = nullptr;
+    float4* d_discount = nullptr;
+    float* d_expl = nullptr;
+    uint32_t* d_iters = nullptr;
+    float* d_root_cfv = nullptr;
+    int* d_counter = nullptr;
+    int* d_order = nullptr;
+
+    void release() {
+        void* ptrs[] = {d_games, d_nodes, d_levels, d_payoff, d_cards, d_same, d_weights, d_terms, d_runouts,
+                        d_chance, d_iso_ref, d_iso_perm, d_u16, d_u32, d_regret, d_strategy, d_reach, d_cfv,
+                        d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_order};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+        d_games = nullptr; d_nodes = nullptr; d_levels = nullptr; d_payoff = nullptr; d_cards = nullptr;
+        d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr; d_chance = nullptr;
+        d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_u32 = nullptr; d_regret = nullptr;
+        d_strategy = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr; d_expl = nullptr;
+        d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_order = nullptr;
+        prepared = false;
+    }
+
+    KernelArgs args() const {
+        KernelArgs a{};
+        a.games = d_games;
+        a.num_games = (int)L.games.size();
+        a.max_hands = L.max_hands;
+        a.nodes = d_nodes;
+        a.level_start = d_levels;
+        a.payoff = d_payoff;
+        a.hand_cards = d_cards;
+        a.hand_same = d_same;
+        a.hand_weight = d_weights;
+        a.terminals = d_terms;
+        a.runouts = d_runouts;
+        a.chance = d_chance;
+        a.iso_ref = d_iso_ref;
+        a.iso_perm = d_iso_perm;
+        a.u16pool = d_u16;
+        a.u32pool = d_u32;
+        a.regret = d_regret;
+        a.strategy = d_strategy;
+        a.reach = d_reach;
+        a.cfv = d_cfv;
+        a.discount = d_discount;
+        a.expl_out = d_expl;
+        a.iters_out = d_iters;
+        a.root_cfv = d_root_cfv;
+        a.queue_counter = d_counter;
+        a.queue_order = d_order;
+        a.max_iterations = max_iterations;
+        return a;
+    }
+
+    int run(KernelArgs a) {
+        if (!prepared) return fail("psr_batch_prepare has not been called");
+        HIP_OK(hipSetDevice(device));
+        HIP_OK(hipEventRecord(ev0, stream));
+        HIP_OK(dcfr_launch(a, grid, stream));
+        HIP_OK(hipEventRecord(ev1, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        HIP_OK(hipEventElapsedTime(&last_ms, ev0, ev1));
+        return 0;
+    }
+};
+
+// solver.rs:26-44, computed in f64 and rounded to f32 as the reference does
+static float4 discount_params(uint32_t t) {
+    uint32_t p4 = t == 0 ? 0 : 1u << ((31 - __builtin_clz(t)) & ~1u);
+    double ta = (double)std::max<int64_t>((int64_t)t - 1, 0);
+    double tg = (double)(t - p4);
+    double pa = ta * std::sqrt(ta);
+    double x = tg / (tg + 1.0);
+    return make_float4((float)(pa / (pa + 1.0)), 0.5f, (float)(x * x * x), 0.0f);
+}
+
+extern "C" {
+
+int psr_batch_create(int device, psr_batch** out) {
+    if (!out) return fail("null argument");
+    int count = 0;
+    HIP_OK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail("no such HIP device");
+    auto b = new psr_batch;
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e != hipSuccess) {
+        delete b;
+        return fail(std::string("HIP setup failed: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+void psr_batch_destroy(psr_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    b->release();
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_t max_iterations,
+                      const float* targets) {
+    if (!b || !games || n <= 0) return fail("invalid argument");
+    HIP_OK(hipSetDevice(b->device));
+    b->release();
+    b->L = HostLayout{};
+    if (build_layout(games, n, targets, b->L) != 0) return -1;
+    HostLayout& L = b->L;
+    if (upload(&b->d_games, L.games) || upload(&b->d_nodes, L.nodes) || upload(&b->d_levels, L.levels) ||
+        upload(&b->d_payoff, L.payoff) || upload(&b->d_cards, L.cards) || upload(&b->d_same, L.same) ||
+        upload(&b->d_weights, L.weights) || upload(&b->d_terms, L.terminals) || upload(&b->d_runouts, L.runouts) ||
+        upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) || upload(&b->d_iso_perm, L.iso_perm) ||
+        upload(&b->d_u16, L.u16) || upload(&b->d_u32, L.u32) || upload(&b->d_order, L.order))
+        return -1;
+    std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
+    for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
+    if (upload(&b->d_discount, disc)) return -1;
+    b->max_iterations = max_iterations;
+    HIP_OK(hipMalloc((void**)&b->d_regret, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_strategy, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_reach, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_cfv, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_expl, n * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_iters, n * sizeof(uint32_t)));
+    HIP_OK(hipMalloc((void**)&b->d_root_cfv, std::max<size_t>(L.cards.size(), 1) * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_counter, sizeof(int)));
+    HIP_OK(fill_zero(b->d_regret, L.state_total, b->stream));
+    HIP_OK(fill_zero(b->d_strategy, L.state_total, b->stream));
+    HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
+    HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
+    HIP_OK(fill_zero(b->d_root_cfv, L.cards.size(), b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(dcfr_occupancy(L.max_hands, &b->blocks_per_cu));
+    if (b->blocks_per_cu < 1) return fail("kernel does not fit on a CU");
+    b->grid = std::min(n, b->num_cus * b->blocks_per_cu);
+    b->prepared = true;
+    return 0;
+}
+
+int psr_batch_reset(psr_batch* b) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(fill_zero(b->d_regret, b->L.state_total, b->stream));
+    HIP_OK(fill_zero(b->d_strategy, b->L.state_total, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int psr_batch_run_iterations(psr_batch* b, uint32_t start, uint32_t end) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (end > b->max_iterations) return fail("end exceeds the max_iterations given to psr_batch_prepare");
+    if (start > end) return fail("start > end");
+    KernelArgs a = b->args();
+    a.iter_begin = start;
+    a.iter_end = end;
+    a.check_interval = 0;
+    a.flags = JOB_ITERATE;
+    return b->run(a);
+}
+
+int psr_batch_solve(psr_batch* b, uint32_t max_iterations, int32_t check_interval, float* expl_out,
+                    uint32_t* iters_out) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (max_iterations > b->max_iterations) return fail("max_iterations exceeds the prepared discount table");
+    KernelArgs a = b->args();
+    a.iter_begin = 0;
+    a.iter_end = max_iterations;
+    a.max_iterations = max_iterations;
+    a.check_interval = check_interval;
+    a.flags = JOB_INITIAL_CHECK | JOB_ITERATE | (check_interval > 0 ? 0 : JOB_FINAL_CHECK);
+    if (int rc = b->run(a)) return rc;
+    const size_t n = b->L.games.size();
+    if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (iters_out) HIP_OK(hipMemcpy(iters_out, b->d_iters, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int psr_batch_exploitability(psr_batch* b, float* expl_out) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    KernelArgs a = b->args();
+    a.flags = JOB_FINAL_CHECK;
+    if (int rc = b->run(a)) return rc;
+    if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, b->L.games.size() * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int psr_batch_compute_root_cfvalues(psr_batch* b) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    KernelArgs a = b->args();
+    a.flags = JOB_ROOT_CFV;
+    return b->run(a);
+}
+
+int psr_batch_root_cfvalues(psr_batch* b, int game, int player, float* out) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || player < 0 || player > 1 || !out) return fail("invalid argument");
+    const GameDesc& d = b->L.games[game];
+    HIP_OK(hipMemcpy(out, b->d_root_cfv + d.hand_base[player], d.H[player] * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int copy_state(psr_batch* b, const float* src, int game, float* out, int64_t cap) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || !out) return fail("invalid argument");
+    int64_t n = b->L.storage_size[game];
+    if (cap < n) return fail("output buffer too small");
+    HIP_OK(hipMemcpy(out, src + b->L.games[game].state_base, n * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int psr_batch_strategy_sum(psr_batch* b, int game, float* out, int64_t cap) {
+    return copy_state(b, b ? b->d_strategy : nullptr, game, out, cap);
+}
+
+int psr_batch_regrets(psr_batch* b, int game, float* out, int64_t cap) {
+    return copy_state(b, b ? b->d_regret : nullptr, game, out, cap);
+}
+
+float psr_batch_last_kernel_ms(psr_batch* b) { return b ? b->last_ms : -1.0f; }
+
+int psr_batch_launch_info(psr_batch* b, int* num_cus, int* grid, int* blocks_per_cu) {
+    if (!b) return fail("null batch");
+    if (num_cus) *num_cus = b->num_cus;
+    if (grid) *grid = b->grid;
+    if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+    return 0;
+}
+
+int psr_gpu_solve(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result, float* root_strategy,
+                  int64_t cap) {
+    if (!game || !cfg || !result) return fail("null argument");
+    psr_batch* b = nullptr;
+    if (psr_batch_create(0, &b)) return -1;
+    float target = cfg->target_exploitability;
+    int rc = psr_batch_prepare(b, &game, 1, std::max<uint32_t>(cfg->max_iterations, 1), &target);
+    float expl = 0.0f;
+    uint32_t iters = 0;
+    if (rc == 0) rc = psr_batch_solve(b, cfg->max_iterations, cfg->check_interval > 0 ? cfg->check_interval : 5, &expl, &iters);
+    if (rc == 0) {
+        result->exploitability = expl;
+        result->iterations_run = iters;
+        if (root_strategy) {
+            const PostFlopGame& g = game_of(game);
+            const GameNode& root = g.nodes()[0];
+            int64_t n = (int64_t)root.num_children * g.num_private_hands(root.player & 1);
+            if (root.is_terminal() || root.is_chance()) n = 0;
+            if (cap < n) rc = fail("root_strategy buffer too small");
+            else if (n > 0) {
+                std::vector<float> s(b->L.storage_size[0]);
+                rc = psr_batch_strategy_sum(b, 0, s.data(), (int64_t)s.size());
+                if (rc == 0) {
+                    // normalised average strategy at the root (utility.rs:229-245)
+                    const int A = root.num_children, H = g.num_private_hands(root.player & 1);
+                    for (int h = 0; h < H; ++h) {
+                        float denom = s[h];
+                        for (int a = 1; a < A; ++a) denom += s[(size_t)a * H + h];
+                        for (int a = 0; a < A; ++a)
+                            root_strategy[(size_t)a * H + h] = denom == 0.0f ? 1.0f / (float)A : s[(size_t)a * H + h] / denom;
+                    }
+                }
+            }
+        }
+    }
+    psr_batch_destroy(b);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,479 @@
+"""ctypes mirror of the reference range-solver API over libpsr_dcfr.so.
+
+Reference map (crates/range-solver/src unless noted):
+  card_pair_to_index / index_to_card_pair   card.rs:50-65
+  Range (from_str)                           range.rs:721-760
+  TreeConfig / ActionTree                    action_tree.rs:130-334
+  CardConfig                                 card.rs:20-32
+  PostFlopGame.with_config                   game/interpreter.rs:128-166
+  GpuSolverConfig / GpuSolveResult / gpu_solve_hand_parallel
+                                             gpu-range-solver lib.rs:14-87
+  GpuBatchSolver                             gpu-range-solver batch.rs:79-610
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+NOT_DEALT = 0xFF
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class PsrError(RuntimeError):
+    """Error reported by the library (the reference returns Err / panics)."""
+
+
+def lib_path() -> str:
+    return os.path.join(_HERE, "libpsr_dcfr.so")
+
+
+class _TreeConfigC(C.Structure):
+    _fields_ = [
+        ("initial_state", C.c_int32),
+        ("starting_pot", C.c_int32),
+        ("effective_stack", C.c_int32),
+        ("rake_rate", C.c_double),
+        ("rake_cap", C.c_double),
+        ("bet", (C.c_char_p * 2) * 3),
+        ("raise_", (C.c_char_p * 2) * 3),
+        ("donk", C.c_char_p * 2),
+        ("add_allin_threshold", C.c_double),
+        ("force_allin_threshold", C.c_double),
+        ("merging_threshold", C.c_double),
+        ("depth_limit", C.c_int32),
+        ("max_raises", C.c_int32),
+    ]
+
+
+class _ActionC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("amount", C.c_int32)]
+
+
+class _CardConfigC(C.Structure):
+    _fields_ = [
+        ("range", C.POINTER(C.c_float) * 2),
+        ("flop", C.c_uint8 * 3),
+        ("turn", C.c_uint8),
+        ("river", C.c_uint8),
+    ]
+
+
+class _SolverConfigC(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_uint32),
+        ("target_exploitability", C.c_float),
+        ("print_progress", C.c_int32),
+        ("check_interval", C.c_int32),
+    ]
+
+
+class _SolveResultC(C.Structure):
+    _fields_ = [("exploitability", C.c_float), ("iterations_run", C.c_uint32)]
+
+
+def load_library(path: Optional[str] = None):
+    """Loads libpsr_dcfr.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = path or lib_path()
+    if not os.path.exists(p):
+        raise PsrError(f"{p} is missing: build it with `python -m poker_solver_rust_amd.build`")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    sig = {
+        "psr_last_error": (C.c_char_p, []),
+        "psr_card_pair_to_index": (C.c_int, [C.c_uint8, C.c_uint8]),
+        "psr_index_to_card_pair": (C.c_int, [C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8)]),
+        "psr_card_from_str": (C.c_int, [C.c_char_p]),
+        "psr_range_parse": (C.c_int, [C.c_char_p, C.POINTER(C.c_float)]),
+        "psr_hand_evaluate": (C.c_int, [C.POINTER(C.c_uint8), C.c_int]),
+        "psr_action_tree_build": (C.c_int, [C.POINTER(_TreeConfigC), C.POINTER(vp)]),
+        "psr_action_tree_free": (None, [vp]),
+        "psr_action_tree_actions": (C.c_int, [vp, C.POINTER(_ActionC), C.c_int, C.POINTER(_ActionC), C.c_int]),
+        "psr_action_tree_count_nodes": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+        "psr_game_build": (C.c_int, [C.POINTER(_CardConfigC), vp, C.POINTER(vp)]),
+        "psr_game_free": (None, [vp]),
+        "psr_game_num_hands": (C.c_int, [vp, C.c_int]),
+        "psr_game_private_cards": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint8)]),
+        "psr_game_initial_weights": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float)]),
+        "psr_game_num_nodes": (C.c_int64, [vp]),
+        "psr_game_num_combinations": (C.c_double, [vp]),
+        "psr_game_num_storage": (C.c_int64, [vp]),
+        "psr_game_serialize": (C.c_int64, [vp, C.POINTER(C.c_uint8), C.c_int64]),
+        "psr_gpu_solve": (C.c_int, [vp, C.POINTER(_SolverConfigC), C.POINTER(_SolveResultC), C.POINTER(C.c_float), C.c_int64]),
+        "psr_batch_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "psr_batch_destroy": (None, [vp]),
+        "psr_batch_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
+        "psr_batch_reset": (C.c_int, [vp]),
+        "psr_batch_run_iterations": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "psr_batch_solve": (C.c_int, [vp, C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
+        "psr_batch_exploitability": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "psr_batch_compute_root_cfvalues": (C.c_int, [vp]),
+        "psr_batch_root_cfvalues": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "psr_batch_strategy_sum": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
+        "psr_batch_regrets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
+        "psr_batch_last_kernel_ms": (C.c_float, [vp]),
+        "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _check(rc):
+    if rc is not None and rc < 0:
+        raise PsrError(load_library().psr_last_error().decode())
+    return rc
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ------------------------------------------------------------------ cards --
+
+def card_pair_to_index(c1: int, c2: int) -> int:
+    return _check(load_library().psr_card_pair_to_index(c1, c2))
+
+
+def index_to_card_pair(index: int) -> Tuple[int, int]:
+    a, b = C.c_uint8(), C.c_uint8()
+    _check(load_library().psr_index_to_card_pair(index, C.byref(a), C.byref(b)))
+    return a.value, b.value
+
+
+def card_from_str(s: str) -> int:
+    return _check(load_library().psr_card_from_str(s.encode()))
+
+
+def flop_from_str(s: str) -> List[int]:
+    """card.rs flop_from_str: 3 cards, optionally space separated, sorted."""
+    s = s.replace(" ", "")
+    if len(s) != 6:
+        raise PsrError("Expected exactly three cards")
+    cards = sorted(card_from_str(s[i:i + 2]) for i in range(0, 6, 2))
+    if cards[0] == cards[1] or cards[1] == cards[2]:
+        raise PsrError("Cards must be unique")
+    return cards
+
+
+def hand_evaluate(cards: Sequence[int]) -> int:
+    arr = (C.c_uint8 * len(cards))(*cards)
+    return _check(load_library().psr_hand_evaluate(arr, len(cards)))
+
+
+class Range:
+    """1326 weights in card_pair_to_index order (range.rs:38-40)."""
+
+    def __init__(self, data=None):
+        self.data = np.zeros(1326, np.float32) if data is None else np.asarray(data, np.float32).copy()
+        if self.data.shape != (1326,):
+            raise PsrError("Expected exactly 1326 elements")
+
+    @classmethod
+    def parse(cls, s: str) -> "Range":
+        out = np.zeros(1326, np.float32)
+        _check(load_library().psr_range_parse(s.encode(), _fptr(out)))
+        return cls(out)
+
+    from_str = parse
+
+    @classmethod
+    def ones(cls) -> "Range":
+        return cls(np.ones(1326, np.float32))
+
+    def get_weight_by_cards(self, c1: int, c2: int) -> float:
+        return float(self.data[card_pair_to_index(c1, c2)])
+
+
+# ------------------------------------------------------------------- tree --
+
+class BoardState(enum.IntEnum):
+    Flop = 0
+    Turn = 1
+    River = 2
+
+
+class Action(enum.IntEnum):
+    None_ = 0
+    Fold = 1
+    Check = 2
+    Call = 3
+    Bet = 4
+    Raise = 5
+    AllIn = 6
+    Chance = 7
+
+
+def _act(kind, amount=0):
+    return (int(kind), int(amount))
+
+
+@dataclass
+class TreeConfig:
+    """action_tree.rs:130-192.  Bet sizes are (bet, raise) strings per player
+    and street, e.g. river_bet_sizes=[("50%, a", "2.5x"), ("50%, a", "2.5x")]."""
+    initial_state: BoardState = BoardState.Flop
+    starting_pot: int = 0
+    effective_stack: int = 0
+    rake_rate: float = 0.0
+    rake_cap: float = 0.0
+    flop_bet_sizes: Sequence[Tuple[str, str]] = (("", ""), ("", ""))
+    turn_bet_sizes: Sequence[Tuple[str, str]] = (("", ""), ("", ""))
+    river_bet_sizes: Sequence[Tuple[str, str]] = (("", ""), ("", ""))
+    turn_donk_sizes: Optional[str] = None
+    river_donk_sizes: Optional[str] = None
+    add_allin_threshold: float = 0.0
+    force_allin_threshold: float = 0.0
+    merging_threshold: float = 0.0
+    depth_limit: Optional[int] = None
+    max_raises: int = 0   # extension: 0 = unlimited (reference behaviour)
+
+    def _to_c(self):
+        c = _TreeConfigC()
+        c.initial_state = int(self.initial_state)
+        c.starting_pot = self.starting_pot
+        c.effective_stack = self.effective_stack
+        c.rake_rate = self.rake_rate
+        c.rake_cap = self.rake_cap
+        keep = []
+        for s, sizes in enumerate((self.flop_bet_sizes, self.turn_bet_sizes, self.river_bet_sizes)):
+            for p in range(2):
+                b, r = sizes[p]
+                bb, rb = b.encode(), r.encode()
+                keep += [bb, rb]
+                c.bet[s][p] = bb
+                c.raise_[s][p] = rb
+        for i, d in enumerate((self.turn_donk_sizes, self.river_donk_sizes)):
+            if d is not None:
+                db = d.encode()
+                keep.append(db)
+                c.donk[i] = db
+        c.add_allin_threshold = self.add_allin_threshold
+        c.force_allin_threshold = self.force_allin_threshold
+        c.merging_threshold = self.merging_threshold
+        c.depth_limit = -1 if self.depth_limit is None else int(self.depth_limit)
+        c.max_raises = int(self.max_raises)
+        return c, keep
+
+
+class ActionTree:
+    """action_tree.rs:326-334 ActionTree::new and its query helpers."""
+
+    def __init__(self, config: TreeConfig):
+        lib = load_library()
+        c, keep = config._to_c()
+        h = C.c_void_p()
+        _check(lib.psr_action_tree_build(C.byref(c), C.byref(h)))
+        self._h = h
+        self.config = config
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load_library().psr_action_tree_free(self._h)
+            self._h = None
+
+    def available_actions(self, history: Sequence[Tuple[int, int]] = ()) -> List[Tuple[Action, int]]:
+        lib = load_library()
+        hist = (_ActionC * max(len(history), 1))(*[_ActionC(int(k), int(a)) for k, a in history])
+        out = (_ActionC * 64)()
+        n = _check(lib.psr_action_tree_actions(self._h, hist, len(history), out, 64))
+        return [(Action(out[i].kind), out[i].amount) for i in range(n)]
+
+    def count_nodes(self) -> List[int]:
+        out = (C.c_uint64 * 3)()
+        _check(load_library().psr_action_tree_count_nodes(self._h, out))
+        return list(out)
+
+
+@dataclass
+class CardConfig:
+    """card.rs:20-32."""
+    range: Tuple[Range, Range] = field(default_factory=lambda: (Range(), Range()))
+    flop: Sequence[int] = (NOT_DEALT, NOT_DEALT, NOT_DEALT)
+    turn: int = NOT_DEALT
+    river: int = NOT_DEALT
+
+
+class PostFlopGame:
+    """game/interpreter.rs:128-166 PostFlopGame::with_config."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def with_config(cls, card_config: CardConfig, action_tree: ActionTree) -> "PostFlopGame":
+        lib = load_library()
+        r0 = np.ascontiguousarray(card_config.range[0].data, np.float32)
+        r1 = np.ascontiguousarray(card_config.range[1].data, np.float32)
+        cc = _CardConfigC()
+        cc.range[0] = _fptr(r0)
+        cc.range[1] = _fptr(r1)
+        for i in range(3):
+            cc.flop[i] = card_config.flop[i]
+        cc.turn = card_config.turn
+        cc.river = card_config.river
+        h = C.c_void_p()
+        _check(lib.psr_game_build(C.byref(cc), action_tree._h, C.byref(h)))
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load_library().psr_game_free(self._h)
+            self._h = None
+
+    def num_private_hands(self, player: int) -> int:
+        return _check(load_library().psr_game_num_hands(self._h, player))
+
+    def private_cards(self, player: int) -> List[Tuple[int, int]]:
+        n = self.num_private_hands(player)
+        buf = (C.c_uint8 * (2 * n))()
+        _check(load_library().psr_game_private_cards(self._h, player, buf))
+        return [(buf[2 * i], buf[2 * i + 1]) for i in range(n)]
+
+    def initial_weights(self, player: int) -> np.ndarray:
+        out = np.zeros(self.num_private_hands(player), np.float32)
+        _check(load_library().psr_game_initial_weights(self._h, player, _fptr(out)))
+        return out
+
+    def num_nodes(self) -> int:
+        return _check(load_library().psr_game_num_nodes(self._h))
+
+    def num_combinations(self) -> float:
+        return load_library().psr_game_num_combinations(self._h)
+
+    def num_storage(self) -> int:
+        return _check(load_library().psr_game_num_storage(self._h))
+
+    def serialize(self) -> bytes:
+        lib = load_library()
+        n = _check(lib.psr_game_serialize(self._h, None, 0))
+        buf = (C.c_uint8 * n)()
+        _check(lib.psr_game_serialize(self._h, buf, n))
+        return bytes(buf)
+
+
+# ---------------------------------------------------------------- solving --
+
+@dataclass
+class GpuSolverConfig:
+    """gpu-range-solver lib.rs:14-18 (+ check_interval, reference value 5)."""
+    max_iterations: int = 1000
+    target_exploitability: float = 0.0
+    print_progress: bool = False
+    check_interval: int = 5
+
+
+@dataclass
+class GpuSolveResult:
+    """gpu-range-solver lib.rs:21-28."""
+    exploitability: float
+    iterations_run: int
+    root_strategy: np.ndarray
+
+
+def gpu_solve_hand_parallel(game: PostFlopGame, config: GpuSolverConfig) -> GpuSolveResult:
+    """gpu-range-solver lib.rs:71-87: solve one game on the GPU."""
+    lib = load_library()
+    cfg = _SolverConfigC(config.max_iterations, config.target_exploitability, int(config.print_progress),
+                         config.check_interval)
+    res = _SolveResultC()
+    cap = 64 * 1326
+    strat = np.zeros(cap, np.float32)
+    _check(lib.psr_gpu_solve(game._h, C.byref(cfg), C.byref(res), _fptr(strat), cap))
+    return GpuSolveResult(res.exploitability, res.iterations_run, strat)
+
+
+gpu_solve_game = gpu_solve_hand_parallel
+gpu_solve_mega = gpu_solve_hand_parallel
+
+
+class GpuBatchSolver:
+    """gpu-range-solver batch.rs:79-610, with one tree per game."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.psr_batch_create(device, C.byref(h)))
+        self._h = h
+        self.games: List[PostFlopGame] = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load_library().psr_batch_destroy(self._h)
+            self._h = None
+
+    def prepare_batch(self, games: Sequence[PostFlopGame], max_iterations: int,
+                      targets: Optional[Sequence[float]] = None):
+        lib = load_library()
+        self.games = list(games)
+        arr = (C.c_void_p * len(games))(*[g._h.value for g in games])
+        t = None
+        if targets is not None:
+            tt = np.ascontiguousarray(targets, np.float32)
+            self._targets = tt
+            t = _fptr(tt)
+        _check(lib.psr_batch_prepare(self._h, arr, len(games), max_iterations, t))
+
+    def reset(self):
+        _check(load_library().psr_batch_reset(self._h))
+
+    def run_iterations(self, start: int, end: int):
+        _check(load_library().psr_batch_run_iterations(self._h, start, end))
+
+    def solve(self, max_iterations: int, check_interval: int = 5):
+        n = len(self.games)
+        expl = np.zeros(n, np.float32)
+        iters = np.zeros(n, np.uint32)
+        _check(load_library().psr_batch_solve(self._h, max_iterations, check_interval, _fptr(expl),
+                                              iters.ctypes.data_as(C.POINTER(C.c_uint32))))
+        return expl, iters
+
+    def exploitability(self) -> np.ndarray:
+        expl = np.zeros(len(self.games), np.float32)
+        _check(load_library().psr_batch_exploitability(self._h, _fptr(expl)))
+        return expl
+
+    def root_cfvalues(self) -> List[Tuple[np.ndarray, np.ndarray]]:
+        lib = load_library()
+        _check(lib.psr_batch_compute_root_cfvalues(self._h))
+        out = []
+        for i, g in enumerate(self.games):
+            pair = []
+            for p in range(2):
+                v = np.zeros(g.num_private_hands(p), np.float32)
+                _check(lib.psr_batch_root_cfvalues(self._h, i, p, _fptr(v)))
+                pair.append(v)
+            out.append(tuple(pair))
+        return out
+
+    def strategy_sum(self, i: int) -> np.ndarray:
+        n = self.games[i].num_storage()
+        v = np.zeros(n, np.float32)
+        _check(load_library().psr_batch_strategy_sum(self._h, i, _fptr(v), n))
+        return v
+
+    def regrets(self, i: int) -> np.ndarray:
+        n = self.games[i].num_storage()
+        v = np.zeros(n, np.float32)
+        _check(load_library().psr_batch_regrets(self._h, i, _fptr(v), n))
+        return v
+
+    def last_kernel_ms(self) -> float:
+        return float(load_library().psr_batch_last_kernel_ms(self._h))
+
+    def launch_info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(load_library().psr_batch_launch_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"num_cus": a.value, "grid": b.value, "blocks_per_cu": c.value}

@@ -1,0 +1,151 @@
+"""ctypes binding of oracle/liboracle.so (the CPU oracle).  Test
+infrastructure: used by tests/, __graft_entry__.smoke() as the checker and
+bench.py's cpu_baseline leg only."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+_LIB = None
+
+
+def build_oracle():
+    subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(ORACLE_LIB):
+        build_oracle()
+    lib = C.CDLL(ORACLE_LIB)
+    vp = C.c_void_p
+    dp = C.POINTER(C.c_double)
+    sig = {
+        "oracle_last_error": (C.c_char_p, []),
+        "oracle_game_load": (vp, [C.POINTER(C.c_uint8), C.c_size_t, C.c_int]),
+        "oracle_game_free": (None, [vp]),
+        "oracle_storage_size": (C.c_int64, [vp]),
+        "oracle_num_hands": (C.c_int, [vp, C.c_int]),
+        "oracle_discount": (None, [C.c_uint32, C.POINTER(C.c_float)]),
+        "oracle_solve_step": (None, [vp, C.c_uint32]),
+        "oracle_exploitability": (C.c_double, [vp]),
+        "oracle_solve": (C.c_double, [vp, C.c_uint32, C.c_double, C.POINTER(C.c_uint32)]),
+        "oracle_mes_ev": (None, [vp, dp]),
+        "oracle_current_ev": (None, [vp, dp]),
+        "oracle_root_cfvalues": (None, [vp, C.c_int, dp]),
+        "oracle_state": (None, [vp, dp, dp]),
+        "oracle_set_state": (None, [vp, dp, dp]),
+        "oracle_evaluate": (C.c_int, [vp, C.c_uint32, C.c_int, dp, dp]),
+        "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "oracle_solve_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
+                                           C.c_uint32, C.c_int, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def discount(t):
+    out = (C.c_float * 3)()
+    load().oracle_discount(t, out)
+    return tuple(out)
+
+
+class OracleGame:
+    """One game loaded from PostFlopGame.serialize(); precision 32 follows the
+    reference's f32 arithmetic, 64 is the all-f64 oracle."""
+
+    def __init__(self, blob: bytes, precision: int = 32):
+        lib = load()
+        self._blob = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+        self._h = lib.oracle_game_load(self._blob, len(blob), precision)
+        if not self._h:
+            raise RuntimeError(lib.oracle_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            load().oracle_game_free(self._h)
+            self._h = None
+
+    def num_hands(self, p):
+        return load().oracle_num_hands(self._h, p)
+
+    def storage_size(self):
+        return load().oracle_storage_size(self._h)
+
+    def solve_step(self, t):
+        load().oracle_solve_step(self._h, t)
+
+    def exploitability(self):
+        return load().oracle_exploitability(self._h)
+
+    def solve(self, max_iterations, target=0.0):
+        it = C.c_uint32()
+        e = load().oracle_solve(self._h, max_iterations, target, C.byref(it))
+        return e, it.value
+
+    def mes_ev(self):
+        out = np.zeros(2)
+        load().oracle_mes_ev(self._h, _dptr(out))
+        return out
+
+    def current_ev(self):
+        out = np.zeros(2)
+        load().oracle_current_ev(self._h, _dptr(out))
+        return out
+
+    def root_cfvalues(self, p):
+        out = np.zeros(self.num_hands(p))
+        load().oracle_root_cfvalues(self._h, p, _dptr(out))
+        return out
+
+    def state(self):
+        n = self.storage_size()
+        r, s = np.zeros(n), np.zeros(n)
+        load().oracle_state(self._h, _dptr(r), _dptr(s))
+        return r, s
+
+    def set_state(self, regret, strategy):
+        r = np.ascontiguousarray(regret, np.float64)
+        s = np.ascontiguousarray(strategy, np.float64)
+        load().oracle_set_state(self._h, _dptr(r), _dptr(s))
+
+    def evaluate(self, node, player, cfreach):
+        cr = np.ascontiguousarray(cfreach, np.float64)
+        out = np.zeros(self.num_hands(player))
+        rc = load().oracle_evaluate(self._h, node, player, _dptr(cr), _dptr(out))
+        if rc != 0:
+            raise ValueError("not a terminal node")
+        return out
+
+    def set_boundary_cfvs(self, ordinal, player, values):
+        v = np.ascontiguousarray(values, np.float32)
+        rc = load().oracle_set_boundary_cfvs(self._h, ordinal, player, v.ctypes.data_as(C.POINTER(C.c_float)), len(v))
+        if rc != 0:
+            raise ValueError("bad boundary ordinal or length")
+
+
+def solve_many(blobs, iterations, threads):
+    """Solves serialized games with `threads` worker threads (one game per
+    thread at a time).  Returns (wall seconds, exploitability array)."""
+    lib = load()
+    n = len(blobs)
+    bufs = [(C.c_uint8 * len(b)).from_buffer_copy(b) for b in blobs]
+    ptrs = (C.POINTER(C.c_uint8) * n)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
+    lens = (C.c_size_t * n)(*[len(b) for b in blobs])
+    expl = np.zeros(n)
+    secs = lib.oracle_solve_many(ptrs, lens, n, iterations, threads, _dptr(expl))
+    return secs, expl
