@@ -734,7 +734,7 @@ int oracle_set_boundary_cfvs(void* hv, int ord, int player, const float* v, int 
 // solve per thread at a time (the reference runs river subgames single-
 // threaded, node.rs:347-349).  Returns wall seconds; writes exploitability.
 double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n, uint32_t iters, int threads,
-                         double* expl_out) {
+                         const double* targets, double* expl_out, uint32_t* iters_out) {
     std::atomic<int> next{0};
     auto t0 = std::chrono::steady_clock::now();
     auto work = [&]() {
@@ -745,7 +745,8 @@ double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n,
             if (!load(blobs[i], lens[i], g)) { expl_out[i] = NAN; continue; }
             Solver<float> s(g);
             uint32_t run = 0;
-            expl_out[i] = (double)s.solve(iters, 0.0f, &run);
+            expl_out[i] = (double)s.solve(iters, targets ? (float)targets[i] : 0.0f, &run);
+            if (iters_out) iters_out[i] = run;
         }
     };
     std::vector<std::thread> pool;

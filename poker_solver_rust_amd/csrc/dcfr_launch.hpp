@@ -31,7 +31,7 @@ struct KernelArgs {
     const int32_t* iso_ref;
     const int32_t* iso_perm;
     const uint16_t* u16pool;
-    const uint32_t* u32pool;
+    const HandScan* hand_scan;
     float* regret;
     float* strategy;
     float* reach;

@@ -13,12 +13,14 @@
 //                (action-major, hand contiguous: the reference's [a*H + h])
 //   reach[], cfv[]         f32 per-node rows of `row_stride` floats, scratch
 //                used inside one traversal
-//   runout tables (per game, runout and traversing player): the opponent's
-//                valid hands in ascending strength, the same hands grouped by
-//                card (52 segments, strength-ascending inside a segment), and
-//                per traversing hand the count of opponent hands weaker /
-//                not stronger than it, globally and inside each of its two
-//                cards' segments.
+//   runout tables (per game, runout and traversing player): one index list of
+//                53 segments scanned in a single pass -- segment 0 holds the
+//                opponent's valid hands in ascending strength, segment 1+c the
+//                valid opponent hands holding card c (strength-ascending);
+//                every segment ends with one extra slot whose exclusive prefix
+//                is the segment total.  Per traversing hand a 16-byte record
+//                gives the scan positions it reads: weaker / not-stronger
+//                bounds in segment 0 and inside the segments of its two cards.
 #pragma once
 #include <cstdint>
 
@@ -36,14 +38,26 @@ struct NodeRec {
 };
 static_assert(sizeof(NodeRec) == 16, "NodeRec is 16 bytes");
 
+// Scan index entry: bits 0..14 = opponent hand index (SCAN_ZERO = the zero
+// slot closing a segment), bit 15 = first entry of a segment.
+constexpr uint16_t SCAN_HEAD = 0x8000;
+constexpr uint16_t SCAN_ZERO = 0x7FFF;
+
+// Per traversing hand: positions in the scan output (0xFFFF in .x low half =
+// hand not valid on this runout).
+//   x = lo | hi << 16          weaker / not-stronger bound in segment 0
+//   y = c1lo | c1hi << 16      same bounds inside card 1's segment
+//   z = c2lo | c2hi << 16      same bounds inside card 2's segment
+//   w = c1end | c2end << 16    the closing slot (total) of the two segments
+struct HandScan {
+    uint32_t x, y, z, w;
+};
+
 struct RunoutTab {
-    int32_t sorted;         // u16[H_opp]  opponent valid hands, strength ascending
-    int32_t card_off;       // u16[53]     segment offsets into card_hands
-    int32_t card_hands;     // u16[n]      opponent valid hands grouped by card
-    int32_t lohi;           // u32[H_player]  lo | hi << 16, 0xFFFF = hand not valid
-    int32_t card_lohi;      // u16x4[H_player] lo1, hi1, lo2, hi2 inside the two card segments
-    int32_t num_valid_opp;  // number of entries in `sorted`
-    int32_t pad[2];
+    int32_t scan_idx;       // u16[scan_len] index list (u16 pool)
+    int32_t scan_len;       // 3 * num_valid_opp + 53
+    int32_t hand_scan;      // HandScan[H_player] (HandScan pool)
+    int32_t num_valid_opp;  // entries of segment 0 before its closing slot
 };
 
 struct ChanceTab {          // chance node: reach scale and isomorphic-card swaps

@@ -40,7 +40,7 @@ struct HostLayout {
     std::vector<ChanceTab> chance;
     std::vector<int32_t> iso_ref, iso_perm;
     std::vector<uint16_t> u16;
-    std::vector<uint32_t> u32;
+    std::vector<HandScan> hand_scan;
     std::vector<int64_t> storage_size;   // per game
     int64_t state_total = 0, scratch_total = 0;
     int max_hands = 0;
@@ -54,15 +54,18 @@ int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, si
     return off;
 }
 
-// Showdown / fold tables of one runout for traversing player p.
+// Scan tables of one runout for traversing player p (layout in
+// dcfr_layout.hpp).  Segment 0 lists the opponent's valid hands in ascending
+// strength (valid order when the river is not dealt: folds only need the
+// totals); segment 1+c the valid opponent hands holding card c in the same
+// order.  Each segment ends with a SCAN_ZERO slot holding the total.
 RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayout& L) {
     const int o = p ^ 1;
     const auto& oc = g.private_cards(o);
     const auto& pcards = g.private_cards(p);
     const bool showdown = ro.river != NOT_DEALT;
-    // opponent valid hands in ascending strength (valid order for fold-only runouts)
     std::vector<uint16_t> sorted;
-    std::vector<uint16_t> opp_strength(oc.size(), 0);
+    std::vector<int> opp_strength(oc.size(), 0);
     if (showdown) {
         for (size_t k = 1; k + 1 < ro.strength[o].size(); ++k) {
             sorted.push_back(ro.strength[o][k].index);
@@ -71,65 +74,63 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     } else {
         sorted = ro.valid[o];
     }
-    // group by card, keeping the strength order inside each card
-    std::vector<uint16_t> card_off(53, 0), card_hands;
     std::vector<std::vector<uint16_t>> by_card(52);
     for (uint16_t oi : sorted) {
         by_card[oc[oi].first].push_back(oi);
         by_card[oc[oi].second].push_back(oi);
     }
-    for (int c = 0; c < 52; ++c) {
-        card_off[c] = (uint16_t)card_hands.size();
-        card_hands.insert(card_hands.end(), by_card[c].begin(), by_card[c].end());
-    }
-    card_off[52] = (uint16_t)card_hands.size();
+    std::vector<uint16_t> idx;
+    std::vector<int> seg_start(53);
+    auto push_segment = [&](int sid, const std::vector<uint16_t>& list) {
+        seg_start[sid] = (int)idx.size();
+        for (size_t k = 0; k < list.size(); ++k) idx.push_back((uint16_t)(list[k] | (k == 0 ? SCAN_HEAD : 0)));
+        idx.push_back((uint16_t)(SCAN_ZERO | (list.empty() ? SCAN_HEAD : 0)));
+    };
+    push_segment(0, sorted);
+    for (int c = 0; c < 52; ++c) push_segment(1 + c, by_card[c]);
 
     const int Hp = (int)pcards.size();
-    std::vector<uint32_t> lohi(Hp, 0xFFFFFFFFu);
-    std::vector<uint16_t> clohi((size_t)4 * Hp, 0);
     std::vector<int> pstrength(Hp, -1);
     if (showdown) {
         for (size_t k = 1; k + 1 < ro.strength[p].size(); ++k) pstrength[ro.strength[p][k].index] = ro.strength[p][k].strength;
     } else {
         for (uint16_t i : ro.valid[p]) pstrength[i] = 0;
     }
-    auto count_below = [&](const std::vector<uint16_t>& list, int s, bool inclusive) {
+    // number of list entries with strength < s (strict) or <= s
+    auto bound = [&](const std::vector<uint16_t>& list, int s, bool inclusive) {
         int n = 0;
         for (uint16_t oi : list) {
             int os = opp_strength[oi];
             if (os < s || (inclusive && os == s)) ++n;
-            else break;   // lists are strength-ascending
+            else break;
         }
         return n;
     };
-    for (int i = 0; i < Hp; ++i) {
-        if (pstrength[i] < 0) continue;
-        if (!showdown) {
-            lohi[i] = 0;
-            continue;
-        }
-        const int s = pstrength[i];
-        // sorted is ascending: binary search for the bounds
-        int lo = (int)(std::lower_bound(sorted.begin(), sorted.end(), s,
-                                        [&](uint16_t oi, int v) { return (int)opp_strength[oi] < v; }) -
-                       sorted.begin());
-        int hi = (int)(std::upper_bound(sorted.begin(), sorted.end(), s,
-                                        [&](int v, uint16_t oi) { return v < (int)opp_strength[oi]; }) -
-                       sorted.begin());
-        lohi[i] = (uint32_t)lo | ((uint32_t)hi << 16);
-        const int c1 = pcards[i].first, c2 = pcards[i].second;
-        clohi[4 * i + 0] = (uint16_t)count_below(by_card[c1], s, false);
-        clohi[4 * i + 1] = (uint16_t)count_below(by_card[c1], s, true);
-        clohi[4 * i + 2] = (uint16_t)count_below(by_card[c2], s, false);
-        clohi[4 * i + 3] = (uint16_t)count_below(by_card[c2], s, true);
-    }
+    auto pack = [](int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); };
     RunoutTab t{};
-    t.sorted = push_u16(L.u16, sorted);
-    t.card_off = push_u16(L.u16, card_off);
-    t.card_hands = push_u16(L.u16, card_hands);
-    t.card_lohi = push_u16(L.u16, clohi);
-    t.lohi = (int32_t)L.u32.size();
-    L.u32.insert(L.u32.end(), lohi.begin(), lohi.end());
+    t.hand_scan = (int32_t)L.hand_scan.size();
+    for (int i = 0; i < Hp; ++i) {
+        HandScan h{0xFFFFFFFFu, 0, 0, 0};
+        if (pstrength[i] >= 0) {
+            const int c1 = pcards[i].first, c2 = pcards[i].second;
+            const int s1 = seg_start[1 + c1], s2 = seg_start[1 + c2];
+            const int e1 = s1 + (int)by_card[c1].size(), e2 = s2 + (int)by_card[c2].size();
+            if (showdown) {
+                const int s = pstrength[i];
+                h.x = pack(bound(sorted, s, false), bound(sorted, s, true));
+                h.y = pack(s1 + bound(by_card[c1], s, false), s1 + bound(by_card[c1], s, true));
+                h.z = pack(s2 + bound(by_card[c2], s, false), s2 + bound(by_card[c2], s, true));
+            } else {
+                h.x = pack(0, 0);
+                h.y = pack(s1, s1);
+                h.z = pack(s2, s2);
+            }
+            h.w = pack(e1, e2);
+        }
+        L.hand_scan.push_back(h);
+    }
+    t.scan_idx = push_u16(L.u16, idx);
+    t.scan_len = (int32_t)idx.size();
     t.num_valid_opp = (int32_t)sorted.size();
     return t;
 }
@@ -281,7 +282,7 @@ struct psr_batch {
     ChanceTab* d_chance = nullptr;
     int32_t *d_iso_ref = nullptr, *d_iso_perm = nullptr;
     uint16_t* d_u16 = nullptr;
-    uint32_t* d_u32 = nullptr;
+    HandScan* d_hand_scan = nullptr;
     float *d_regret = nullptr, *d_strategy = nullptr, *d_reach = nullptr, *d_cfv = nullptr;
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
@@ -292,13 +293,13 @@ struct psr_batch {
 
     void release() {
         void* ptrs[] = {d_games, d_nodes, d_levels, d_payoff, d_cards, d_same, d_weights, d_terms, d_runouts,
-                        d_chance, d_iso_ref, d_iso_perm, d_u16, d_u32, d_regret, d_strategy, d_reach, d_cfv,
+                        d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_reach, d_cfv,
                         d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_order};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_nodes = nullptr; d_levels = nullptr; d_payoff = nullptr; d_cards = nullptr;
         d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr; d_chance = nullptr;
-        d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_u32 = nullptr; d_regret = nullptr;
+        d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr; d_regret = nullptr;
         d_strategy = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr; d_expl = nullptr;
         d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_order = nullptr;
         prepared = false;
@@ -321,7 +322,7 @@ struct psr_batch {
         a.iso_ref = d_iso_ref;
         a.iso_perm = d_iso_perm;
         a.u16pool = d_u16;
-        a.u32pool = d_u32;
+        a.hand_scan = d_hand_scan;
         a.regret = d_regret;
         a.strategy = d_strategy;
         a.reach = d_reach;
@@ -402,7 +403,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         upload(&b->d_payoff, L.payoff) || upload(&b->d_cards, L.cards) || upload(&b->d_same, L.same) ||
         upload(&b->d_weights, L.weights) || upload(&b->d_terms, L.terminals) || upload(&b->d_runouts, L.runouts) ||
         upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) || upload(&b->d_iso_perm, L.iso_perm) ||
-        upload(&b->d_u16, L.u16) || upload(&b->d_u32, L.u32) || upload(&b->d_order, L.order))
+        upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) || upload(&b->d_order, L.order))
         return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);

@@ -44,7 +44,7 @@ def load():
         "oracle_evaluate": (C.c_int, [vp, C.c_uint32, C.c_int, dp, dp]),
         "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "oracle_solve_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
-                                           C.c_uint32, C.c_int, dp]),
+                                           C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -138,14 +138,18 @@ class OracleGame:
             raise ValueError("bad boundary ordinal or length")
 
 
-def solve_many(blobs, iterations, threads):
-    """Solves serialized games with `threads` worker threads (one game per
-    thread at a time).  Returns (wall seconds, exploitability array)."""
+def solve_many(blobs, iterations, threads, targets=None):
+    """Solves serialized games with the reference solve() schedule using
+    `threads` worker threads (one game per thread at a time).  Returns
+    (wall seconds, exploitability array, iterations array)."""
     lib = load()
     n = len(blobs)
     bufs = [(C.c_uint8 * len(b)).from_buffer_copy(b) for b in blobs]
     ptrs = (C.POINTER(C.c_uint8) * n)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
     lens = (C.c_size_t * n)(*[len(b) for b in blobs])
     expl = np.zeros(n)
-    secs = lib.oracle_solve_many(ptrs, lens, n, iterations, threads, _dptr(expl))
-    return secs, expl
+    its = np.zeros(n, np.uint32)
+    tg = None if targets is None else np.ascontiguousarray(targets, np.float64)
+    secs = lib.oracle_solve_many(ptrs, lens, n, iterations, threads, None if tg is None else _dptr(tg), _dptr(expl),
+                                 its.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return secs, expl, its

@@ -1,0 +1,200 @@
+"""Parity of the HIP DCFR path (through the C ABI) with the CPU oracle.
+
+Bar: per-hand CFVs, regrets, strategy sums and exploitability within 1e-4
+relative to the vector's largest magnitude (f32 values; the device uses the
+reference operation order, so in practice they agree bit for bit), the
+reference solve() schedule (iterations run, early stop) identical.  Full-size
+batches are checked through size-independent properties.
+"""
+import numpy as np
+import pytest
+
+import oracle_binding as O
+import poker_solver_rust_amd as P
+from poker_solver_rust_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-4
+
+
+def close(gpu, ref, rtol=RTOL):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    assert gpu.shape == ref.shape
+    scale = max(np.abs(ref).max(), 1e-30)
+    err = np.abs(gpu - ref).max() / scale
+    assert err <= rtol, f"max rel err {err:.3e} ({int((gpu != ref).sum())} of {gpu.size} differ)"
+
+
+def std_game(bets="50%, a", raises=""):
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
+    tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
+                      river_bet_sizes=[(bets, raises), (bets, raises)])
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def test_stepwise_state_matches_oracle():
+    g = std_game()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 20)
+    o = O.OracleGame(g.serialize())
+    for t in range(12):
+        b.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, s = o.state()
+        close(b.regrets(0), r)
+        close(b.strategy_sum(0), s)
+    assert abs(float(b.exploitability()[0]) - o.exploitability()) <= RTOL * max(1.0, o.exploitability())
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+    close(cf[1], o.root_cfvalues(1))
+
+
+def test_solve_schedule_matches_oracle():
+    g = std_game("33%, 75%, a", "2.5x")
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 200, [0.5])
+    e, it = b.solve(200, 5)
+    o = O.OracleGame(g.serialize())
+    eo, ito = o.solve(200, 0.5)
+    assert int(it[0]) == ito
+    assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
+    r, s = o.state()
+    close(b.regrets(0), r)
+    close(b.strategy_sum(0), s)
+
+
+def test_gpu_solve_hand_parallel_api():
+    """gpu-range-solver lib.rs:214-245: exploitability >= ~0, < 1.0 after 500."""
+    g = std_game("100%")
+    res = P.gpu_solve_hand_parallel(g, P.GpuSolverConfig(max_iterations=500, target_exploitability=0.0))
+    assert -1e-4 < res.exploitability < 1.0 and res.iterations_run == 500
+    o = O.OracleGame(g.serialize())
+    eo, _ = o.solve(500)
+    assert abs(res.exploitability - eo) < 0.01
+    A, H = 2, g.num_private_hands(0)
+    strat = res.root_strategy[:A * H].reshape(A, H)
+    assert np.allclose(strat.sum(0), 1.0, atol=1e-6)
+
+
+def test_river_uniform_1000_iterations():
+    """1000-iteration river subgame, 1081 hands per player."""
+    s = W.river_uniform(0)
+    g = s.build()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 1000)
+    e, it = b.solve(1000, 5)
+    o = O.OracleGame(g.serialize())
+    eo, ito = o.solve(1000)
+    assert int(it[0]) == ito == 1000
+    assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
+    r, st = o.state()
+    close(b.strategy_sum(0), st)
+    close(b.regrets(0), r)
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+    close(cf[1], o.root_cfvalues(1))
+    assert float(e[0]) <= 0.003 * s.pot
+
+
+def test_river_batch_matches_oracle():
+    """Heterogeneous batch: every game agrees with its own oracle solve."""
+    subs = W.river_batch(12, seed=11, iterations=300)
+    games = [x.build() for x in subs]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 300, [x.target for x in subs])
+    e, it = b.solve(300, 5)
+    cfs = b.root_cfvalues()
+    for i, g in enumerate(games):
+        o = O.OracleGame(g.serialize())
+        eo, ito = o.solve(300, subs[i].target)
+        assert int(it[i]) == ito, i
+        assert abs(float(e[i]) - eo) <= RTOL * max(1.0, abs(eo)), i
+        r, st = o.state()
+        close(b.strategy_sum(i), st)
+        close(cfs[i][0], o.root_cfvalues(0))
+        close(cfs[i][1], o.root_cfvalues(1))
+
+
+def test_batch_equals_single():
+    subs = W.river_batch(6, seed=12, iterations=50)
+    games = [x.build() for x in subs]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 50)
+    b.run_iterations(0, 50)
+    single = P.GpuBatchSolver(0)
+    single.prepare_batch([games[3]], 50)
+    single.run_iterations(0, 50)
+    assert np.array_equal(b.strategy_sum(3), single.strategy_sum(0))
+    assert np.array_equal(b.regrets(3), single.regrets(0))
+
+
+def test_incremental_iterations_equal_one_launch():
+    """batch.rs:411-534: state persists across run_iterations calls."""
+    g = std_game()
+    a = P.GpuBatchSolver(0)
+    a.prepare_batch([g], 40)
+    a.run_iterations(0, 40)
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 40)
+    for s in range(0, 40, 7):
+        b.run_iterations(s, min(s + 7, 40))
+    assert np.array_equal(a.regrets(0), b.regrets(0))
+    b.reset()
+    assert not b.regrets(0).any() and not b.strategy_sum(0).any()
+
+
+def test_edge_games():
+    """Tiny stacks (all-in only), single-hand ranges and zero-weight combos."""
+    a = np.zeros(1326, np.float32)
+    c = np.zeros(1326, np.float32)
+    a[P.card_pair_to_index(P.card_from_str("Ah"), P.card_from_str("Ad"))] = 1.0
+    c[P.card_pair_to_index(P.card_from_str("Kh"), P.card_from_str("Kd"))] = 1.0
+    c[P.card_pair_to_index(P.card_from_str("Qh"), P.card_from_str("Qd"))] = 0.25
+    games = []
+    for stack, bets in [(1, "50%"), (100, "50%, a"), (1000, "25%, 75%, 150%, a")]:
+        cc = P.CardConfig(range=(P.Range(a), P.Range(c)), flop=P.flop_from_str("2c7s9h"),
+                          turn=P.card_from_str("Js"), river=P.card_from_str("3c"))
+        tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=stack,
+                          river_bet_sizes=[(bets, "2x, a"), (bets, "2x, a")])
+        games.append(P.PostFlopGame.with_config(cc, P.ActionTree(tc)))
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 100)
+    e, it = b.solve(100, 5)
+    for i, g in enumerate(games):
+        o = O.OracleGame(g.serialize())
+        eo, ito = o.solve(100)
+        assert int(it[i]) == ito and abs(float(e[i]) - eo) <= 1e-5
+        close(b.strategy_sum(i), o.state()[1])
+
+
+def test_full_size_batch_properties():
+    """4096 seeded subgames at full size: exploitability >= 0, every game at
+    or near 0.3% pot, and the unraked zero-sum identity of the root values
+    under the average strategy (sum_h w_h * cfv_h over both players = 0)."""
+    subs = W.river_batch(4096, seed=20260, iterations=300)
+    games = [x.build() for x in subs]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 300, [x.target for x in subs])
+    e, it = b.solve(300, 5)
+    pots = np.array([x.pot for x in subs], np.float64)
+    assert np.all(e >= -1e-4 * pots)
+    assert np.mean(e <= 0.003 * pots + 1e-6) >= 0.99
+    cfs = b.root_cfvalues()
+    for i in range(0, 4096, 97):
+        w0, w1 = games[i].initial_weights(0), games[i].initial_weights(1)
+        v0 = float(np.dot(cfs[i][0].astype(np.float64), w0))
+        v1 = float(np.dot(cfs[i][1].astype(np.float64), w1))
+        assert abs(v0 + v1) <= 1e-4 * max(abs(v0), abs(v1), 1e-3 * pots[i]), i
+
+
+def test_error_paths():
+    b = P.GpuBatchSolver(0)
+    with pytest.raises(P.PsrError):
+        b.run_iterations(0, 1)           # before prepare_batch (batch.rs:419-422)
+    b.prepare_batch([std_game()], 10)
+    with pytest.raises(P.PsrError):
+        b.run_iterations(0, 11)          # beyond the prepared iterations
+    with pytest.raises(P.PsrError):
+        b.solve(11, 5)

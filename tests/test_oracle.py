@@ -1,0 +1,176 @@
+"""Pins for the CPU oracle (oracle/dcfr_oracle.cpp), independent of the
+oracle's own formulas:
+  - DiscountParams known values (solver.rs:688-722)
+  - fold / showdown values against an O(H^2) pairwise brute force (the
+    dense formulation of gpu-range-solver extract.rs:306-341) built from the
+    7-card evaluator on random reaches
+  - convergence thresholds of the reference's tests (solver.rs:772-805,
+    gpu-range-solver lib.rs:214-245)
+  - invariants: exploitability >= 0, best response >= current value,
+    unraked zero sum
+  - a closed-form game value (complete information: the winner takes the pot)
+"""
+import numpy as np
+import pytest
+
+import oracle_binding as O
+import poker_solver_rust_amd as P
+from poker_solver_rust_amd import workloads as W
+
+
+def std_game(bets="50%, a"):
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
+    tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
+                      river_bet_sizes=[(bets, ""), (bets, "")])
+    return cc, P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def test_discount_params_known_values():
+    a, b, g = O.discount(0)
+    assert (a, b, g) == (0.0, 0.5, 0.0)
+    a, b, g = O.discount(1)
+    assert a == 0.0 and b == 0.5
+    assert O.discount(4)[2] == 0.0          # strategy-sum restart at powers of 4
+    assert abs(O.discount(5)[2] - 0.125) < 1e-6
+    assert O.discount(16)[2] == 0.0 and O.discount(64)[2] == 0.0
+    assert O.discount(100)[0] > 0.9 and O.discount(100)[2] > 0.0
+    # closed forms: alpha = (t-1)^1.5 / ((t-1)^1.5 + 1); gamma = (t'/(t'+1))^3
+    for t in [2, 3, 7, 33, 250, 999]:
+        x = (t - 1) ** 1.5
+        p4 = 4 ** int(np.floor(np.log(t) / np.log(4)))
+        tg = t - p4
+        assert O.discount(t)[0] == np.float32(x / (x + 1))
+        assert abs(O.discount(t)[2] - (tg / (tg + 1)) ** 3) < 1e-7
+
+
+def terminal_nodes(blob):
+    """(node index, is_fold, folded player, amount) of every terminal node,
+    read from the serialized arena (format in game.cpp)."""
+    import struct
+    off = 8 + 4 + 4 + 8 + 8 + 8 + 8
+    hands = []
+    for p in range(2):
+        (h,) = struct.unpack_from("<I", blob, off)
+        off += 4 + 2 * h + 4 * h + 2 * h
+        hands.append(h)
+    (n,) = struct.unpack_from("<I", blob, off)
+    off += 4
+    out = []
+    for i in range(n):
+        player, turn, river, _, amount, first, num, runout = struct.unpack_from("<BBBBiIIi", blob, off)
+        off += 20
+        if player & 8:
+            out.append((i, (player & 24) == 24, player & 3, amount))
+    return out
+
+
+def brute_force(game, board, node, player, reach, pot):
+    """O(H^2): sum over non-conflicting opponent hands of reach * payoff."""
+    pc, oc = game.private_cards(player), game.private_cards(player ^ 1)
+    ncomb = game.num_combinations()
+    half = 0.5 * (pot)
+    if node[1]:
+        pay = half / ncomb if node[2] != player else -half / ncomb
+    s_p = [P.hand_evaluate(board + [a, b]) for a, b in pc]
+    s_o = [P.hand_evaluate(board + [a, b]) for a, b in oc]
+    out = np.zeros(len(pc))
+    for i, (a, b) in enumerate(pc):
+        acc = 0.0
+        for j, (c, d) in enumerate(oc):
+            if {a, b} & {c, d}:
+                continue
+            if node[1]:
+                acc += pay * reach[j]
+            elif s_p[i] > s_o[j]:
+                acc += half / ncomb * reach[j]
+            elif s_p[i] < s_o[j]:
+                acc -= half / ncomb * reach[j]
+        out[i] = acc
+    return out
+
+
+@pytest.mark.parametrize("which", ["std", "uniform"])
+def test_terminal_values_match_brute_force(which):
+    rng = np.random.default_rng(7)
+    if which == "std":
+        cc, g = std_game()
+        board = list(cc.flop) + [cc.turn, cc.river]
+        pot0 = 100
+    else:
+        s = W.river_uniform(3)
+        g = s.build()
+        board = s.board
+        pot0 = s.pot
+    blob = g.serialize()
+    og = O.OracleGame(blob)
+    terms = terminal_nodes(blob)
+    assert any(t[1] for t in terms) and any(not t[1] for t in terms)
+    for t in terms[:6] if which == "uniform" else terms:
+        for p in range(2):
+            reach = rng.random(g.num_private_hands(p ^ 1)).astype(np.float32).astype(np.float64)
+            reach[rng.random(len(reach)) < 0.2] = 0.0
+            got = og.evaluate(t[0], p, reach)
+            ref = brute_force(g, board, t, p, reach, pot0 + 2 * t[3])
+            scale = np.abs(ref).max()
+            assert np.abs(got - ref).max() <= 1e-6 * scale + 1e-12, (t, p)
+
+
+def test_convergence_thresholds():
+    """solver.rs:772-805: expl < 10 after 100 iterations; gpu lib.rs:214-245:
+    below 1.0 (reference GPU) after 500, CPU reaches < 0.01."""
+    _, g = std_game()
+    og = O.OracleGame(g.serialize())
+    e, it = og.solve(100)
+    assert e < 10.0 and it == 100
+    og = O.OracleGame(g.serialize())
+    e, it = og.solve(500)
+    assert 0.0 <= e < 0.01
+
+
+def test_early_termination():
+    _, g = std_game()
+    og = O.OracleGame(g.serialize())
+    e, it = og.solve(1000, target=1.0)
+    assert e <= 1.0 and it < 1000 and it % 5 == 0
+
+
+def test_invariants_along_the_solve():
+    _, g = std_game("33%, 75%, a")
+    og = O.OracleGame(g.serialize())
+    for t in range(60):
+        og.solve_step(t)
+        if t % 7 == 0:
+            mes, cur = og.mes_ev(), og.current_ev()
+            assert og.exploitability() >= -1e-5
+            assert np.all(mes >= cur - 1e-4)
+            assert abs(cur[0] + cur[1]) <= 1e-4 * max(1.0, np.abs(cur).max())
+
+
+def test_complete_information_value():
+    """One hand each: the winner takes half the pot whatever is played, so the
+    solved root value is exactly +-half pot per combination."""
+    a = np.zeros(1326, np.float32)
+    b = np.zeros(1326, np.float32)
+    a[P.card_pair_to_index(P.card_from_str("Ah"), P.card_from_str("Ad"))] = 1.0
+    b[P.card_pair_to_index(P.card_from_str("Kh"), P.card_from_str("Kd"))] = 1.0
+    cc = P.CardConfig(range=(P.Range(a), P.Range(b)), flop=P.flop_from_str("2c7s9h"), turn=P.card_from_str("Js"),
+                      river=P.card_from_str("3c"))
+    tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
+                      river_bet_sizes=[("50%, a", "2x"), ("50%, a", "2x")])
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    og = O.OracleGame(g.serialize())
+    e, _ = og.solve(300)
+    assert e < 1e-3
+    assert abs(og.root_cfvalues(0)[0] - 50.0) < 0.05
+    assert abs(og.root_cfvalues(1)[0] + 50.0) < 0.05
+
+
+def test_f64_oracle_close_to_f32():
+    _, g = std_game()
+    o32 = O.OracleGame(g.serialize(), 32)
+    o64 = O.OracleGame(g.serialize(), 64)
+    e32, _ = o32.solve(200)
+    e64, _ = o64.solve(200)
+    assert abs(e32 - e64) < 1e-3
+    assert np.abs(o32.root_cfvalues(0) - o64.root_cfvalues(0)).max() < 1e-3
