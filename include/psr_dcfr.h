@@ -170,6 +170,11 @@ int psr_batch_regrets(psr_batch* batch, int game, float* out, int64_t cap);
 /* Wall time of the last device job in milliseconds (HIP events on the batch
  * stream) -- used for roofline accounting. */
 float psr_batch_last_kernel_ms(psr_batch* batch);
+/* Host-only check of the device layout of one game (no HIP calls): tiles,
+ * levels, child placement, state and scan-table bounds.  out = tiles, levels,
+ * terminals, widest level, state floats, nodes.  No reference counterpart
+ * (layout self-check of this library). */
+int psr_layout_validate(const psr_game* game, int64_t out[6]);
 /* Device properties: number of CUs and the workgroups launched. */
 int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
 

@@ -34,8 +34,6 @@ constexpr int WAVE = 64;
 constexpr int NWAVES = BLOCK / WAVE;
 constexpr int MAXA = 16;   // largest action count of a decision node on the device
 
-enum Mode { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2 };
-
 __device__ __forceinline__ float fmax0(float x) { return x > 0.0f ? x : 0.0f; }
 __device__ __forceinline__ float fmaxr(float x, float y) { return x > y ? x : y; }
 
@@ -49,7 +47,6 @@ struct Ctx {
     const KernelArgs& A;
     GameDesc G;
     const NodeRec* nodes;
-    const int32_t* levels;
     float* reach;
     float* cfv;
     int stride;
@@ -92,18 +89,27 @@ __device__ __forceinline__ float strat_of(float v, float denom, float def) {
     return denom == 0.0f ? def : __fdiv_rn(v, denom);
 }
 
+// Forward pass over one tile: reach of the opponent of p, level by level.
+// The game-root tile seeds its root row with the opponent's initial weights;
+// a subtree tile finds its root row already written by the chance node above.
 template <int MODE>
-__device__ void forward(Ctx& c, int p) {
+__device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Ho = c.G.H[o];
-    const float* w = A.hand_weight + c.G.hand_base[o];
-    for (int h = threadIdx.x; h < Ho; h += BLOCK) c.reach[h] = w[h];
-    __syncthreads();
-    for (int L = 0; L + 1 < c.G.num_levels; ++L) {
-        const int n0 = c.levels[L], n1 = c.levels[L + 1];
+    const LevelRange* lv = A.level_ranges + T.level_base;
+    if (T.flags & TILE_GAME_ROOT) {
+        const float* w = A.hand_weight + c.G.hand_base[o];
+        float* r0 = c.reach + (size_t)lv[0].start * c.stride;
+        for (int h = threadIdx.x; h < Ho; h += BLOCK) r0[h] = w[h];
+        __syncthreads();
+    }
+    for (int L = 0; L < T.num_levels; ++L) {
+        const int n0 = lv[L].start, n1 = lv[L].end;
+        bool any = false;
         for (int n = n0; n < n1; ++n) {
             const NodeRec nd = c.nodes[n];
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
+            any = true;
             const int na = nd.num_children;
             const float* rn = c.reach + (size_t)n * c.stride;
             float* rc = c.reach + (size_t)nd.first_child * c.stride;
@@ -125,7 +131,7 @@ __device__ void forward(Ctx& c, int p) {
                 }
             }
         }
-        __syncthreads();
+        if (any) __syncthreads();
     }
 }
 
@@ -231,15 +237,19 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     __syncthreads();
 }
 
+// Backward pass over one tile, deepest level first.
 template <int MODE>
-__device__ void backward(Ctx& c, int p, const float4 disc) {
+__device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int Hp = c.G.H[p];
-    for (int L = c.G.num_levels - 2; L >= 0; --L) {
-        const int n0 = c.levels[L], n1 = c.levels[L + 1];
+    const LevelRange* lv = A.level_ranges + T.level_base;
+    for (int L = T.num_levels - 1; L >= 0; --L) {
+        const int n0 = lv[L].start, n1 = lv[L].end;
+        bool any = false;
         for (int n = n0; n < n1; ++n) {
             const NodeRec nd = c.nodes[n];
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
+            any = true;
             const int na = nd.num_children;
             float* out = c.cfv + (size_t)n * c.stride;
             const float* cc = c.cfv + (size_t)nd.first_child * c.stride;
@@ -294,43 +304,67 @@ __device__ void backward(Ctx& c, int p, const float4 disc) {
                 }
             }
         }
-        __syncthreads();
+        if (any) __syncthreads();
     }
 }
 
-template <int MODE>
-__device__ void traverse(Ctx& c, int p, const float4 disc) {
-    forward<MODE>(c, p);
-    const int32_t* terms = c.A.terminals + c.G.term_base;
-    for (int k = 0; k < c.G.num_terminals; ++k) {
+__device__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
+    const int32_t* terms = c.A.terminals + T.term_base;
+    for (int k = 0; k < T.num_terminals; ++k) {
         const int n = terms[k];
         const NodeRec nd = c.nodes[n];
         eval_terminal(c, nd, n, p);
     }
-    backward<MODE>(c, p, disc);
 }
 
-// Exploitability in chips, utility.rs:380-392 (unraked): mean of the two
-// best-response values, each an f64 weighted sum cast to f32.
-__device__ float exploitability(Ctx& c) {
+template <int MODE>
+__device__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
+    tile_forward<MODE>(c, T, p);
+    tile_terminals(c, T, p);
+    tile_backward<MODE>(c, T, p, disc);
+}
+
+__device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, float* rs, Shared* sh) {
+    Ctx c{A, A.games[gi], nullptr, nullptr, nullptr, 0, scan, rs, sh};
+    c.nodes = A.nodes + c.G.node_base;
+    c.reach = A.reach + c.G.scratch_base;
+    c.cfv = A.cfv + c.G.scratch_base;
+    c.stride = c.G.row_stride;
+    return c;
+}
+
+// f64 weighted sum of the game-root cfvalues of p with p's initial weights
+// (utility.rs:99-103), identical on every thread.
+__device__ double root_value(Ctx& c, int p) {
+    const float* w = c.A.hand_weight + c.G.hand_base[p];
+    const float* root = c.cfv + (size_t)c.A.level_ranges[c.A.tiles[c.G.tile_base].level_base].start * c.stride;
+    double s = 0.0;
+    for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) s += (double)root[h] * (double)w[h];
+    return block_sum(s, c.sh);
+}
+
+// Exploitability in chips of a single-tile game, utility.rs:380-392
+// (unraked): mean of the two best-response values, each f64 cast to f32.
+__device__ float exploitability(Ctx& c, const TileDesc& T) {
     float ev[2];
     const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int p = 0; p < 2; ++p) {
-        traverse<MODE_BR>(c, p, none);
-        const float* w = c.A.hand_weight + c.G.hand_base[p];
-        double s = 0.0;
-        for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) s += (double)c.cfv[h] * (double)w[h];
-        ev[p] = (float)block_sum(s, c.sh);
+        tile_traverse<MODE_BR>(c, T, p, none);
+        ev[p] = (float)root_value(c, p);
     }
     return __fmul_rn(__fadd_rn(ev[0], ev[1]), 0.5f);
 }
 
-__global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
-    extern __shared__ double lds[];
-    __shared__ Shared sh;
-    double* scan = lds;
-    float* rs = reinterpret_cast<float*>(lds + 3 * A.max_hands + 53);
+#define LDS_SETUP()                                                        \
+    extern __shared__ double lds[];                                        \
+    __shared__ Shared sh;                                                  \
+    double* scan = lds;                                                    \
+    float* rs = reinterpret_cast<float*>(lds + 3 * A.max_hands + 53)
 
+// Whole-game solve of single-tile games (river subgames): one workgroup runs
+// every iteration of one game, games come from the work queue.
+__global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
+    LDS_SETUP();
     for (;;) {
         if (threadIdx.x == 0) sh.job = atomicAdd(A.queue_counter, 1);
         __syncthreads();
@@ -338,33 +372,30 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
         __syncthreads();
         if (q >= A.num_games) break;
         const int gi = A.queue_order[q];
-        Ctx c{A, A.games[gi], nullptr, nullptr, nullptr, nullptr, 0, scan, rs, &sh};
-        c.nodes = A.nodes + c.G.node_base;
-        c.levels = A.level_start + c.G.level_base;
-        c.reach = A.reach + c.G.scratch_base;
-        c.cfv = A.cfv + c.G.scratch_base;
-        c.stride = c.G.row_stride;
+        Ctx c = make_ctx(A, gi, scan, rs, &sh);
+        const TileDesc T = A.tiles[c.G.tile_base];
 
         float expl = __int_as_float(0x7f800000);   // +inf: no check done yet
-        if (A.flags & JOB_INITIAL_CHECK) expl = exploitability(c);
+        if (A.flags & JOB_INITIAL_CHECK) expl = exploitability(c, T);
         uint32_t t = A.iter_begin;
         if (A.flags & JOB_ITERATE) {
             for (; t < A.iter_end; ++t) {
                 if (expl <= c.G.target) break;
                 const float4 d = A.discount[t];
-                traverse<MODE_DCFR>(c, 0, d);
-                traverse<MODE_DCFR>(c, 1, d);
+                tile_traverse<MODE_DCFR>(c, T, 0, d);
+                tile_traverse<MODE_DCFR>(c, T, 1, d);
                 if (A.check_interval > 0 && ((t + 1) % A.check_interval == 0 || t + 1 == A.max_iterations))
-                    expl = exploitability(c);
+                    expl = exploitability(c, T);
             }
         }
-        if (A.flags & JOB_FINAL_CHECK) expl = exploitability(c);
+        if (A.flags & JOB_FINAL_CHECK) expl = exploitability(c, T);
         if (A.flags & JOB_ROOT_CFV) {
             const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
             for (int p = 0; p < 2; ++p) {
-                traverse<MODE_CFV>(c, p, none);
+                tile_traverse<MODE_CFV>(c, T, p, none);
                 float* dst = A.root_cfv + c.G.hand_base[p];
-                for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = c.cfv[h];
+                const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
+                for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = root[h];
                 __syncthreads();
             }
         }
@@ -373,6 +404,59 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
             A.iters_out[gi] = t;
         }
     }
+}
+
+// One phase of one traversal of multi-tile games (turn subgames).  The host
+// (or a captured graph) issues ROOT_FORWARD, SUBTREES, ROOT_BACKWARD per
+// traversing player.  Games whose expl_out is already <= target are skipped
+// (the reference solve() stops iterating a converged game).
+template <int MODE>
+__device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared& sh) {
+    const int p = A.player;
+    const float4 disc = MODE == MODE_DCFR ? A.discount[A.iteration] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (A.phase == PHASE_ROOT_FORWARD || A.phase == PHASE_ROOT_BACKWARD) {
+        if (A.phase == PHASE_ROOT_FORWARD && blockIdx.x == 0 && threadIdx.x == 0) A.queue_counter[1] = 0;
+        for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {
+            const int gi = A.multi_games[k];
+            Ctx c = make_ctx(A, gi, scan, rs, &sh);
+            if (MODE == MODE_DCFR && A.expl_out[gi] <= c.G.target) continue;
+            const TileDesc T = A.tiles[c.G.tile_base];
+            if (A.phase == PHASE_ROOT_FORWARD) {
+                tile_forward<MODE>(c, T, p);
+            } else {
+                tile_terminals(c, T, p);
+                tile_backward<MODE>(c, T, p, disc);
+                if (MODE == MODE_BR) {
+                    const double v = root_value(c, p);
+                    if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
+                } else if (MODE == MODE_CFV) {
+                    float* dst = A.root_cfv + c.G.hand_base[p];
+                    const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
+                    for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = root[h];
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) sh.job = atomicAdd(A.queue_counter + 1, 1);
+        __syncthreads();
+        const int q = sh.job;
+        __syncthreads();
+        if (q >= A.num_subtree_tiles) break;
+        const TileDesc T = A.tiles[A.subtree_tiles[q]];
+        Ctx c = make_ctx(A, T.game, scan, rs, &sh);
+        if (MODE == MODE_DCFR && A.expl_out[T.game] <= c.G.target) continue;
+        tile_traverse<MODE>(c, T, p, disc);
+    }
+}
+
+__global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs A) {
+    LDS_SETUP();
+    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR>(A, scan, rs, sh);
+    else if (A.mode == MODE_BR) phase_body<MODE_BR>(A, scan, rs, sh);
+    else phase_body<MODE_CFV>(A, scan, rs, sh);
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -393,6 +477,11 @@ size_t dcfr_lds_bytes(int max_hands) {
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, args.queue_counter);
     hipLaunchKernelGGL(dcfr_solve_kernel, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+    return hipGetLastError();
+}
+
+hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(dcfr_phase_kernel, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
     return hipGetLastError();
 }
 

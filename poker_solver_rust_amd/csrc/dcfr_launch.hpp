@@ -15,12 +15,21 @@ enum JobFlags : int {
     JOB_ROOT_CFV = 8,        // root per-hand cfvalues under the average strategy
 };
 
+enum Phase : int {
+    PHASE_ROOT_FORWARD = 0,   // root tile forward, one block per multi-tile game
+    PHASE_SUBTREES = 1,       // every subtree tile: forward, terminals, backward (work queue)
+    PHASE_ROOT_BACKWARD = 2,  // root tile terminals + backward, one block per game
+};
+
+enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2 };
+
 struct KernelArgs {
     const GameDesc* games;
     int num_games;
     int max_hands;
+    const TileDesc* tiles;
+    const LevelRange* level_ranges;
     const NodeRec* nodes;
-    const int32_t* level_start;
     const double* payoff;
     const uint16_t* hand_cards;
     const uint16_t* hand_same;
@@ -41,14 +50,23 @@ struct KernelArgs {
     uint32_t* iters_out;
     float* root_cfv;          // indexed like the hand arrays
     int* queue_counter;
-    const int* queue_order;
+    const int* queue_order;   // single-tile games (whole-game solve), largest first
     uint32_t iter_begin, iter_end, max_iterations;
     int check_interval;       // 0: no periodic check
     int flags;
+    // phase launches (multi-tile games)
+    const int* multi_games;   // game ids of multi-tile games
+    int num_multi_games;
+    const int* subtree_tiles; // global tile ids of all subtree tiles, largest first
+    int num_subtree_tiles;
+    int phase, player, mode;
+    uint32_t iteration;
+    double* ev_out;           // per game and player: weighted root value (best response)
 };
 
 size_t dcfr_lds_bytes(int max_hands);
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream);
+hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_occupancy(int max_hands, int* blocks_per_cu);
 int dcfr_block_size();
 int dcfr_max_actions();

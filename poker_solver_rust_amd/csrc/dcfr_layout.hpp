@@ -3,9 +3,14 @@
 // plain structs only.
 //
 // HBM layout (all arrays concatenated over the batch):
-//   nodes[]      NodeRec per node, each game's nodes in breadth-first order,
-//                so depth level L of a game is the contiguous range
-//                level_start[level_base+L] .. level_start[level_base+L+1]
+//   nodes[]      NodeRec per node in device order.  A game is cut into tiles:
+//                tile 0 holds the nodes down to (and including) the first
+//                chance level, every child of a chance node roots one more
+//                tile (a river subtree).  Inside a tile nodes are breadth-
+//                first, so each depth level is one contiguous node range
+//                (level_ranges[level_base + L]); the roots of one chance
+//                node's tiles are contiguous so the chance node addresses
+//                them as its children.  A river game is a single tile.
 //   payoff[]     3 x f64 per node (win, lose, tie amount per combination)
 //   hands[]      per game and player: u16 card pair (c1 | c2 << 8), u16 index
 //                of the same hand in the opponent's list, f32 initial weight
@@ -67,18 +72,32 @@ struct ChanceTab {          // chance node: reach scale and isomorphic-card swap
     int32_t pad;
 };
 
+struct LevelRange {
+    int32_t start, end;     // game-local device node ids [start, end)
+};
+
+enum TileFlags : int32_t { TILE_GAME_ROOT = 1 };
+
+struct TileDesc {
+    int32_t game;
+    int32_t level_base;     // into level_ranges
+    int32_t num_levels;
+    int32_t term_base;      // into terminals (game-local device node ids)
+    int32_t num_terminals;
+    int32_t flags;          // TILE_GAME_ROOT: level 0 is the game root
+    int32_t pad[2];
+};
+
 struct GameDesc {
     int32_t node_base, num_nodes;
-    int32_t level_base, num_levels;
     int32_t H[2];
     int32_t hand_base[2];   // into hands arrays
     int64_t state_base;     // float offset into regret/strategy
     int64_t scratch_base;   // float offset into reach/cfv
     int32_t row_stride;     // floats per scratch row (multiple of 64)
-    int32_t num_terminals;
-    int32_t term_base;      // into terminal list (local node ids)
+    int32_t tile_base;      // tile 0 of the game is its root tile
+    int32_t num_tiles;
     float target;           // early-stop exploitability (chips)
-    int32_t pad[2];
 };
 
 }  // namespace psr

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -41,10 +42,15 @@ struct HostLayout {
     std::vector<int32_t> iso_ref, iso_perm;
     std::vector<uint16_t> u16;
     std::vector<HandScan> hand_scan;
+    std::vector<TileDesc> tiles;
+    std::vector<LevelRange> level_ranges;
     std::vector<int64_t> storage_size;   // per game
     int64_t state_total = 0, scratch_total = 0;
     int max_hands = 0;
-    std::vector<int> order;   // work-queue order, most expensive first
+    std::vector<int> single_games;   // whole-game work queue, most expensive first
+    std::vector<int> multi_games;    // games solved by phase launches (turn)
+    std::vector<std::pair<double, int>> subtree_cost;
+    std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
 };
 
 int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, size_t align = 4) {
@@ -135,6 +141,105 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     return t;
 }
 
+// Device node order of one game: tile 0 = nodes reachable from the root
+// without entering a chance node's children (breadth-first), then for each
+// chance node its children (subtree roots, contiguous), then the body of each
+// subtree (breadth-first).  Returns per tile the level ranges and terminals.
+struct TilePlan {
+    std::vector<int> order;                         // device position -> arena index
+    std::vector<std::vector<std::pair<int, int>>> levels;   // per tile, per level [start,end)
+    std::vector<std::vector<int>> terminals;        // per tile, device ids
+    std::vector<double> cost;                       // per tile
+};
+
+int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
+    const auto& nodes = g.nodes();
+    const int N = (int)nodes.size();
+    std::vector<int> depth(N, 0);
+    for (int i = 0; i < N; ++i)
+        for (uint32_t a = 0; a < nodes[i].num_children; ++a) depth[nodes[i].first_child + a] = depth[i] + 1;
+    // root tile: breadth-first from the root, not entering chance children
+    std::vector<int> root_tile, chance_nodes;
+    std::vector<char> in_root(N, 0);
+    root_tile.push_back(0);
+    in_root[0] = 1;
+    for (size_t k = 0; k < root_tile.size(); ++k) {
+        const GameNode& n = nodes[root_tile[k]];
+        if (n.is_terminal()) continue;
+        if (n.is_chance()) {
+            chance_nodes.push_back(root_tile[k]);
+            continue;
+        }
+        for (uint32_t a = 0; a < n.num_children; ++a) {
+            root_tile.push_back((int)(n.first_child + a));
+            in_root[n.first_child + a] = 1;
+        }
+    }
+    std::stable_sort(root_tile.begin(), root_tile.end(), [&](int a, int b) { return depth[a] < depth[b]; });
+    plan.order = root_tile;
+    plan.levels.assign(1, {});
+    plan.terminals.assign(1, {});
+    plan.cost.assign(1, 0.0);
+    std::vector<int> pos(N, -1);
+    for (size_t k = 0; k < plan.order.size(); ++k) pos[plan.order[k]] = (int)k;
+    for (size_t k = 0; k < root_tile.size(); ++k) {
+        const int d = depth[root_tile[k]];
+        if (plan.levels[0].empty() || depth[plan.order[plan.levels[0].back().first]] != d)
+            plan.levels[0].push_back({(int)k, (int)k + 1});
+        else
+            plan.levels[0].back().second = (int)k + 1;
+        if (nodes[root_tile[k]].is_terminal()) plan.terminals[0].push_back((int)k);
+        plan.cost[0] += nodes[root_tile[k]].num_children;
+    }
+    // subtree roots, contiguous per chance node
+    std::vector<int> roots;
+    for (int c : chance_nodes) {
+        const GameNode& n = nodes[c];
+        for (uint32_t a = 0; a < n.num_children; ++a) {
+            const int r = (int)(n.first_child + a);
+            const GameNode& rn = nodes[r];
+            if (rn.is_chance()) return fail("nested chance levels (flop games) are not supported by the device path");
+            pos[r] = (int)plan.order.size();
+            plan.order.push_back(r);
+            roots.push_back(r);
+        }
+    }
+    // subtree bodies
+    for (int r : roots) {
+        std::vector<std::pair<int, int>> lv;
+        std::vector<int> terms;
+        lv.push_back({pos[r], pos[r] + 1});
+        if (nodes[r].is_terminal()) terms.push_back(pos[r]);
+        double cost = 1.0;
+        std::vector<int> body;
+        std::vector<int> frontier{r};
+        while (!frontier.empty()) {
+            std::vector<int> next;
+            for (int v : frontier) {
+                const GameNode& n = nodes[v];
+                if (n.is_terminal()) continue;
+                if (n.is_chance()) return fail("nested chance levels (flop games) are not supported by the device path");
+                for (uint32_t a = 0; a < n.num_children; ++a) next.push_back((int)(n.first_child + a));
+            }
+            if (next.empty()) break;
+            const int start = (int)plan.order.size();
+            for (int v : next) {
+                pos[v] = (int)plan.order.size();
+                plan.order.push_back(v);
+                if (nodes[v].is_terminal()) terms.push_back(pos[v]);
+                cost += 1.0 + nodes[v].num_children;
+            }
+            lv.push_back({start, (int)plan.order.size()});
+            frontier.swap(next);
+        }
+        plan.levels.push_back(lv);
+        plan.terminals.push_back(terms);
+        plan.cost.push_back(cost);
+    }
+    if ((int)plan.order.size() != N) return fail("tile plan does not cover the game");
+    return 0;
+}
+
 int build_layout(const psr_game* const* games, int n, const float* targets, HostLayout& L) {
     const int maxa = dcfr_max_actions();
     std::vector<double> cost(n);
@@ -143,6 +248,11 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         const TreeConfig& tc = g.tree_config();
         if (tc.rake_rate > 0.0 && tc.rake_cap > 0.0) return fail("raked games are not supported by the device path");
         const auto& nodes = g.nodes();
+        TilePlan plan;
+        if (plan_tiles(g, plan) != 0) return -1;
+        std::vector<int> pos(nodes.size());
+        for (size_t k = 0; k < plan.order.size(); ++k) pos[plan.order[k]] = (int)k;
+
         GameDesc d{};
         d.node_base = (int32_t)L.nodes.size();
         d.num_nodes = (int32_t)nodes.size();
@@ -155,7 +265,6 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         d.scratch_base = L.scratch_total;
         d.target = targets ? targets[gi] : 0.0f;
 
-        // hands
         for (int p = 0; p < 2; ++p) {
             d.hand_base[p] = (int32_t)L.cards.size();
             const auto& hc = g.private_cards(p);
@@ -165,49 +274,46 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
                 L.weights.push_back(g.initial_weights(p)[i]);
             }
         }
-        // runout tables (global ids)
         const int runout_base = (int)L.runouts.size() / 2;
         for (const Runout& ro : g.runouts())
             for (int p = 0; p < 2; ++p) L.runouts.push_back(build_runout(g, ro, p, L));
 
-        // levels (arena is breadth-first)
-        std::vector<int> depth(nodes.size(), 0);
-        for (size_t i = 0; i < nodes.size(); ++i)
-            for (uint32_t a = 0; a < nodes[i].num_children; ++a) depth[nodes[i].first_child + a] = depth[i] + 1;
-        d.level_base = (int32_t)L.levels.size();
-        int nlev = depth.empty() ? 0 : depth.back() + 1;
-        for (int lv = 0, i = 0; lv <= nlev; ++lv) {
-            while (i < (int)nodes.size() && depth[i] < lv) ++i;
-            L.levels.push_back(i);
-        }
-        d.num_levels = nlev;
-
-        // nodes
+        // state offsets in arena order (the oracle's order)
+        std::vector<int64_t> state_off(nodes.size(), -1);
         int64_t off = 0;
-        d.term_base = (int32_t)L.terminals.size();
-        const double half_div = g.num_combinations();
         for (size_t i = 0; i < nodes.size(); ++i) {
             const GameNode& gn = nodes[i];
+            if (gn.is_terminal() || gn.is_chance()) continue;
+            if (gn.num_children > maxa) return fail("node has more actions than the device supports");
+            state_off[i] = off;
+            off += (int64_t)gn.num_children * d.H[gn.player];
+        }
+        if (off > INT32_MAX) return fail("game too large for 32-bit state offsets");
+
+        // nodes in device order
+        const double ncomb = g.num_combinations();
+        for (size_t k = 0; k < plan.order.size(); ++k) {
+            const int i = plan.order[k];
+            const GameNode& gn = nodes[i];
             NodeRec r{};
-            r.first_child = (int32_t)gn.first_child;
+            r.first_child = gn.num_children ? pos[gn.first_child] : 0;
             r.num_children = gn.num_children;
-            r.state_off = -1;
+            r.state_off = (int32_t)state_off[i];
             r.aux = -1;
             double win = 0, lose = 0, tie = 0;
             if (gn.is_terminal()) {
                 if (gn.is_boundary()) return fail("depth-boundary leaves are not supported by the device path yet");
                 r.kind = gn.is_fold() ? KIND_FOLD : KIND_SHOWDOWN;
                 r.player = gn.player & PLAYER_MASK;
-                int ro = g.runout_index(gn.turn, gn.river);
+                const int ro = g.runout_index(gn.turn, gn.river);
                 if (ro < 0) return fail("terminal without runout");
                 r.aux = runout_base + ro;
-                L.terminals.push_back((int32_t)i);
-                double pot = (double)(tc.starting_pot + 2 * gn.amount);
-                double half = 0.5 * pot;
-                double rake = std::min(pot * tc.rake_rate, tc.rake_cap);
-                win = (half - rake) / half_div;
-                lose = -half / half_div;
-                tie = -0.5 * rake / half_div;
+                const double pot = (double)(tc.starting_pot + 2 * gn.amount);
+                const double half = 0.5 * pot;
+                const double rake = std::min(pot * tc.rake_rate, tc.rake_cap);
+                win = (half - rake) / ncomb;
+                lose = -half / ncomb;
+                tie = -0.5 * rake / ncomb;
             } else if (gn.is_chance()) {
                 r.kind = KIND_CHANCE;
                 ChanceTab ct{};
@@ -220,7 +326,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
                     for (int p = 0; p < 2; ++p) {
                         std::vector<uint16_t> perm(g.num_private_hands(p));
                         std::iota(perm.begin(), perm.end(), 0);
-                        for (auto& s : ic.swap[p]) std::swap(perm[s.first], perm[s.second]);
+                        for (auto& sw : ic.swap[p]) std::swap(perm[sw.first], perm[sw.second]);
                         L.iso_perm.push_back(push_u16(L.u16, perm, 1));
                     }
                 }
@@ -229,25 +335,42 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             } else {
                 r.kind = KIND_DECISION;
                 r.player = gn.player;
-                if (gn.num_children > maxa) return fail("node has more actions than the device supports");
-                r.state_off = (int32_t)off;
-                off += (int64_t)gn.num_children * d.H[gn.player];
             }
             L.nodes.push_back(r);
             L.payoff.push_back(win);
             L.payoff.push_back(lose);
             L.payoff.push_back(tie);
         }
-        d.num_terminals = (int32_t)L.terminals.size() - d.term_base;
+
+        // tiles
+        d.tile_base = (int32_t)L.tiles.size();
+        d.num_tiles = (int32_t)plan.levels.size();
+        for (size_t t = 0; t < plan.levels.size(); ++t) {
+            TileDesc td{};
+            td.game = gi;
+            td.level_base = (int32_t)L.level_ranges.size();
+            td.num_levels = (int32_t)plan.levels[t].size();
+            for (auto& lr : plan.levels[t]) L.level_ranges.push_back({lr.first, lr.second});
+            td.term_base = (int32_t)L.terminals.size();
+            td.num_terminals = (int32_t)plan.terminals[t].size();
+            for (int v : plan.terminals[t]) L.terminals.push_back(v);
+            td.flags = t == 0 ? TILE_GAME_ROOT : 0;
+            if (t > 0) L.subtree_cost.push_back({plan.cost[t] * std::max(d.H[0], d.H[1]), (int)L.tiles.size()});
+            L.tiles.push_back(td);
+        }
+        if (d.num_tiles > 1) L.multi_games.push_back(gi);
+        else L.single_games.push_back(gi);
+
         L.storage_size.push_back(off);
         L.state_total += (off + 63) / 64 * 64;
         L.scratch_total += (int64_t)nodes.size() * d.row_stride;
         cost[gi] = (double)off + (double)nodes.size() * d.row_stride;
         L.games.push_back(d);
     }
-    L.order.resize(n);
-    std::iota(L.order.begin(), L.order.end(), 0);
-    std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(L.single_games.begin(), L.single_games.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::stable_sort(L.subtree_cost.begin(), L.subtree_cost.end(),
+                     [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first > b.first; });
+    for (auto& sc : L.subtree_cost) L.subtree_tiles.push_back(sc.second);
     return 0;
 }
 
@@ -272,8 +395,9 @@ struct psr_batch {
     float last_ms = 0.0f;
     // device buffers
     GameDesc* d_games = nullptr;
+    TileDesc* d_tiles = nullptr;
+    LevelRange* d_levels = nullptr;
     NodeRec* d_nodes = nullptr;
-    int32_t* d_levels = nullptr;
     double* d_payoff = nullptr;
     uint16_t *d_cards = nullptr, *d_same = nullptr;
     float* d_weights = nullptr;
@@ -289,29 +413,33 @@ struct psr_batch {
     uint32_t* d_iters = nullptr;
     float* d_root_cfv = nullptr;
     int* d_counter = nullptr;
-    int* d_order = nullptr;
+    int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr;
+    double* d_ev = nullptr;
 
     void release() {
-        void* ptrs[] = {d_games, d_nodes, d_levels, d_payoff, d_cards, d_same, d_weights, d_terms, d_runouts,
-                        d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_reach, d_cfv,
-                        d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_order};
+        void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
+                        d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy,
+                        d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
+                        d_subtrees, d_ev};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
-        d_games = nullptr; d_nodes = nullptr; d_levels = nullptr; d_payoff = nullptr; d_cards = nullptr;
-        d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr; d_chance = nullptr;
-        d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr; d_regret = nullptr;
-        d_strategy = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr; d_expl = nullptr;
-        d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_order = nullptr;
+        d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
+        d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
+        d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
+        d_regret = nullptr; d_strategy = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
+        d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
+        d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         prepared = false;
     }
 
     KernelArgs args() const {
         KernelArgs a{};
         a.games = d_games;
-        a.num_games = (int)L.games.size();
+        a.num_games = (int)L.single_games.size();
         a.max_hands = L.max_hands;
+        a.tiles = d_tiles;
+        a.level_ranges = d_levels;
         a.nodes = d_nodes;
-        a.level_start = d_levels;
         a.payoff = d_payoff;
         a.hand_cards = d_cards;
         a.hand_same = d_same;
@@ -332,16 +460,116 @@ struct psr_batch {
         a.iters_out = d_iters;
         a.root_cfv = d_root_cfv;
         a.queue_counter = d_counter;
-        a.queue_order = d_order;
+        a.queue_order = d_single;
         a.max_iterations = max_iterations;
+        a.multi_games = d_multi;
+        a.num_multi_games = (int)L.multi_games.size();
+        a.subtree_tiles = d_subtrees;
+        a.num_subtree_tiles = (int)L.subtree_tiles.size();
+        a.ev_out = d_ev;
         return a;
     }
 
-    int run(KernelArgs a) {
+    // one phase launch of a multi-tile traversal
+    int phase(int ph, int player, int mode, uint32_t t) {
+        KernelArgs a = args();
+        a.phase = ph;
+        a.player = player;
+        a.mode = mode;
+        a.iteration = t;
+        int g = ph == PHASE_SUBTREES ? std::min(a.num_subtree_tiles, grid_all()) : a.num_multi_games;
+        if (g <= 0) return 0;
+        HIP_OK(dcfr_launch_phase(a, g, stream));
+        return 0;
+    }
+    int grid_all() const { return num_cus * blocks_per_cu; }
+    int traverse(int player, int mode, uint32_t t) {
+        if (phase(PHASE_ROOT_FORWARD, player, mode, t)) return -1;
+        if (phase(PHASE_SUBTREES, player, mode, t)) return -1;
+        return phase(PHASE_ROOT_BACKWARD, player, mode, t);
+    }
+
+    // exploitability of every multi-tile game into host_expl (f32, as
+    // utility.rs:380-392: (f32(ev0) + f32(ev1)) * 0.5)
+    int multi_exploitability(std::vector<float>& host_expl) {
+        for (int p = 0; p < 2; ++p)
+            if (traverse(p, MODE_BR, 0)) return -1;
+        std::vector<double> ev(2 * L.games.size());
+        HIP_OK(hipMemcpyAsync(ev.data(), d_ev, ev.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        for (int gi : L.multi_games) {
+            volatile float e0 = (float)ev[2 * gi], e1 = (float)ev[2 * gi + 1];
+            volatile float sum = e0 + e1;
+            host_expl[gi] = sum * 0.5f;
+        }
+        return 0;
+    }
+
+    int set_expl(const std::vector<float>& host_expl) {
+        HIP_OK(hipMemcpyAsync(d_expl, host_expl.data(), host_expl.size() * sizeof(float), hipMemcpyHostToDevice,
+                              stream));
+        return 0;
+    }
+
+    // Runs a job on every game: the whole-game kernel for single-tile games,
+    // then the phase loop (reference solve() schedule driven from the host)
+    // for multi-tile games.  Timed with events on the batch stream.
+    int run(int flags, uint32_t begin, uint32_t end, int check_interval, uint32_t max_iters) {
         if (!prepared) return fail("psr_batch_prepare has not been called");
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipEventRecord(ev0, stream));
-        HIP_OK(dcfr_launch(a, grid, stream));
+        if (!L.single_games.empty()) {
+            KernelArgs a = args();
+            a.iter_begin = begin;
+            a.iter_end = end;
+            a.max_iterations = max_iters;
+            a.check_interval = check_interval;
+            a.flags = flags;
+            HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
+        }
+        if (!L.multi_games.empty()) {
+            const float inf = INFINITY;
+            std::vector<float> expl(L.games.size(), inf);
+            std::vector<uint32_t> iters(L.games.size(), begin);
+            if (set_expl(expl)) return -1;   // +inf: every multi-tile game active
+            if (flags & JOB_INITIAL_CHECK) {
+                if (multi_exploitability(expl) || set_expl(expl)) return -1;
+            }
+            if (flags & JOB_ITERATE) {
+                for (uint32_t t = begin; t < end; ++t) {
+                    bool active = false;
+                    for (int gi : L.multi_games)
+                        if (!(expl[gi] <= L.games[gi].target)) {
+                            active = true;
+                            iters[gi] = t + 1;
+                        }
+                    if (!active) break;
+                    if (traverse(0, MODE_DCFR, t) || traverse(1, MODE_DCFR, t)) return -1;
+                    if (check_interval > 0 && ((t + 1) % check_interval == 0 || t + 1 == max_iters)) {
+                        std::vector<float> fresh(expl);
+                        if (multi_exploitability(fresh)) return -1;
+                        for (int gi : L.multi_games)
+                            if (!(expl[gi] <= L.games[gi].target)) expl[gi] = fresh[gi];
+                        if (set_expl(expl)) return -1;
+                    }
+                }
+            }
+            if (flags & JOB_FINAL_CHECK) {
+                if (multi_exploitability(expl)) return -1;
+            }
+            if (flags & JOB_ROOT_CFV) {
+                std::vector<float> active(L.games.size(), inf);
+                if (set_expl(active)) return -1;
+                for (int p = 0; p < 2; ++p)
+                    if (traverse(p, MODE_CFV, 0)) return -1;
+            }
+            // publish per-game results next to the single-tile ones
+            for (int gi : L.multi_games) {
+                HIP_OK(hipMemcpyAsync(d_expl + gi, &expl[gi], sizeof(float), hipMemcpyHostToDevice, stream));
+                HIP_OK(hipMemcpyAsync(d_iters + gi, &iters[gi], sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+                HIP_OK(hipStreamSynchronize(stream));
+            }
+        }
         HIP_OK(hipEventRecord(ev1, stream));
         HIP_OK(hipStreamSynchronize(stream));
         HIP_OK(hipEventElapsedTime(&last_ms, ev0, ev1));
@@ -399,11 +627,13 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     b->L = HostLayout{};
     if (build_layout(games, n, targets, b->L) != 0) return -1;
     HostLayout& L = b->L;
-    if (upload(&b->d_games, L.games) || upload(&b->d_nodes, L.nodes) || upload(&b->d_levels, L.levels) ||
-        upload(&b->d_payoff, L.payoff) || upload(&b->d_cards, L.cards) || upload(&b->d_same, L.same) ||
-        upload(&b->d_weights, L.weights) || upload(&b->d_terms, L.terminals) || upload(&b->d_runouts, L.runouts) ||
-        upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) || upload(&b->d_iso_perm, L.iso_perm) ||
-        upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) || upload(&b->d_order, L.order))
+    if (upload(&b->d_games, L.games) || upload(&b->d_tiles, L.tiles) || upload(&b->d_levels, L.level_ranges) ||
+        upload(&b->d_nodes, L.nodes) || upload(&b->d_payoff, L.payoff) || upload(&b->d_cards, L.cards) ||
+        upload(&b->d_same, L.same) || upload(&b->d_weights, L.weights) || upload(&b->d_terms, L.terminals) ||
+        upload(&b->d_runouts, L.runouts) || upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) ||
+        upload(&b->d_iso_perm, L.iso_perm) || upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) ||
+        upload(&b->d_single, L.single_games) || upload(&b->d_multi, L.multi_games) ||
+        upload(&b->d_subtrees, L.subtree_tiles))
         return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
@@ -416,7 +646,8 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(hipMalloc((void**)&b->d_expl, n * sizeof(float)));
     HIP_OK(hipMalloc((void**)&b->d_iters, n * sizeof(uint32_t)));
     HIP_OK(hipMalloc((void**)&b->d_root_cfv, std::max<size_t>(L.cards.size(), 1) * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_counter, sizeof(int)));
+    HIP_OK(hipMalloc((void**)&b->d_counter, 2 * sizeof(int)));
+    HIP_OK(hipMalloc((void**)&b->d_ev, 2 * n * sizeof(double)));
     HIP_OK(fill_zero(b->d_regret, L.state_total, b->stream));
     HIP_OK(fill_zero(b->d_strategy, L.state_total, b->stream));
     HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
@@ -425,7 +656,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(hipStreamSynchronize(b->stream));
     HIP_OK(dcfr_occupancy(L.max_hands, &b->blocks_per_cu));
     if (b->blocks_per_cu < 1) return fail("kernel does not fit on a CU");
-    b->grid = std::min(n, b->num_cus * b->blocks_per_cu);
+    b->grid = b->num_cus * b->blocks_per_cu;
     b->prepared = true;
     return 0;
 }
@@ -443,25 +674,15 @@ int psr_batch_run_iterations(psr_batch* b, uint32_t start, uint32_t end) {
     if (!b || !b->prepared) return fail("batch not prepared");
     if (end > b->max_iterations) return fail("end exceeds the max_iterations given to psr_batch_prepare");
     if (start > end) return fail("start > end");
-    KernelArgs a = b->args();
-    a.iter_begin = start;
-    a.iter_end = end;
-    a.check_interval = 0;
-    a.flags = JOB_ITERATE;
-    return b->run(a);
+    return b->run(JOB_ITERATE, start, end, 0, b->max_iterations);
 }
 
 int psr_batch_solve(psr_batch* b, uint32_t max_iterations, int32_t check_interval, float* expl_out,
                     uint32_t* iters_out) {
     if (!b || !b->prepared) return fail("batch not prepared");
     if (max_iterations > b->max_iterations) return fail("max_iterations exceeds the prepared discount table");
-    KernelArgs a = b->args();
-    a.iter_begin = 0;
-    a.iter_end = max_iterations;
-    a.max_iterations = max_iterations;
-    a.check_interval = check_interval;
-    a.flags = JOB_INITIAL_CHECK | JOB_ITERATE | (check_interval > 0 ? 0 : JOB_FINAL_CHECK);
-    if (int rc = b->run(a)) return rc;
+    const int flags = JOB_INITIAL_CHECK | JOB_ITERATE | (check_interval > 0 ? 0 : JOB_FINAL_CHECK);
+    if (int rc = b->run(flags, 0, max_iterations, check_interval, max_iterations)) return rc;
     const size_t n = b->L.games.size();
     if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, n * sizeof(float), hipMemcpyDeviceToHost));
     if (iters_out) HIP_OK(hipMemcpy(iters_out, b->d_iters, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -470,18 +691,14 @@ int psr_batch_solve(psr_batch* b, uint32_t max_iterations, int32_t check_interva
 
 int psr_batch_exploitability(psr_batch* b, float* expl_out) {
     if (!b || !b->prepared) return fail("batch not prepared");
-    KernelArgs a = b->args();
-    a.flags = JOB_FINAL_CHECK;
-    if (int rc = b->run(a)) return rc;
+    if (int rc = b->run(JOB_FINAL_CHECK, 0, 0, 0, b->max_iterations)) return rc;
     if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, b->L.games.size() * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
 }
 
 int psr_batch_compute_root_cfvalues(psr_batch* b) {
     if (!b || !b->prepared) return fail("batch not prepared");
-    KernelArgs a = b->args();
-    a.flags = JOB_ROOT_CFV;
-    return b->run(a);
+    return b->run(JOB_ROOT_CFV, 0, 0, 0, b->max_iterations);
 }
 
 int psr_batch_root_cfvalues(psr_batch* b, int game, int player, float* out) {
@@ -516,6 +733,92 @@ int psr_batch_launch_info(psr_batch* b, int* num_cus, int* grid, int* blocks_per
     if (num_cus) *num_cus = b->num_cus;
     if (grid) *grid = b->grid;
     if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu;
+    return 0;
+}
+
+// Host-only check of the device layout of one game (no HIP calls): every
+// node lies in exactly one tile level, a node's children lie in the next
+// level of its tile (or, for a chance node, are the roots of subtree tiles),
+// indices stay inside their arrays, and the scan tables reference valid
+// positions.  out[0..5] = tiles, levels, terminals, widest level,
+// regret/strategy floats, scratch rows.
+int psr_layout_validate(const psr_game* game, int64_t out[6]) {
+    if (!game || !out) return fail("null argument");
+    HostLayout L;
+    if (build_layout(&game, 1, nullptr, L) != 0) return -1;
+    const GameDesc& d = L.games[0];
+    const int N = d.num_nodes;
+    std::vector<int> tile_of(N, -1), level_of(N, -1);
+    int64_t levels = 0, terms = 0, widest = 0;
+    for (int t = 0; t < d.num_tiles; ++t) {
+        const TileDesc& td = L.tiles[d.tile_base + t];
+        levels += td.num_levels;
+        for (int l = 0; l < td.num_levels; ++l) {
+            const LevelRange& lr = L.level_ranges[td.level_base + l];
+            if (lr.start < 0 || lr.end > N || lr.start >= lr.end) return fail("bad level range");
+            widest = std::max<int64_t>(widest, lr.end - lr.start);
+            for (int v = lr.start; v < lr.end; ++v) {
+                if (tile_of[v] != -1) return fail("node in two levels");
+                tile_of[v] = t;
+                level_of[v] = l;
+            }
+        }
+        if (t > 0 && td.num_levels > 0 && L.level_ranges[td.level_base].end - L.level_ranges[td.level_base].start != 1)
+            return fail("subtree level 0 must be its root");
+        for (int k = 0; k < td.num_terminals; ++k) {
+            const int v = L.terminals[td.term_base + k];
+            if (v < 0 || v >= N || tile_of[v] != t) return fail("terminal outside its tile");
+            const uint8_t kind = L.nodes[d.node_base + v].kind;
+            if (kind != KIND_FOLD && kind != KIND_SHOWDOWN) return fail("terminal list holds a non-terminal");
+        }
+        terms += td.num_terminals;
+    }
+    for (int v = 0; v < N; ++v) {
+        if (tile_of[v] < 0) return fail("node in no level");
+        const NodeRec& r = L.nodes[d.node_base + v];
+        const bool term = r.kind == KIND_FOLD || r.kind == KIND_SHOWDOWN;
+        if (term) {
+            if (r.aux < 0 || r.aux * 2 + 1 >= (int)L.runouts.size()) return fail("bad runout table");
+            continue;
+        }
+        if (r.first_child < 0 || r.first_child + r.num_children > N) return fail("children out of range");
+        for (int a = 0; a < r.num_children; ++a) {
+            const int ch = r.first_child + a;
+            if (r.kind == KIND_CHANCE) {
+                if (tile_of[ch] == tile_of[v] || level_of[ch] != 0) return fail("chance child is not a subtree root");
+            } else if (tile_of[ch] != tile_of[v] || level_of[ch] != level_of[v] + 1) {
+                return fail("child not in the next level of its tile");
+            }
+        }
+        if (r.kind == KIND_DECISION &&
+            (r.state_off < 0 || r.state_off + (int64_t)r.num_children * d.H[r.player] > L.storage_size[0]))
+            return fail("state block out of range");
+    }
+    for (size_t k = 0; k < L.runouts.size(); ++k) {
+        const RunoutTab& t = L.runouts[k];
+        const int p = (int)(k & 1), o = p ^ 1;
+        if (t.scan_idx < 0 || t.scan_idx + t.scan_len > (int)L.u16.size()) return fail("scan list out of range");
+        if (t.scan_len != 3 * t.num_valid_opp + 53) return fail("scan length");
+        for (int e = 0; e < t.scan_len; ++e) {
+            const uint16_t u = L.u16[t.scan_idx + e];
+            const int idx = u & SCAN_ZERO;
+            if (idx != SCAN_ZERO && idx >= d.H[o]) return fail("scan entry out of range");
+        }
+        for (int i = 0; i < d.H[p]; ++i) {
+            const HandScan& h = L.hand_scan[t.hand_scan + i];
+            if ((h.x & 0xFFFF) == 0xFFFF) continue;
+            const uint32_t v[8] = {h.x & 0xFFFF, h.x >> 16, h.y & 0xFFFF, h.y >> 16,
+                                   h.z & 0xFFFF, h.z >> 16, h.w & 0xFFFF, h.w >> 16};
+            for (uint32_t x : v)
+                if ((int)x >= t.scan_len) return fail("hand scan position out of range");
+        }
+    }
+    out[0] = d.num_tiles;
+    out[1] = levels;
+    out[2] = terms;
+    out[3] = widest;
+    out[4] = L.storage_size[0];
+    out[5] = N;
     return 0;
 }
 

@@ -121,6 +121,7 @@ def load_library(path: Optional[str] = None):
         "psr_batch_regrets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_last_kernel_ms": (C.c_float, [vp]),
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -355,6 +356,12 @@ class PostFlopGame:
 
     def num_storage(self) -> int:
         return _check(load_library().psr_game_num_storage(self._h))
+
+    def validate_layout(self) -> dict:
+        """Host-only check of the device layout (no GPU needed)."""
+        out = (C.c_int64 * 6)()
+        _check(load_library().psr_layout_validate(self._h, out))
+        return dict(zip(["tiles", "levels", "terminals", "widest_level", "state", "nodes"], list(out)))
 
     def serialize(self) -> bytes:
         lib = load_library()

@@ -198,3 +198,99 @@ def test_error_paths():
         b.run_iterations(0, 11)          # beyond the prepared iterations
     with pytest.raises(P.PsrError):
         b.solve(11, 5)
+
+
+# ------------------------------------------------------------------ turn --
+
+def turn_game(oop="AA,KK,QQ,AKs", ip="QQ-JJ,AQs,AJs", board="Qs Jh 2c", turn="8d", bets="50%, a", raises="",
+              pot=100, stack=200, max_raises=0):
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board),
+                      turn=P.card_from_str(turn))
+    s = [(bets, raises), (bets, raises)]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=pot, effective_stack=stack, turn_bet_sizes=s,
+                      river_bet_sizes=s, max_raises=max_raises)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def test_turn_stepwise_state_matches_oracle():
+    g = turn_game()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 20)
+    o = O.OracleGame(g.serialize())
+    for t in range(8):
+        b.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, s = o.state()
+        close(b.regrets(0), r)
+        close(b.strategy_sum(0), s)
+    assert abs(float(b.exploitability()[0]) - o.exploitability()) <= RTOL * max(1.0, o.exploitability())
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+    close(cf[1], o.root_cfvalues(1))
+
+
+def test_turn_suit_isomorphic_rivers():
+    """No clubs or diamonds on the board and suit-symmetric ranges: diamond
+    rivers are skipped and added back through the swap lists
+    (solver.rs:307-320, isomorphism.rs)."""
+    g = turn_game(oop="AA,KK,QQ,AKs,T9s", ip="QQ-TT,AQs,AJs,87s", board="Ah Kh Qs", turn="Js")
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 40, [0.0])
+    e, it = b.solve(40, 5)
+    o = O.OracleGame(g.serialize())
+    eo, ito = o.solve(40)
+    assert int(it[0]) == ito
+    assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
+    close(b.strategy_sum(0), o.state()[1])
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+
+
+def test_turn_basic_example_game():
+    """examples/basic.rs turn game (250 IP hands), solve() schedule."""
+    oop = P.Range.parse("66+,A8s+,A5s-A4s,AJo+,K9s+,KQo,QTs+,JTs,96s+,85s+,75s+,65s,54s")
+    ip = P.Range.parse("QQ-22,AQs-A2s,ATo+,K5s+,KJo+,Q8s+,J8s+,T7s+,96s+,86s+,75s+,64s+,53s+")
+    cc = P.CardConfig(range=(oop, ip), flop=P.flop_from_str("Td9d6h"), turn=P.card_from_str("Qc"))
+    s = [("60%, e, a", "2.5x"), ("60%, e, a", "2.5x")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=200, effective_stack=900, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s, add_allin_threshold=1.5, force_allin_threshold=0.15,
+                      merging_threshold=0.1)
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 30, [1.0])
+    e, it = b.solve(30, 5)
+    o = O.OracleGame(g.serialize())
+    eo, ito = o.solve(30, 1.0)
+    assert int(it[0]) == ito
+    assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
+    close(b.strategy_sum(0), o.state()[1])
+
+
+def test_turn_full_size_iterations_match_oracle():
+    """The 500-iteration turn config at full size (1128 hands, ~1.7e4 nodes):
+    the first iterations agree with the oracle state element for element."""
+    s = W.turn_uniform01(0)
+    g = s.build()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 500)
+    b.run_iterations(0, 3)
+    o = O.OracleGame(g.serialize())
+    for t in range(3):
+        o.solve_step(t)
+    r, st = o.state()
+    close(b.regrets(0), r)
+    close(b.strategy_sum(0), st)
+
+
+def test_mixed_river_and_turn_batch():
+    """River (whole-game kernel) and turn (phase launches) games in one batch."""
+    games = [std_game(), turn_game(), W.river_batch(1, seed=5, iterations=30)[0].build()]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 30)
+    e, it = b.solve(30, 5)
+    for i, g in enumerate(games):
+        o = O.OracleGame(g.serialize())
+        eo, ito = o.solve(30)
+        assert int(it[i]) == ito
+        assert abs(float(e[i]) - eo) <= RTOL * max(1.0, eo)
+        close(b.strategy_sum(i), o.state()[1])
