@@ -167,6 +167,18 @@ int psr_batch_root_cfvalues(psr_batch* batch, int game, int player, float* out /
  * psr_game_num_storage() floats each, decision nodes in game arena order. */
 int psr_batch_strategy_sum(psr_batch* batch, int game, float* out, int64_t cap);
 int psr_batch_regrets(psr_batch* batch, int game, float* out, int64_t cap);
+/* Storage of regrets and strategy sums, set before psr_batch_prepare:
+ * 0 f32 (default, the reference's uncompressed storage); 1 the reference's
+ * compressed storage, i16 regrets / u16 strategy sums with a per-node scale
+ * (allocate_memory(true), utility.rs:142-202, solver.rs:486-553); 2 FP16 and
+ * 3 BF16 words of value/scale with a per-node f32 scale and f32 arithmetic
+ * (no reference counterpart). */
+int psr_batch_set_storage(psr_batch* batch, int mode);
+/* Raw 16-bit words and per-node scales of one game (decision nodes in arena
+ * order); returns the number of decision nodes.  game/node.rs:229-344
+ * (regrets_compressed / strategy_compressed / scales). */
+int psr_batch_raw_state(psr_batch* batch, int game, uint16_t* regret, uint16_t* strategy, float* rscale,
+                        float* sscale, int64_t cap, int32_t cap_nodes);
 /* Wall time of the last device job in milliseconds (HIP events on the batch
  * stream) -- used for roofline accounting. */
 float psr_batch_last_kernel_ms(psr_batch* batch);

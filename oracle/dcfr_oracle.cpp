@@ -43,6 +43,7 @@ struct Node {
     uint32_t first_child, num_children;
     int32_t runout;
     int64_t storage = -1;   // offset of the A*H block of a decision node
+    int dec = -1;           // decision-node ordinal (per-node scales)
     int iso = -1;           // index into Game::iso
 };
 
@@ -68,6 +69,7 @@ struct Game {
     std::vector<Runout> runouts;
     std::vector<std::vector<Iso>> iso;
     int64_t storage_size = 0;
+    int num_decisions = 0;
     // leaf values injected at depth boundaries: [ordinal][player] -> H values
     std::vector<int> boundary_ordinal;   // per node, -1 if not a boundary
     std::vector<std::vector<float>> boundary_cfv[2];
@@ -169,6 +171,7 @@ bool load(const uint8_t* blob, size_t len, Game& g) {
         }
         if (nd.player & CHANCE) continue;
         nd.storage = off;
+        nd.dec = g.num_decisions++;
         off += (int64_t)nd.num_children * g.H(nd.player);
     }
     g.storage_size = off;
@@ -195,12 +198,87 @@ DiscountParams discount(uint32_t t) {
     return {(float)(pa / (pa + 1.0)), 0.5f, (float)pg};
 }
 
+// utility.rs:142-202: 16-bit encoding with a per-slice scale
+inline float abs_max(const float* x, size_t n) {
+    float m = 0.0f;
+    for (size_t i = 0; i < n; ++i) m = max_(m, std::fabs(x[i]));
+    return m;
+}
+inline float nonneg_max(const float* x, size_t n) {
+    float m = 0.0f;
+    for (size_t i = 0; i < n; ++i) m = max_(m, x[i]);
+    return m;
+}
+inline float encode_signed(int16_t* dst, const float* src, size_t n) {
+    const float scale = abs_max(src, n);
+    const float nz = scale == 0.0f ? 1.0f : scale;
+    const float enc = 32767.0f / nz;
+    for (size_t i = 0; i < n; ++i) dst[i] = (int16_t)(int32_t)std::round(src[i] * enc);
+    return scale;
+}
+inline float encode_unsigned(uint16_t* dst, const float* src, size_t n) {
+    const float scale = nonneg_max(src, n);
+    const float nz = scale == 0.0f ? 1.0f : scale;
+    const float enc = 65535.0f / nz;
+    for (size_t i = 0; i < n; ++i) dst[i] = (uint16_t)(int32_t)(src[i] * enc + 0.49999997f);
+    return scale;
+}
+
 template <class T>
 struct Solver {
     Game& g;
     std::vector<T> regret, strategy;   // storage2 / storage1 of every decision node
+    // compressed storage (solver.rs:486-553): i16 regrets, u16 strategy sums,
+    // one f32 scale each per decision node
+    bool compressed = false;
+    std::vector<int16_t> regret16;
+    std::vector<uint16_t> strategy16;
+    std::vector<float> rscale, sscale;
 
-    explicit Solver(Game& game) : g(game), regret(game.storage_size, T(0)), strategy(game.storage_size, T(0)) {}
+    explicit Solver(Game& game, bool comp = false)
+        : g(game), regret(comp ? 0 : game.storage_size, T(0)), strategy(comp ? 0 : game.storage_size, T(0)),
+          compressed(comp) {
+        if (comp) {
+            regret16.assign(game.storage_size, 0);
+            strategy16.assign(game.storage_size, 0);
+            rscale.assign(game.num_decisions, 0.0f);
+            sscale.assign(game.num_decisions, 0.0f);
+        }
+    }
+
+    // regret matching from the node's storage (solver.rs:600-654)
+    void node_regret_match(const Node& nd, int A, int H, std::vector<T>& s) const {
+        if (!compressed) {
+            regret_match(&regret[nd.storage], A, H, s);
+            return;
+        }
+        s.resize((size_t)A * H);
+        const int16_t* r = &regret16[nd.storage];
+        for (int k = 0; k < A * H; ++k) s[k] = (T)(float)std::max<int16_t>(r[k], 0);
+        normalize_rows(s, A, H);
+    }
+    // normalised average strategy from storage (utility.rs:229-327)
+    void node_average(const Node& nd, int A, int H, std::vector<T>& s) const {
+        if (!compressed) {
+            normalized(&strategy[nd.storage], A, H, s);
+            return;
+        }
+        s.resize((size_t)A * H);
+        const uint16_t* q = &strategy16[nd.storage];
+        for (int k = 0; k < A * H; ++k) s[k] = (T)(float)q[k];
+        normalize_rows(s, A, H);
+    }
+    void normalize_rows(std::vector<T>& s, int A, int H) const {
+        std::vector<T> denom(s.begin(), s.begin() + H);
+        for (int a = 1; a < A; ++a)
+            for (int h = 0; h < H; ++h) denom[h] += s[(size_t)a * H + h];
+        T def = T(1) / (T)A;
+        for (int a = 0; a < A; ++a)
+            for (int h = 0; h < H; ++h) {
+                T& x = s[(size_t)a * H + h];
+                x = denom[h] == T(0) ? def : x / denom[h];
+            }
+    }
 
     // ------------------------------------------------ terminal evaluation --
     // game/evaluation.rs:57-538
@@ -417,6 +495,25 @@ struct Solver {
         return T(1) / (T)(node.turn == 0xFF ? 45 : 44);
     }
 
+    // solver.rs:486-553 (update_compressed_strategy / _regret)
+    void update_compressed(const Node& node, int A, int H, std::vector<T>& sig, std::vector<T>& cfv,
+                           const T* result, const DiscountParams& dp) {
+        const size_t n = (size_t)A * H;
+        uint16_t* S = &strategy16[node.storage];
+        int16_t* R = &regret16[node.storage];
+        float* sp = reinterpret_cast<float*>(sig.data());
+        float* cp = reinterpret_cast<float*>(cfv.data());
+        const float sdec = dp.gamma * sscale[node.dec] / 65535.0f;
+        for (size_t k = 0; k < n; ++k) sp[k] += (float)S[k] * sdec;
+        sscale[node.dec] = encode_unsigned(S, sp, n);
+        const float adec = dp.alpha * rscale[node.dec] / 32767.0f;
+        const float bdec = dp.beta * rscale[node.dec] / 32767.0f;
+        for (size_t k = 0; k < n; ++k) cp[k] += (float)R[k] * (R[k] >= 0 ? adec : bdec);
+        for (int a = 0; a < A; ++a)
+            for (int h = 0; h < H; ++h) cp[(size_t)a * H + h] -= (float)result[h];
+        rscale[node.dec] = encode_signed(R, cp, n);
+    }
+
     // --------------------------------------------------- solve_recursive --
     void solve_rec(T* result, uint32_t ni, int player, const T* cfreach, const DiscountParams& dp) {
         const Node& node = g.nodes[ni];
@@ -435,13 +532,17 @@ struct Solver {
             chance_sum(result, node, player, cfv, H);
         } else if (node.player == player) {
             for (int a = 0; a < A; ++a) solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach, dp);
-            T* R = &regret[node.storage];
-            T* S = &strategy[node.storage];
             std::vector<T> sig;
-            regret_match(R, A, H, sig);
+            node_regret_match(node, A, H, sig);
             for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
             for (int a = 1; a < A; ++a)
                 for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
+            if (compressed) {
+                update_compressed(node, A, H, sig, cfv, result, dp);
+                return;
+            }
+            T* R = &regret[node.storage];
+            T* S = &strategy[node.storage];
             const T gamma = (T)dp.gamma, alpha = (T)dp.alpha, beta = (T)dp.beta;
             for (int k = 0; k < A * H; ++k) S[k] = S[k] * gamma + sig[k];
             for (int k = 0; k < A * H; ++k) {
@@ -453,7 +554,7 @@ struct Solver {
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
-            regret_match(&regret[node.storage], A, Ho, sig);
+            node_regret_match(node, A, Ho, sig);
             for (int a = 0; a < A; ++a)
                 for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
             for (int a = 0; a < A; ++a)
@@ -489,7 +590,7 @@ struct Solver {
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
-            normalized(&strategy[node.storage], A, Ho, sig);
+            node_average(node, A, Ho, sig);
             for (int a = 0; a < A; ++a)
                 for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
             for (int a = 0; a < A; ++a)
@@ -522,14 +623,14 @@ struct Solver {
         } else if (node.player == player) {
             for (int a = 0; a < A; ++a) cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach);
             std::vector<T> sig;
-            normalized(&strategy[node.storage], A, H, sig);
+            node_average(node, A, H, sig);
             for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
             for (int a = 1; a < A; ++a)
                 for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
-            normalized(&strategy[node.storage], A, Ho, sig);
+            node_average(node, A, Ho, sig);
             for (int a = 0; a < A; ++a)
                 for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
             for (int a = 0; a < A; ++a)
@@ -607,6 +708,8 @@ struct Handle {
     std::unique_ptr<Solver<double>> f64;
 };
 
+static_assert(sizeof(float) == 4, "f32");
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -615,7 +718,8 @@ extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
 
-// precision: 32 = the reference's f32 arithmetic, 64 = all-f64 oracle
+// precision: 32 = the reference's f32 arithmetic, 64 = all-f64 oracle,
+// 16 = the reference's compressed storage (i16 regrets / u16 strategy sums)
 void* oracle_game_load(const uint8_t* blob, size_t len, int precision) {
     auto h = std::make_unique<Handle>();
     if (!load(blob, len, h->game)) {
@@ -623,9 +727,27 @@ void* oracle_game_load(const uint8_t* blob, size_t len, int precision) {
         return nullptr;
     }
     if (precision == 64) h->f64 = std::make_unique<Solver<double>>(h->game);
-    else h->f32 = std::make_unique<Solver<float>>(h->game);
+    else h->f32 = std::make_unique<Solver<float>>(h->game, precision == 16);
     return h.release();
 }
+
+// compressed storage of a precision-16 handle: raw 16-bit words and scales
+int oracle_compressed_state(void* hv, int16_t* regret, uint16_t* strategy, float* rscale, float* sscale) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (!h->f32 || !h->f32->compressed) return -1;
+    auto& S = *h->f32;
+    std::copy(S.regret16.begin(), S.regret16.end(), regret);
+    std::copy(S.strategy16.begin(), S.strategy16.end(), strategy);
+    std::copy(S.rscale.begin(), S.rscale.end(), rscale);
+    std::copy(S.sscale.begin(), S.sscale.end(), sscale);
+    return 0;
+}
+
+// utility.rs:179-202 encoders, exported for their known-answer tests
+float oracle_encode_signed(const float* src, size_t n, int16_t* dst) { return encode_signed(dst, src, n); }
+float oracle_encode_unsigned(const float* src, size_t n, uint16_t* dst) { return encode_unsigned(dst, src, n); }
+
+int oracle_num_decisions(void* hv) { return static_cast<Handle*>(hv)->game.num_decisions; }
 
 void oracle_game_free(void* h) { delete static_cast<Handle*>(h); }
 
@@ -691,6 +813,18 @@ void oracle_root_cfvalues(void* h, int p, double* out) {
 // regrets / strategy sums of every decision node in arena order (A*H blocks)
 void oracle_state(void* h, double* regret, double* strategy) {
     DISPATCH(h, {
+        if (S.compressed) {
+            for (const Node& nd : S.g.nodes) {
+                if (nd.storage < 0) continue;
+                const size_t n = (size_t)nd.num_children * S.g.H(nd.player);
+                for (size_t k = 0; k < n; ++k) {
+                    const size_t i = nd.storage + k;
+                    if (regret) regret[i] = (double)S.regret16[i] * S.rscale[nd.dec] / 32767.0;
+                    if (strategy) strategy[i] = (double)S.strategy16[i] * S.sscale[nd.dec] / 65535.0;
+                }
+            }
+            break;
+        }
         for (size_t i = 0; i < S.regret.size(); ++i) {
             if (regret) regret[i] = (double)S.regret[i];
             if (strategy) strategy[i] = (double)S.strategy[i];

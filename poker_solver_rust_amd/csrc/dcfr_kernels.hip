@@ -40,6 +40,7 @@ __device__ __forceinline__ float fmaxr(float x, float y) { return x > y ? x : y;
 struct Shared {
     int job;
     int wave_f[NWAVES];
+    float wave_m[NWAVES];
     double wave_s[NWAVES];
 };
 
@@ -70,14 +71,80 @@ __device__ double block_sum(double v, Shared* sh) {
     return s;
 }
 
+// ---------------------------------------------------------------- state --
+// Regrets and strategy sums of one decision node in one of four storage
+// modes: f32; the reference's compressed i16 / u16 words with a per-node f32
+// scale (solver.rs:486-553, utility.rs:142-202); FP16 or BF16 words holding
+// value / scale with a per-node f32 scale.  reg()/str() return the f32 value
+// regret matching and averaging work on (for the compressed mode the raw
+// integer, as the reference does: the scale cancels).
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float x) {   // round to nearest even
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float fp16_to_f32(uint16_t h) {
+    _Float16 v;
+    __builtin_memcpy(&v, &h, 2);
+    return (float)v;
+}
+__device__ __forceinline__ uint16_t f32_to_fp16(float x) {
+    _Float16 v = (_Float16)x;   // round to nearest even
+    uint16_t h;
+    __builtin_memcpy(&h, &v, 2);
+    return h;
+}
+
+template <int SM>
+struct NodeState {
+    float* R;
+    float* S;
+    uint16_t* R16;
+    uint16_t* S16;
+    float rs, ss;   // per-node scales
+    __device__ float reg(size_t k) const {
+        if (SM == SM_F32) return R[k];
+        if (SM == SM_REF16) return (float)(int16_t)R16[k];
+        if (SM == SM_FP16) return __fmul_rn(fp16_to_f32(R16[k]), rs);
+        return __fmul_rn(bf16_to_f32(R16[k]), rs);
+    }
+    __device__ float str(size_t k) const {
+        if (SM == SM_F32) return S[k];
+        if (SM == SM_REF16) return (float)S16[k];
+        if (SM == SM_FP16) return __fmul_rn(fp16_to_f32(S16[k]), ss);
+        return __fmul_rn(bf16_to_f32(S16[k]), ss);
+    }
+};
+
+template <int SM>
+__device__ __forceinline__ NodeState<SM> node_state(const KernelArgs& A, const GameDesc& G, const NodeRec& nd) {
+    NodeState<SM> v{};
+    const size_t off = (size_t)G.state_base + nd.state_off;
+    if (SM == SM_F32) {
+        v.R = A.regret + off;
+        v.S = A.strategy + off;
+    } else {
+        v.R16 = A.regret16 + off;
+        v.S16 = A.strategy16 + off;
+        v.rs = A.rscale[nd.aux];
+        v.ss = A.sscale[nd.aux];
+    }
+    return v;
+}
+
 // Strategy of one hand at a node from regrets (regret matching,
 // solver.rs:600-626) or strategy sums (normalised average,
 // utility.rs:229-245): denominator pass, then sigma(a) on demand.
-template <int MODE>
-__device__ __forceinline__ float strat_denom(const float* base, int H, int h, int na) {
+template <int MODE, int SM>
+__device__ __forceinline__ float node_value(const NodeState<SM>& st, size_t k) {
+    return MODE == MODE_DCFR ? st.reg(k) : st.str(k);
+}
+template <int MODE, int SM>
+__device__ __forceinline__ float strat_denom(const NodeState<SM>& st, int H, int h, int na) {
     float d = 0.0f;
     for (int a = 0; a < na; ++a) {
-        float v = base[(size_t)a * H + h];
+        float v = node_value<MODE>(st, (size_t)a * H + h);
         if (MODE == MODE_DCFR) v = fmax0(v);
         d = a == 0 ? v : __fadd_rn(d, v);
     }
@@ -89,10 +156,24 @@ __device__ __forceinline__ float strat_of(float v, float denom, float def) {
     return denom == 0.0f ? def : __fdiv_rn(v, denom);
 }
 
+// max over the workgroup of one non-negative f32 per thread (exact, any order)
+__device__ float block_max(float v, Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, WAVE));
+    if (lane == 0) sh->wave_m[wave] = v;
+    __syncthreads();
+    float m = sh->wave_m[0];
+#pragma unroll
+    for (int w = 1; w < NWAVES; ++w) m = fmaxf(m, sh->wave_m[w]);
+    __syncthreads();
+    return m;
+}
+
 // Forward pass over one tile: reach of the opponent of p, level by level.
 // The game-root tile seeds its root row with the opponent's initial weights;
 // a subtree tile finds its root row already written by the chance node above.
-template <int MODE>
+template <int MODE, int SM>
 __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Ho = c.G.H[o];
@@ -114,13 +195,14 @@ __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
             const float* rn = c.reach + (size_t)n * c.stride;
             float* rc = c.reach + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_DECISION && nd.player == o && na > 1) {
-                const float* base = (MODE == MODE_DCFR ? A.regret : A.strategy) + c.G.state_base + nd.state_off;
+                const NodeState<SM> st = node_state<SM>(A, c.G, nd);
                 const float def = __fdiv_rn(1.0f, (float)na);
                 for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                    const float denom = strat_denom<MODE>(base, Ho, h, na);
+                    const float denom = strat_denom<MODE>(st, Ho, h, na);
                     const float r = rn[h];
                     for (int a = 0; a < na; ++a)
-                        rc[(size_t)a * c.stride + h] = __fmul_rn(strat_of<MODE>(base[(size_t)a * Ho + h], denom, def), r);
+                        rc[(size_t)a * c.stride + h] =
+                            __fmul_rn(strat_of<MODE>(node_value<MODE>(st, (size_t)a * Ho + h), denom, def), r);
                 }
             } else {
                 const bool chance = nd.kind == KIND_CHANCE;
@@ -238,7 +320,73 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
 }
 
 // Backward pass over one tile, deepest level first.
-template <int MODE>
+// 16-bit storage: the traverser's update computes the new f32 values first
+// (into the children's cfv and reach rows, dead at this point), takes the
+// node-wide max for the new scale, then encodes.
+template <int SM>
+__device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st, int Hp, const float4 disc) {
+    const KernelArgs& A = c.A;
+    const int na = nd.num_children;
+    float* cc = c.cfv + (size_t)nd.first_child * c.stride;    // cfv of action a, then new regret
+    float* rc = c.reach + (size_t)nd.first_child * c.stride;  // new strategy sum
+    const float def = __fdiv_rn(1.0f, (float)na);
+    float rmax = 0.0f, smax = 0.0f;
+    const float sdec = __fdiv_rn(__fmul_rn(disc.z, st.ss), 65535.0f);
+    const float adec = __fdiv_rn(__fmul_rn(disc.x, st.rs), 32767.0f);
+    const float bdec = __fdiv_rn(__fmul_rn(disc.y, st.rs), 32767.0f);
+    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
+        const float denom = strat_denom<MODE_DCFR>(st, Hp, h, na);
+        const float result = c.rs[h];   // node value, stashed by the caller
+        for (int a = 0; a < na; ++a) {
+            const size_t k = (size_t)a * Hp + h;
+            const size_t row = (size_t)a * c.stride + h;
+            const float sig = strat_of<MODE_DCFR>(st.reg(k), denom, def);
+            float nr, ns;
+            if (SM == SM_REF16) {
+                const int16_t r = (int16_t)st.R16[k];
+                ns = __fadd_rn(sig, __fmul_rn((float)st.S16[k], sdec));
+                nr = __fsub_rn(__fadd_rn(cc[row], __fmul_rn((float)r, r >= 0 ? adec : bdec)), result);
+            } else {
+                const float rv = st.reg(k);
+                const float coef = signbit(rv) ? disc.y : disc.x;
+                ns = __fadd_rn(__fmul_rn(st.str(k), disc.z), sig);
+                nr = __fsub_rn(__fadd_rn(__fmul_rn(rv, coef), cc[row]), result);
+            }
+            cc[row] = nr;
+            rc[row] = ns;
+            rmax = fmaxf(rmax, fabsf(nr));
+            smax = fmaxf(smax, ns);
+        }
+    }
+    rmax = block_max(rmax, c.sh);
+    smax = block_max(smax, c.sh);
+    const float rnz = rmax == 0.0f ? 1.0f : rmax, snz = smax == 0.0f ? 1.0f : smax;
+    const float renc = SM == SM_REF16 ? __fdiv_rn(32767.0f, rnz) : __fdiv_rn(1.0f, rnz);
+    const float senc = SM == SM_REF16 ? __fdiv_rn(65535.0f, snz) : __fdiv_rn(1.0f, snz);
+    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
+        for (int a = 0; a < na; ++a) {
+            const size_t k = (size_t)a * Hp + h;
+            const size_t row = (size_t)a * c.stride + h;
+            const float nr = cc[row], ns = rc[row];
+            if (SM == SM_REF16) {
+                st.R16[k] = (uint16_t)(int16_t)(int32_t)roundf(__fmul_rn(nr, renc));
+                st.S16[k] = (uint16_t)(int32_t)__fadd_rn(__fmul_rn(ns, senc), 0.49999997f);
+            } else if (SM == SM_FP16) {
+                st.R16[k] = f32_to_fp16(__fmul_rn(nr, renc));
+                st.S16[k] = f32_to_fp16(__fmul_rn(ns, senc));
+            } else {
+                st.R16[k] = f32_to_bf16(__fmul_rn(nr, renc));
+                st.S16[k] = f32_to_bf16(__fmul_rn(ns, senc));
+            }
+        }
+    }
+    if (threadIdx.x == 0) {
+        A.rscale[nd.aux] = rmax;
+        A.sscale[nd.aux] = smax;
+    }
+}
+
+template <int MODE, int SM>
 __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int Hp = c.G.H[p];
@@ -266,8 +414,9 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
                     out[h] = (float)acc;
                 }
             } else if (nd.player == p && na > 1) {
-                float* R = A.regret + c.G.state_base + nd.state_off;
-                float* S = A.strategy + c.G.state_base + nd.state_off;
+                const NodeState<SM> st = node_state<SM>(A, c.G, nd);
+                float* R = st.R;
+                float* S = st.S;
                 const float def = __fdiv_rn(1.0f, (float)na);
                 for (int h = threadIdx.x; h < Hp; h += BLOCK) {
                     float res;
@@ -275,15 +424,17 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
                         res = cc[h];
                         for (int a = 1; a < na; ++a) res = fmaxr(res, cc[(size_t)a * c.stride + h]);
                     } else {
-                        const float* base = MODE == MODE_DCFR ? R : S;
-                        const float denom = strat_denom<MODE>(base, Hp, h, na);
+                        const float denom = strat_denom<MODE>(st, Hp, h, na);
                         res = 0.0f;
                         for (int a = 0; a < na; ++a) {
-                            const float t = __fmul_rn(strat_of<MODE>(base[(size_t)a * Hp + h], denom, def),
-                                                      cc[(size_t)a * c.stride + h]);
+                            const float t = __fmul_rn(
+                                strat_of<MODE>(node_value<MODE>(st, (size_t)a * Hp + h), denom, def),
+                                cc[(size_t)a * c.stride + h]);
                             res = a == 0 ? t : __fadd_rn(res, t);
                         }
-                        if (MODE == MODE_DCFR) {
+                        if (MODE == MODE_DCFR && SM != SM_F32) {
+                            c.rs[h] = res;   // stashed for update_16bit
+                        } else if (MODE == MODE_DCFR) {
                             for (int a = 0; a < na; ++a) {
                                 const size_t k = (size_t)a * Hp + h;
                                 const float rv = R[k];
@@ -295,6 +446,10 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
                         }
                     }
                     out[h] = res;
+                }
+                if (MODE == MODE_DCFR && SM != SM_F32) {
+                    __syncthreads();
+                    update_16bit<SM>(c, nd, st, Hp, disc);
                 }
             } else {
                 for (int h = threadIdx.x; h < Hp; h += BLOCK) {
@@ -317,11 +472,11 @@ __device__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
     }
 }
 
-template <int MODE>
+template <int MODE, int SM>
 __device__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
-    tile_forward<MODE>(c, T, p);
+    tile_forward<MODE, SM>(c, T, p);
     tile_terminals(c, T, p);
-    tile_backward<MODE>(c, T, p, disc);
+    tile_backward<MODE, SM>(c, T, p, disc);
 }
 
 __device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, float* rs, Shared* sh) {
@@ -345,11 +500,12 @@ __device__ double root_value(Ctx& c, int p) {
 
 // Exploitability in chips of a single-tile game, utility.rs:380-392
 // (unraked): mean of the two best-response values, each f64 cast to f32.
+template <int SM>
 __device__ float exploitability(Ctx& c, const TileDesc& T) {
     float ev[2];
     const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int p = 0; p < 2; ++p) {
-        tile_traverse<MODE_BR>(c, T, p, none);
+        tile_traverse<MODE_BR, SM>(c, T, p, none);
         ev[p] = (float)root_value(c, p);
     }
     return __fmul_rn(__fadd_rn(ev[0], ev[1]), 0.5f);
@@ -363,6 +519,7 @@ __device__ float exploitability(Ctx& c, const TileDesc& T) {
 
 // Whole-game solve of single-tile games (river subgames): one workgroup runs
 // every iteration of one game, games come from the work queue.
+template <int SM>
 __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
     LDS_SETUP();
     for (;;) {
@@ -376,23 +533,23 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
         const TileDesc T = A.tiles[c.G.tile_base];
 
         float expl = __int_as_float(0x7f800000);   // +inf: no check done yet
-        if (A.flags & JOB_INITIAL_CHECK) expl = exploitability(c, T);
+        if (A.flags & JOB_INITIAL_CHECK) expl = exploitability<SM>(c, T);
         uint32_t t = A.iter_begin;
         if (A.flags & JOB_ITERATE) {
             for (; t < A.iter_end; ++t) {
                 if (expl <= c.G.target) break;
                 const float4 d = A.discount[t];
-                tile_traverse<MODE_DCFR>(c, T, 0, d);
-                tile_traverse<MODE_DCFR>(c, T, 1, d);
+                tile_traverse<MODE_DCFR, SM>(c, T, 0, d);
+                tile_traverse<MODE_DCFR, SM>(c, T, 1, d);
                 if (A.check_interval > 0 && ((t + 1) % A.check_interval == 0 || t + 1 == A.max_iterations))
-                    expl = exploitability(c, T);
+                    expl = exploitability<SM>(c, T);
             }
         }
-        if (A.flags & JOB_FINAL_CHECK) expl = exploitability(c, T);
+        if (A.flags & JOB_FINAL_CHECK) expl = exploitability<SM>(c, T);
         if (A.flags & JOB_ROOT_CFV) {
             const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
             for (int p = 0; p < 2; ++p) {
-                tile_traverse<MODE_CFV>(c, T, p, none);
+                tile_traverse<MODE_CFV, SM>(c, T, p, none);
                 float* dst = A.root_cfv + c.G.hand_base[p];
                 const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
                 for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = root[h];
@@ -410,7 +567,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
 // (or a captured graph) issues ROOT_FORWARD, SUBTREES, ROOT_BACKWARD per
 // traversing player.  Games whose expl_out is already <= target are skipped
 // (the reference solve() stops iterating a converged game).
-template <int MODE>
+template <int MODE, int SM>
 __device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared& sh) {
     const int p = A.player;
     const float4 disc = MODE == MODE_DCFR ? A.discount[A.iteration] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -422,10 +579,10 @@ __device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared&
             if (MODE == MODE_DCFR && A.expl_out[gi] <= c.G.target) continue;
             const TileDesc T = A.tiles[c.G.tile_base];
             if (A.phase == PHASE_ROOT_FORWARD) {
-                tile_forward<MODE>(c, T, p);
+                tile_forward<MODE, SM>(c, T, p);
             } else {
                 tile_terminals(c, T, p);
-                tile_backward<MODE>(c, T, p, disc);
+                tile_backward<MODE, SM>(c, T, p, disc);
                 if (MODE == MODE_BR) {
                     const double v = root_value(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
@@ -448,15 +605,16 @@ __device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared&
         const TileDesc T = A.tiles[A.subtree_tiles[q]];
         Ctx c = make_ctx(A, T.game, scan, rs, &sh);
         if (MODE == MODE_DCFR && A.expl_out[T.game] <= c.G.target) continue;
-        tile_traverse<MODE>(c, T, p, disc);
+        tile_traverse<MODE, SM>(c, T, p, disc);
     }
 }
 
+template <int SM>
 __global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs A) {
     LDS_SETUP();
-    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR>(A, scan, rs, sh);
-    else if (A.mode == MODE_BR) phase_body<MODE_BR>(A, scan, rs, sh);
-    else phase_body<MODE_CFV>(A, scan, rs, sh);
+    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR, SM>(A, scan, rs, sh);
+    else if (A.mode == MODE_BR) phase_body<MODE_BR, SM>(A, scan, rs, sh);
+    else phase_body<MODE_CFV, SM>(A, scan, rs, sh);
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -474,19 +632,40 @@ size_t dcfr_lds_bytes(int max_hands) {
     return sizeof(double) * (3 * (size_t)max_hands + 53) + sizeof(float) * (size_t)max_hands;
 }
 
+template <int SM>
+static void launch_solve(const KernelArgs& args, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(dcfr_solve_kernel<SM>, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+}
+template <int SM>
+static void launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
+    hipLaunchKernelGGL(dcfr_phase_kernel<SM>, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+}
+
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, args.queue_counter);
-    hipLaunchKernelGGL(dcfr_solve_kernel, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+    switch (args.storage) {
+        case SM_F32: launch_solve<SM_F32>(args, grid, stream); break;
+        case SM_REF16: launch_solve<SM_REF16>(args, grid, stream); break;
+        case SM_FP16: launch_solve<SM_FP16>(args, grid, stream); break;
+        case SM_BF16: launch_solve<SM_BF16>(args, grid, stream); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(dcfr_phase_kernel, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+    switch (args.storage) {
+        case SM_F32: launch_phase<SM_F32>(args, grid, stream); break;
+        case SM_REF16: launch_phase<SM_REF16>(args, grid, stream); break;
+        case SM_FP16: launch_phase<SM_FP16>(args, grid, stream); break;
+        case SM_BF16: launch_phase<SM_BF16>(args, grid, stream); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t dcfr_occupancy(int max_hands, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel, BLOCK,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_REF16>, BLOCK,
                                                         dcfr_lds_bytes(max_hands));
 }
 

@@ -23,6 +23,14 @@ enum Phase : int {
 
 enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2 };
 
+// Storage of regrets and strategy sums.
+enum StorageMode : int {
+    SM_F32 = 0,     // f32 (reference uncompressed)
+    SM_REF16 = 1,   // reference compressed: i16 regrets, u16 strategy sums, per-node scale
+    SM_FP16 = 2,    // FP16 value/scale words, per-node f32 scale, f32 arithmetic
+    SM_BF16 = 3,    // BF16 value/scale words, per-node f32 scale, f32 arithmetic
+};
+
 struct KernelArgs {
     const GameDesc* games;
     int num_games;
@@ -43,6 +51,11 @@ struct KernelArgs {
     const HandScan* hand_scan;
     float* regret;
     float* strategy;
+    uint16_t* regret16;       // 16-bit storage modes
+    uint16_t* strategy16;
+    float* rscale;            // per decision node (NodeRec.aux), 16-bit modes
+    float* sscale;
+    int storage;
     float* reach;
     float* cfv;
     const float4* discount;   // per iteration: alpha, beta, gamma, 0

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -46,6 +47,9 @@ struct HostLayout {
     std::vector<LevelRange> level_ranges;
     std::vector<int64_t> storage_size;   // per game
     int64_t state_total = 0, scratch_total = 0;
+    int32_t num_decisions = 0;
+    // per game: decision nodes in arena order as (state offset, A*H, scale index)
+    std::vector<std::vector<std::array<int64_t, 3>>> dec_blocks;
     int max_hands = 0;
     std::vector<int> single_games;   // whole-game work queue, most expensive first
     std::vector<int> multi_games;    // games solved by phase launches (turn)
@@ -280,12 +284,16 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
 
         // state offsets in arena order (the oracle's order)
         std::vector<int64_t> state_off(nodes.size(), -1);
+        std::vector<int32_t> dec_of(nodes.size(), -1);
+        std::vector<std::array<int64_t, 3>> blocks;
         int64_t off = 0;
         for (size_t i = 0; i < nodes.size(); ++i) {
             const GameNode& gn = nodes[i];
             if (gn.is_terminal() || gn.is_chance()) continue;
             if (gn.num_children > maxa) return fail("node has more actions than the device supports");
             state_off[i] = off;
+            dec_of[i] = L.num_decisions++;
+            blocks.push_back({off, (int64_t)gn.num_children * d.H[gn.player], dec_of[i]});
             off += (int64_t)gn.num_children * d.H[gn.player];
         }
         if (off > INT32_MAX) return fail("game too large for 32-bit state offsets");
@@ -335,6 +343,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             } else {
                 r.kind = KIND_DECISION;
                 r.player = gn.player;
+                r.aux = dec_of[i];   // per-node scale slot (16-bit storage)
             }
             L.nodes.push_back(r);
             L.payoff.push_back(win);
@@ -362,6 +371,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         else L.single_games.push_back(gi);
 
         L.storage_size.push_back(off);
+        L.dec_blocks.push_back(std::move(blocks));
         L.state_total += (off + 63) / 64 * 64;
         L.scratch_total += (int64_t)nodes.size() * d.row_stride;
         cost[gi] = (double)off + (double)nodes.size() * d.row_stride;
@@ -408,6 +418,9 @@ struct psr_batch {
     uint16_t* d_u16 = nullptr;
     HandScan* d_hand_scan = nullptr;
     float *d_regret = nullptr, *d_strategy = nullptr, *d_reach = nullptr, *d_cfv = nullptr;
+    uint16_t *d_regret16 = nullptr, *d_strategy16 = nullptr;
+    float *d_rscale = nullptr, *d_sscale = nullptr;
+    int storage = SM_F32;
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
     uint32_t* d_iters = nullptr;
@@ -418,7 +431,7 @@ struct psr_batch {
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
-                        d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy,
+                        d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev};
         for (void* p : ptrs)
@@ -426,7 +439,8 @@ struct psr_batch {
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
-        d_regret = nullptr; d_strategy = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
+        d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
+        d_sscale = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
         d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         prepared = false;
@@ -453,6 +467,11 @@ struct psr_batch {
         a.hand_scan = d_hand_scan;
         a.regret = d_regret;
         a.strategy = d_strategy;
+        a.regret16 = d_regret16;
+        a.strategy16 = d_strategy16;
+        a.rscale = d_rscale;
+        a.sscale = d_sscale;
+        a.storage = storage;
         a.reach = d_reach;
         a.cfv = d_cfv;
         a.discount = d_discount;
@@ -483,6 +502,20 @@ struct psr_batch {
         return 0;
     }
     int grid_all() const { return num_cus * blocks_per_cu; }
+
+    int zero_state() {
+        if (storage == SM_F32) {
+            HIP_OK(fill_zero(d_regret, L.state_total, stream));
+            HIP_OK(fill_zero(d_strategy, L.state_total, stream));
+        } else {
+            // 16-bit words: state_total is a multiple of 64 per game, so pairs of words
+            HIP_OK(fill_zero(reinterpret_cast<float*>(d_regret16), (L.state_total + 1) / 2, stream));
+            HIP_OK(fill_zero(reinterpret_cast<float*>(d_strategy16), (L.state_total + 1) / 2, stream));
+            HIP_OK(fill_zero(d_rscale, L.num_decisions, stream));
+            HIP_OK(fill_zero(d_sscale, L.num_decisions, stream));
+        }
+        return 0;
+    }
     int traverse(int player, int mode, uint32_t t) {
         if (phase(PHASE_ROOT_FORWARD, player, mode, t)) return -1;
         if (phase(PHASE_SUBTREES, player, mode, t)) return -1;
@@ -639,8 +672,15 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
     if (upload(&b->d_discount, disc)) return -1;
     b->max_iterations = max_iterations;
-    HIP_OK(hipMalloc((void**)&b->d_regret, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_strategy, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+    if (b->storage == SM_F32) {
+        HIP_OK(hipMalloc((void**)&b->d_regret, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+        HIP_OK(hipMalloc((void**)&b->d_strategy, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+    } else {
+        HIP_OK(hipMalloc((void**)&b->d_regret16, std::max<int64_t>(L.state_total, 2) * sizeof(uint16_t)));
+        HIP_OK(hipMalloc((void**)&b->d_strategy16, std::max<int64_t>(L.state_total, 2) * sizeof(uint16_t)));
+        HIP_OK(hipMalloc((void**)&b->d_rscale, std::max<int32_t>(L.num_decisions, 1) * sizeof(float)));
+        HIP_OK(hipMalloc((void**)&b->d_sscale, std::max<int32_t>(L.num_decisions, 1) * sizeof(float)));
+    }
     HIP_OK(hipMalloc((void**)&b->d_reach, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
     HIP_OK(hipMalloc((void**)&b->d_cfv, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
     HIP_OK(hipMalloc((void**)&b->d_expl, n * sizeof(float)));
@@ -648,8 +688,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(hipMalloc((void**)&b->d_root_cfv, std::max<size_t>(L.cards.size(), 1) * sizeof(float)));
     HIP_OK(hipMalloc((void**)&b->d_counter, 2 * sizeof(int)));
     HIP_OK(hipMalloc((void**)&b->d_ev, 2 * n * sizeof(double)));
-    HIP_OK(fill_zero(b->d_regret, L.state_total, b->stream));
-    HIP_OK(fill_zero(b->d_strategy, L.state_total, b->stream));
+    if (b->zero_state()) return -1;
     HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_root_cfv, L.cards.size(), b->stream));
@@ -664,8 +703,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
 int psr_batch_reset(psr_batch* b) {
     if (!b || !b->prepared) return fail("batch not prepared");
     HIP_OK(hipSetDevice(b->device));
-    HIP_OK(fill_zero(b->d_regret, b->L.state_total, b->stream));
-    HIP_OK(fill_zero(b->d_strategy, b->L.state_total, b->stream));
+    if (b->zero_state()) return -1;
     HIP_OK(hipStreamSynchronize(b->stream));
     return 0;
 }
@@ -709,21 +747,87 @@ int psr_batch_root_cfvalues(psr_batch* b, int game, int player, float* out) {
     return 0;
 }
 
-static int copy_state(psr_batch* b, const float* src, int game, float* out, int64_t cap) {
+static float half_bits_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h >> 15) << 31;
+    const int exp = (h >> 10) & 0x1F;
+    const uint32_t man = h & 0x3FF;
+    float v;
+    if (exp == 0) v = std::ldexp((float)man, -24);
+    else if (exp == 31) v = man ? NAN : INFINITY;
+    else v = std::ldexp((float)(man | 0x400), exp - 25);
+    return sign ? -v : v;
+}
+
+static float bf16_bits_to_float(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float v;
+    std::memcpy(&v, &u, 4);
+    return v;
+}
+
+// regrets (which = 0) or strategy sums (which = 1) of one game as f32, in
+// arena order; 16-bit storage is decoded with its per-node scale
+static int copy_state(psr_batch* b, int which, int game, float* out, int64_t cap) {
     if (!b || !b->prepared) return fail("batch not prepared");
     if (game < 0 || game >= (int)b->L.games.size() || !out) return fail("invalid argument");
-    int64_t n = b->L.storage_size[game];
+    const int64_t n = b->L.storage_size[game];
     if (cap < n) return fail("output buffer too small");
-    HIP_OK(hipMemcpy(out, src + b->L.games[game].state_base, n * sizeof(float), hipMemcpyDeviceToHost));
+    const int64_t base = b->L.games[game].state_base;
+    if (b->storage == SM_F32) {
+        const float* src = which == 0 ? b->d_regret : b->d_strategy;
+        HIP_OK(hipMemcpy(out, src + base, n * sizeof(float), hipMemcpyDeviceToHost));
+        return 0;
+    }
+    std::vector<uint16_t> raw(n);
+    std::vector<float> scale(std::max<int32_t>(b->L.num_decisions, 1));
+    HIP_OK(hipMemcpy(raw.data(), (which == 0 ? b->d_regret16 : b->d_strategy16) + base, n * sizeof(uint16_t),
+                     hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(scale.data(), which == 0 ? b->d_rscale : b->d_sscale, b->L.num_decisions * sizeof(float),
+                     hipMemcpyDeviceToHost));
+    for (const auto& blk : b->L.dec_blocks[game]) {
+        const float sc = scale[blk[2]];
+        for (int64_t k = blk[0]; k < blk[0] + blk[1]; ++k) {
+            const uint16_t w = raw[k];
+            if (b->storage == SM_REF16)
+                out[k] = which == 0 ? (float)((double)(int16_t)w * sc / 32767.0) : (float)((double)w * sc / 65535.0);
+            else if (b->storage == SM_FP16)
+                out[k] = half_bits_to_float(w) * sc;
+            else
+                out[k] = bf16_bits_to_float(w) * sc;
+        }
+    }
     return 0;
 }
 
-int psr_batch_strategy_sum(psr_batch* b, int game, float* out, int64_t cap) {
-    return copy_state(b, b ? b->d_strategy : nullptr, game, out, cap);
+int psr_batch_strategy_sum(psr_batch* b, int game, float* out, int64_t cap) { return copy_state(b, 1, game, out, cap); }
+
+int psr_batch_regrets(psr_batch* b, int game, float* out, int64_t cap) { return copy_state(b, 0, game, out, cap); }
+
+int psr_batch_set_storage(psr_batch* b, int mode) {
+    if (!b) return fail("null batch");
+    if (mode < SM_F32 || mode > SM_BF16) return fail("unknown storage mode");
+    if (b->prepared) return fail("set the storage mode before psr_batch_prepare");
+    b->storage = mode;
+    return 0;
 }
 
-int psr_batch_regrets(psr_batch* b, int game, float* out, int64_t cap) {
-    return copy_state(b, b ? b->d_regret : nullptr, game, out, cap);
+int psr_batch_raw_state(psr_batch* b, int game, uint16_t* regret, uint16_t* strategy, float* rscale, float* sscale,
+                        int64_t cap, int32_t cap_nodes) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (b->storage == SM_F32) return fail("raw 16-bit state needs a 16-bit storage mode");
+    if (game < 0 || game >= (int)b->L.games.size()) return fail("invalid argument");
+    const auto& blocks = b->L.dec_blocks[game];
+    const int64_t n = b->L.storage_size[game];
+    if (cap < n || cap_nodes < (int32_t)blocks.size()) return fail("output buffer too small");
+    const int64_t base = b->L.games[game].state_base;
+    HIP_OK(hipMemcpy(regret, b->d_regret16 + base, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(strategy, b->d_strategy16 + base, n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+    if (!blocks.empty()) {
+        const int32_t d0 = (int32_t)blocks.front()[2];
+        HIP_OK(hipMemcpy(rscale, b->d_rscale + d0, blocks.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(sscale, b->d_sscale + d0, blocks.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return (int)blocks.size();
 }
 
 float psr_batch_last_kernel_ms(psr_batch* b) { return b ? b->last_ms : -1.0f; }

@@ -122,6 +122,9 @@ def load_library(path: Optional[str] = None):
         "psr_batch_last_kernel_ms": (C.c_float, [vp]),
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "psr_batch_set_storage": (C.c_int, [vp, C.c_int]),
+        "psr_batch_raw_state": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16),
+                                          C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -406,14 +409,23 @@ gpu_solve_game = gpu_solve_hand_parallel
 gpu_solve_mega = gpu_solve_hand_parallel
 
 
-class GpuBatchSolver:
-    """gpu-range-solver batch.rs:79-610, with one tree per game."""
+STORAGE_MODES = {"f32": 0, "compressed": 1, "fp16": 2, "bf16": 3}
 
-    def __init__(self, device: int = 0):
+
+class GpuBatchSolver:
+    """gpu-range-solver batch.rs:79-610, with one tree per game.
+
+    storage: "f32" (default), "compressed" (the reference's i16/u16 with a
+    per-node scale, allocate_memory(true)), "fp16" or "bf16" (value/scale
+    words with a per-node f32 scale)."""
+
+    def __init__(self, device: int = 0, storage: str = "f32"):
         lib = load_library()
         h = C.c_void_p()
         _check(lib.psr_batch_create(device, C.byref(h)))
         self._h = h
+        self.storage = storage
+        _check(lib.psr_batch_set_storage(h, STORAGE_MODES[storage]))
         self.games: List[PostFlopGame] = []
 
     def __del__(self):
@@ -476,6 +488,20 @@ class GpuBatchSolver:
         v = np.zeros(n, np.float32)
         _check(load_library().psr_batch_regrets(self._h, i, _fptr(v), n))
         return v
+
+    def raw_state(self, i: int):
+        """16-bit words and per-node scales: (regret u16, strategy u16,
+        regret scales, strategy scales)."""
+        n = self.games[i].num_storage()
+        r = np.zeros(n, np.uint16)
+        s = np.zeros(n, np.uint16)
+        cap = 1 << 20
+        rs = np.zeros(cap, np.float32)
+        ss = np.zeros(cap, np.float32)
+        k = _check(load_library().psr_batch_raw_state(self._h, i, r.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                                      s.ctypes.data_as(C.POINTER(C.c_uint16)), _fptr(rs), _fptr(ss),
+                                                      n, cap))
+        return r, s, rs[:k], ss[:k]
 
     def last_kernel_ms(self) -> float:
         return float(load_library().psr_batch_last_kernel_ms(self._h))

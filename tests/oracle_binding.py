@@ -43,6 +43,11 @@ def load():
         "oracle_set_state": (None, [vp, dp, dp]),
         "oracle_evaluate": (C.c_int, [vp, C.c_uint32, C.c_int, dp, dp]),
         "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "oracle_compressed_state": (C.c_int, [vp, C.POINTER(C.c_int16), C.POINTER(C.c_uint16),
+                                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "oracle_num_decisions": (C.c_int, [vp]),
+        "oracle_encode_signed": (C.c_float, [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_int16)]),
+        "oracle_encode_unsigned": (C.c_float, [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_uint16)]),
         "oracle_solve_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
                                            C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32)]),
     }
@@ -131,11 +136,44 @@ class OracleGame:
             raise ValueError("not a terminal node")
         return out
 
+    def compressed_state(self):
+        """precision-16 handle: (i16 regrets, u16 strategy, regret scales,
+        strategy scales), decision nodes in arena order."""
+        lib = load()
+        n = self.storage_size()
+        k = lib.oracle_num_decisions(self._h)
+        r = np.zeros(n, np.int16)
+        s = np.zeros(n, np.uint16)
+        rs = np.zeros(k, np.float32)
+        ss = np.zeros(k, np.float32)
+        rc = lib.oracle_compressed_state(self._h, r.ctypes.data_as(C.POINTER(C.c_int16)),
+                                         s.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                         rs.ctypes.data_as(C.POINTER(C.c_float)), ss.ctypes.data_as(C.POINTER(C.c_float)))
+        if rc != 0:
+            raise ValueError("not a compressed handle")
+        return r, s, rs, ss
+
     def set_boundary_cfvs(self, ordinal, player, values):
         v = np.ascontiguousarray(values, np.float32)
         rc = load().oracle_set_boundary_cfvs(self._h, ordinal, player, v.ctypes.data_as(C.POINTER(C.c_float)), len(v))
         if rc != 0:
             raise ValueError("bad boundary ordinal or length")
+
+
+def encode_signed(values):
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(len(v), np.int16)
+    sc = load().oracle_encode_signed(v.ctypes.data_as(C.POINTER(C.c_float)), len(v),
+                                     out.ctypes.data_as(C.POINTER(C.c_int16)))
+    return sc, out
+
+
+def encode_unsigned(values):
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(len(v), np.uint16)
+    sc = load().oracle_encode_unsigned(v.ctypes.data_as(C.POINTER(C.c_float)), len(v),
+                                       out.ctypes.data_as(C.POINTER(C.c_uint16)))
+    return sc, out
 
 
 def solve_many(blobs, iterations, threads, targets=None):

@@ -174,3 +174,27 @@ def test_f64_oracle_close_to_f32():
     e64, _ = o64.solve(200)
     assert abs(e32 - e64) < 1e-3
     assert np.abs(o32.root_cfvalues(0) - o64.root_cfvalues(0)).max() < 1e-3
+
+
+def test_encode_known_answers():
+    """utility.rs:1063-1081."""
+    sc, out = O.encode_signed([1.0, -1.0, 0.5, -0.5])
+    assert sc == 1.0 and out[0] == 32767 and out[1] == -32767 and out[2] == 16384 and out[3] == -16384
+    sc, out = O.encode_unsigned([1.0, 0.0, 0.5])
+    assert sc == 1.0 and out[0] == 65535 and out[1] == 0 and out[2] == 32768
+    sc, out = O.encode_signed([0.0, 0.0])
+    assert sc == 0.0 and not out.any()
+
+
+def test_compressed_oracle_tracks_uncompressed():
+    """The compressed mode (allocate_memory(true)) converges like f32 and its
+    decoded state stays within the 16-bit quantisation of the f32 state."""
+    _, g = std_game()
+    o32 = O.OracleGame(g.serialize(), 32)
+    o16 = O.OracleGame(g.serialize(), 16)
+    e32, _ = o32.solve(300)
+    e16, _ = o16.solve(300)
+    assert e16 < 10.0 and abs(e16 - e32) < 0.05
+    s32 = o32.state()[1]
+    s16 = o16.state()[1]
+    assert np.abs(s16 - s32).max() <= 0.05 * np.abs(s32).max()
