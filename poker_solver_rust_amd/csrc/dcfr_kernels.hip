@@ -48,13 +48,27 @@ struct Ctx {
     const KernelArgs& A;
     GameDesc G;
     const NodeRec* nodes;
-    float* reach;
+    float* reach;      // f32 rows (solve, f32 best response, cfvalues)
     float* cfv;
+    double* reach64;   // f64 rows (FP64 best response)
+    double* cfv64;
     int stride;
-    double* scan;   // LDS, 3 * max_hands + 53
-    float* rs;      // LDS, max_hands
+    double* scan;      // LDS, 3 * max_hands + 53
+    void* rs;          // LDS, max_hands values of the row type (staged reach)
     Shared* sh;
 };
+
+// Row storage of a traversal: V = float (f32 rows) or double (f64 rows).
+template <typename V>
+__device__ __forceinline__ V* reach_rows(const Ctx& c) {
+    if constexpr (sizeof(V) == 4) return c.reach;
+    else return c.reach64;
+}
+template <typename V>
+__device__ __forceinline__ V* cfv_rows(const Ctx& c) {
+    if constexpr (sizeof(V) == 4) return c.cfv;
+    else return c.cfv64;
+}
 
 // Sum over the workgroup of one f64 per thread; every thread gets the same
 // value (same association order on every thread).
@@ -173,15 +187,17 @@ __device__ float block_max(float v, Shared* sh) {
 // Forward pass over one tile: reach of the opponent of p, level by level.
 // The game-root tile seeds its root row with the opponent's initial weights;
 // a subtree tile finds its root row already written by the chance node above.
-template <int MODE, int SM>
+// V = double is the FP64 best response: average strategy and reach in f64.
+template <int MODE, int SM, typename V = float>
 __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Ho = c.G.H[o];
     const LevelRange* lv = A.level_ranges + T.level_base;
+    V* reach = reach_rows<V>(c);
     if (T.flags & TILE_GAME_ROOT) {
         const float* w = A.hand_weight + c.G.hand_base[o];
-        float* r0 = c.reach + (size_t)lv[0].start * c.stride;
-        for (int h = threadIdx.x; h < Ho; h += BLOCK) r0[h] = w[h];
+        V* r0 = reach + (size_t)lv[0].start * c.stride;
+        for (int h = threadIdx.x; h < Ho; h += BLOCK) r0[h] = (V)w[h];
         __syncthreads();
     }
     for (int L = 0; L < T.num_levels; ++L) {
@@ -192,23 +208,45 @@ __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
             any = true;
             const int na = nd.num_children;
-            const float* rn = c.reach + (size_t)n * c.stride;
-            float* rc = c.reach + (size_t)nd.first_child * c.stride;
+            const V* rn = reach + (size_t)n * c.stride;
+            V* rc = reach + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_DECISION && nd.player == o && na > 1) {
                 const NodeState<SM> st = node_state<SM>(A, c.G, nd);
-                const float def = __fdiv_rn(1.0f, (float)na);
-                for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                    const float denom = strat_denom<MODE>(st, Ho, h, na);
-                    const float r = rn[h];
-                    for (int a = 0; a < na; ++a)
-                        rc[(size_t)a * c.stride + h] =
-                            __fmul_rn(strat_of<MODE>(node_value<MODE>(st, (size_t)a * Ho + h), denom, def), r);
+                if constexpr (sizeof(V) == 8) {
+                    for (int h = threadIdx.x; h < Ho; h += BLOCK) {
+                        double denom = 0.0;
+                        for (int a = 0; a < na; ++a) denom += (double)st.str((size_t)a * Ho + h);
+                        const double r = rn[h];
+                        for (int a = 0; a < na; ++a) {
+                            const double sig =
+                                denom == 0.0 ? 1.0 / (double)na : (double)st.str((size_t)a * Ho + h) / denom;
+                            rc[(size_t)a * c.stride + h] = sig * r;
+                        }
+                    }
+                } else {
+                    const float def = __fdiv_rn(1.0f, (float)na);
+                    for (int h = threadIdx.x; h < Ho; h += BLOCK) {
+                        const float denom = strat_denom<MODE>(st, Ho, h, na);
+                        const float r = rn[h];
+                        for (int a = 0; a < na; ++a)
+                            rc[(size_t)a * c.stride + h] =
+                                __fmul_rn(strat_of<MODE>(node_value<MODE>(st, (size_t)a * Ho + h), denom, def), r);
+                    }
                 }
             } else {
                 const bool chance = nd.kind == KIND_CHANCE;
-                const float sc = chance ? A.chance[nd.aux].scale : 1.0f;
+                V sc = 1;
+                if (chance) {
+                    const ChanceTab ct = A.chance[nd.aux];
+                    if constexpr (sizeof(V) == 8) sc = 1.0 / (double)ct.factor;
+                    else sc = ct.scale;
+                }
                 for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                    const float r = chance ? __fmul_rn(rn[h], sc) : rn[h];
+                    V r = rn[h];
+                    if (chance) {
+                        if constexpr (sizeof(V) == 8) r = r * sc;
+                        else r = __fmul_rn(r, sc);
+                    }
                     for (int a = 0; a < na; ++a) rc[(size_t)a * c.stride + h] = r;
                 }
             }
@@ -221,7 +259,9 @@ __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
 // by the index list (SCAN_HEAD starts a segment, SCAN_ZERO adds 0).  Every
 // thread owns a contiguous run of entries; runs are combined with a wave scan
 // of (segment-started, sum) pairs and a carry across the four waves.
+template <typename V>
 __device__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
+    const V* rs = static_cast<const V*>(c.rs);
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     const int C = (len + BLOCK - 1) / BLOCK;
     const int e0 = threadIdx.x * C;
@@ -231,7 +271,7 @@ __device__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
     for (int e = e0; e < e1; ++e) {
         const uint16_t u = idx[e];
         const uint16_t k = u & SCAN_ZERO;
-        const double v = k == SCAN_ZERO ? 0.0 : (double)c.rs[k];
+        const double v = k == SCAN_ZERO ? 0.0 : (double)rs[k];
         if (u & SCAN_HEAD) {
             f = true;
             s = v;
@@ -267,22 +307,24 @@ __device__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
         const uint16_t k = u & SCAN_ZERO;
         if (u & SCAN_HEAD) run = 0.0;
         c.scan[e] = run;
-        run += k == SCAN_ZERO ? 0.0 : (double)c.rs[k];
+        run += k == SCAN_ZERO ? 0.0 : (double)rs[k];
     }
     __syncthreads();
 }
 
 // Fold / showdown values of one terminal for traversing player p
 // (game/evaluation.rs:293-425), whole workgroup.
+template <typename V>
 __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     const KernelArgs& A = c.A;
     const int Hp = c.G.H[p], Ho = c.G.H[p ^ 1];
     const RunoutTab T = A.runouts[nd.aux * 2 + p];
-    const float* r = c.reach + (size_t)n * c.stride;
-    float* out = c.cfv + (size_t)n * c.stride;
-    for (int h = threadIdx.x; h < Ho; h += BLOCK) c.rs[h] = r[h];
+    const V* r = reach_rows<V>(c) + (size_t)n * c.stride;
+    V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
+    V* rs = static_cast<V*>(c.rs);
+    for (int h = threadIdx.x; h < Ho; h += BLOCK) rs[h] = r[h];
     __syncthreads();
-    segmented_scan(c, A.u16pool + T.scan_idx, T.scan_len);
+    segmented_scan<V>(c, A.u16pool + T.scan_idx, T.scan_len);
 
     const HandScan* hs = A.hand_scan + T.hand_scan;
     const double win = A.payoff[(size_t)3 * (c.G.node_base + n) + 0];
@@ -292,26 +334,27 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
         const double payoff = (int)nd.player != p ? win : lose;
         const uint16_t* same = A.hand_same + c.G.hand_base[p];
         for (int i = threadIdx.x; i < Hp; i += BLOCK) {
-            float res = 0.0f;
+            V res = 0;
             const HandScan q = hs[i];
             if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
                 const uint16_t si = same[i];
-                const double sm = si == 0xFFFF ? 0.0 : (double)c.rs[si];
+                const double sm = si == 0xFFFF ? 0.0 : (double)rs[si];
                 const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
-                res = (float)(payoff * cr);
+                res = (V)(payoff * cr);
             }
             out[i] = res;
         }
     } else {
         for (int i = threadIdx.x; i < Hp; i += BLOCK) {
-            float res = 0.0f;
+            V res = 0;
             const HandScan q = hs[i];
             if ((q.x & 0xFFFF) != 0xFFFF) {
                 const double* S = c.scan;
                 const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
                 const double cl = ((total - S[q.x >> 16]) - (S[q.w & 0xFFFF] - S[q.y >> 16])) -
                                   (S[q.w >> 16] - S[q.z >> 16]);
-                res = __fadd_rn((float)(win * cw), (float)(lose * cl));
+                if constexpr (sizeof(V) == 8) res = win * cw + lose * cl;
+                else res = __fadd_rn((float)(win * cw), (float)(lose * cl));
             }
             out[i] = res;
         }
@@ -319,7 +362,6 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     __syncthreads();
 }
 
-// Backward pass over one tile, deepest level first.
 // 16-bit storage: the traverser's update computes the new f32 values first
 // (into the children's cfv and reach rows, dead at this point), takes the
 // node-wide max for the new scale, then encodes.
@@ -329,6 +371,7 @@ __device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st,
     const int na = nd.num_children;
     float* cc = c.cfv + (size_t)nd.first_child * c.stride;    // cfv of action a, then new regret
     float* rc = c.reach + (size_t)nd.first_child * c.stride;  // new strategy sum
+    const float* stash = static_cast<const float*>(c.rs);     // node value of each hand
     const float def = __fdiv_rn(1.0f, (float)na);
     float rmax = 0.0f, smax = 0.0f;
     const float sdec = __fdiv_rn(__fmul_rn(disc.z, st.ss), 65535.0f);
@@ -336,7 +379,7 @@ __device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st,
     const float bdec = __fdiv_rn(__fmul_rn(disc.y, st.rs), 32767.0f);
     for (int h = threadIdx.x; h < Hp; h += BLOCK) {
         const float denom = strat_denom<MODE_DCFR>(st, Hp, h, na);
-        const float result = c.rs[h];   // node value, stashed by the caller
+        const float result = stash[h];
         for (int a = 0; a < na; ++a) {
             const size_t k = (size_t)a * Hp + h;
             const size_t row = (size_t)a * c.stride + h;
@@ -386,11 +429,13 @@ __device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st,
     }
 }
 
-template <int MODE, int SM>
+// Backward pass over one tile, deepest level first.
+template <int MODE, int SM, typename V = float>
 __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int Hp = c.G.H[p];
     const LevelRange* lv = A.level_ranges + T.level_base;
+    V* cfv = cfv_rows<V>(c);
     for (int L = T.num_levels - 1; L >= 0; --L) {
         const int n0 = lv[L].start, n1 = lv[L].end;
         bool any = false;
@@ -399,8 +444,8 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
             any = true;
             const int na = nd.num_children;
-            float* out = c.cfv + (size_t)n * c.stride;
-            const float* cc = c.cfv + (size_t)nd.first_child * c.stride;
+            V* out = cfv + (size_t)n * c.stride;
+            const V* cc = cfv + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_CHANCE) {
                 const ChanceTab ct = A.chance[nd.aux];
                 for (int h = threadIdx.x; h < Hp; h += BLOCK) {
@@ -411,50 +456,65 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
                         const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
                         acc += (double)cc[(size_t)A.iso_ref[e] * c.stride + perm[h]];
                     }
-                    out[h] = (float)acc;
+                    out[h] = (V)acc;
                 }
             } else if (nd.player == p && na > 1) {
-                const NodeState<SM> st = node_state<SM>(A, c.G, nd);
-                float* R = st.R;
-                float* S = st.S;
-                const float def = __fdiv_rn(1.0f, (float)na);
-                for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                    float res;
-                    if (MODE == MODE_BR) {
-                        res = cc[h];
-                        for (int a = 1; a < na; ++a) res = fmaxr(res, cc[(size_t)a * c.stride + h]);
-                    } else {
-                        const float denom = strat_denom<MODE>(st, Hp, h, na);
-                        res = 0.0f;
-                        for (int a = 0; a < na; ++a) {
-                            const float t = __fmul_rn(
-                                strat_of<MODE>(node_value<MODE>(st, (size_t)a * Hp + h), denom, def),
-                                cc[(size_t)a * c.stride + h]);
-                            res = a == 0 ? t : __fadd_rn(res, t);
+                if constexpr (sizeof(V) == 8) {
+                    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
+                        double res = cc[h];
+                        for (int a = 1; a < na; ++a) {
+                            const double v = cc[(size_t)a * c.stride + h];
+                            res = v > res ? v : res;
                         }
-                        if (MODE == MODE_DCFR && SM != SM_F32) {
-                            c.rs[h] = res;   // stashed for update_16bit
-                        } else if (MODE == MODE_DCFR) {
+                        out[h] = res;
+                    }
+                } else {
+                    const NodeState<SM> st = node_state<SM>(A, c.G, nd);
+                    float* R = st.R;
+                    float* S = st.S;
+                    const float def = __fdiv_rn(1.0f, (float)na);
+                    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
+                        float res;
+                        if (MODE == MODE_BR) {
+                            res = cc[h];
+                            for (int a = 1; a < na; ++a) res = fmaxr(res, cc[(size_t)a * c.stride + h]);
+                        } else {
+                            const float denom = strat_denom<MODE>(st, Hp, h, na);
+                            res = 0.0f;
                             for (int a = 0; a < na; ++a) {
-                                const size_t k = (size_t)a * Hp + h;
-                                const float rv = R[k];
-                                const float sig = strat_of<MODE>(rv, denom, def);
-                                S[k] = __fadd_rn(__fmul_rn(S[k], disc.z), sig);
-                                const float coef = signbit(rv) ? disc.y : disc.x;
-                                R[k] = __fsub_rn(__fadd_rn(__fmul_rn(rv, coef), cc[(size_t)a * c.stride + h]), res);
+                                const float t = __fmul_rn(
+                                    strat_of<MODE>(node_value<MODE>(st, (size_t)a * Hp + h), denom, def),
+                                    cc[(size_t)a * c.stride + h]);
+                                res = a == 0 ? t : __fadd_rn(res, t);
+                            }
+                            if (MODE == MODE_DCFR && SM != SM_F32) {
+                                static_cast<float*>(c.rs)[h] = res;   // stashed for update_16bit
+                            } else if (MODE == MODE_DCFR) {
+                                for (int a = 0; a < na; ++a) {
+                                    const size_t k = (size_t)a * Hp + h;
+                                    const float rv = R[k];
+                                    const float sig = strat_of<MODE>(rv, denom, def);
+                                    S[k] = __fadd_rn(__fmul_rn(S[k], disc.z), sig);
+                                    const float coef = signbit(rv) ? disc.y : disc.x;
+                                    R[k] = __fsub_rn(__fadd_rn(__fmul_rn(rv, coef), cc[(size_t)a * c.stride + h]),
+                                                     res);
+                                }
                             }
                         }
+                        out[h] = res;
                     }
-                    out[h] = res;
-                }
-                if (MODE == MODE_DCFR && SM != SM_F32) {
-                    __syncthreads();
-                    update_16bit<SM>(c, nd, st, Hp, disc);
+                    if (MODE == MODE_DCFR && SM != SM_F32) {
+                        __syncthreads();
+                        update_16bit<SM>(c, nd, st, Hp, disc);
+                    }
                 }
             } else {
                 for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                    float res = cc[h];
-                    for (int a = 1; a < na; ++a) res = __fadd_rn(res, cc[(size_t)a * c.stride + h]);
+                    V res = cc[h];
+                    for (int a = 1; a < na; ++a) {
+                        if constexpr (sizeof(V) == 8) res += cc[(size_t)a * c.stride + h];
+                        else res = __fadd_rn(res, cc[(size_t)a * c.stride + h]);
+                    }
                     out[h] = res;
                 }
             }
@@ -463,36 +523,42 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
     }
 }
 
+template <typename V = float>
 __device__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
     const int32_t* terms = c.A.terminals + T.term_base;
     for (int k = 0; k < T.num_terminals; ++k) {
         const int n = terms[k];
         const NodeRec nd = c.nodes[n];
-        eval_terminal(c, nd, n, p);
+        eval_terminal<V>(c, nd, n, p);
     }
 }
 
-template <int MODE, int SM>
+template <int MODE, int SM, typename V = float>
 __device__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
-    tile_forward<MODE, SM>(c, T, p);
-    tile_terminals(c, T, p);
-    tile_backward<MODE, SM>(c, T, p, disc);
+    tile_forward<MODE, SM, V>(c, T, p);
+    tile_terminals<V>(c, T, p);
+    tile_backward<MODE, SM, V>(c, T, p, disc);
 }
 
-__device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, float* rs, Shared* sh) {
-    Ctx c{A, A.games[gi], nullptr, nullptr, nullptr, 0, scan, rs, sh};
+__device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
+    Ctx c{A, A.games[gi], nullptr, nullptr, nullptr, nullptr, nullptr, 0, scan, rs, sh};
     c.nodes = A.nodes + c.G.node_base;
     c.reach = A.reach + c.G.scratch_base;
     c.cfv = A.cfv + c.G.scratch_base;
+    if (A.reach64) {
+        c.reach64 = A.reach64 + c.G.scratch_base;
+        c.cfv64 = A.cfv64 + c.G.scratch_base;
+    }
     c.stride = c.G.row_stride;
     return c;
 }
 
 // f64 weighted sum of the game-root cfvalues of p with p's initial weights
 // (utility.rs:99-103), identical on every thread.
+template <typename V = float>
 __device__ double root_value(Ctx& c, int p) {
     const float* w = c.A.hand_weight + c.G.hand_base[p];
-    const float* root = c.cfv + (size_t)c.A.level_ranges[c.A.tiles[c.G.tile_base].level_base].start * c.stride;
+    const V* root = cfv_rows<V>(c) + (size_t)c.A.level_ranges[c.A.tiles[c.G.tile_base].level_base].start * c.stride;
     double s = 0.0;
     for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) s += (double)root[h] * (double)w[h];
     return block_sum(s, c.sh);
@@ -511,11 +577,24 @@ __device__ float exploitability(Ctx& c, const TileDesc& T) {
     return __fmul_rn(__fadd_rn(ev[0], ev[1]), 0.5f);
 }
 
+// The same best response carried out entirely in f64 (average strategy,
+// reach, terminal values and backups), the FP64 exploitability pass.
+template <int SM>
+__device__ double exploitability64(Ctx& c, const TileDesc& T) {
+    double ev[2];
+    const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int p = 0; p < 2; ++p) {
+        tile_traverse<MODE_BR, SM, double>(c, T, p, none);
+        ev[p] = root_value<double>(c, p);
+    }
+    return (ev[0] + ev[1]) * 0.5;
+}
+
 #define LDS_SETUP()                                                        \
     extern __shared__ double lds[];                                        \
     __shared__ Shared sh;                                                  \
     double* scan = lds;                                                    \
-    float* rs = reinterpret_cast<float*>(lds + 3 * A.max_hands + 53)
+    void* rs = lds + 3 * A.max_hands + 53
 
 // Whole-game solve of single-tile games (river subgames): one workgroup runs
 // every iteration of one game, games come from the work queue.
@@ -556,6 +635,10 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
                 __syncthreads();
             }
         }
+        if (A.flags & JOB_EXPL64) {
+            const double e = exploitability64<SM>(c, T);
+            if (threadIdx.x == 0) A.expl64_out[gi] = e;
+        }
         if (threadIdx.x == 0) {
             A.expl_out[gi] = expl;
             A.iters_out[gi] = t;
@@ -564,11 +647,11 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
 }
 
 // One phase of one traversal of multi-tile games (turn subgames).  The host
-// (or a captured graph) issues ROOT_FORWARD, SUBTREES, ROOT_BACKWARD per
-// traversing player.  Games whose expl_out is already <= target are skipped
-// (the reference solve() stops iterating a converged game).
-template <int MODE, int SM>
-__device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared& sh) {
+// issues ROOT_FORWARD, SUBTREES, ROOT_BACKWARD per traversing player.  In
+// DCFR mode a game whose expl_out is already <= target is skipped (the
+// reference solve() stops iterating a converged game).
+template <int MODE, int SM, typename V>
+__device__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& sh) {
     const int p = A.player;
     const float4 disc = MODE == MODE_DCFR ? A.discount[A.iteration] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (A.phase == PHASE_ROOT_FORWARD || A.phase == PHASE_ROOT_BACKWARD) {
@@ -579,12 +662,12 @@ __device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared&
             if (MODE == MODE_DCFR && A.expl_out[gi] <= c.G.target) continue;
             const TileDesc T = A.tiles[c.G.tile_base];
             if (A.phase == PHASE_ROOT_FORWARD) {
-                tile_forward<MODE, SM>(c, T, p);
+                tile_forward<MODE, SM, V>(c, T, p);
             } else {
-                tile_terminals(c, T, p);
-                tile_backward<MODE, SM>(c, T, p, disc);
+                tile_terminals<V>(c, T, p);
+                tile_backward<MODE, SM, V>(c, T, p, disc);
                 if (MODE == MODE_BR) {
-                    const double v = root_value(c, p);
+                    const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
                 } else if (MODE == MODE_CFV) {
                     float* dst = A.root_cfv + c.G.hand_base[p];
@@ -605,16 +688,17 @@ __device__ void phase_body(const KernelArgs& A, double* scan, float* rs, Shared&
         const TileDesc T = A.tiles[A.subtree_tiles[q]];
         Ctx c = make_ctx(A, T.game, scan, rs, &sh);
         if (MODE == MODE_DCFR && A.expl_out[T.game] <= c.G.target) continue;
-        tile_traverse<MODE, SM>(c, T, p, disc);
+        tile_traverse<MODE, SM, V>(c, T, p, disc);
     }
 }
 
 template <int SM>
 __global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs A) {
     LDS_SETUP();
-    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR, SM>(A, scan, rs, sh);
-    else if (A.mode == MODE_BR) phase_body<MODE_BR, SM>(A, scan, rs, sh);
-    else phase_body<MODE_CFV, SM>(A, scan, rs, sh);
+    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR, SM, float>(A, scan, rs, sh);
+    else if (A.mode == MODE_BR) phase_body<MODE_BR, SM, float>(A, scan, rs, sh);
+    else if (A.mode == MODE_BR64) phase_body<MODE_BR, SM, double>(A, scan, rs, sh);
+    else phase_body<MODE_CFV, SM, float>(A, scan, rs, sh);
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -629,7 +713,7 @@ __global__ void fill_zero_kernel(float* p, size_t n) {
 // ------------------------------------------------------------------ host --
 
 size_t dcfr_lds_bytes(int max_hands) {
-    return sizeof(double) * (3 * (size_t)max_hands + 53) + sizeof(float) * (size_t)max_hands;
+    return sizeof(double) * (3 * (size_t)max_hands + 53) + sizeof(double) * (size_t)max_hands;
 }
 
 template <int SM>

@@ -13,6 +13,7 @@ enum JobFlags : int {
     JOB_ITERATE = 2,         // DCFR iterations [iter_begin, iter_end)
     JOB_FINAL_CHECK = 4,     // exploitability after the iterations
     JOB_ROOT_CFV = 8,        // root per-hand cfvalues under the average strategy
+    JOB_EXPL64 = 16,         // FP64 best-response exploitability
 };
 
 enum Phase : int {
@@ -21,7 +22,7 @@ enum Phase : int {
     PHASE_ROOT_BACKWARD = 2,  // root tile terminals + backward, one block per game
 };
 
-enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2 };
+enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3 };
 
 // Storage of regrets and strategy sums.
 enum StorageMode : int {
@@ -58,6 +59,8 @@ struct KernelArgs {
     int storage;
     float* reach;
     float* cfv;
+    double* reach64;          // f64 scratch rows, FP64 best response (may be null)
+    double* cfv64;
     const float4* discount;   // per iteration: alpha, beta, gamma, 0
     float* expl_out;
     uint32_t* iters_out;
@@ -75,6 +78,7 @@ struct KernelArgs {
     int phase, player, mode;
     uint32_t iteration;
     double* ev_out;           // per game and player: weighted root value (best response)
+    double* expl64_out;       // per game: FP64 exploitability
 };
 
 size_t dcfr_lds_bytes(int max_hands);

@@ -68,8 +68,8 @@ struct RunoutTab {
 struct ChanceTab {          // chance node: reach scale and isomorphic-card swaps
     float scale;            // 1 / chance factor (f32, as the reference computes it)
     int32_t iso_count;
-    int32_t iso_base;       // into iso_ref[] / iso_swap_off[]
-    int32_t pad;
+    int32_t iso_base;       // into iso_ref[] / iso_perm[]
+    int32_t factor;         // chance factor (44 or 45), for the f64 pass
 };
 
 struct LevelRange {

@@ -326,6 +326,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
                 r.kind = KIND_CHANCE;
                 ChanceTab ct{};
                 ct.scale = 1.0f / (float)g.chance_factor(gn);
+                ct.factor = g.chance_factor(gn);
                 const auto& iso = g.iso_chances((uint32_t)i);
                 ct.iso_count = (int32_t)iso.size();
                 ct.iso_base = (int32_t)L.iso_ref.size();
@@ -420,6 +421,7 @@ struct psr_batch {
     float *d_regret = nullptr, *d_strategy = nullptr, *d_reach = nullptr, *d_cfv = nullptr;
     uint16_t *d_regret16 = nullptr, *d_strategy16 = nullptr;
     float *d_rscale = nullptr, *d_sscale = nullptr;
+    double *d_reach64 = nullptr, *d_cfv64 = nullptr, *d_expl64 = nullptr;
     int storage = SM_F32;
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
@@ -431,7 +433,7 @@ struct psr_batch {
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
-                        d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale,
+                        d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev};
         for (void* p : ptrs)
@@ -440,7 +442,7 @@ struct psr_batch {
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
         d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
-        d_sscale = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
+        d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
         d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         prepared = false;
@@ -472,6 +474,9 @@ struct psr_batch {
         a.rscale = d_rscale;
         a.sscale = d_sscale;
         a.storage = storage;
+        a.reach64 = d_reach64;
+        a.cfv64 = d_cfv64;
+        a.expl64_out = d_expl64;
         a.reach = d_reach;
         a.cfv = d_cfv;
         a.discount = d_discount;
@@ -502,6 +507,16 @@ struct psr_batch {
         return 0;
     }
     int grid_all() const { return num_cus * blocks_per_cu; }
+
+    // f64 scratch rows for the FP64 best response, allocated on first use
+    int ensure64() {
+        if (d_reach64) return 0;
+        const size_t rows = std::max<int64_t>(L.scratch_total, 1);
+        HIP_OK(hipMalloc((void**)&d_reach64, rows * sizeof(double)));
+        HIP_OK(hipMalloc((void**)&d_cfv64, rows * sizeof(double)));
+        HIP_OK(hipMalloc((void**)&d_expl64, std::max<size_t>(L.games.size(), 1) * sizeof(double)));
+        return 0;
+    }
 
     int zero_state() {
         if (storage == SM_F32) {
@@ -589,6 +604,17 @@ struct psr_batch {
             }
             if (flags & JOB_FINAL_CHECK) {
                 if (multi_exploitability(expl)) return -1;
+            }
+            if (flags & JOB_EXPL64) {
+                for (int p = 0; p < 2; ++p)
+                    if (traverse(p, MODE_BR64, 0)) return -1;
+                std::vector<double> ev(2 * L.games.size());
+                HIP_OK(hipMemcpyAsync(ev.data(), d_ev, ev.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+                HIP_OK(hipStreamSynchronize(stream));
+                for (int gi : L.multi_games) {
+                    const double e = (ev[2 * gi] + ev[2 * gi + 1]) * 0.5;
+                    HIP_OK(hipMemcpy(d_expl64 + gi, &e, sizeof(double), hipMemcpyHostToDevice));
+                }
             }
             if (flags & JOB_ROOT_CFV) {
                 std::vector<float> active(L.games.size(), inf);
@@ -731,6 +757,15 @@ int psr_batch_exploitability(psr_batch* b, float* expl_out) {
     if (!b || !b->prepared) return fail("batch not prepared");
     if (int rc = b->run(JOB_FINAL_CHECK, 0, 0, 0, b->max_iterations)) return rc;
     if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, b->L.games.size() * sizeof(float), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int psr_batch_exploitability_f64(psr_batch* b, double* expl_out) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (b->ensure64()) return -1;
+    if (int rc = b->run(JOB_EXPL64, 0, 0, 0, b->max_iterations)) return rc;
+    if (expl_out)
+        HIP_OK(hipMemcpy(expl_out, b->d_expl64, b->L.games.size() * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -123,6 +123,7 @@ def load_library(path: Optional[str] = None):
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "psr_batch_set_storage": (C.c_int, [vp, C.c_int]),
+        "psr_batch_exploitability_f64": (C.c_int, [vp, C.POINTER(C.c_double)]),
         "psr_batch_raw_state": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16),
                                           C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int64, C.c_int32]),
     }
@@ -463,6 +464,12 @@ class GpuBatchSolver:
         expl = np.zeros(len(self.games), np.float32)
         _check(load_library().psr_batch_exploitability(self._h, _fptr(expl)))
         return expl
+
+    def exploitability_f64(self) -> np.ndarray:
+        """FP64 best-response exploitability of every game (device)."""
+        out = np.zeros(len(self.games), np.float64)
+        _check(load_library().psr_batch_exploitability_f64(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
 
     def root_cfvalues(self) -> List[Tuple[np.ndarray, np.ndarray]]:
         lib = load_library()

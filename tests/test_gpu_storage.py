@@ -87,3 +87,21 @@ def test_storage_mode_must_precede_prepare():
     b.prepare_batch([std_game()], 5)
     with pytest.raises(P.PsrError):
         P.range_solver._check(P.load_library().psr_batch_set_storage(b._h, 1))
+
+
+@pytest.mark.parametrize("make", [std_game, turn_game, lambda: W.river_uniform(4).build()])
+def test_fp64_exploitability_matches_oracle(make):
+    """FP64 best response on the device vs the f64 oracle's best response of
+    the very same strategy sums (loaded into an f64 handle)."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 60)
+    b.run_iterations(0, 60)
+    e64 = float(b.exploitability_f64()[0])
+    o = O.OracleGame(g.serialize(), 64)
+    o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    eo = o.exploitability()
+    assert abs(e64 - eo) <= 1e-9 * max(1.0, abs(eo)), (e64, eo)
+    # and the f32 pass agrees to f32 precision
+    e32 = float(b.exploitability()[0])
+    assert abs(e32 - eo) <= 1e-4 * max(1.0, abs(eo))
