@@ -58,6 +58,11 @@ struct Ctx {
     Shared* sh;
 };
 
+// Per-player fields by select, not by index: a runtime index into a local
+// struct sends the whole struct to per-lane scratch memory.
+__device__ __forceinline__ int num_hands(const GameDesc& G, int p) { return p ? G.H[1] : G.H[0]; }
+__device__ __forceinline__ int hand_base(const GameDesc& G, int p) { return p ? G.hand_base[1] : G.hand_base[0]; }
+
 // Row storage of a traversal: V = float (f32 rows) or double (f64 rows).
 template <typename V>
 __device__ __forceinline__ V* reach_rows(const Ctx& c) {
@@ -70,9 +75,41 @@ __device__ __forceinline__ V* cfv_rows(const Ctx& c) {
     else return c.cfv64;
 }
 
+// ------------------------------------------------------------- uniform --
+// Every branch that encloses a barrier must be decided identically by all
+// lanes, and the compiler must know it: a loop whose exit it considers
+// per-lane is run under an exec mask, lanes that leave wait at the exit while
+// the rest of the wave iterates, and a barrier inside then pairs up with the
+// wrong barrier of the other waves (the workgroup never finishes).  Values that
+// steer such branches (job indices, descriptors, node records, reductions) go
+// through readfirstlane, which puts them in scalar registers.
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ float uni(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uni(double x) {
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xFFFFFFFFll));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// a descriptor (any struct of 32-bit words) read by every lane, made uniform
+template <typename T>
+__device__ __forceinline__ T uni_load(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "descriptor of 32-bit words");
+    constexpr int N = sizeof(T) / 4;
+    int w[N];
+    __builtin_memcpy(w, p, sizeof(T));
+#pragma unroll
+    for (int k = 0; k < N; ++k) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    T v;
+    __builtin_memcpy(&v, w, sizeof(T));
+    return v;
+}
+
 // Sum over the workgroup of one f64 per thread; every thread gets the same
 // value (same association order on every thread).
-__device__ double block_sum(double v, Shared* sh) {
+__device__ __forceinline__ double block_sum(double v, Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) v += __shfl_xor(v, d, WAVE);
@@ -82,7 +119,7 @@ __device__ double block_sum(double v, Shared* sh) {
 #pragma unroll
     for (int w = 1; w < NWAVES; ++w) s += sh->wave_s[w];
     __syncthreads();
-    return s;
+    return uni(s);
 }
 
 // ---------------------------------------------------------------- state --
@@ -171,7 +208,7 @@ __device__ __forceinline__ float strat_of(float v, float denom, float def) {
 }
 
 // max over the workgroup of one non-negative f32 per thread (exact, any order)
-__device__ float block_max(float v, Shared* sh) {
+__device__ __forceinline__ float block_max(float v, Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
 #pragma unroll
     for (int d = WAVE / 2; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, WAVE));
@@ -181,7 +218,7 @@ __device__ float block_max(float v, Shared* sh) {
 #pragma unroll
     for (int w = 1; w < NWAVES; ++w) m = fmaxf(m, sh->wave_m[w]);
     __syncthreads();
-    return m;
+    return uni(m);
 }
 
 // Forward pass over one tile: reach of the opponent of p, level by level.
@@ -189,22 +226,22 @@ __device__ float block_max(float v, Shared* sh) {
 // a subtree tile finds its root row already written by the chance node above.
 // V = double is the FP64 best response: average strategy and reach in f64.
 template <int MODE, int SM, typename V = float>
-__device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
+__device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     const KernelArgs& A = c.A;
-    const int o = p ^ 1, Ho = c.G.H[o];
+    const int o = p ^ 1, Ho = num_hands(c.G, o);
     const LevelRange* lv = A.level_ranges + T.level_base;
     V* reach = reach_rows<V>(c);
     if (T.flags & TILE_GAME_ROOT) {
-        const float* w = A.hand_weight + c.G.hand_base[o];
+        const float* w = A.hand_weight + hand_base(c.G, o);
         V* r0 = reach + (size_t)lv[0].start * c.stride;
         for (int h = threadIdx.x; h < Ho; h += BLOCK) r0[h] = (V)w[h];
         __syncthreads();
     }
     for (int L = 0; L < T.num_levels; ++L) {
-        const int n0 = lv[L].start, n1 = lv[L].end;
+        const LevelRange lr = uni_load(lv + L);
         bool any = false;
-        for (int n = n0; n < n1; ++n) {
-            const NodeRec nd = c.nodes[n];
+        for (int n = lr.start; n < lr.end; ++n) {
+            const NodeRec nd = uni_load(c.nodes + n);
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
             any = true;
             const int na = nd.num_children;
@@ -237,7 +274,7 @@ __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                 const bool chance = nd.kind == KIND_CHANCE;
                 V sc = 1;
                 if (chance) {
-                    const ChanceTab ct = A.chance[nd.aux];
+                    const ChanceTab ct = uni_load(A.chance + nd.aux);
                     if constexpr (sizeof(V) == 8) sc = 1.0 / (double)ct.factor;
                     else sc = ct.scale;
                 }
@@ -260,7 +297,7 @@ __device__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
 // thread owns a contiguous run of entries; runs are combined with a wave scan
 // of (segment-started, sum) pairs and a carry across the four waves.
 template <typename V>
-__device__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
+__device__ __forceinline__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
     const V* rs = static_cast<const V*>(c.rs);
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     const int C = (len + BLOCK - 1) / BLOCK;
@@ -315,10 +352,10 @@ __device__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
 // Fold / showdown values of one terminal for traversing player p
 // (game/evaluation.rs:293-425), whole workgroup.
 template <typename V>
-__device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
+__device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     const KernelArgs& A = c.A;
-    const int Hp = c.G.H[p], Ho = c.G.H[p ^ 1];
-    const RunoutTab T = A.runouts[nd.aux * 2 + p];
+    const int Hp = num_hands(c.G, p), Ho = num_hands(c.G, p ^ 1);
+    const RunoutTab T = uni_load(A.runouts + nd.aux * 2 + p);
     const V* r = reach_rows<V>(c) + (size_t)n * c.stride;
     V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
     V* rs = static_cast<V*>(c.rs);
@@ -332,7 +369,7 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     const double total = c.scan[T.num_valid_opp];
     if (nd.kind == KIND_FOLD) {
         const double payoff = (int)nd.player != p ? win : lose;
-        const uint16_t* same = A.hand_same + c.G.hand_base[p];
+        const uint16_t* same = A.hand_same + hand_base(c.G, p);
         for (int i = threadIdx.x; i < Hp; i += BLOCK) {
             V res = 0;
             const HandScan q = hs[i];
@@ -366,7 +403,7 @@ __device__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
 // (into the children's cfv and reach rows, dead at this point), takes the
 // node-wide max for the new scale, then encodes.
 template <int SM>
-__device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st, int Hp, const float4 disc) {
+__device__ __forceinline__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st, int Hp, const float4 disc) {
     const KernelArgs& A = c.A;
     const int na = nd.num_children;
     float* cc = c.cfv + (size_t)nd.first_child * c.stride;    // cfv of action a, then new regret
@@ -431,23 +468,23 @@ __device__ void update_16bit(Ctx& c, const NodeRec& nd, const NodeState<SM>& st,
 
 // Backward pass over one tile, deepest level first.
 template <int MODE, int SM, typename V = float>
-__device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
+__device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
-    const int Hp = c.G.H[p];
+    const int Hp = num_hands(c.G, p);
     const LevelRange* lv = A.level_ranges + T.level_base;
     V* cfv = cfv_rows<V>(c);
     for (int L = T.num_levels - 1; L >= 0; --L) {
-        const int n0 = lv[L].start, n1 = lv[L].end;
+        const LevelRange lr = uni_load(lv + L);
         bool any = false;
-        for (int n = n0; n < n1; ++n) {
-            const NodeRec nd = c.nodes[n];
+        for (int n = lr.start; n < lr.end; ++n) {
+            const NodeRec nd = uni_load(c.nodes + n);
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
             any = true;
             const int na = nd.num_children;
             V* out = cfv + (size_t)n * c.stride;
             const V* cc = cfv + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_CHANCE) {
-                const ChanceTab ct = A.chance[nd.aux];
+                const ChanceTab ct = uni_load(A.chance + nd.aux);
                 for (int h = threadIdx.x; h < Hp; h += BLOCK) {
                     double acc = (double)cc[h];
                     for (int a = 1; a < na; ++a) acc += (double)cc[(size_t)a * c.stride + h];
@@ -524,24 +561,24 @@ __device__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 dis
 }
 
 template <typename V = float>
-__device__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
+__device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
     const int32_t* terms = c.A.terminals + T.term_base;
     for (int k = 0; k < T.num_terminals; ++k) {
-        const int n = terms[k];
-        const NodeRec nd = c.nodes[n];
+        const int n = uni(terms[k]);
+        const NodeRec nd = uni_load(c.nodes + n);
         eval_terminal<V>(c, nd, n, p);
     }
 }
 
 template <int MODE, int SM, typename V = float>
-__device__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
+__device__ __forceinline__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     tile_forward<MODE, SM, V>(c, T, p);
     tile_terminals<V>(c, T, p);
     tile_backward<MODE, SM, V>(c, T, p, disc);
 }
 
-__device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
-    Ctx c{A, A.games[gi], nullptr, nullptr, nullptr, nullptr, nullptr, 0, scan, rs, sh};
+__device__ __forceinline__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
+    Ctx c{A, uni_load(A.games + gi), nullptr, nullptr, nullptr, nullptr, nullptr, 0, scan, rs, sh};
     c.nodes = A.nodes + c.G.node_base;
     c.reach = A.reach + c.G.scratch_base;
     c.cfv = A.cfv + c.G.scratch_base;
@@ -556,60 +593,72 @@ __device__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Sha
 // f64 weighted sum of the game-root cfvalues of p with p's initial weights
 // (utility.rs:99-103), identical on every thread.
 template <typename V = float>
-__device__ double root_value(Ctx& c, int p) {
-    const float* w = c.A.hand_weight + c.G.hand_base[p];
+__device__ __forceinline__ double root_value(Ctx& c, int p) {
+    const float* w = c.A.hand_weight + hand_base(c.G, p);
     const V* root = cfv_rows<V>(c) + (size_t)c.A.level_ranges[c.A.tiles[c.G.tile_base].level_base].start * c.stride;
     double s = 0.0;
-    for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) s += (double)root[h] * (double)w[h];
+    for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) s += (double)root[h] * (double)w[h];
     return block_sum(s, c.sh);
 }
 
 // Exploitability in chips of a single-tile game, utility.rs:380-392
 // (unraked): mean of the two best-response values, each f64 cast to f32.
 template <int SM>
-__device__ float exploitability(Ctx& c, const TileDesc& T) {
-    float ev[2];
+__device__ __forceinline__ float exploitability(Ctx& c, const TileDesc& T) {
+    float ev0 = 0.0f, ev1 = 0.0f;   // scalars: a per-player array would live in scratch
     const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 1
     for (int p = 0; p < 2; ++p) {
         tile_traverse<MODE_BR, SM>(c, T, p, none);
-        ev[p] = (float)root_value(c, p);
+        const float v = (float)root_value(c, p);
+        if (p == 0) ev0 = v;
+        else ev1 = v;
     }
-    return __fmul_rn(__fadd_rn(ev[0], ev[1]), 0.5f);
+    return __fmul_rn(__fadd_rn(ev0, ev1), 0.5f);
 }
 
 // The same best response carried out entirely in f64 (average strategy,
 // reach, terminal values and backups), the FP64 exploitability pass.
 template <int SM>
-__device__ double exploitability64(Ctx& c, const TileDesc& T) {
-    double ev[2];
+__device__ __forceinline__ double exploitability64(Ctx& c, const TileDesc& T) {
+    double ev0 = 0.0, ev1 = 0.0;
     const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 1
     for (int p = 0; p < 2; ++p) {
         tile_traverse<MODE_BR, SM, double>(c, T, p, none);
-        ev[p] = root_value<double>(c, p);
+        const double v = root_value<double>(c, p);
+        if (p == 0) ev0 = v;
+        else ev1 = v;
     }
-    return (ev[0] + ev[1]) * 0.5;
+    return (ev0 + ev1) * 0.5;
 }
 
-#define LDS_SETUP()                                                        \
-    extern __shared__ double lds[];                                        \
-    __shared__ Shared sh;                                                  \
-    double* scan = lds;                                                    \
+// The kernel argument block is read in place through the kernarg segment
+// pointer (the only explicit argument, at offset 0): scalar loads, and no
+// per-lane copy of the 352-byte block, which the compiler otherwise makes as
+// soon as the parameter's address is taken (Ctx keeps a reference to it) and
+// which turns every bound read from it into a per-lane value.
+#define KERNEL_SETUP()                                                                          \
+    const KernelArgs& A = *(const KernelArgs*)__builtin_amdgcn_kernarg_segment_ptr();           \
+    extern __shared__ double lds[];                                                             \
+    __shared__ Shared sh;                                                                       \
+    double* scan = lds;                                                                         \
     void* rs = lds + 3 * A.max_hands + 53
 
 // Whole-game solve of single-tile games (river subgames): one workgroup runs
 // every iteration of one game, games come from the work queue.
 template <int SM>
-__global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
-    LDS_SETUP();
+__global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
+    KERNEL_SETUP();
     for (;;) {
         if (threadIdx.x == 0) sh.job = atomicAdd(A.queue_counter, 1);
         __syncthreads();
-        const int q = sh.job;
+        const int q = uni(sh.job);
         __syncthreads();
         if (q >= A.num_games) break;
-        const int gi = A.queue_order[q];
+        const int gi = uni(A.queue_order[q]);
         Ctx c = make_ctx(A, gi, scan, rs, &sh);
-        const TileDesc T = A.tiles[c.G.tile_base];
+        const TileDesc T = uni_load(A.tiles + c.G.tile_base);
 
         float expl = __int_as_float(0x7f800000);   // +inf: no check done yet
         if (A.flags & JOB_INITIAL_CHECK) expl = exploitability<SM>(c, T);
@@ -618,8 +667,8 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
             for (; t < A.iter_end; ++t) {
                 if (expl <= c.G.target) break;
                 const float4 d = A.discount[t];
-                tile_traverse<MODE_DCFR, SM>(c, T, 0, d);
-                tile_traverse<MODE_DCFR, SM>(c, T, 1, d);
+#pragma unroll 1
+                for (int p = 0; p < 2; ++p) tile_traverse<MODE_DCFR, SM>(c, T, p, d);
                 if (A.check_interval > 0 && ((t + 1) % A.check_interval == 0 || t + 1 == A.max_iterations))
                     expl = exploitability<SM>(c, T);
             }
@@ -627,11 +676,12 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
         if (A.flags & JOB_FINAL_CHECK) expl = exploitability<SM>(c, T);
         if (A.flags & JOB_ROOT_CFV) {
             const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 1
             for (int p = 0; p < 2; ++p) {
                 tile_traverse<MODE_CFV, SM>(c, T, p, none);
-                float* dst = A.root_cfv + c.G.hand_base[p];
+                float* dst = A.root_cfv + hand_base(c.G, p);
                 const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
-                for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = root[h];
+                for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) dst[h] = root[h];
                 __syncthreads();
             }
         }
@@ -643,6 +693,9 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
             A.expl_out[gi] = expl;
             A.iters_out[gi] = t;
         }
+        // keeps this lane-0 store and the next job's lane-0 fetch in separate
+        // regions (merged, they would become a per-lane loop exit)
+        __syncthreads();
     }
 }
 
@@ -651,16 +704,16 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs A) {
 // DCFR mode a game whose expl_out is already <= target is skipped (the
 // reference solve() stops iterating a converged game).
 template <int MODE, int SM, typename V>
-__device__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& sh) {
+__device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& sh) {
     const int p = A.player;
     const float4 disc = MODE == MODE_DCFR ? A.discount[A.iteration] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (A.phase == PHASE_ROOT_FORWARD || A.phase == PHASE_ROOT_BACKWARD) {
         if (A.phase == PHASE_ROOT_FORWARD && blockIdx.x == 0 && threadIdx.x == 0) A.queue_counter[1] = 0;
         for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {
-            const int gi = A.multi_games[k];
+            const int gi = uni(A.multi_games[k]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
-            if (MODE == MODE_DCFR && A.expl_out[gi] <= c.G.target) continue;
-            const TileDesc T = A.tiles[c.G.tile_base];
+            if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
+            const TileDesc T = uni_load(A.tiles + c.G.tile_base);
             if (A.phase == PHASE_ROOT_FORWARD) {
                 tile_forward<MODE, SM, V>(c, T, p);
             } else {
@@ -670,9 +723,9 @@ __device__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& 
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
                 } else if (MODE == MODE_CFV) {
-                    float* dst = A.root_cfv + c.G.hand_base[p];
+                    float* dst = A.root_cfv + hand_base(c.G, p);
                     const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
-                    for (int h = threadIdx.x; h < c.G.H[p]; h += BLOCK) dst[h] = root[h];
+                    for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) dst[h] = root[h];
                 }
             }
             __syncthreads();
@@ -682,23 +735,22 @@ __device__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& 
     for (;;) {
         if (threadIdx.x == 0) sh.job = atomicAdd(A.queue_counter + 1, 1);
         __syncthreads();
-        const int q = sh.job;
+        const int q = uni(sh.job);
         __syncthreads();
         if (q >= A.num_subtree_tiles) break;
-        const TileDesc T = A.tiles[A.subtree_tiles[q]];
+        const TileDesc T = uni_load(A.tiles + uni(A.subtree_tiles[q]));
         Ctx c = make_ctx(A, T.game, scan, rs, &sh);
-        if (MODE == MODE_DCFR && A.expl_out[T.game] <= c.G.target) continue;
+        if (MODE == MODE_DCFR && uni(A.expl_out[T.game]) <= c.G.target) continue;
         tile_traverse<MODE, SM, V>(c, T, p, disc);
     }
 }
 
-template <int SM>
-__global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs A) {
-    LDS_SETUP();
-    if (A.mode == MODE_DCFR) phase_body<MODE_DCFR, SM, float>(A, scan, rs, sh);
-    else if (A.mode == MODE_BR) phase_body<MODE_BR, SM, float>(A, scan, rs, sh);
-    else if (A.mode == MODE_BR64) phase_body<MODE_BR, SM, double>(A, scan, rs, sh);
-    else phase_body<MODE_CFV, SM, float>(A, scan, rs, sh);
+// One kernel per (storage, traversal mode), so that each body is inlined whole
+// and the kernel arguments stay in scalar registers.
+template <int SM, int MODE, typename V>
+__global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs) {
+    KERNEL_SETUP();
+    phase_body<MODE, SM, V>(A, scan, rs, sh);
 }
 
 __global__ void reset_counter_kernel(int* counter) {
@@ -722,7 +774,21 @@ static void launch_solve(const KernelArgs& args, int grid, hipStream_t stream) {
 }
 template <int SM>
 static void launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(dcfr_phase_kernel<SM>, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+    const size_t lds = dcfr_lds_bytes(args.max_hands);
+    switch (args.mode) {
+        case MODE_DCFR:
+            hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_DCFR, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
+            break;
+        case MODE_BR:
+            hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_BR, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
+            break;
+        case MODE_BR64:
+            hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_BR, double>), dim3(grid), dim3(BLOCK), lds, stream, args);
+            break;
+        default:
+            hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_CFV, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
+            break;
+    }
 }
 
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream) {

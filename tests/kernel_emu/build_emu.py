@@ -1,0 +1,41 @@
+"""Builds tests/kernel_emu/libpsr_dcfr_emu.so: the library's own sources
+(host game build, device layout, C ABI and the HIP kernels) compiled for the
+CPU against hip_cpu_shim.hpp.  The kernel source is used unchanged except
+that its dynamic LDS declaration is pointed at the emulated block's buffer.
+"""
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+CSRC = os.path.join(ROOT, "poker_solver_rust_amd", "csrc")
+OUT = os.path.join(HERE, "libpsr_dcfr_emu.so")
+CXX = os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
+
+
+def build():
+    work = os.path.join(HERE, "_build")
+    os.makedirs(work, exist_ok=True)
+    src = open(os.path.join(CSRC, "dcfr_kernels.hip")).read()
+    src, n = re.subn(r"extern __shared__ double (\w+)\[\];", r"double* \1 = emu::dynamic_lds<double>();", src)
+    assert n == 1, "dynamic LDS declaration not found"
+    kern = os.path.join(work, "dcfr_kernels_emu.cpp")
+    with open(kern, "w") as f:
+        f.write(src)
+    flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-U_FORTIFY_SOURCE", "-ffp-contract=off", "-pthread", "-I" + HERE, "-I" + CSRC,
+             "-Wno-unused-function"]
+    objs = []
+    for s in [os.path.join(CSRC, "game.cpp"), os.path.join(CSRC, "capi_game.cpp"), os.path.join(CSRC, "gpu_solver.cpp"),
+              kern]:
+        o = os.path.join(work, os.path.basename(s) + ".o")
+        subprocess.check_call([CXX, "-c", *flags, s, "-o", o])
+        objs.append(o)
+    subprocess.check_call([CXX, "-shared", "-pthread", *objs, "-o", OUT])
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build())
+    sys.exit(0)

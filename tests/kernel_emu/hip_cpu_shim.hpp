@@ -1,0 +1,291 @@
+// CPU execution of the library's HIP sources, for tests only.
+//
+// Each workgroup runs as blockDim.x cooperative fibers on the calling thread
+// (see emu::launch); __syncthreads and wave shuffles are synchronisation
+// points between them; device memory is host memory and streams execute
+// synchronously.  Blocks of a launch run one after another, so __shared__
+// variables become function statics.  A barrier that only part of a workgroup
+// reaches is reported as a deadlock.  The scheduler does not model lockstep
+// execution of a wave under an exec mask, so it cannot show hazards that come
+// from the compiler's handling of divergent control flow: the assembly check
+// in tests/isa_barrier_lint.py covers those.
+#pragma once
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <csetjmp>
+#include <ucontext.h>
+#include <vector>
+
+struct dim3 {
+    unsigned x = 1, y = 1, z = 1;
+    dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+struct float4 {
+    float x, y, z, w;
+};
+inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
+
+typedef int hipError_t;
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2 };
+typedef void* hipStream_t;
+typedef void* hipEvent_t;
+enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
+enum { hipStreamNonBlocking = 1 };
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
+
+#define __global__
+#define __device__
+#define __host__
+#define __forceinline__ inline
+#define __launch_bounds__(x)
+#define __shared__ static
+
+namespace emu {
+// Every lane of a workgroup is a fiber (ucontext) on the calling thread.  A
+// lane runs until it reaches a synchronisation point: a workgroup barrier or a
+// wave exchange (shuffle, readfirstlane).  The scheduler releases a wave
+// exchange when all 64 lanes of the wave wait on it and a barrier when every
+// lane of the workgroup waits on it.  When lanes wait and nothing can be
+// released (part of a workgroup at a barrier, part finished or at another
+// kind of point) the launch is reported as a deadlock and the process aborts.
+enum Wait : int { RUNNING = 0, AT_BARRIER = 1, AT_WAVE = 2, DONE = 3 };
+
+struct Lane {
+    ucontext_t ctx;   // first entry only
+    jmp_buf jb;       // resume point (switches use _setjmp/_longjmp, no signal-mask system call)
+    std::unique_ptr<char[]> stack;
+    int wait = RUNNING;
+    bool started = false;
+};
+
+struct Group {
+    std::vector<Lane> lanes;
+    std::vector<uint64_t> slots;   // wave exchange values, one per lane
+    void* lds = nullptr;
+    ucontext_t sched;
+    jmp_buf sched_jb;
+    int current = 0;
+    std::function<void()> body;
+};
+
+inline Group* g_group = nullptr;
+inline dim3 cur_thread, cur_block, cur_block_dim, cur_grid;
+inline const void* cur_kernarg = nullptr;
+
+template <class T>
+T* dynamic_lds() {
+    return static_cast<T*>(g_group->lds);
+}
+
+inline void wait_for(int what) {
+    Lane& l = g_group->lanes[g_group->current];
+    l.wait = what;
+    if (!_setjmp(l.jb)) _longjmp(g_group->sched_jb, 1);
+}
+
+inline void syncthreads() { wait_for(AT_BARRIER); }
+
+template <class T>
+T exchange(T v, int src) {
+    static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+    const int me = g_group->current, wave = me / 64;
+    uint64_t bits = 0;
+    std::memcpy(&bits, &v, sizeof(T));
+    g_group->slots[me] = bits;
+    wait_for(AT_WAVE);   // every lane of the wave has written its slot
+    T r = v;
+    if (src >= 0 && src < 64) {
+        const uint64_t b = g_group->slots[wave * 64 + src];
+        std::memcpy(&r, &b, sizeof(T));
+    }
+    wait_for(AT_WAVE);   // every lane has read before the slots are reused
+    return r;
+}
+
+inline void lane_entry() {
+    g_group->body();
+    g_group->lanes[g_group->current].wait = DONE;
+    _longjmp(g_group->sched_jb, 1);
+}
+
+[[noreturn]] inline void deadlock(const Group& G, unsigned block) {
+    int n[4] = {0, 0, 0, 0};
+    for (const Lane& l : G.lanes) n[l.wait]++;
+    std::fprintf(stderr,
+                 "emulated workgroup %u cannot proceed: %d lanes at a barrier, %d at a wave exchange, %d finished\n",
+                 block, n[AT_BARRIER], n[AT_WAVE], n[DONE]);
+    std::abort();
+}
+
+template <class A0, class... Rest>
+const void* first_arg(const A0& a, const Rest&...) {
+    return &a;
+}
+
+template <class K, class... Args>
+void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
+    constexpr size_t STACK = size_t(1) << 20;
+    const int nt = (int)block.x;
+    Group G;
+    G.lanes.resize(nt);
+    for (Lane& l : G.lanes) l.stack.reset(new char[STACK]);
+    G.slots.resize(((nt + 63) / 64) * 64);
+    std::vector<double> lds((lds_bytes + 7) / 8 + 1);
+    G.lds = lds.data();
+    G.body = [&]() { kernel(args...); };
+    Group* saved = g_group;
+    g_group = &G;
+    cur_kernarg = first_arg(args...);
+    cur_block_dim = block;
+    cur_grid = grid;
+    for (unsigned bx = 0; bx < grid.x; ++bx) {
+        cur_block = dim3(bx);
+        for (int t = 0; t < nt; ++t) {
+            Lane& l = G.lanes[t];
+            getcontext(&l.ctx);
+            l.ctx.uc_stack.ss_sp = l.stack.get();
+            l.ctx.uc_stack.ss_size = STACK;
+            l.ctx.uc_link = nullptr;
+            makecontext(&l.ctx, lane_entry, 0);
+            l.wait = RUNNING;
+            l.started = false;
+        }
+        for (;;) {
+            // run every runnable lane up to its next synchronisation point
+            for (int t = 0; t < nt; ++t) {
+                if (G.lanes[t].wait != RUNNING) continue;
+                G.current = t;
+                cur_thread = dim3((unsigned)t);
+                if (!_setjmp(G.sched_jb)) {
+                    Lane& l = G.lanes[t];
+                    if (l.started) _longjmp(l.jb, 1);
+                    l.started = true;
+                    swapcontext(&G.sched, &l.ctx);
+                }
+            }
+            int done = 0, at_bar = 0;
+            for (const Lane& l : G.lanes) {
+                done += l.wait == DONE;
+                at_bar += l.wait == AT_BARRIER;
+            }
+            if (done == nt) break;
+            bool released = false;
+            if (at_bar == nt) {
+                for (Lane& l : G.lanes) l.wait = RUNNING;
+                released = true;
+            }
+            for (int w = 0; w * 64 < nt && !released; ++w) {
+                const int hi = std::min(nt, 64 * (w + 1));
+                bool all = true;
+                for (int t = 64 * w; t < hi; ++t) all = all && G.lanes[t].wait == AT_WAVE;
+                if (all) {
+                    for (int t = 64 * w; t < hi; ++t) G.lanes[t].wait = RUNNING;
+                    released = true;
+                }
+            }
+            if (!released) deadlock(G, bx);
+        }
+    }
+    g_group = saved;
+}
+}  // namespace emu
+
+#define threadIdx (emu::cur_thread)
+#define blockIdx (emu::cur_block)
+#define blockDim (emu::cur_block_dim)
+#define gridDim (emu::cur_grid)
+#define __syncthreads() emu::syncthreads()
+#define hipLaunchKernelGGL(k, g, b, s, st, ...) emu::launch(k, dim3(g), dim3(b), (size_t)(s), __VA_ARGS__)
+
+template <class T>
+inline T __shfl_up(T v, unsigned d, int = 64) {
+    const int lane = threadIdx.x & 63;
+    return emu::exchange(v, lane >= (int)d ? lane - (int)d : -1);
+}
+template <class T>
+inline T __shfl_xor(T v, int m, int = 64) {
+    const int lane = threadIdx.x & 63;
+    return emu::exchange(v, lane ^ m);
+}
+// readfirstlane: lane 0's value; the kernels only apply it to values every
+// lane of the wave already holds, which is checked here
+inline int __builtin_amdgcn_readfirstlane(int x) {
+    static const bool check = std::getenv("EMU_NO_UNIFORM_CHECK") == nullptr;
+    if (!check) return x;
+    const int first = emu::exchange(x, 0);
+    if (x != first) {
+        std::fprintf(stderr, "readfirstlane of a value that differs across the wave\n");
+        std::abort();
+    }
+    return first;
+}
+inline const void* __builtin_amdgcn_kernarg_segment_ptr() { return emu::cur_kernarg; }
+inline long long __double_as_longlong(double d) { long long i; std::memcpy(&i, &d, 8); return i; }
+inline double __longlong_as_double(long long i) { double d; std::memcpy(&d, &i, 8); return d; }
+inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
+inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline float __fadd_rn(float a, float b) { return a + b; }
+inline float __fsub_rn(float a, float b) { return a - b; }
+inline float __fmul_rn(float a, float b) { return a * b; }
+inline float __fdiv_rn(float a, float b) { return a / b; }
+inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
+inline float __uint_as_float(unsigned i) { float f; std::memcpy(&f, &i, 4); return f; }
+inline unsigned __float_as_uint(float f) { unsigned i; std::memcpy(&i, &f, 4); return i; }
+using std::min;
+using std::max;
+using std::signbit;
+
+// ------------------------------------------------------------ runtime ----
+inline const char* hipGetErrorString(hipError_t e) { return e == 0 ? "hipSuccess" : "emulated HIP error"; }
+inline hipError_t hipGetLastError() { return hipSuccess; }
+inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
+template <class T>
+inline hipError_t hipMalloc(T** p, size_t n) {
+    // device memory is not cleared by hipMalloc: fill it with a pattern
+    // (EMU_FILL, default 0xA5) so reads of never-written memory show up
+    const char* f = std::getenv("EMU_FILL");
+    *p = static_cast<T*>(std::malloc(std::max<size_t>(n, 8)));
+    if (*p) std::memset((void*)*p, f ? (int)std::strtol(f, nullptr, 0) : 0xA5, std::max<size_t>(n, 8));
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
+    return hipMemcpy(d, s, n, k);
+}
+inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipEventCreate(hipEvent_t* e) {
+    *e = new std::chrono::steady_clock::time_point();
+    return hipSuccess;
+}
+inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
+    *static_cast<std::chrono::steady_clock::time_point*>(e) = std::chrono::steady_clock::now();
+    return hipSuccess;
+}
+inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
+    auto& ta = *static_cast<std::chrono::steady_clock::time_point*>(a);
+    auto& tb = *static_cast<std::chrono::steady_clock::time_point*>(b);
+    *ms = std::chrono::duration<float, std::milli>(tb - ta).count();
+    return hipSuccess;
+}
+inline hipError_t hipEventDestroy(hipEvent_t e) {
+    delete static_cast<std::chrono::steady_clock::time_point*>(e);
+    return hipSuccess;
+}
+template <class K>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, size_t) {
+    *n = 1;
+    return hipSuccess;
+}

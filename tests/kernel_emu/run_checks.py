@@ -1,0 +1,172 @@
+"""Runs the library's DCFR kernels on the CPU (libpsr_dcfr_emu.so, built by
+build_emu.py) against the oracle.  Started in its own process by
+tests/test_kernel_emu.py so that the emulated library never shares a process
+with the HIP one.  Usage: run_checks.py CHECK [CHECK ...]; exit status 0 when
+every check passes, the failing check named on stderr otherwise.
+"""
+import os
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+import oracle_binding as O  # noqa: E402
+import poker_solver_rust_amd as P  # noqa: E402
+from poker_solver_rust_amd import range_solver as R  # noqa: E402
+from poker_solver_rust_amd import workloads as W  # noqa: E402
+
+R._LIB = R.load_library(os.path.join(HERE, "libpsr_dcfr_emu.so"))
+
+
+def std_game(bets="50%, a", raises=""):
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
+    tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
+                      river_bet_sizes=[(bets, raises), (bets, raises)])
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def turn_game(oop="AA,KK,AKs", ip="QQ-JJ,AQs", board="Qs Jh 2c", turn="8d"):
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board),
+                      turn=P.card_from_str(turn))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200, turn_bet_sizes=s,
+                      river_bet_sizes=s)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def same(a, b, what):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.shape != b.shape or not np.array_equal(a, b):
+        n = int((a != b).sum()) if a.shape == b.shape else -1
+        raise AssertionError(f"{what}: {n} entries differ")
+
+
+def check_stepwise(game, iters, storage="f32"):
+    b = P.GpuBatchSolver(0, storage=storage)
+    b.prepare_batch([game], iters + 1)
+    o = O.OracleGame(game.serialize(), 16 if storage == "compressed" else 32)
+    for t in range(iters):
+        b.run_iterations(t, t + 1)
+        o.solve_step(t)
+        if storage == "compressed":
+            r, s, rs, ss = b.raw_state(0)
+            orr, os_, ors, oss = o.compressed_state()
+            same(r.view(np.int16), orr, f"iteration {t} regret words")
+            same(s, os_, f"iteration {t} strategy words")
+            same(rs, ors, f"iteration {t} regret scales")
+            same(ss, oss, f"iteration {t} strategy scales")
+        else:
+            r, s = o.state()
+            same(b.regrets(0), r, f"iteration {t} regrets")
+            same(b.strategy_sum(0), s, f"iteration {t} strategy sums")
+    e = float(b.exploitability()[0])
+    if abs(e - o.exploitability()) > 1e-6 * max(1.0, abs(o.exploitability())):
+        raise AssertionError(f"exploitability {e} vs oracle {o.exploitability()}")
+    if storage == "f32":
+        cf = b.root_cfvalues()[0]
+        same(cf[0], o.root_cfvalues(0), "root cfvalues player 0")
+        same(cf[1], o.root_cfvalues(1), "root cfvalues player 1")
+
+
+def run_std_stepwise():
+    check_stepwise(std_game(), 6)
+
+
+def run_std_raises_stepwise():
+    check_stepwise(std_game("33%, 75%, a", "2x, a"), 4)
+
+
+def run_std_solve():
+    g = std_game()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 60, [1.0])
+    e, it = b.solve(60, 5)
+    o = O.OracleGame(g.serialize())
+    eo, ito = o.solve(60, 1.0)
+    if int(it[0]) != ito or float(e[0]) != np.float32(eo):
+        raise AssertionError(f"solve: ({float(e[0])}, {int(it[0])}) vs oracle ({eo}, {ito})")
+
+
+def run_batch():
+    """Three games of different sizes in one batch (work queue order, rows of
+    different strides), each compared with its own oracle."""
+    games = [std_game(), std_game("33%, a"), W.river_batch(1, seed=5, iterations=10)[0].build()]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 8)
+    b.run_iterations(0, 3)
+    for i, g in enumerate(games):
+        o = O.OracleGame(g.serialize())
+        for t in range(3):
+            o.solve_step(t)
+        r, s = o.state()
+        same(b.regrets(i), r, f"game {i} regrets")
+        same(b.strategy_sum(i), s, f"game {i} strategy sums")
+
+
+def run_turn_stepwise():
+    """A turn game: root tile plus one subtree tile per river card, driven by
+    the phase kernel; then the FP64 best response through the phase kernel."""
+    g = turn_game("AA,AKs", "KK,QQ")
+    check_stepwise(g, 2)
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 2)
+    b.run_iterations(0, 2)
+    e64 = float(b.exploitability_f64()[0])
+    o = O.OracleGame(g.serialize(), 64)
+    o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    eo = o.exploitability()
+    if abs(e64 - eo) > 1e-9 * max(1.0, abs(eo)):
+        raise AssertionError(f"turn fp64 exploitability {e64} vs oracle {eo}")
+
+
+def run_compressed_stepwise():
+    check_stepwise(std_game(), 6, storage="compressed")
+
+
+def run_fp64_exploitability():
+    """FP64 best response vs the f64 oracle's best response of the same
+    strategy sums, on two river trees."""
+    for g in (std_game(), std_game("33%, 75%, a", "2x, a")):
+        b = P.GpuBatchSolver(0)
+        b.prepare_batch([g], 10)
+        b.run_iterations(0, 10)
+        e64 = float(b.exploitability_f64()[0])
+        o = O.OracleGame(g.serialize(), 64)
+        o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+        eo = o.exploitability()
+        if abs(e64 - eo) > 1e-9 * max(1.0, abs(eo)):
+            raise AssertionError(f"fp64 exploitability {e64} vs oracle {eo}")
+
+
+def run_half_storage():
+    """FP16 / BF16 words with per-node scales: the solve converges like the
+    FP64 oracle on the same tree (no reference counterpart)."""
+    g = std_game()
+    o = O.OracleGame(g.serialize(), 64)
+    eo, _ = o.solve(100)
+    for storage in ("fp16", "bf16"):
+        b = P.GpuBatchSolver(0, storage=storage)
+        b.prepare_batch([g], 100)
+        e, _ = b.solve(100, 0)
+        cf = b.root_cfvalues()[0]
+        err = max(np.abs(cf[0] - o.root_cfvalues(0)).max(), np.abs(cf[1] - o.root_cfvalues(1)).max())
+        if not (abs(float(e[0]) - eo) <= 0.5 and err <= 1.0):   # chips, pot 100
+            raise AssertionError(f"{storage}: expl {float(e[0])} vs fp64 {eo}, root cfv error {err}")
+
+
+CHECKS = {name[4:]: fn for name, fn in globals().items() if name.startswith("run_")}
+
+if __name__ == "__main__":
+    import time
+    for name in sys.argv[1:]:
+        print(f"check {name}", flush=True)
+        t0 = time.time()
+        CHECKS[name]()
+        print(f"check {name} passed in {time.time() - t0:.1f} s", flush=True)
+    sys.exit(0)
