@@ -82,6 +82,31 @@ def cpu_baseline(subgames, iters, mode, interval, seconds_target=15.0):
                       f"(mean {float(its.mean()):.1f}), {threads} threads on '{cpu_model()}'"}
 
 
+def turn_bench(P, W, iters, seed):
+    """Turn DCFR iterations/s on the full turn subgame (4 seeded board cards,
+    48 rivers, U[0,1] ranges, pot 60, stack 200, 33%/75%/all-in, 2 raises at
+    3x): solve_step iterations only, timed over `iters` iterations after one
+    warm-up iteration."""
+    s = W.turn_uniform01(seed)
+    g = s.build()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters + 1)
+    b.run_iterations(0, 1)
+    t = time.perf_counter()
+    b.run_iterations(1, iters + 1)
+    secs = time.perf_counter() - t
+    kms = b.last_kernel_ms()
+    edge_hands = float(g.num_storage())
+    abytes = 20.0 * edge_hands * iters
+    achieved = abytes / (kms * 1e-3) / 1e9
+    log(f"turn: {iters} iterations in {secs * 1e3:.1f} ms ({iters / secs:.1f} it/s), {g.num_nodes()} nodes, "
+        f"{edge_hands:.0f} edge-hands")
+    return {"value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
+            "edge_hands": edge_hands, "ms_per_iter": round(secs * 1e3 / iters, 4),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 5)}}
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
@@ -100,6 +125,9 @@ def main():
     ap.add_argument("--mode", choices=["solve", "fixed"], default="solve")
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--single-sample", type=int, default=64,
+                    help="subgames solved one per launch for the single-subgame yardstick (0: skip)")
+    ap.add_argument("--turn-iters", type=int, default=200, help="timed turn DCFR iterations (0: skip)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -179,6 +207,22 @@ def main():
         "kernel_ms": round(kms, 3),
         "algorithmic_bytes_per_launch": abytes,
     }
+    # the plain single-subgame path on the same subgames: one game per batch,
+    # one launch each (the batched launch is the product; this is its yardstick)
+    n_single = min(args.single_sample, len(games))
+    if n_single > 0:
+        t = time.perf_counter()
+        for i in range(n_single):
+            one = P.GpuBatchSolver(0)
+            one.prepare_batch([games[i]], args.iterations, targets[i:i + 1] if args.mode == "solve" else None)
+            one.solve(args.iterations, interval)
+        single_rate = n_single / (time.perf_counter() - t)
+        out["single_subgame"] = {"value": round(single_rate, 3), "unit": "solves/s", "sample": n_single,
+                                 "batched_over_single": round(value / single_rate, 2)}
+        log(f"single-subgame path: {single_rate:.1f} solves/s on {n_single} subgames; batched is "
+            f"{value / single_rate:.1f}x")
+    if args.turn_iters > 0:
+        out["turn"] = turn_bench(P, W, args.turn_iters, args.seed)
     if not args.no_cpu_baseline:
         n_sample = min(len(subs), 512)
         out["cpu_baseline"] = cpu_baseline(subs[:n_sample], args.iterations, args.mode, interval)

@@ -109,6 +109,13 @@ double psr_game_num_combinations(const psr_game* game);
 /* sum over decision nodes of actions * hands: the size of the regret and of
  * the strategy-sum arrays (game/interpreter.rs:212-249 num_storage). */
 int64_t psr_game_num_storage(const psr_game* game);
+/* Depth-boundary leaves (TreeConfig depth_limit): their number
+ * (interpreter.rs:313-317 num_boundary_nodes), the arena index of ordinal b
+ * (interpreter.rs:320-330 boundary_node_indices; ordinals in the reference's
+ * arena order) and the pot there (interpreter.rs:354-358 boundary_pot). */
+int psr_game_num_boundaries(const psr_game* game);
+int64_t psr_game_boundary_node(const psr_game* game, int ordinal);
+int psr_game_boundary_pot(const psr_game* game, int ordinal);
 /* Flat description of the built game (format in game.cpp) -- the input of the
  * CPU oracle and of the device layout.  Returns the byte size; writes it when
  * cap is large enough.  Replaces extract_topology/extract_terminal_data
@@ -183,6 +190,25 @@ int psr_batch_set_storage(psr_batch* batch, int mode);
  * (regrets_compressed / strategy_compressed / scales). */
 int psr_batch_raw_state(psr_batch* batch, int game, uint16_t* regret, uint16_t* strategy, float* rscale,
                         float* sscale, int64_t cap, int32_t cap_nodes);
+/* Depth-boundary leaves of a prepared game (single continuation).
+ * psr_batch_set_boundary_cfvs: interpreter.rs:343-352 set_boundary_cfvs --
+ * H_player values for `ordinal`; raw == 0: pot-normalised values, the leaf
+ * value of hand i is v_i * (half_pot / num_combinations) * (opponent reach
+ * not blocked by hand i) (evaluation.rs:544-604); raw != 0: chip cfvalues
+ * already integrated over the opponent reach, used as they are
+ * (evaluation.rs:606-630).  The mode is per ordinal.  A boundary with no
+ * values contributes 0.
+ * psr_batch_update_leaf_cfvs: batch.rs:557-585 update_leaf_cfvs -- raw
+ * values of every boundary of every game at once: p0 (p1) holds, per game in
+ * batch order and per ordinal, H_0 (H_1) values.
+ * psr_batch_boundary_reach: interpreter.rs:396-403 boundary_reach -- the
+ * reach of `player` recorded when the opponent last traversed the boundary
+ * (every traversal records it, evaluation.rs:91-99); zeros before any visit. */
+int psr_batch_num_boundaries(psr_batch* batch, int game);
+int psr_batch_set_boundary_cfvs(psr_batch* batch, int game, int ordinal, int player, const float* cfvs, int n,
+                                int raw);
+int psr_batch_update_leaf_cfvs(psr_batch* batch, const float* p0, int64_t n0, const float* p1, int64_t n1);
+int psr_batch_boundary_reach(psr_batch* batch, int game, int ordinal, int player, float* out, int n);
 /* Wall time of the last device job in milliseconds (HIP events on the batch
  * stream) -- used for roofline accounting. */
 float psr_batch_last_kernel_ms(psr_batch* batch);

@@ -73,6 +73,9 @@ struct Game {
     // leaf values injected at depth boundaries: [ordinal][player] -> H values
     std::vector<int> boundary_ordinal;   // per node, -1 if not a boundary
     std::vector<std::vector<float>> boundary_cfv[2];
+    std::vector<char> boundary_raw;      // per ordinal: values are raw chip cfvalues
+    // opponent reach at the last visit, [ordinal][opponent] (evaluation.rs:91-99)
+    std::vector<std::vector<float>> boundary_reach[2];
     int H(int p) const { return (int)cards[p].size(); }
     bool raked() const { return rake_rate > 0.0 && rake_cap > 0.0; }
 };
@@ -166,7 +169,12 @@ bool load(const uint8_t* blob, size_t len, Game& g) {
     for (uint32_t i = 0; i < n; ++i) {
         Node& nd = g.nodes[i];
         if (nd.player & TERMINAL) {
-            if ((nd.player & BOUNDARY) == BOUNDARY) g.boundary_ordinal[i] = bcount++;
+            // a boundary leaf carries its ordinal (reference arena order,
+            // interpreter.rs:786-797) in the child field
+            if ((nd.player & BOUNDARY) == BOUNDARY) {
+                g.boundary_ordinal[i] = (int)nd.first_child;
+                ++bcount;
+            }
             continue;
         }
         if (nd.player & CHANCE) continue;
@@ -175,7 +183,13 @@ bool load(const uint8_t* blob, size_t len, Game& g) {
         off += (int64_t)nd.num_children * g.H(nd.player);
     }
     g.storage_size = off;
-    for (int p = 0; p < 2; ++p) g.boundary_cfv[p].assign(bcount, std::vector<float>());
+    for (int i : g.boundary_ordinal)
+        if (i >= bcount) return false;
+    for (int p = 0; p < 2; ++p) {
+        g.boundary_cfv[p].assign(bcount, std::vector<float>());
+        g.boundary_reach[p].assign(bcount, std::vector<float>());
+    }
+    g.boundary_raw.assign(bcount, 0);
     return true;
 }
 
@@ -424,9 +438,16 @@ struct Solver {
     void evaluate_boundary(T* result, const Node& node, uint32_t node_index, int player, const T* cfreach,
                            double half_pot) const {
         int ord = g.boundary_ordinal[node_index];
+        // record the opponent reach of this visit (evaluation.rs:91-99)
+        g.boundary_reach[player ^ 1][ord].assign(cfreach, cfreach + g.H(player ^ 1));
         const auto& bcfv = g.boundary_cfv[player][ord];
         if (bcfv.empty()) return;
         const Runout& ro = g.runouts[node.runout];
+        if (g.boundary_raw[ord]) {
+            // raw chip cfvalues for the valid hands (evaluation.rs:606-630)
+            for (uint16_t i : ro.valid[player]) result[i] = (T)bcfv[i];
+            return;
+        }
         const auto& pc = g.cards[player];
         const auto& oc = g.cards[player ^ 1];
         double scale = half_pot / g.num_combinations;
@@ -856,13 +877,31 @@ int oracle_evaluate(void* h, uint32_t node, int player, const double* cfreach, d
 }
 
 // leaf values for depth-boundary ordinal `ord` (evaluation.rs:544-604)
-int oracle_set_boundary_cfvs(void* hv, int ord, int player, const float* v, int n) {
+// (interpreter.rs:343-352); raw != 0: the values are raw chip cfvalues
+// (the boundary_is_raw mode, evaluation.rs:606-630), set per ordinal
+int oracle_set_boundary_cfvs(void* hv, int ord, int player, const float* v, int n, int raw) {
     Handle* h = static_cast<Handle*>(hv);
     auto& b = h->game.boundary_cfv[player];
-    if (ord < 0 || ord >= (int)b.size() || n != h->game.H(player)) return -1;
+    if (player < 0 || player > 1 || ord < 0 || ord >= (int)b.size() || n != h->game.H(player)) return -1;
     b[ord].assign(v, v + n);
+    h->game.boundary_raw[ord] = raw ? 1 : 0;
     return 0;
 }
+
+// opponent reach recorded at the last visit of boundary `ord` by the
+// traverser `player ^ 1` (interpreter.rs:396-403); returns its length (0 if
+// never visited)
+int oracle_boundary_reach(void* hv, int ord, int player, float* out, int cap) {
+    Handle* h = static_cast<Handle*>(hv);
+    auto& b = h->game.boundary_reach[player];
+    if (player < 0 || player > 1 || ord < 0 || ord >= (int)b.size()) return -1;
+    const auto& v = b[ord];
+    if ((int)v.size() > cap) return -1;
+    std::copy(v.begin(), v.end(), out);
+    return (int)v.size();
+}
+
+int oracle_num_boundaries(void* hv) { return (int)static_cast<Handle*>(hv)->game.boundary_raw.size(); }
 
 // CPU baseline: solve `n` serialized games with `threads` worker threads, one
 // solve per thread at a time (the reference runs river subgames single-

@@ -176,6 +176,22 @@ double psr_game_num_combinations(const psr_game* g) { return g ? g->game.num_com
 
 int64_t psr_game_num_storage(const psr_game* g) { return g ? (int64_t)g->game.num_storage() : fail("null game"); }
 
+int psr_game_num_boundaries(const psr_game* g) {
+    return g ? (int)g->game.boundary_nodes().size() : fail("null game");
+}
+
+int64_t psr_game_boundary_node(const psr_game* g, int ordinal) {
+    if (!g) return fail("null game");
+    if (ordinal < 0 || ordinal >= (int)g->game.boundary_nodes().size()) return fail("boundary ordinal out of range");
+    return (int64_t)g->game.boundary_nodes()[ordinal];
+}
+
+int psr_game_boundary_pot(const psr_game* g, int ordinal) {
+    if (!g) return fail("null game");
+    if (ordinal < 0 || ordinal >= (int)g->game.boundary_nodes().size()) return fail("boundary ordinal out of range");
+    return g->game.boundary_pot(ordinal);
+}
+
 int64_t psr_game_serialize(const psr_game* g, uint8_t* buf, int64_t cap) {
     if (!g) return fail("null game");
     std::vector<uint8_t> b = g->game.serialize();

@@ -367,7 +367,30 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     const double win = A.payoff[(size_t)3 * (c.G.node_base + n) + 0];
     const double lose = A.payoff[(size_t)3 * (c.G.node_base + n) + 1];
     const double total = c.scan[T.num_valid_opp];
-    if (nd.kind == KIND_FOLD) {
+    if (nd.kind == KIND_BOUNDARY) {
+        // depth-boundary leaf (evaluation.rs:87-120, 544-630): record the
+        // opponent reach of this visit, then the leaf value of every hand.
+        // win holds half_pot / num_combinations.
+        const int b = nd.state_off;
+        float* rec = A.bnd_reach + c.G.bnd_base + (size_t)(2 * b + (p ^ 1)) * c.stride;
+        for (int h = threadIdx.x; h < Ho; h += BLOCK) rec[h] = (float)rs[h];
+        const int mode = uni(A.bnd_mode[c.G.bnd_index + b]);
+        const float* bv = A.bnd_cfv + c.G.bnd_base + (size_t)(2 * b + p) * c.stride;
+        for (int i = threadIdx.x; i < Hp; i += BLOCK) {
+            V res = 0;
+            const HandScan q = hs[i];
+            const bool valid = (q.x & 0xFFFF) != 0xFFFF;
+            if (mode == BND_RAW && valid) {
+                res = (V)bv[i];
+            } else if (mode == BND_SCALED && valid && total != 0.0) {
+                const uint16_t si = A.hand_same[hand_base(c.G, p) + i];
+                const double sm = si == 0xFFFF ? 0.0 : (double)rs[si];
+                const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
+                res = (V)(((double)bv[i] * win) * cr);
+            }
+            out[i] = res;
+        }
+    } else if (nd.kind == KIND_FOLD) {
         const double payoff = (int)nd.player != p ? win : lose;
         const uint16_t* same = A.hand_same + hand_base(c.G, p);
         for (int i = threadIdx.x; i < Hp; i += BLOCK) {

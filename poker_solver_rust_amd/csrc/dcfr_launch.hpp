@@ -79,6 +79,10 @@ struct KernelArgs {
     uint32_t iteration;
     double* ev_out;           // per game and player: weighted root value (best response)
     double* expl64_out;       // per game: FP64 exploitability
+    // depth-boundary leaves (GameDesc::bnd_base / bnd_index)
+    const float* bnd_cfv;     // given values, row (2b + traversing player)
+    float* bnd_reach;         // opponent reach at the last visit, row (2b + opponent)
+    const int32_t* bnd_mode;  // BoundaryMode per boundary
 };
 
 size_t dcfr_lds_bytes(int max_hands);

@@ -35,8 +35,9 @@ enum NodeKind : uint8_t { KIND_DECISION = 0, KIND_FOLD = 1, KIND_SHOWDOWN = 2, K
 
 struct NodeRec {
     int32_t first_child;    // local node id of child 0
-    int32_t state_off;      // float offset of the A x H block in regret/strategy (decision)
-    int32_t aux;            // terminal: runout table (global id); chance: chance-table id
+    int32_t state_off;      // decision: float offset of the A x H block in regret/strategy;
+                            // depth boundary: the boundary ordinal within its game
+    int32_t aux;            // terminal / boundary: runout table (global id); chance: chance-table id
     uint16_t num_children;
     uint8_t kind;
     uint8_t player;         // decision: acting player; fold: folded player
@@ -98,6 +99,20 @@ struct GameDesc {
     int32_t tile_base;      // tile 0 of the game is its root tile
     int32_t num_tiles;
     float target;           // early-stop exploitability (chips)
+    int64_t bnd_base;       // float offset into the boundary rows: per depth-boundary
+                            // ordinal b and player q, row (2b + q) of row_stride floats
+    int32_t bnd_index;      // index of boundary 0 of the game in the per-boundary mode array
+    int32_t num_boundaries;
+};
+
+// How a depth-boundary leaf produces its values (single continuation).
+enum BoundaryMode : int32_t {
+    BND_UNSET = 0,    // no values given: the leaf contributes 0
+    BND_SCALED = 1,   // value * (half_pot / num_combinations) * blocked opponent reach
+                      // (evaluation.rs:544-604)
+    BND_RAW = 2,      // values are chip cfvalues already integrated over the opponent
+                      // reach, written as they are (evaluation.rs:606-630, the GPU
+                      // solver's leaf injection, kernels.rs:546-554)
 };
 
 }  // namespace psr

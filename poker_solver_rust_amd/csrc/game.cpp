@@ -857,8 +857,43 @@ bool PostFlopGame::with_config(const CardConfig& cards, const ActionTree& tree, 
     if (!g.check_card_config(err)) return false;
     g.init_card_fields();
     g.build_arena(tree.root());
+    g.number_boundaries();
     out = std::move(g);
     return true;
+}
+
+// Boundary ordinals in the reference's arena order (interpreter.rs:786-797,
+// 815-864): the reference arena is street-major and a street's nodes are
+// allocated block by block as its recursive build reaches each parent (a
+// decision node's children stay on its street, a chance node's children go
+// to the next street).  Replaying that recursion over this arena gives every
+// node its (street, allocation rank) key; boundaries are numbered by it.
+void PostFlopGame::number_boundaries() {
+    const size_t n = nodes_.size();
+    boundary_of_.assign(n, -1);
+    boundary_nodes_.clear();
+    auto street = [&](const GameNode& nd) { return nd.turn == NOT_DEALT ? 0 : nd.river == NOT_DEALT ? 1 : 2; };
+    std::vector<std::pair<int, uint64_t>> key(n, {0, 0});
+    uint64_t counter[3] = {0, 0, 0};
+    key[0] = {street(nodes_[0]), counter[street(nodes_[0])]++};
+    std::vector<uint32_t> stack{0};
+    while (!stack.empty()) {
+        const uint32_t v = stack.back();
+        stack.pop_back();
+        const GameNode& nd = nodes_[v];
+        if (nd.is_terminal() || nd.num_children == 0) continue;
+        for (uint32_t k = 0; k < nd.num_children; ++k) {
+            const uint32_t c = nd.first_child + k;
+            const int st = street(nodes_[c]);
+            key[c] = {st, counter[st]++};
+        }
+        // children are visited in order: push in reverse
+        for (uint32_t k = nd.num_children; k-- > 0;) stack.push_back(nd.first_child + k);
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (nodes_[i].is_boundary()) boundary_nodes_.push_back(i);
+    std::sort(boundary_nodes_.begin(), boundary_nodes_.end(), [&](uint32_t a, uint32_t b) { return key[a] < key[b]; });
+    for (size_t b = 0; b < boundary_nodes_.size(); ++b) boundary_of_[boundary_nodes_[b]] = (int32_t)b;
 }
 
 // game/interpreter.rs:542-642
@@ -1144,7 +1179,8 @@ void PostFlopGame::build_arena(const ActionTreeNode* root) {
 //   u8 flop[3], u8 turn, u8 river, u8 initial_state, u8 pad[2]
 //   2 x { u32 H; H x (u8 c1, u8 c2); H x f32 weight; H x u16 same_hand_index }
 //   u32 N; N x { u8 player, u8 turn, u8 river, u8 pad, i32 amount, u32 first_child,
-//                u32 num_children, i32 runout }
+//                u32 num_children, i32 runout }   (first_child of a depth-boundary
+//                leaf holds its boundary ordinal)
 //   u32 R; R x { u8 turn, u8 river, u16 pad; 2 x {u32 n; n x u16 valid};
 //                2 x {u32 n; n x (u16 strength, u16 index)} }
 //   u32 C; C x { u32 node, u32 count; count x { u32 ref; 2 x {u32 n; n x (u16, u16)} } }
@@ -1186,7 +1222,7 @@ std::vector<uint8_t> PostFlopGame::serialize() const {
         w.put<uint8_t>(n.river);
         w.put<uint8_t>(0);
         w.put<int32_t>(n.amount);
-        w.put<uint32_t>(n.first_child);
+        w.put<uint32_t>(n.is_boundary() ? (uint32_t)boundary_of_[&n - nodes_.data()] : n.first_child);
         w.put<uint32_t>(n.num_children);
         w.put<int32_t>(n.is_terminal() ? runout_index(n.turn, n.river) : -1);
     }

@@ -204,6 +204,16 @@ public:
     int chance_factor(const GameNode& n) const { return n.turn == NOT_DEALT ? 45 : 44; }
     const std::vector<IsoChance>& iso_chances(uint32_t node) const;
     uint64_t num_storage() const { return num_storage_; }   // sum of A*H over decisions
+    // Depth-boundary leaves: arena indices by boundary ordinal, and the
+    // ordinal of a node (-1 if it is not a boundary).  Ordinals follow the
+    // reference arena (interpreter.rs:786-797): street-major, and within a
+    // street the order in which its recursive build allocates children.
+    const std::vector<uint32_t>& boundary_nodes() const { return boundary_nodes_; }
+    int boundary_ordinal(uint32_t node) const {
+        return node < boundary_of_.size() ? boundary_of_[node] : -1;
+    }
+    // pot at a boundary, interpreter.rs:354-358
+    int boundary_pot(int ordinal) const { return tcfg_.starting_pot + 2 * nodes_[boundary_nodes_[ordinal]].amount; }
 
     // Flat little-endian description consumed by the CPU oracle and the GPU
     // layout builder (format documented in game.cpp).
@@ -214,6 +224,7 @@ private:
     void init_card_fields();
     void build_arena(const ActionTreeNode* root);
     void compute_isomorphism();
+    void number_boundaries();
 
     CardConfig ccfg_;
     TreeConfig tcfg_;
@@ -234,6 +245,8 @@ private:
     std::vector<std::vector<IsoChance>> iso_by_node_;   // per chance node (by arena index)
     std::vector<IsoChance> empty_iso_;
     uint64_t num_storage_ = 0;
+    std::vector<uint32_t> boundary_nodes_;
+    std::vector<int32_t> boundary_of_;
 };
 
 }  // namespace psr

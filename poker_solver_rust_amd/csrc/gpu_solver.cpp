@@ -55,6 +55,8 @@ struct HostLayout {
     std::vector<int> multi_games;    // games solved by phase launches (turn)
     std::vector<std::pair<double, int>> subtree_cost;
     std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
+    int32_t num_boundaries = 0;      // depth-boundary leaves over the batch
+    int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
 };
 
 int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, size_t align = 4) {
@@ -268,6 +270,11 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         d.state_base = L.state_total;
         d.scratch_base = L.scratch_total;
         d.target = targets ? targets[gi] : 0.0f;
+        d.num_boundaries = (int32_t)g.boundary_nodes().size();
+        d.bnd_index = L.num_boundaries;
+        d.bnd_base = L.bnd_rows_total;
+        L.num_boundaries += d.num_boundaries;
+        L.bnd_rows_total += (int64_t)d.num_boundaries * 2 * d.row_stride;
 
         for (int p = 0; p < 2; ++p) {
             d.hand_base[p] = (int32_t)L.cards.size();
@@ -309,8 +316,17 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             r.state_off = (int32_t)state_off[i];
             r.aux = -1;
             double win = 0, lose = 0, tie = 0;
-            if (gn.is_terminal()) {
-                if (gn.is_boundary()) return fail("depth-boundary leaves are not supported by the device path yet");
+            if (gn.is_boundary()) {
+                // depth-boundary leaf: ordinal in state_off, payoff scale
+                // half_pot / num_combinations (evaluation.rs:544-555)
+                r.kind = KIND_BOUNDARY;
+                r.player = 0;
+                r.state_off = g.boundary_ordinal((uint32_t)i);
+                const int ro = g.runout_index(gn.turn, gn.river);
+                if (ro < 0 || r.state_off < 0) return fail("boundary without runout or ordinal");
+                r.aux = runout_base + ro;
+                win = 0.5 * (double)(tc.starting_pot + 2 * gn.amount) / ncomb;
+            } else if (gn.is_terminal()) {
                 r.kind = gn.is_fold() ? KIND_FOLD : KIND_SHOWDOWN;
                 r.player = gn.player & PLAYER_MASK;
                 const int ro = g.runout_index(gn.turn, gn.river);
@@ -430,12 +446,14 @@ struct psr_batch {
     int* d_counter = nullptr;
     int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr;
     double* d_ev = nullptr;
+    float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
+    int32_t* d_bnd_mode = nullptr;
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
-                        d_subtrees, d_ev};
+                        d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
@@ -445,6 +463,7 @@ struct psr_batch {
         d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
         d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
+        d_bnd_cfv = nullptr; d_bnd_reach = nullptr; d_bnd_mode = nullptr;
         prepared = false;
     }
 
@@ -491,6 +510,9 @@ struct psr_batch {
         a.subtree_tiles = d_subtrees;
         a.num_subtree_tiles = (int)L.subtree_tiles.size();
         a.ev_out = d_ev;
+        a.bnd_cfv = d_bnd_cfv;
+        a.bnd_reach = d_bnd_reach;
+        a.bnd_mode = d_bnd_mode;
         return a;
     }
 
@@ -559,22 +581,13 @@ struct psr_batch {
         return 0;
     }
 
-    // Runs a job on every game: the whole-game kernel for single-tile games,
-    // then the phase loop (reference solve() schedule driven from the host)
-    // for multi-tile games.  Timed with events on the batch stream.
+    // Runs a job on every game: the phase loop (reference solve() schedule
+    // driven from the host) for multi-tile games, then the whole-game kernel
+    // for single-tile games.  Timed with events on the batch stream.
     int run(int flags, uint32_t begin, uint32_t end, int check_interval, uint32_t max_iters) {
         if (!prepared) return fail("psr_batch_prepare has not been called");
         HIP_OK(hipSetDevice(device));
         HIP_OK(hipEventRecord(ev0, stream));
-        if (!L.single_games.empty()) {
-            KernelArgs a = args();
-            a.iter_begin = begin;
-            a.iter_end = end;
-            a.max_iterations = max_iters;
-            a.check_interval = check_interval;
-            a.flags = flags;
-            HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
-        }
         if (!L.multi_games.empty()) {
             const float inf = INFINITY;
             std::vector<float> expl(L.games.size(), inf);
@@ -628,6 +641,18 @@ struct psr_batch {
                 HIP_OK(hipMemcpyAsync(d_iters + gi, &iters[gi], sizeof(uint32_t), hipMemcpyHostToDevice, stream));
                 HIP_OK(hipStreamSynchronize(stream));
             }
+        }
+        // Single-tile games after the multi-tile loop: that loop rewrites the
+        // whole per-game exploitability array (its skip flags), which would
+        // clobber results the whole-game kernel had already written.
+        if (!L.single_games.empty()) {
+            KernelArgs a = args();
+            a.iter_begin = begin;
+            a.iter_end = end;
+            a.max_iterations = max_iters;
+            a.check_interval = check_interval;
+            a.flags = flags;
+            HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
         }
         HIP_OK(hipEventRecord(ev1, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -714,6 +739,14 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(hipMalloc((void**)&b->d_root_cfv, std::max<size_t>(L.cards.size(), 1) * sizeof(float)));
     HIP_OK(hipMalloc((void**)&b->d_counter, 2 * sizeof(int)));
     HIP_OK(hipMalloc((void**)&b->d_ev, 2 * n * sizeof(double)));
+    const int64_t brows = std::max<int64_t>(L.bnd_rows_total, 1);
+    const int64_t nbnd = std::max<int32_t>(L.num_boundaries, 1);
+    HIP_OK(hipMalloc((void**)&b->d_bnd_cfv, brows * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_bnd_reach, brows * sizeof(float)));
+    HIP_OK(hipMalloc((void**)&b->d_bnd_mode, nbnd * sizeof(int32_t)));
+    HIP_OK(fill_zero(b->d_bnd_cfv, brows, b->stream));
+    HIP_OK(fill_zero(b->d_bnd_reach, brows, b->stream));
+    HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_bnd_mode), nbnd, b->stream));   // BND_UNSET
     if (b->zero_state()) return -1;
     HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
@@ -866,6 +899,63 @@ int psr_batch_raw_state(psr_batch* b, int game, uint16_t* regret, uint16_t* stra
 }
 
 float psr_batch_last_kernel_ms(psr_batch* b) { return b ? b->last_ms : -1.0f; }
+
+int psr_batch_num_boundaries(psr_batch* b, int game) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size()) return fail("game index out of range");
+    return b->L.games[game].num_boundaries;
+}
+
+int psr_batch_set_boundary_cfvs(psr_batch* b, int game, int ordinal, int player, const float* cfvs, int n, int raw) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size()) return fail("game index out of range");
+    const GameDesc& d = b->L.games[game];
+    if (ordinal < 0 || ordinal >= d.num_boundaries) return fail("boundary ordinal out of range");
+    if (player < 0 || player > 1) return fail("player must be 0 or 1");
+    if (!cfvs || n != d.H[player]) return fail("cfvs must hold one value per private hand of the player");
+    HIP_OK(hipSetDevice(b->device));
+    float* row = b->d_bnd_cfv + d.bnd_base + (int64_t)(2 * ordinal + player) * d.row_stride;
+    const int32_t mode = raw ? BND_RAW : BND_SCALED;
+    HIP_OK(hipMemcpyAsync(row, cfvs, (size_t)n * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_bnd_mode + d.bnd_index + ordinal, &mode, sizeof(int32_t), hipMemcpyHostToDevice,
+                          b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int psr_batch_update_leaf_cfvs(psr_batch* b, const float* p0, int64_t n0, const float* p1, int64_t n1) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    int64_t need[2] = {0, 0};
+    for (const GameDesc& d : b->L.games)
+        for (int p = 0; p < 2; ++p) need[p] += (int64_t)d.num_boundaries * d.H[p];
+    if (!p0 || !p1 || n0 != need[0] || n1 != need[1])
+        return fail("leaf cfvs must hold, per game in batch order, per boundary ordinal, one value per hand");
+    int64_t off[2] = {0, 0};
+    const float* src[2] = {p0, p1};
+    for (size_t gi = 0; gi < b->L.games.size(); ++gi) {
+        const GameDesc& d = b->L.games[gi];
+        for (int o = 0; o < d.num_boundaries; ++o)
+            for (int p = 0; p < 2; ++p) {
+                if (psr_batch_set_boundary_cfvs(b, (int)gi, o, p, src[p] + off[p], d.H[p], 1)) return -1;
+                off[p] += d.H[p];
+            }
+    }
+    return 0;
+}
+
+int psr_batch_boundary_reach(psr_batch* b, int game, int ordinal, int player, float* out, int n) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size()) return fail("game index out of range");
+    const GameDesc& d = b->L.games[game];
+    if (ordinal < 0 || ordinal >= d.num_boundaries) return fail("boundary ordinal out of range");
+    if (player < 0 || player > 1) return fail("player must be 0 or 1");
+    if (!out || n != d.H[player]) return fail("out must hold one value per private hand of the player");
+    HIP_OK(hipSetDevice(b->device));
+    const float* row = b->d_bnd_reach + d.bnd_base + (int64_t)(2 * ordinal + player) * d.row_stride;
+    HIP_OK(hipMemcpyAsync(out, row, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return 0;
+}
 
 int psr_batch_launch_info(psr_batch* b, int* num_cus, int* grid, int* blocks_per_cu) {
     if (!b) return fail("null batch");

@@ -121,6 +121,15 @@ def load_library(path: Optional[str] = None):
         "psr_batch_regrets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_last_kernel_ms": (C.c_float, [vp]),
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "psr_game_num_boundaries": (C.c_int, [vp]),
+        "psr_game_boundary_node": (C.c_int64, [vp, C.c_int]),
+        "psr_game_boundary_pot": (C.c_int, [vp, C.c_int]),
+        "psr_batch_num_boundaries": (C.c_int, [vp, C.c_int]),
+        "psr_batch_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int,
+                                                  C.c_int]),
+        "psr_batch_update_leaf_cfvs": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float),
+                                                 C.c_int64]),
+        "psr_batch_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "psr_batch_set_storage": (C.c_int, [vp, C.c_int]),
         "psr_batch_exploitability_f64": (C.c_int, [vp, C.POINTER(C.c_double)]),
@@ -361,6 +370,18 @@ class PostFlopGame:
     def num_storage(self) -> int:
         return _check(load_library().psr_game_num_storage(self._h))
 
+    def num_boundary_nodes(self) -> int:
+        """interpreter.rs:313-317: depth-boundary leaves of the tree."""
+        return _check(load_library().psr_game_num_boundaries(self._h))
+
+    def boundary_node_indices(self) -> list:
+        """interpreter.rs:320-330: arena index of each boundary ordinal."""
+        return [_check(load_library().psr_game_boundary_node(self._h, b)) for b in range(self.num_boundary_nodes())]
+
+    def boundary_pot(self, ordinal: int) -> int:
+        """interpreter.rs:354-358."""
+        return _check(load_library().psr_game_boundary_pot(self._h, ordinal))
+
     def validate_layout(self) -> dict:
         """Host-only check of the device layout (no GPU needed)."""
         out = (C.c_int64 * 6)()
@@ -509,6 +530,30 @@ class GpuBatchSolver:
                                                       s.ctypes.data_as(C.POINTER(C.c_uint16)), _fptr(rs), _fptr(ss),
                                                       n, cap))
         return r, s, rs[:k], ss[:k]
+
+    def num_boundaries(self, i: int) -> int:
+        return _check(load_library().psr_batch_num_boundaries(self._h, i))
+
+    def set_boundary_cfvs(self, i: int, ordinal: int, player: int, cfvs, raw: bool = False):
+        """interpreter.rs:343-352 set_boundary_cfvs on game i of the batch;
+        raw=True: chip cfvalues used as they are (evaluation.rs:606-630)."""
+        v = np.ascontiguousarray(cfvs, np.float32)
+        _check(load_library().psr_batch_set_boundary_cfvs(self._h, i, ordinal, player, _fptr(v), len(v),
+                                                          1 if raw else 0))
+
+    def update_leaf_cfvs(self, p0, p1):
+        """batch.rs:557-585: raw leaf values of every boundary of every game
+        (per game in batch order, per ordinal, H_player values)."""
+        a = np.ascontiguousarray(p0, np.float32)
+        b = np.ascontiguousarray(p1, np.float32)
+        _check(load_library().psr_batch_update_leaf_cfvs(self._h, _fptr(a), len(a), _fptr(b), len(b)))
+
+    def boundary_reach(self, i: int, ordinal: int, player: int) -> np.ndarray:
+        """interpreter.rs:396-403: reach of `player` recorded at the last
+        visit of the boundary by the opponent."""
+        out = np.zeros(self.games[i].num_private_hands(player), np.float32)
+        _check(load_library().psr_batch_boundary_reach(self._h, i, ordinal, player, _fptr(out), len(out)))
+        return out
 
     def last_kernel_ms(self) -> float:
         return float(load_library().psr_batch_last_kernel_ms(self._h))

@@ -160,6 +160,64 @@ def run_half_storage():
             raise AssertionError(f"{storage}: expl {float(e[0])} vs fp64 {eo}, root cfv error {err}")
 
 
+def run_mixed_batch():
+    """River games (whole-game kernel) and a turn game (phase launches) in one
+    batch: every game's exploitability and iteration count as its own oracle
+    solve (the turn loop must not overwrite the river games' results)."""
+    games = [std_game(), turn_game("AA", "KK,QQ"), std_game("33%, a")]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 10)
+    e, it = b.solve(10, 5)
+    for i, g in enumerate(games):
+        o = O.OracleGame(g.serialize())
+        eo, ito = o.solve(10)
+        if int(it[i]) != ito or float(e[i]) != np.float32(eo):
+            raise AssertionError(f"game {i}: ({float(e[i])}, {int(it[i])}) vs oracle ({eo}, {ito})")
+
+
+def boundary_game(depth=0, flop="Qs Jh 2c", turn="8d", oop="AA,KK,QQ,AKs", ip="QQ-JJ,AQs,AJs"):
+    """Turn-rooted tree cut at the river deal (depth_limit 0): every line that
+    reaches the river ends in a depth-boundary leaf."""
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(flop),
+                      turn=P.card_from_str(turn) if turn else P.NOT_DEALT)
+    s = [("50%, a", "2x"), ("50%, a", "2x")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn if turn else P.BoardState.Flop, starting_pot=100,
+                      effective_stack=200, flop_bet_sizes=s, turn_bet_sizes=s, river_bet_sizes=s, depth_limit=depth)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def set_boundaries(g, targets, seed=3):
+    """Seeded values on every boundary; odd ordinals raw, even scaled."""
+    rng = np.random.default_rng(seed)
+    for b in range(g.num_boundary_nodes()):
+        for q in range(2):
+            v = rng.uniform(-1.0, 1.0, g.num_private_hands(q)).astype(np.float32)
+            for t in targets:
+                t(b, q, v, b % 2 == 1)
+
+
+def run_boundary_stepwise():
+    g = boundary_game()
+    assert g.num_boundary_nodes() > 2
+    bt = P.GpuBatchSolver(0)
+    bt.prepare_batch([g], 6)
+    o = O.OracleGame(g.serialize())
+    set_boundaries(g, [lambda b, q, v, raw: bt.set_boundary_cfvs(0, b, q, v, raw),
+                       lambda b, q, v, raw: o.set_boundary_cfvs(b, q, v, raw)])
+    for t in range(4):
+        bt.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, st = o.state()
+        same(bt.regrets(0), r, f"iteration {t} regrets")
+        same(bt.strategy_sum(0), st, f"iteration {t} strategy sums")
+        for b in range(g.num_boundary_nodes()):
+            for q in range(2):
+                same(bt.boundary_reach(0, b, q), o.boundary_reach(b, q), f"iteration {t} boundary {b} reach {q}")
+    e = float(bt.exploitability()[0])
+    if abs(e - o.exploitability()) > 1e-6 * max(1.0, abs(o.exploitability())):
+        raise AssertionError(f"exploitability {e} vs oracle {o.exploitability()}")
+
+
 CHECKS = {name[4:]: fn for name, fn in globals().items() if name.startswith("run_")}
 
 if __name__ == "__main__":

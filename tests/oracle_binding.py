@@ -42,7 +42,9 @@ def load():
         "oracle_state": (None, [vp, dp, dp]),
         "oracle_set_state": (None, [vp, dp, dp]),
         "oracle_evaluate": (C.c_int, [vp, C.c_uint32, C.c_int, dp, dp]),
-        "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_int]),
+        "oracle_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "oracle_num_boundaries": (C.c_int, [vp]),
         "oracle_compressed_state": (C.c_int, [vp, C.POINTER(C.c_int16), C.POINTER(C.c_uint16),
                                               C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "oracle_num_decisions": (C.c_int, [vp]),
@@ -153,11 +155,24 @@ class OracleGame:
             raise ValueError("not a compressed handle")
         return r, s, rs, ss
 
-    def set_boundary_cfvs(self, ordinal, player, values):
+    def set_boundary_cfvs(self, ordinal, player, values, raw=False):
         v = np.ascontiguousarray(values, np.float32)
-        rc = load().oracle_set_boundary_cfvs(self._h, ordinal, player, v.ctypes.data_as(C.POINTER(C.c_float)), len(v))
+        rc = load().oracle_set_boundary_cfvs(self._h, ordinal, player, v.ctypes.data_as(C.POINTER(C.c_float)), len(v),
+                                             1 if raw else 0)
         if rc != 0:
             raise ValueError("bad boundary ordinal or length")
+
+    def num_boundaries(self):
+        return load().oracle_num_boundaries(self._h)
+
+    def boundary_reach(self, ordinal, player):
+        """Opponent reach recorded at the last visit of boundary `ordinal`,
+        indexed like the reference's boundary_reach(ordinal, player)."""
+        out = np.zeros(1326, np.float32)
+        n = load().oracle_boundary_reach(self._h, ordinal, player, out.ctypes.data_as(C.POINTER(C.c_float)), 1326)
+        if n < 0:
+            raise ValueError("bad boundary ordinal")
+        return out[:n].copy()
 
 
 def encode_signed(values):

@@ -24,12 +24,21 @@ def emu_lib():
     return build_emu.build()
 
 
-@pytest.mark.parametrize("check", ["std_stepwise", "std_raises_stepwise", "std_solve", "batch",
-                                   "compressed_stepwise", "fp64_exploitability", "half_storage",
-                                   "turn_stepwise"])
-def test_emulated_kernels_match_oracle(emu_lib, check):
+# (check, verify that every readfirstlane argument is wave-uniform): the
+# uniformity check costs two wave exchanges per call, so the long turn
+# solves run without it (the other checks cover the same code with it)
+CHECKS = [("std_stepwise", True), ("std_raises_stepwise", True), ("std_solve", True), ("batch", True),
+          ("compressed_stepwise", True), ("fp64_exploitability", True), ("half_storage", True),
+          ("turn_stepwise", True), ("mixed_batch", False), ("boundary_stepwise", True)]
+
+
+@pytest.mark.parametrize("check,uniform", CHECKS)
+def test_emulated_kernels_match_oracle(emu_lib, check, uniform):
+    env = dict(os.environ)
+    if not uniform:
+        env["EMU_NO_UNIFORM_CHECK"] = "1"
     r = subprocess.run([sys.executable, os.path.join(HERE, "kernel_emu", "run_checks.py"), check],
-                       capture_output=True, text=True, timeout=600)
+                       capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
 
 
