@@ -107,6 +107,21 @@ def turn_bench(P, W, iters, seed):
                          "frac": round(achieved / HBM_PEAK_GBS, 5)}}
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of `nbytes` (float32 tensors, after a
+    warm-up copy), the known-size kernel the FETCH_SIZE / WRITE_SIZE counters
+    of the DCFR kernels are scaled by."""
+    import torch
+    n = nbytes // 4
+    x = torch.rand(n, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}), flush=True)
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
@@ -128,7 +143,13 @@ def main():
     ap.add_argument("--single-sample", type=int, default=64,
                     help="subgames solved one per launch for the single-subgame yardstick (0: skip)")
     ap.add_argument("--turn-iters", type=int, default=200, help="timed turn DCFR iterations (0: skip)")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_bytes > 0:
+        copy_probe(args.copy_bytes)
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); see the sharding notes in DESIGN.md",

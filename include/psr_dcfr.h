@@ -190,6 +190,18 @@ int psr_batch_set_storage(psr_batch* batch, int mode);
  * (regrets_compressed / strategy_compressed / scales). */
 int psr_batch_raw_state(psr_batch* batch, int game, uint16_t* regret, uint16_t* strategy, float* rscale,
                         float* sscale, int64_t cap, int32_t cap_nodes);
+/* Query interface at the game root (the reference's state after
+ * back_to_root): psr_batch_normalized_weights -- query.rs:498-567
+ * cache_normalized_weights / normalized_weights (weights with dead hands
+ * zeroed, query.rs:886-920, times the unblocked opponent weight);
+ * psr_batch_expected_values_detail -- query.rs:672-760 (chip EV per hand; per
+ * action and hand for the player acting at the root; returns the count);
+ * psr_batch_expected_values -- query.rs:639-666 (weighted by the average
+ * strategy for the acting player).  Both run the cfvalue job (finalize,
+ * utility.rs:348-372) on the device first. */
+int psr_batch_normalized_weights(psr_batch* batch, int game, int player, float* out, int n);
+int psr_batch_expected_values_detail(psr_batch* batch, int game, int player, float* out, int64_t cap);
+int psr_batch_expected_values(psr_batch* batch, int game, int player, float* out, int n);
 /* Depth-boundary leaves of a prepared game (single continuation).
  * psr_batch_set_boundary_cfvs: interpreter.rs:343-352 set_boundary_cfvs --
  * H_player values for `ordinal`; raw == 0: pot-normalised values, the leaf

@@ -721,6 +721,101 @@ struct Solver {
         std::vector<T> reach = weights_T(p ^ 1);
         cfvalue_rec(out, 0, p, reach.data());
     }
+
+    // Query interface at the root after back_to_root (query.rs): weights with
+    // dead hands zeroed (assign_zero_weights, :886-920), normalized weights
+    // (cache_normalized_weights, :498-567), expected_values_detail (:672-760)
+    // and expected_values (:639-666).  f32 values as the reference.
+    void root_weights(std::vector<float> w[2], std::vector<float> nw[2]) const {
+        uint64_t board = 0;
+        if (g.turn != 0xFF) board |= 1ull << g.turn;
+        if (g.river != 0xFF) board |= 1ull << g.river;
+        auto mask = [&](int p, int i) { return (1ull << g.cards[p][i].first) | (1ull << g.cards[p][i].second); };
+        for (int p = 0; p < 2; ++p) {
+            w[p] = g.weights[p];
+            uint64_t dead = (1ull << 52) - 1;
+            for (int i = 0; i < g.H(p ^ 1); ++i) {
+                if ((mask(p ^ 1, i) & board) == 0) dead &= mask(p ^ 1, i);
+                if (dead == 0) break;
+            }
+            dead |= board;
+            for (int i = 0; i < g.H(p); ++i)
+                if (mask(p, i) & dead) w[p][i] = 0.0f;
+        }
+        double sum[2] = {0, 0}, minus[2][52] = {};
+        for (int p = 0; p < 2; ++p)
+            for (int i = 0; i < g.H(p); ++i)
+                if ((mask(p, i) & board) == 0) {
+                    double x = w[p][i];
+                    sum[p] += x;
+                    minus[p][g.cards[p][i].first] += x;
+                    minus[p][g.cards[p][i].second] += x;
+                }
+        for (int p = 0; p < 2; ++p) {
+            int o = p ^ 1;
+            nw[p].assign(g.H(p), 0.0f);
+            for (int i = 0; i < g.H(p); ++i) {
+                if (mask(p, i) & board) continue;
+                uint16_t s = g.same[p][i];
+                double same = s == NONE16 ? 0.0 : (double)w[o][s];
+                double opp = sum[o] + same - minus[o][g.cards[p][i].first] - minus[o][g.cards[p][i].second];
+                nw[p][i] = w[p][i] * (float)opp;
+            }
+        }
+    }
+
+    // returns the number of values (A*H for the player acting at the root)
+    int expected_values_detail(int p, std::vector<float>& out) const {
+        const Node& root = g.nodes[0];
+        if (root.player & (TERMINAL | CHANCE)) return -1;
+        const int r = root.player & MASK, A = (int)root.num_children, H = g.H(p);
+        const bool have = p == r;
+        std::vector<T> v((size_t)(have ? A : 1) * H, T(0));
+        if (have) {
+            // the per-action values finalize saves at the root (utility.rs:712-722):
+            // each child's cfvalues under the average strategy, same opponent reach
+            std::vector<T> reach = weights_T(p ^ 1);
+            for (int a = 0; a < A; ++a) cfvalue_rec(&v[(size_t)a * H], root.first_child + a, p, reach.data());
+        } else if (r == 0) {
+            root_cfvalues(p, v.data());   // cfvalues_ip at an OOP root (utility.rs:782-794)
+        }
+        std::vector<float> w[2], nw[2];
+        root_weights(w, nw);
+        const float normalizer = (float)g.num_combinations;
+        const float add = (float)g.starting_pot * 0.5f + (float)(root.amount + 0);
+        out.assign(v.size(), 0.0f);
+        for (int a = 0; a < (have ? A : 1); ++a) {
+            const bool fold = have && (g.nodes[root.first_child + a].player & FOLD) == FOLD;
+            for (int i = 0; i < H; ++i) {
+                float x = (float)v[(size_t)a * H + i];
+                if (fold || nw[p][i] == 0.0f) x = 0.0f;
+                else {
+                    x *= normalizer * (w[p][i] / nw[p][i]);
+                    x += add;
+                }
+                out[(size_t)a * H + i] = x;
+            }
+        }
+        return (int)out.size();
+    }
+
+    void expected_values(int p, float* out) const {
+        std::vector<float> d;
+        expected_values_detail(p, d);
+        const Node& root = g.nodes[0];
+        const int r = root.player & MASK, A = (int)root.num_children, H = g.H(p);
+        if (p != r) {
+            std::copy(d.begin(), d.begin() + H, out);
+            return;
+        }
+        std::vector<T> sig;
+        node_average(root, A, H, sig);
+        for (int i = 0; i < H; ++i) {
+            float ev = 0.0f;
+            for (int a = 0; a < A; ++a) ev += d[(size_t)a * H + i] * (float)sig[(size_t)a * H + i];
+            out[i] = ev;
+        }
+    }
 };
 
 struct Handle {
@@ -899,6 +994,35 @@ int oracle_boundary_reach(void* hv, int ord, int player, float* out, int cap) {
     if ((int)v.size() > cap) return -1;
     std::copy(v.begin(), v.end(), out);
     return (int)v.size();
+}
+
+// query.rs:639-760 at the root: detail != 0 gives expected_values_detail
+// (A*H values for the player acting at the root); returns the count
+int oracle_expected_values(void* h, int p, int detail, float* out, int cap) {
+    int rc = -1;
+    DISPATCH(h, {
+        std::vector<float> d;
+        int n = S.expected_values_detail(p, d);
+        if (n < 0) break;
+        if (detail) {
+            if (n > cap) break;
+            std::copy(d.begin(), d.end(), out);
+            rc = n;
+        } else {
+            if (S.g.H(p) > cap) break;
+            S.expected_values(p, out);
+            rc = S.g.H(p);
+        }
+    });
+    return rc;
+}
+
+void oracle_normalized_weights(void* h, int p, float* out) {
+    DISPATCH(h, {
+        std::vector<float> w[2], nw[2];
+        S.root_weights(w, nw);
+        std::copy(nw[p].begin(), nw[p].end(), out);
+    });
 }
 
 int oracle_num_boundaries(void* hv) { return (int)static_cast<Handle*>(hv)->game.boundary_raw.size(); }

@@ -661,6 +661,30 @@ __device__ __forceinline__ double exploitability64(Ctx& c, const TileDesc& T) {
 // per-lane copy of the 352-byte block, which the compiler otherwise makes as
 // soon as the parameter's address is taken (Ctx keeps a reference to it) and
 // which turns every bound read from it into a per-lane value.
+// Root cfvalues of player p after a cfvalue traversal (utility.rs:348-372,
+// 552-795): the root row, and when p acts at the root the rows of its
+// children -- the per-action values the reference saves at the root
+// (node.cfvalues(), read by expected_values_detail, query.rs:672-760).
+__device__ __forceinline__ void store_root_cfvalues(Ctx& c, const TileDesc& T, int p) {
+    const KernelArgs& A = c.A;
+    const int root_id = uni_load(A.level_ranges + T.level_base).start;
+    const NodeRec root = uni_load(c.nodes + root_id);
+    const int Hp = num_hands(c.G, p);
+    float* dst = A.root_cfv + hand_base(c.G, p);
+    const float* row = c.cfv + (size_t)root_id * c.stride;
+    for (int h = threadIdx.x; h < Hp; h += BLOCK) dst[h] = row[h];
+    if (root.kind == KIND_DECISION && root.player == p) {
+        const int64_t off = A.root_act_off[T.game];
+        if (off >= 0) {
+            const float* kids = c.cfv + (size_t)root.first_child * c.stride;
+            for (int a = 0; a < root.num_children; ++a)
+                for (int h = threadIdx.x; h < Hp; h += BLOCK)
+                    A.root_action_cfv[off + (int64_t)a * Hp + h] = kids[(size_t)a * c.stride + h];
+        }
+    }
+    __syncthreads();
+}
+
 #define KERNEL_SETUP()                                                                          \
     const KernelArgs& A = *(const KernelArgs*)__builtin_amdgcn_kernarg_segment_ptr();           \
     extern __shared__ double lds[];                                                             \
@@ -702,10 +726,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
 #pragma unroll 1
             for (int p = 0; p < 2; ++p) {
                 tile_traverse<MODE_CFV, SM>(c, T, p, none);
-                float* dst = A.root_cfv + hand_base(c.G, p);
-                const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
-                for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) dst[h] = root[h];
-                __syncthreads();
+                store_root_cfvalues(c, T, p);
             }
         }
         if (A.flags & JOB_EXPL64) {
@@ -746,9 +767,7 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
                 } else if (MODE == MODE_CFV) {
-                    float* dst = A.root_cfv + hand_base(c.G, p);
-                    const float* root = c.cfv + (size_t)A.level_ranges[T.level_base].start * c.stride;
-                    for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) dst[h] = root[h];
+                    store_root_cfvalues(c, T, p);
                 }
             }
             __syncthreads();

@@ -83,6 +83,10 @@ struct KernelArgs {
     const float* bnd_cfv;     // given values, row (2b + traversing player)
     float* bnd_reach;         // opponent reach at the last visit, row (2b + opponent)
     const int32_t* bnd_mode;  // BoundaryMode per boundary
+    // per-action cfvalues of the root player at the game root (cfvalue job):
+    // A_root x H rows at root_act_off[game] (-1: root is not a decision node)
+    float* root_action_cfv;
+    const int64_t* root_act_off;
 };
 
 size_t dcfr_lds_bytes(int max_hands);

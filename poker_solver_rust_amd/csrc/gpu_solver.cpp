@@ -57,6 +57,17 @@ struct HostLayout {
     std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
     int32_t num_boundaries = 0;      // depth-boundary leaves over the batch
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
+    std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
+    int64_t root_act_total = 0;
+    // per game: what the expected-value queries need about the root
+    struct Root {
+        bool decision = false;
+        int player = 0, num_actions = 0, amount = 0, starting_pot = 0;
+        std::vector<char> fold;   // per action: the child is a fold
+        uint64_t board_mask = 0;  // turn / river of the card config (query.rs:886-893)
+        double num_combinations = 0.0;
+    };
+    std::vector<Root> roots;
 };
 
 int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, size_t align = 4) {
@@ -387,6 +398,30 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         if (d.num_tiles > 1) L.multi_games.push_back(gi);
         else L.single_games.push_back(gi);
 
+        {
+            HostLayout::Root ri;
+            const GameNode& rn = nodes[0];
+            ri.decision = !rn.is_terminal() && !rn.is_chance();
+            ri.player = rn.player & PLAYER_MASK;
+            ri.num_actions = rn.num_children;
+            ri.amount = rn.amount;
+            ri.starting_pot = tc.starting_pot;
+            ri.num_combinations = g.num_combinations();
+            const CardConfig& ccfg = g.card_config();
+            if (ccfg.turn != NOT_DEALT) ri.board_mask |= 1ull << ccfg.turn;
+            if (ccfg.river != NOT_DEALT) ri.board_mask |= 1ull << ccfg.river;
+            for (int a = 0; a < rn.num_children; ++a) ri.fold.push_back(nodes[rn.first_child + a].is_fold() ? 1 : 0);
+            L.roots.push_back(std::move(ri));
+        }
+        {
+            const GameNode& root = nodes[0];
+            if (!root.is_terminal() && !root.is_chance()) {
+                L.root_act_off.push_back(L.root_act_total);
+                L.root_act_total += (int64_t)root.num_children * d.H[root.player];
+            } else {
+                L.root_act_off.push_back(-1);
+            }
+        }
         L.storage_size.push_back(off);
         L.dec_blocks.push_back(std::move(blocks));
         L.state_total += (off + 63) / 64 * 64;
@@ -448,12 +483,14 @@ struct psr_batch {
     double* d_ev = nullptr;
     float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
     int32_t* d_bnd_mode = nullptr;
+    float* d_root_action_cfv = nullptr;
+    int64_t* d_root_act_off = nullptr;
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
-                        d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode};
+                        d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
@@ -464,6 +501,7 @@ struct psr_batch {
         d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         d_bnd_cfv = nullptr; d_bnd_reach = nullptr; d_bnd_mode = nullptr;
+        d_root_action_cfv = nullptr; d_root_act_off = nullptr;
         prepared = false;
     }
 
@@ -513,6 +551,8 @@ struct psr_batch {
         a.bnd_cfv = d_bnd_cfv;
         a.bnd_reach = d_bnd_reach;
         a.bnd_mode = d_bnd_mode;
+        a.root_action_cfv = d_root_action_cfv;
+        a.root_act_off = d_root_act_off;
         return a;
     }
 
@@ -747,6 +787,9 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(fill_zero(b->d_bnd_cfv, brows, b->stream));
     HIP_OK(fill_zero(b->d_bnd_reach, brows, b->stream));
     HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_bnd_mode), nbnd, b->stream));   // BND_UNSET
+    if (upload(&b->d_root_act_off, L.root_act_off)) return -1;
+    HIP_OK(hipMalloc((void**)&b->d_root_action_cfv, std::max<int64_t>(L.root_act_total, 1) * sizeof(float)));
+    HIP_OK(fill_zero(b->d_root_action_cfv, std::max<int64_t>(L.root_act_total, 1), b->stream));
     if (b->zero_state()) return -1;
     HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
@@ -813,6 +856,52 @@ int psr_batch_root_cfvalues(psr_batch* b, int game, int player, float* out) {
     const GameDesc& d = b->L.games[game];
     HIP_OK(hipMemcpy(out, b->d_root_cfv + d.hand_base[player], d.H[player] * sizeof(float), hipMemcpyDeviceToHost));
     return 0;
+}
+
+// Root weights of one game as the reference query interface sees them after
+// back_to_root: initial weights with the dead hands zeroed
+// (assign_zero_weights, query.rs:886-920), and the normalized weights
+// (cache_normalized_weights, query.rs:498-567).
+static void root_weights(const HostLayout& L, int gi, std::vector<float> w[2], std::vector<float> nw[2]) {
+    const GameDesc& d = L.games[gi];
+    const uint64_t board = L.roots[gi].board_mask;
+    auto mask_of = [&](int p, int i) {
+        const uint16_t c = L.cards[d.hand_base[p] + i];
+        return (1ull << (c & 0xFF)) | (1ull << (c >> 8));
+    };
+    for (int p = 0; p < 2; ++p) {
+        w[p].assign(L.weights.begin() + d.hand_base[p], L.weights.begin() + d.hand_base[p] + d.H[p]);
+        uint64_t dead = (1ull << 52) - 1;
+        for (int i = 0; i < d.H[p ^ 1] && dead; ++i) {
+            const uint64_t m = mask_of(p ^ 1, i);
+            if ((m & board) == 0) dead &= m;
+        }
+        dead |= board;
+        for (int i = 0; i < d.H[p]; ++i)
+            if (mask_of(p, i) & dead) w[p][i] = 0.0f;
+    }
+    double sum[2] = {0.0, 0.0}, minus[2][52] = {};
+    for (int p = 0; p < 2; ++p)
+        for (int i = 0; i < d.H[p]; ++i) {
+            if (mask_of(p, i) & board) continue;
+            const uint16_t c = L.cards[d.hand_base[p] + i];
+            const double x = (double)w[p][i];
+            sum[p] += x;
+            minus[p][c & 0xFF] += x;
+            minus[p][c >> 8] += x;
+        }
+    for (int p = 0; p < 2; ++p) {
+        nw[p].assign(d.H[p], 0.0f);
+        const int o = p ^ 1;
+        for (int i = 0; i < d.H[p]; ++i) {
+            if (mask_of(p, i) & board) continue;
+            const uint16_t c = L.cards[d.hand_base[p] + i];
+            const uint16_t si = L.same[d.hand_base[p] + i];
+            const double same = si == 0xFFFF ? 0.0 : (double)w[o][si];
+            const double opp = sum[o] + same - minus[o][c & 0xFF] - minus[o][c >> 8];
+            nw[p][i] = w[p][i] * (float)opp;
+        }
+    }
 }
 
 static float half_bits_to_float(uint16_t h) {
@@ -899,6 +988,94 @@ int psr_batch_raw_state(psr_batch* b, int game, uint16_t* regret, uint16_t* stra
 }
 
 float psr_batch_last_kernel_ms(psr_batch* b) { return b ? b->last_ms : -1.0f; }
+
+int psr_batch_normalized_weights(psr_batch* b, int game, int player, float* out, int n) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || player < 0 || player > 1) return fail("invalid argument");
+    if (!out || n != b->L.games[game].H[player]) return fail("out must hold one value per private hand");
+    std::vector<float> w[2], nw[2];
+    root_weights(b->L, game, w, nw);
+    std::copy(nw[player].begin(), nw[player].end(), out);
+    return 0;
+}
+
+// expected_values_detail at the game root (query.rs:672-760): chip EV per hand
+// (per action and hand for the player acting at the root).  Runs the
+// cfvalue job (finalize, utility.rs:348-372) on the device first.  Returns
+// the number of values written.
+int psr_batch_expected_values_detail(psr_batch* b, int game, int player, float* out, int64_t cap) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || player < 0 || player > 1) return fail("invalid argument");
+    const HostLayout& L = b->L;
+    const GameDesc& d = L.games[game];
+    const HostLayout::Root& ri = L.roots[game];
+    if (!ri.decision) return fail("expected values need a decision node at the root");
+    if (b->run(JOB_ROOT_CFV, 0, 0, 0, b->max_iterations)) return -1;
+    const int H = d.H[player];
+    const bool have_actions = player == ri.player;
+    const int rows = have_actions ? ri.num_actions : 1;
+    if (!out || cap < (int64_t)rows * H) return fail("out too small");
+    std::vector<float> v((size_t)rows * H, 0.0f);
+    if (have_actions) {
+        HIP_OK(hipMemcpy(v.data(), b->d_root_action_cfv + L.root_act_off[game], v.size() * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    } else if (ri.player == 0) {
+        // IP values saved at an OOP root (has_cfvalues_ip, utility.rs:782-794)
+        HIP_OK(hipMemcpy(v.data(), b->d_root_cfv + d.hand_base[player], H * sizeof(float), hipMemcpyDeviceToHost));
+    }   // else: the reference reads its zero-initialised cfvalues cache
+    std::vector<float> w[2], nw[2];
+    root_weights(L, game, w, nw);
+    const float normalizer = (float)ri.num_combinations;   // chance factor 1 at the root
+    const float bias_add = (float)ri.starting_pot * 0.5f + (float)(ri.amount + 0);
+    for (int a = 0; a < rows; ++a) {
+        const bool is_fold = have_actions && ri.fold[a];
+        for (int i = 0; i < H; ++i) {
+            float& x = v[(size_t)a * H + i];
+            if (is_fold || nw[player][i] == 0.0f) {
+                x = 0.0f;
+            } else {
+                volatile float f = normalizer * (w[player][i] / nw[player][i]);
+                volatile float y = x * f;
+                x = y + bias_add;
+            }
+        }
+    }
+    std::copy(v.begin(), v.end(), out);
+    return (int)v.size();
+}
+
+// expected_values at the game root (query.rs:639-666): for the player acting
+// at the root the per-action values weighted by the average strategy.
+int psr_batch_expected_values(psr_batch* b, int game, int player, float* out, int n) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || player < 0 || player > 1) return fail("invalid argument");
+    const HostLayout::Root& ri = b->L.roots[game];
+    const int H = b->L.games[game].H[player];
+    if (!out || n != H) return fail("out must hold one value per private hand");
+    std::vector<float> detail((size_t)std::max(ri.num_actions, 1) * H);
+    int k = psr_batch_expected_values_detail(b, game, player, detail.data(), (int64_t)detail.size());
+    if (k < 0) return -1;
+    if (player != ri.player) {
+        std::copy(detail.begin(), detail.begin() + H, out);
+        return 0;
+    }
+    // normalized average strategy at the root (utility.rs:229-245)
+    const int A = ri.num_actions;
+    std::vector<float> S((size_t)b->L.storage_size[game]);
+    if (psr_batch_strategy_sum(b, game, S.data(), (int64_t)S.size())) return -1;
+    for (int i = 0; i < H; ++i) {
+        float denom = S[i];
+        for (int a = 1; a < A; ++a) denom += S[(size_t)a * H + i];
+        float ev = 0.0f;
+        for (int a = 0; a < A; ++a) {
+            const float sig = denom == 0.0f ? 1.0f / (float)A : S[(size_t)a * H + i] / denom;
+            volatile float t = detail[(size_t)a * H + i] * sig;
+            ev = ev + t;
+        }
+        out[i] = ev;
+    }
+    return 0;
+}
 
 int psr_batch_num_boundaries(psr_batch* b, int game) {
     if (!b || !b->prepared) return fail("batch not prepared");

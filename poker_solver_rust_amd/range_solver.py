@@ -130,6 +130,9 @@ def load_library(path: Optional[str] = None):
         "psr_batch_update_leaf_cfvs": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64, C.POINTER(C.c_float),
                                                  C.c_int64]),
         "psr_batch_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "psr_batch_normalized_weights": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "psr_batch_expected_values_detail": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),
+        "psr_batch_expected_values": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "psr_batch_set_storage": (C.c_int, [vp, C.c_int]),
         "psr_batch_exploitability_f64": (C.c_int, [vp, C.POINTER(C.c_double)]),
@@ -506,13 +509,13 @@ class GpuBatchSolver:
         return out
 
     def strategy_sum(self, i: int) -> np.ndarray:
-        n = self.games[i].num_storage()
+        n = self._game(i).num_storage()
         v = np.zeros(n, np.float32)
         _check(load_library().psr_batch_strategy_sum(self._h, i, _fptr(v), n))
         return v
 
     def regrets(self, i: int) -> np.ndarray:
-        n = self.games[i].num_storage()
+        n = self._game(i).num_storage()
         v = np.zeros(n, np.float32)
         _check(load_library().psr_batch_regrets(self._h, i, _fptr(v), n))
         return v
@@ -520,7 +523,7 @@ class GpuBatchSolver:
     def raw_state(self, i: int):
         """16-bit words and per-node scales: (regret u16, strategy u16,
         regret scales, strategy scales)."""
-        n = self.games[i].num_storage()
+        n = self._game(i).num_storage()
         r = np.zeros(n, np.uint16)
         s = np.zeros(n, np.uint16)
         cap = 1 << 20
@@ -530,6 +533,30 @@ class GpuBatchSolver:
                                                       s.ctypes.data_as(C.POINTER(C.c_uint16)), _fptr(rs), _fptr(ss),
                                                       n, cap))
         return r, s, rs[:k], ss[:k]
+
+    def normalized_weights(self, i: int, player: int) -> np.ndarray:
+        """query.rs:498-567 at the root of game i."""
+        out = np.zeros(self._game(i).num_private_hands(player), np.float32)
+        _check(load_library().psr_batch_normalized_weights(self._h, i, player, _fptr(out), len(out)))
+        return out
+
+    def expected_values(self, i: int, player: int) -> np.ndarray:
+        """query.rs:639-666 at the root of game i (chips per hand)."""
+        out = np.zeros(self._game(i).num_private_hands(player), np.float32)
+        _check(load_library().psr_batch_expected_values(self._h, i, player, _fptr(out), len(out)))
+        return out
+
+    def expected_values_detail(self, i: int, player: int) -> np.ndarray:
+        """query.rs:672-760 at the root of game i (per action and hand for the
+        player acting at the root)."""
+        out = np.zeros(16 * self._game(i).num_private_hands(player), np.float32)
+        n = _check(load_library().psr_batch_expected_values_detail(self._h, i, player, _fptr(out), len(out)))
+        return out[:n].copy()
+
+    def _game(self, i: int):
+        if not 0 <= i < len(self.games):
+            raise PsrError(f"game index {i} out of range")
+        return self.games[i]
 
     def num_boundaries(self, i: int) -> int:
         return _check(load_library().psr_batch_num_boundaries(self._h, i))
@@ -551,7 +578,7 @@ class GpuBatchSolver:
     def boundary_reach(self, i: int, ordinal: int, player: int) -> np.ndarray:
         """interpreter.rs:396-403: reach of `player` recorded at the last
         visit of the boundary by the opponent."""
-        out = np.zeros(self.games[i].num_private_hands(player), np.float32)
+        out = np.zeros(self._game(i).num_private_hands(player), np.float32)
         _check(load_library().psr_batch_boundary_reach(self._h, i, ordinal, player, _fptr(out), len(out)))
         return out
 

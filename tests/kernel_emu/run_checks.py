@@ -218,6 +218,22 @@ def run_boundary_stepwise():
         raise AssertionError(f"exploitability {e} vs oracle {o.exploitability()}")
 
 
+def run_expected_values():
+    """query.rs:498-760 at the root: normalized weights, per-action detail and
+    expected values of both players, bit for bit with the oracle."""
+    for g in (std_game("33%, 75%, a", "2x, a"), turn_game("AA,AKs", "KK,QQ")):
+        b = P.GpuBatchSolver(0)
+        b.prepare_batch([g], 8)
+        b.run_iterations(0, 8)
+        o = O.OracleGame(g.serialize())
+        for t in range(8):
+            o.solve_step(t)
+        for p in range(2):
+            same(b.normalized_weights(0, p), o.normalized_weights(p), f"normalized weights {p}")
+            same(b.expected_values_detail(0, p), o.expected_values(p, detail=True), f"detail {p}")
+            same(b.expected_values(0, p), o.expected_values(p), f"expected values {p}")
+
+
 CHECKS = {name[4:]: fn for name, fn in globals().items() if name.startswith("run_")}
 
 if __name__ == "__main__":

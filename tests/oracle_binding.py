@@ -45,6 +45,8 @@ def load():
         "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_int]),
         "oracle_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "oracle_num_boundaries": (C.c_int, [vp]),
+        "oracle_expected_values": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "oracle_normalized_weights": (None, [vp, C.c_int, C.POINTER(C.c_float)]),
         "oracle_compressed_state": (C.c_int, [vp, C.POINTER(C.c_int16), C.POINTER(C.c_uint16),
                                               C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "oracle_num_decisions": (C.c_int, [vp]),
@@ -161,6 +163,21 @@ class OracleGame:
                                              1 if raw else 0)
         if rc != 0:
             raise ValueError("bad boundary ordinal or length")
+
+    def expected_values(self, player, detail=False):
+        """query.rs:639-760 at the root (chips per hand; per action and hand
+        with detail=True for the player acting at the root)."""
+        out = np.zeros(16 * 1326, np.float32)
+        n = load().oracle_expected_values(self._h, player, 1 if detail else 0, out.ctypes.data_as(C.POINTER(C.c_float)),
+                                          len(out))
+        if n < 0:
+            raise ValueError("no decision node at the root")
+        return out[:n].copy()
+
+    def normalized_weights(self, player):
+        out = np.zeros(self.num_hands(player), np.float32)
+        load().oracle_normalized_weights(self._h, player, out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
 
     def num_boundaries(self):
         return load().oracle_num_boundaries(self._h)

@@ -297,3 +297,21 @@ def test_mixed_river_and_turn_batch():
         assert int(it[i]) == ito
         assert abs(float(e[i]) - eo) <= RTOL * max(1.0, eo)
         close(b.strategy_sum(i), o.state()[1])
+
+
+@pytest.mark.parametrize("make,iters", [(lambda: std_game("33%, 75%, a", "2x, a"), 40),
+                                        (lambda: W.river_uniform(2).build(), 60), (turn_game, 10)])
+def test_expected_values_match_oracle(make, iters):
+    """query.rs:498-760 at the root: normalized weights, per-action detail of
+    the acting player and expected values of both players."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters)
+    b.run_iterations(0, iters)
+    o = O.OracleGame(g.serialize())
+    for t in range(iters):
+        o.solve_step(t)
+    for p in range(2):
+        assert np.array_equal(b.normalized_weights(0, p), o.normalized_weights(p))
+        close(b.expected_values_detail(0, p), o.expected_values(p, detail=True))
+        close(b.expected_values(0, p), o.expected_values(p))

@@ -29,7 +29,8 @@ def emu_lib():
 # solves run without it (the other checks cover the same code with it)
 CHECKS = [("std_stepwise", True), ("std_raises_stepwise", True), ("std_solve", True), ("batch", True),
           ("compressed_stepwise", True), ("fp64_exploitability", True), ("half_storage", True),
-          ("turn_stepwise", True), ("mixed_batch", False), ("boundary_stepwise", True)]
+          ("turn_stepwise", True), ("mixed_batch", False), ("boundary_stepwise", True),
+          ("expected_values", False)]
 
 
 @pytest.mark.parametrize("check,uniform", CHECKS)
