@@ -232,6 +232,33 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]);
 /* Device properties: number of CUs and the workgroups launched. */
 int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
 
+/* ------------------------------------------------------------ value net */
+
+/* A small MLP standing in for river chance nodes in depth-limited turn
+ * solves (the reference's value net: cfvnet/src/model/network.rs:10-90,
+ * python/cfvnet/model.py:1-50, masked Huber loss loss.py:6-90; the layer
+ * sizes here are the caller's, e.g. 2706 -> 4 x 512 GELU -> 2652).  BF16
+ * weights with FP32 master copies and FP32 accumulation on MFMA; AdamW.
+ * Parameters are flattened as W_1 (in x out, row-major), b_1, W_2, b_2, ...
+ * Host buffers throughout; the training set stays resident on the device. */
+typedef struct psr_mlp psr_mlp;
+int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint64_t seed, psr_mlp** out);
+void psr_mlp_destroy(psr_mlp* net);
+int64_t psr_mlp_num_params(psr_mlp* net);
+int psr_mlp_get_params(psr_mlp* net, float* out, int64_t n);
+int psr_mlp_set_params(psr_mlp* net, const float* in, int64_t n);
+/* Inference: n rows of dims[0] -> n rows of dims[last]. */
+int psr_mlp_forward(psr_mlp* net, const float* x, int n, float* y);
+/* One forward + backward on n <= max_batch rows: mean masked Huber loss over
+ * all outputs and every parameter's gradient; no update. */
+int psr_mlp_gradients(psr_mlp* net, const float* x, const float* y, const float* mask, int n, float delta,
+                      double* loss, float* grads, int64_t n_grads);
+int psr_mlp_set_dataset(psr_mlp* net, const float* x, const float* y, const float* mask, int n);
+/* Minibatch AdamW (beta 0.9 / 0.999, eps 1e-8, decoupled weight decay). */
+int psr_mlp_train(psr_mlp* net, int epochs, int batch, float lr, float weight_decay, float delta, uint64_t seed,
+                  double* epoch_loss);
+float psr_mlp_last_ms(psr_mlp* net);
+
 #ifdef __cplusplus
 }
 #endif

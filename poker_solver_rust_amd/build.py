@@ -24,8 +24,8 @@ DEVICE_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
-HOST_SOURCES = ["game.cpp", "capi_game.cpp", "gpu_solver.cpp"]
-DEVICE_SOURCES = ["dcfr_kernels.hip"]
+HOST_SOURCES = ["game.cpp", "capi_game.cpp", "gpu_solver.cpp", "mlp.cpp"]
+DEVICE_SOURCES = ["dcfr_kernels.hip", "mlp_kernels.hip"]
 
 
 def run(cmd):

@@ -33,6 +33,14 @@ constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int NWAVES = BLOCK / WAVE;
 constexpr int MAXA = 16;   // largest action count of a decision node on the device
+// Hands per thread: a row of H <= 1326 hands is covered by KH strided slots
+// h = threadIdx.x + k * BLOCK.  The per-hand loops are unrolled over k and
+// split into a load phase and a compute/store phase, so every thread has up
+// to KH independent loads in flight per row instead of one.
+constexpr int KH = 6;
+static_assert(KH * BLOCK >= 1326, "KH slots must cover the largest range");
+#define FOR_K _Pragma("unroll") for (int k = 0; k < KH; ++k)
+__device__ __forceinline__ int hslot(int k) { return (int)threadIdx.x + k * BLOCK; }
 
 __device__ __forceinline__ float fmax0(float x) { return x > 0.0f ? x : 0.0f; }
 __device__ __forceinline__ float fmaxr(float x, float y) { return x > y ? x : y; }
@@ -233,8 +241,10 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     V* reach = reach_rows<V>(c);
     if (T.flags & TILE_GAME_ROOT) {
         const float* w = A.hand_weight + hand_base(c.G, o);
-        V* r0 = reach + (size_t)lv[0].start * c.stride;
-        for (int h = threadIdx.x; h < Ho; h += BLOCK) r0[h] = (V)w[h];
+        V* r0 = reach + (size_t)uni_load(lv).start * c.stride;
+        float wv[KH];
+        FOR_K wv[k] = hslot(k) < Ho ? w[hslot(k)] : 0.0f;
+        FOR_K if (hslot(k) < Ho) r0[hslot(k)] = (V)wv[k];
         __syncthreads();
     }
     for (int L = 0; L < T.num_levels; ++L) {
@@ -247,45 +257,59 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
             const int na = nd.num_children;
             const V* rn = reach + (size_t)n * c.stride;
             V* rc = reach + (size_t)nd.first_child * c.stride;
+            V r[KH];
+            FOR_K r[k] = hslot(k) < Ho ? rn[hslot(k)] : V(0);
             if (nd.kind == KIND_DECISION && nd.player == o && na > 1) {
                 const NodeState<SM> st = node_state<SM>(A, c.G, nd);
                 if constexpr (sizeof(V) == 8) {
-                    for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                        double denom = 0.0;
-                        for (int a = 0; a < na; ++a) denom += (double)st.str((size_t)a * Ho + h);
-                        const double r = rn[h];
-                        for (int a = 0; a < na; ++a) {
-                            const double sig =
-                                denom == 0.0 ? 1.0 / (double)na : (double)st.str((size_t)a * Ho + h) / denom;
-                            rc[(size_t)a * c.stride + h] = sig * r;
+                    // FP64 best response: average strategy in f64
+                    double den[KH];
+                    for (int a = 0; a < na; ++a) {
+                        double v[KH];
+                        FOR_K v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
+                        FOR_K den[k] = a == 0 ? v[k] : den[k] + v[k];
+                    }
+                    for (int a = 0; a < na; ++a) {
+                        double v[KH];
+                        FOR_K v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
+                        FOR_K if (hslot(k) < Ho) {
+                            const double sig = den[k] == 0.0 ? 1.0 / (double)na : v[k] / den[k];
+                            rc[(size_t)a * c.stride + hslot(k)] = sig * r[k];
                         }
                     }
                 } else {
+                    // regret matching / average strategy, solver.rs:600-626,
+                    // utility.rs:229-245: denominator over actions, then sigma
                     const float def = __fdiv_rn(1.0f, (float)na);
-                    for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                        const float denom = strat_denom<MODE>(st, Ho, h, na);
-                        const float r = rn[h];
-                        for (int a = 0; a < na; ++a)
-                            rc[(size_t)a * c.stride + h] =
-                                __fmul_rn(strat_of<MODE>(node_value<MODE>(st, (size_t)a * Ho + h), denom, def), r);
+                    float den[KH];
+                    for (int a = 0; a < na; ++a) {
+                        float v[KH];
+                        FOR_K v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
+                        FOR_K {
+                            const float x = MODE == MODE_DCFR ? fmax0(v[k]) : v[k];
+                            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+                        }
+                    }
+                    for (int a = 0; a < na; ++a) {
+                        float v[KH];
+                        FOR_K v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
+                        FOR_K if (hslot(k) < Ho)
+                            rc[(size_t)a * c.stride + hslot(k)] = __fmul_rn(strat_of<MODE>(v[k], den[k], def), r[k]);
                     }
                 }
             } else {
-                const bool chance = nd.kind == KIND_CHANCE;
-                V sc = 1;
-                if (chance) {
+                // chance node: reach / chance factor; otherwise the reach passes through
+                if (nd.kind == KIND_CHANCE) {
                     const ChanceTab ct = uni_load(A.chance + nd.aux);
-                    if constexpr (sizeof(V) == 8) sc = 1.0 / (double)ct.factor;
-                    else sc = ct.scale;
-                }
-                for (int h = threadIdx.x; h < Ho; h += BLOCK) {
-                    V r = rn[h];
-                    if (chance) {
-                        if constexpr (sizeof(V) == 8) r = r * sc;
-                        else r = __fmul_rn(r, sc);
+                    if constexpr (sizeof(V) == 8) {
+                        const double sc = 1.0 / (double)ct.factor;
+                        FOR_K r[k] = r[k] * sc;
+                    } else {
+                        FOR_K r[k] = __fmul_rn(r[k], ct.scale);
                     }
-                    for (int a = 0; a < na; ++a) rc[(size_t)a * c.stride + h] = r;
                 }
+                for (int a = 0; a < na; ++a)
+                    FOR_K if (hslot(k) < Ho) rc[(size_t)a * c.stride + hslot(k)] = r[k];
             }
         }
         if (any) __syncthreads();
@@ -294,26 +318,37 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
 
 // Block-wide exclusive segmented scan in f64 of the staged reach values named
 // by the index list (SCAN_HEAD starts a segment, SCAN_ZERO adds 0).  Every
-// thread owns a contiguous run of entries; runs are combined with a wave scan
-// of (segment-started, sum) pairs and a carry across the four waves.
+// thread owns a contiguous run of at most SCAN_RUN entries (fully unrolled:
+// the index and reach loads of a run are issued together); runs are combined
+// with a wave scan of (segment-started, sum) pairs and a carry across the
+// four waves.
+constexpr int SCAN_RUN = 16;   // (3 * 1326 + 53) / BLOCK rounded up
+static_assert(SCAN_RUN * BLOCK >= 3 * 1326 + 53, "scan runs must cover the longest index list");
+
 template <typename V>
 __device__ __forceinline__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
     const V* rs = static_cast<const V*>(c.rs);
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     const int C = (len + BLOCK - 1) / BLOCK;
     const int e0 = threadIdx.x * C;
-    const int e1 = min(e0 + C, len);
+    uint16_t u[SCAN_RUN];
+    double x[SCAN_RUN];
+#pragma unroll
+    for (int j = 0; j < SCAN_RUN; ++j) u[j] = (j < C && e0 + j < len) ? idx[e0 + j] : (uint16_t)SCAN_ZERO;
+#pragma unroll
+    for (int j = 0; j < SCAN_RUN; ++j) {
+        const uint16_t k = u[j] & SCAN_ZERO;
+        x[j] = k == SCAN_ZERO ? 0.0 : (double)rs[k];
+    }
     bool f = false;
     double s = 0.0;
-    for (int e = e0; e < e1; ++e) {
-        const uint16_t u = idx[e];
-        const uint16_t k = u & SCAN_ZERO;
-        const double v = k == SCAN_ZERO ? 0.0 : (double)rs[k];
-        if (u & SCAN_HEAD) {
+#pragma unroll
+    for (int j = 0; j < SCAN_RUN; ++j) {
+        if (u[j] & SCAN_HEAD) {
             f = true;
-            s = v;
+            s = x[j];
         } else {
-            s += v;
+            s += x[j];
         }
     }
 #pragma unroll
@@ -339,18 +374,20 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const uint16_t* idx, int 
     double cs = 0.0;
     for (int w = 0; w < wave; ++w) cs = c.sh->wave_f[w] ? c.sh->wave_s[w] : cs + c.sh->wave_s[w];
     double run = fe ? se : cs + se;
-    for (int e = e0; e < e1; ++e) {
-        const uint16_t u = idx[e];
-        const uint16_t k = u & SCAN_ZERO;
-        if (u & SCAN_HEAD) run = 0.0;
-        c.scan[e] = run;
-        run += k == SCAN_ZERO ? 0.0 : (double)rs[k];
+#pragma unroll
+    for (int j = 0; j < SCAN_RUN; ++j) {
+        if (j < C && e0 + j < len) {
+            if (u[j] & SCAN_HEAD) run = 0.0;
+            c.scan[e0 + j] = run;
+            run += x[j];
+        }
     }
     __syncthreads();
 }
 
 // Fold / showdown values of one terminal for traversing player p
-// (game/evaluation.rs:293-425), whole workgroup.
+// (game/evaluation.rs:293-425), and depth-boundary leaves
+// (evaluation.rs:87-120, 544-630), whole workgroup.
 template <typename V>
 __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
     const KernelArgs& A = c.A;
@@ -359,7 +396,11 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     const V* r = reach_rows<V>(c) + (size_t)n * c.stride;
     V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
     V* rs = static_cast<V*>(c.rs);
-    for (int h = threadIdx.x; h < Ho; h += BLOCK) rs[h] = r[h];
+    {
+        V v[KH];
+        FOR_K v[k] = hslot(k) < Ho ? r[hslot(k)] : V(0);
+        FOR_K if (hslot(k) < Ho) rs[hslot(k)] = v[k];
+    }
     __syncthreads();
     segmented_scan<V>(c, A.u16pool + T.scan_idx, T.scan_len);
 
@@ -367,56 +408,61 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     const double win = A.payoff[(size_t)3 * (c.G.node_base + n) + 0];
     const double lose = A.payoff[(size_t)3 * (c.G.node_base + n) + 1];
     const double total = c.scan[T.num_valid_opp];
+    HandScan q[KH];
+    FOR_K q[k] = hslot(k) < Hp ? hs[hslot(k)] : HandScan{0xFFFFFFFFu, 0, 0, 0};
     if (nd.kind == KIND_BOUNDARY) {
-        // depth-boundary leaf (evaluation.rs:87-120, 544-630): record the
-        // opponent reach of this visit, then the leaf value of every hand.
-        // win holds half_pot / num_combinations.
+        // record the opponent reach of this visit, then the leaf value of
+        // every hand; win holds half_pot / num_combinations
         const int b = nd.state_off;
         float* rec = A.bnd_reach + c.G.bnd_base + (size_t)(2 * b + (p ^ 1)) * c.stride;
-        for (int h = threadIdx.x; h < Ho; h += BLOCK) rec[h] = (float)rs[h];
+        FOR_K if (hslot(k) < Ho) rec[hslot(k)] = (float)rs[hslot(k)];
         const int mode = uni(A.bnd_mode[c.G.bnd_index + b]);
         const float* bv = A.bnd_cfv + c.G.bnd_base + (size_t)(2 * b + p) * c.stride;
-        for (int i = threadIdx.x; i < Hp; i += BLOCK) {
+        const uint16_t* same = A.hand_same + hand_base(c.G, p);
+        float val[KH];
+        uint16_t si[KH];
+        FOR_K {
+            val[k] = hslot(k) < Hp ? bv[hslot(k)] : 0.0f;
+            si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
+        }
+        FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            const HandScan q = hs[i];
-            const bool valid = (q.x & 0xFFFF) != 0xFFFF;
+            const bool valid = (q[k].x & 0xFFFF) != 0xFFFF;
             if (mode == BND_RAW && valid) {
-                res = (V)bv[i];
+                res = (V)val[k];
             } else if (mode == BND_SCALED && valid && total != 0.0) {
-                const uint16_t si = A.hand_same[hand_base(c.G, p) + i];
-                const double sm = si == 0xFFFF ? 0.0 : (double)rs[si];
-                const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
-                res = (V)(((double)bv[i] * win) * cr);
+                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
+                const double cr = ((total + sm) - c.scan[q[k].w & 0xFFFF]) - c.scan[q[k].w >> 16];
+                res = (V)(((double)val[k] * win) * cr);
             }
-            out[i] = res;
+            out[hslot(k)] = res;
         }
     } else if (nd.kind == KIND_FOLD) {
         const double payoff = (int)nd.player != p ? win : lose;
         const uint16_t* same = A.hand_same + hand_base(c.G, p);
-        for (int i = threadIdx.x; i < Hp; i += BLOCK) {
+        uint16_t si[KH];
+        FOR_K si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
+        FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            const HandScan q = hs[i];
-            if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
-                const uint16_t si = same[i];
-                const double sm = si == 0xFFFF ? 0.0 : (double)rs[si];
-                const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
+            if (total != 0.0 && (q[k].x & 0xFFFF) != 0xFFFF) {
+                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
+                const double cr = ((total + sm) - c.scan[q[k].w & 0xFFFF]) - c.scan[q[k].w >> 16];
                 res = (V)(payoff * cr);
             }
-            out[i] = res;
+            out[hslot(k)] = res;
         }
     } else {
-        for (int i = threadIdx.x; i < Hp; i += BLOCK) {
+        FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            const HandScan q = hs[i];
-            if ((q.x & 0xFFFF) != 0xFFFF) {
+            if ((q[k].x & 0xFFFF) != 0xFFFF) {
                 const double* S = c.scan;
-                const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
-                const double cl = ((total - S[q.x >> 16]) - (S[q.w & 0xFFFF] - S[q.y >> 16])) -
-                                  (S[q.w >> 16] - S[q.z >> 16]);
+                const double cw = (S[q[k].x & 0xFFFF] - S[q[k].y & 0xFFFF]) - S[q[k].z & 0xFFFF];
+                const double cl = ((total - S[q[k].x >> 16]) - (S[q[k].w & 0xFFFF] - S[q[k].y >> 16])) -
+                                  (S[q[k].w >> 16] - S[q[k].z >> 16]);
                 if constexpr (sizeof(V) == 8) res = win * cw + lose * cl;
                 else res = __fadd_rn((float)(win * cw), (float)(lose * cl));
             }
-            out[i] = res;
+            out[hslot(k)] = res;
         }
     }
     __syncthreads();
@@ -507,76 +553,113 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
             V* out = cfv + (size_t)n * c.stride;
             const V* cc = cfv + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_CHANCE) {
+                // f64 sum over the dealt cards plus the isomorphic swaps
+                // (solver.rs:256-331)
                 const ChanceTab ct = uni_load(A.chance + nd.aux);
-                for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                    double acc = (double)cc[h];
-                    for (int a = 1; a < na; ++a) acc += (double)cc[(size_t)a * c.stride + h];
-                    for (int k = 0; k < ct.iso_count; ++k) {
-                        const int e = ct.iso_base + k;
-                        const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
-                        acc += (double)cc[(size_t)A.iso_ref[e] * c.stride + perm[h]];
-                    }
-                    out[h] = (V)acc;
+                double acc[KH];
+                FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
+                for (int a = 1; a < na; ++a) {
+                    V v[KH];
+                    FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : V(0);
+                    FOR_K acc[k] += (double)v[k];
                 }
+                for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
+                    const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
+                    const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
+                    V v[KH];
+                    FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
+                    FOR_K acc[k] += (double)v[k];
+                }
+                FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
             } else if (nd.player == p && na > 1) {
                 if constexpr (sizeof(V) == 8) {
-                    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                        double res = cc[h];
-                        for (int a = 1; a < na; ++a) {
-                            const double v = cc[(size_t)a * c.stride + h];
-                            res = v > res ? v : res;
-                        }
-                        out[h] = res;
+                    // FP64 best response: max over actions
+                    double res[KH];
+                    FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0;
+                    for (int a = 1; a < na; ++a) {
+                        double v[KH];
+                        FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0;
+                        FOR_K res[k] = v[k] > res[k] ? v[k] : res[k];
                     }
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                } else if (MODE == MODE_BR) {
+                    float res[KH];
+                    FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0f;
+                    for (int a = 1; a < na; ++a) {
+                        float v[KH];
+                        FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0f;
+                        FOR_K res[k] = fmaxr(res[k], v[k]);
+                    }
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
                 } else {
                     const NodeState<SM> st = node_state<SM>(A, c.G, nd);
-                    float* R = st.R;
-                    float* S = st.S;
                     const float def = __fdiv_rn(1.0f, (float)na);
-                    for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                        float res;
-                        if (MODE == MODE_BR) {
-                            res = cc[h];
-                            for (int a = 1; a < na; ++a) res = fmaxr(res, cc[(size_t)a * c.stride + h]);
-                        } else {
-                            const float denom = strat_denom<MODE>(st, Hp, h, na);
-                            res = 0.0f;
-                            for (int a = 0; a < na; ++a) {
-                                const float t = __fmul_rn(
-                                    strat_of<MODE>(node_value<MODE>(st, (size_t)a * Hp + h), denom, def),
-                                    cc[(size_t)a * c.stride + h]);
-                                res = a == 0 ? t : __fadd_rn(res, t);
+                    // sigma of every action (denominator first), node value
+                    // sum_a sigma_a * cfv_a (solver.rs:334-383)
+                    float den[KH];
+                    for (int a = 0; a < na; ++a) {
+                        float v[KH];
+                        FOR_K v[k] = hslot(k) < Hp ? node_value<MODE>(st, (size_t)a * Hp + hslot(k)) : 0.0f;
+                        FOR_K {
+                            const float x = MODE == MODE_DCFR ? fmax0(v[k]) : v[k];
+                            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+                        }
+                    }
+                    float res[KH];
+                    for (int a = 0; a < na; ++a) {
+                        float v[KH], cv[KH];
+                        FOR_K {
+                            v[k] = hslot(k) < Hp ? node_value<MODE>(st, (size_t)a * Hp + hslot(k)) : 0.0f;
+                            cv[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0f;
+                        }
+                        FOR_K {
+                            const float t = __fmul_rn(strat_of<MODE>(v[k], den[k], def), cv[k]);
+                            res[k] = a == 0 ? t : __fadd_rn(res[k], t);
+                        }
+                    }
+                    if (MODE == MODE_DCFR && SM != SM_F32) {
+                        FOR_K if (hslot(k) < Hp) static_cast<float*>(c.rs)[hslot(k)] = res[k];   // for update_16bit
+                    } else if (MODE == MODE_DCFR) {
+                        // S = S * gamma + sigma; R = (R * (alpha | beta) + cfv_a) - value
+                        // (solver.rs:556-589)
+                        for (int a = 0; a < na; ++a) {
+                            float rv[KH], sv[KH], cv[KH];
+                            FOR_K {
+                                const int h = hslot(k);
+                                const size_t q = (size_t)a * Hp + h;
+                                rv[k] = h < Hp ? st.R[q] : 0.0f;
+                                sv[k] = h < Hp ? st.S[q] : 0.0f;
+                                cv[k] = h < Hp ? cc[(size_t)a * c.stride + h] : 0.0f;
                             }
-                            if (MODE == MODE_DCFR && SM != SM_F32) {
-                                static_cast<float*>(c.rs)[h] = res;   // stashed for update_16bit
-                            } else if (MODE == MODE_DCFR) {
-                                for (int a = 0; a < na; ++a) {
-                                    const size_t k = (size_t)a * Hp + h;
-                                    const float rv = R[k];
-                                    const float sig = strat_of<MODE>(rv, denom, def);
-                                    S[k] = __fadd_rn(__fmul_rn(S[k], disc.z), sig);
-                                    const float coef = signbit(rv) ? disc.y : disc.x;
-                                    R[k] = __fsub_rn(__fadd_rn(__fmul_rn(rv, coef), cc[(size_t)a * c.stride + h]),
-                                                     res);
-                                }
+                            FOR_K if (hslot(k) < Hp) {
+                                const size_t q = (size_t)a * Hp + hslot(k);
+                                const float sig = strat_of<MODE>(rv[k], den[k], def);
+                                st.S[q] = __fadd_rn(__fmul_rn(sv[k], disc.z), sig);
+                                const float coef = signbit(rv[k]) ? disc.y : disc.x;
+                                st.R[q] = __fsub_rn(__fadd_rn(__fmul_rn(rv[k], coef), cv[k]), res[k]);
                             }
                         }
-                        out[h] = res;
                     }
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
                     if (MODE == MODE_DCFR && SM != SM_F32) {
                         __syncthreads();
                         update_16bit<SM>(c, nd, st, Hp, disc);
                     }
                 }
             } else {
-                for (int h = threadIdx.x; h < Hp; h += BLOCK) {
-                    V res = cc[h];
-                    for (int a = 1; a < na; ++a) {
-                        if constexpr (sizeof(V) == 8) res += cc[(size_t)a * c.stride + h];
-                        else res = __fadd_rn(res, cc[(size_t)a * c.stride + h]);
+                // opponent (or single-action) node: sum of the children
+                V res[KH];
+                FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : V(0);
+                for (int a = 1; a < na; ++a) {
+                    V v[KH];
+                    FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : V(0);
+                    if constexpr (sizeof(V) == 8) {
+                        FOR_K res[k] += v[k];
+                    } else {
+                        FOR_K res[k] = __fadd_rn(res[k], v[k]);
                     }
-                    out[h] = res;
                 }
+                FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
             }
         }
         if (any) __syncthreads();

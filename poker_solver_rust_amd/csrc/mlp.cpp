@@ -1,0 +1,413 @@
+// Value net host side: parameters, AdamW state, activations and the training
+// set in HBM; forward, backward and training steps built from the kernels in
+// mlp_kernels.hip.  C ABI in include/psr_dcfr.h (psr_mlp_*).
+//
+// Layout: layer l (1..L) maps dims[l-1] -> dims[l]; W_l is stored in x out
+// row-major (FP32 master + BF16 working copy), b_l has dims[l] entries.
+// Hidden layers apply GELU, the last layer is linear.  Activations H_l (BF16,
+// the next layer's input) and pre-activations Z_l (FP32) are kept per batch
+// row for the backward pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "capi_internal.hpp"
+#include "mlp_launch.hpp"
+
+using namespace psr;
+
+namespace {
+
+#define HIP_OK(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) return fail(std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// deterministic parameter init: splitmix64 -> uniform [0, 1)
+struct SplitMix {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+uint16_t bf16_bits(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+}  // namespace
+
+struct psr_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    std::vector<int> dims;
+    int L = 0, max_batch = 0;
+    std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
+    std::vector<uint16_t*> Wb;
+    std::vector<uint16_t*> H;   // H[0] = input, H[l] = gelu(Z[l]) for hidden l
+    std::vector<float*> Z;      // Z[l], l >= 1
+    std::vector<uint16_t*> dZ;  // dZ[l], l >= 1
+    float* row_loss = nullptr;
+    float* yb = nullptr;        // batch targets / mask
+    float* mbuf = nullptr;
+    // resident training set
+    uint16_t* dx = nullptr;
+    float *dy = nullptr, *dmask = nullptr;
+    int32_t* drows = nullptr;
+    int n_data = 0;
+    int64_t step = 0;
+    std::vector<void*> allocs;
+
+    int alloc(void** p, size_t bytes) {
+        HIP_OK(hipMalloc(p, std::max<size_t>(bytes, 4)));
+        allocs.push_back(*p);
+        return 0;
+    }
+    void release() {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+    }
+    int64_t num_params() const {
+        int64_t n = 0;
+        for (int l = 1; l <= L; ++l) n += (int64_t)dims[l - 1] * dims[l] + dims[l];
+        return n;
+    }
+
+    // forward of n <= max_batch rows already in H[0]
+    int forward(int n) {
+        for (int l = 1; l <= L; ++l) {
+            GemmArgs g{};
+            g.A = H[l - 1];
+            g.sam = dims[l - 1];
+            g.sak = 1;
+            g.B = Wb[l];
+            g.sbk = dims[l];
+            g.sbn = 1;
+            g.M = n;
+            g.N = dims[l];
+            g.K = dims[l - 1];
+            g.ldc = dims[l];
+            g.Cf = Z[l];
+            g.Cb = l < L ? H[l] : nullptr;
+            g.bias = b[l];
+            HIP_OK(gemm_launch(g, l < L ? EPI_BIAS_GELU : EPI_BIAS, stream));
+        }
+        return 0;
+    }
+
+    // backward of n rows given dZ[L]; fills gW, gb
+    int backward(int n) {
+        for (int l = L; l >= 1; --l) {
+            GemmArgs g{};   // gW_l = H_{l-1}^T dZ_l
+            g.A = H[l - 1];
+            g.sam = 1;
+            g.sak = dims[l - 1];
+            g.B = dZ[l];
+            g.sbk = dims[l];
+            g.sbn = 1;
+            g.M = dims[l - 1];
+            g.N = dims[l];
+            g.K = n;
+            g.ldc = dims[l];
+            g.Cf = gW[l];
+            HIP_OK(gemm_launch(g, EPI_NONE, stream));
+            HIP_OK(colsum_launch(dZ[l], n, dims[l], gb[l], stream));
+            if (l > 1) {
+                GemmArgs h{};   // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})
+                h.A = dZ[l];
+                h.sam = dims[l];
+                h.sak = 1;
+                h.B = Wb[l];
+                h.sbk = 1;
+                h.sbn = dims[l];
+                h.M = n;
+                h.N = dims[l - 1];
+                h.K = dims[l];
+                h.ldc = dims[l - 1];
+                h.Cb = dZ[l - 1];
+                h.aux = Z[l - 1];
+                HIP_OK(gemm_launch(h, EPI_GELU_GRAD, stream));
+            }
+        }
+        return 0;
+    }
+
+    // mean Huber loss over all outputs of n rows (targets yb, mask mbuf); dZ[L]
+    int loss(int n, float delta, double* out) {
+        const int D = dims[L];
+        const float inv = 1.0f / ((float)n * (float)D);
+        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, delta, inv, dZ[L], row_loss, stream));
+        if (out) {
+            std::vector<float> rl(n);
+            HIP_OK(hipMemcpyAsync(rl.data(), row_loss, n * sizeof(float), hipMemcpyDeviceToHost, stream));
+            HIP_OK(hipStreamSynchronize(stream));
+            double s = 0.0;
+            for (float x : rl) s += x;
+            *out = s * (double)inv;
+        }
+        return 0;
+    }
+
+    int adamw(float lr, float wd) {
+        ++step;
+        const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+        const float bc1 = 1.0f - (float)std::pow(0.9, (double)step);
+        const float bc2 = 1.0f - (float)std::pow(0.999, (double)step);
+        for (int l = 1; l <= L; ++l) {
+            HIP_OK(adamw_launch(W[l], Wb[l], gW[l], mW[l], vW[l], (int64_t)dims[l - 1] * dims[l], lr, b1, b2, eps, wd,
+                                bc1, bc2, stream));
+            HIP_OK(adamw_launch(b[l], nullptr, gb[l], mb[l], vb[l], dims[l], lr, b1, b2, eps, 0.0f, bc1, bc2, stream));
+        }
+        return 0;
+    }
+
+    // host f32 rows -> H[0] (BF16)
+    int upload_input(const float* x, int n) {
+        std::vector<uint16_t> t((size_t)n * dims[0]);
+        for (size_t i = 0; i < t.size(); ++i) t[i] = bf16_bits(x[i]);
+        HIP_OK(hipMemcpyAsync(H[0], t.data(), t.size() * 2, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return 0;
+    }
+};
+
+extern "C" {
+
+int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint64_t seed, psr_mlp** out) {
+    if (!dims || n_dims < 2 || max_batch <= 0 || !out) return fail("invalid argument");
+    for (int i = 0; i < n_dims; ++i)
+        if (dims[i] <= 0) return fail("layer widths must be positive");
+    auto m = new psr_mlp;
+    m->device = device;
+    m->dims.assign(dims, dims + n_dims);
+    m->L = n_dims - 1;
+    m->max_batch = max_batch;
+    auto bad = [&](int rc) {
+        if (rc) {
+            m->release();
+            delete m;
+        }
+        return rc;
+    };
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess) {
+        delete m;
+        return fail("HIP setup failed");
+    }
+    const int L = m->L;
+    m->W.assign(L + 1, nullptr);
+    m->b.assign(L + 1, nullptr);
+    m->gW.assign(L + 1, nullptr);
+    m->gb.assign(L + 1, nullptr);
+    m->mW.assign(L + 1, nullptr);
+    m->vW.assign(L + 1, nullptr);
+    m->mb.assign(L + 1, nullptr);
+    m->vb.assign(L + 1, nullptr);
+    m->Wb.assign(L + 1, nullptr);
+    m->H.assign(L + 1, nullptr);
+    m->Z.assign(L + 1, nullptr);
+    m->dZ.assign(L + 1, nullptr);
+    SplitMix rng{seed};
+    for (int l = 1; l <= L; ++l) {
+        const int64_t nw = (int64_t)dims[l - 1] * dims[l];
+        if (bad(m->alloc((void**)&m->W[l], nw * 4)) || bad(m->alloc((void**)&m->Wb[l], nw * 2)) ||
+            bad(m->alloc((void**)&m->gW[l], nw * 4)) || bad(m->alloc((void**)&m->mW[l], nw * 4)) ||
+            bad(m->alloc((void**)&m->vW[l], nw * 4)) || bad(m->alloc((void**)&m->b[l], dims[l] * 4)) ||
+            bad(m->alloc((void**)&m->gb[l], dims[l] * 4)) || bad(m->alloc((void**)&m->mb[l], dims[l] * 4)) ||
+            bad(m->alloc((void**)&m->vb[l], dims[l] * 4)) ||
+            bad(m->alloc((void**)&m->Z[l], (size_t)max_batch * dims[l] * 4)) ||
+            bad(m->alloc((void**)&m->dZ[l], (size_t)max_batch * dims[l] * 2)))
+            return -1;
+        // U(-1/sqrt(in), 1/sqrt(in)) for weights and biases
+        const double bound = 1.0 / std::sqrt((double)dims[l - 1]);
+        std::vector<float> w(nw), bb(dims[l]);
+        for (auto& x : w) x = (float)((2.0 * rng.uniform() - 1.0) * bound);
+        for (auto& x : bb) x = (float)((2.0 * rng.uniform() - 1.0) * bound);
+        std::vector<float> zero(nw, 0.0f);
+        if (hipMemcpy(m->W[l], w.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->b[l], bb.data(), dims[l] * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->mW[l], zero.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->vW[l], zero.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->mb[l], zero.data(), dims[l] * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->vb[l], zero.data(), dims[l] * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            to_bf16_launch(m->W[l], m->Wb[l], nw, m->stream) != hipSuccess)
+            return bad(fail("parameter upload failed"));
+    }
+    for (int l = 0; l < L; ++l)
+        if (bad(m->alloc((void**)&m->H[l], (size_t)max_batch * dims[l] * 2))) return -1;
+    if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||
+        bad(m->alloc((void**)&m->yb, (size_t)max_batch * dims[L] * 4)) ||
+        bad(m->alloc((void**)&m->mbuf, (size_t)max_batch * dims[L] * 4)))
+        return -1;
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return bad(fail("HIP error during init"));
+    *out = m;
+    return 0;
+}
+
+void psr_mlp_destroy(psr_mlp* m) {
+    if (!m) return;
+    (void)hipSetDevice(m->device);
+    m->release();
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int64_t psr_mlp_num_params(psr_mlp* m) { return m ? m->num_params() : fail("null net"); }
+
+int psr_mlp_get_params(psr_mlp* m, float* out, int64_t n) {
+    if (!m || !out || n != m->num_params()) return fail("out must hold psr_mlp_num_params floats");
+    HIP_OK(hipSetDevice(m->device));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    int64_t off = 0;
+    for (int l = 1; l <= m->L; ++l) {
+        const int64_t nw = (int64_t)m->dims[l - 1] * m->dims[l];
+        HIP_OK(hipMemcpy(out + off, m->W[l], nw * 4, hipMemcpyDeviceToHost));
+        off += nw;
+        HIP_OK(hipMemcpy(out + off, m->b[l], m->dims[l] * 4, hipMemcpyDeviceToHost));
+        off += m->dims[l];
+    }
+    return 0;
+}
+
+int psr_mlp_set_params(psr_mlp* m, const float* in, int64_t n) {
+    if (!m || !in || n != m->num_params()) return fail("in must hold psr_mlp_num_params floats");
+    HIP_OK(hipSetDevice(m->device));
+    int64_t off = 0;
+    for (int l = 1; l <= m->L; ++l) {
+        const int64_t nw = (int64_t)m->dims[l - 1] * m->dims[l];
+        HIP_OK(hipMemcpy(m->W[l], in + off, nw * 4, hipMemcpyHostToDevice));
+        HIP_OK(to_bf16_launch(m->W[l], m->Wb[l], nw, m->stream));
+        off += nw;
+        HIP_OK(hipMemcpy(m->b[l], in + off, m->dims[l] * 4, hipMemcpyHostToDevice));
+        off += m->dims[l];
+    }
+    HIP_OK(hipStreamSynchronize(m->stream));
+    return 0;
+}
+
+// Inference: n rows of dims[0] inputs -> n rows of dims[L] outputs (host memory).
+int psr_mlp_forward(psr_mlp* m, const float* x, int n, float* y) {
+    if (!m || !x || !y || n < 0) return fail("invalid argument");
+    HIP_OK(hipSetDevice(m->device));
+    const int D0 = m->dims[0], DL = m->dims[m->L];
+    HIP_OK(hipEventRecord(m->ev0, m->stream));
+    for (int r0 = 0; r0 < n; r0 += m->max_batch) {
+        const int k = std::min(m->max_batch, n - r0);
+        if (m->upload_input(x + (int64_t)r0 * D0, k) || m->forward(k)) return -1;
+        HIP_OK(hipMemcpyAsync(y + (int64_t)r0 * DL, m->Z[m->L], (size_t)k * DL * 4, hipMemcpyDeviceToHost, m->stream));
+    }
+    HIP_OK(hipEventRecord(m->ev1, m->stream));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    HIP_OK(hipEventElapsedTime(&m->last_ms, m->ev0, m->ev1));
+    return 0;
+}
+
+// One forward + backward on n <= max_batch rows: mean Huber loss (delta) over
+// all outputs with the mask applied, and the gradient of every parameter
+// (flattened like psr_mlp_get_params).  Parameters are not updated.
+int psr_mlp_gradients(psr_mlp* m, const float* x, const float* y, const float* mask, int n, float delta, double* loss,
+                      float* grads, int64_t ng) {
+    if (!m || !x || !y || !mask || n <= 0 || n > m->max_batch) return fail("invalid argument");
+    if (grads && ng != m->num_params()) return fail("grads must hold psr_mlp_num_params floats");
+    HIP_OK(hipSetDevice(m->device));
+    const int DL = m->dims[m->L];
+    if (m->upload_input(x, n)) return -1;
+    HIP_OK(hipMemcpyAsync(m->yb, y, (size_t)n * DL * 4, hipMemcpyHostToDevice, m->stream));
+    HIP_OK(hipMemcpyAsync(m->mbuf, mask, (size_t)n * DL * 4, hipMemcpyHostToDevice, m->stream));
+    if (m->forward(n) || m->loss(n, delta, loss) || m->backward(n)) return -1;
+    HIP_OK(hipStreamSynchronize(m->stream));
+    if (grads) {
+        int64_t off = 0;
+        for (int l = 1; l <= m->L; ++l) {
+            const int64_t nw = (int64_t)m->dims[l - 1] * m->dims[l];
+            HIP_OK(hipMemcpy(grads + off, m->gW[l], nw * 4, hipMemcpyDeviceToHost));
+            off += nw;
+            HIP_OK(hipMemcpy(grads + off, m->gb[l], m->dims[l] * 4, hipMemcpyDeviceToHost));
+            off += m->dims[l];
+        }
+    }
+    return 0;
+}
+
+// Training set resident in HBM: n rows of inputs (stored as BF16), targets
+// and per-output masks (0 / 1).
+int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float* mask, int n) {
+    if (!m || !x || !y || !mask || n <= 0) return fail("invalid argument");
+    HIP_OK(hipSetDevice(m->device));
+    for (void* p : {(void*)m->dx, (void*)m->dy, (void*)m->dmask, (void*)m->drows}) {
+        if (!p) continue;
+        auto it = std::find(m->allocs.begin(), m->allocs.end(), p);
+        if (it != m->allocs.end()) m->allocs.erase(it);
+        (void)hipFree(p);
+    }
+    const int D0 = m->dims[0], DL = m->dims[m->L];
+    if (m->alloc((void**)&m->dx, (size_t)n * D0 * 2) || m->alloc((void**)&m->dy, (size_t)n * DL * 4) ||
+        m->alloc((void**)&m->dmask, (size_t)n * DL * 4) || m->alloc((void**)&m->drows, (size_t)m->max_batch * 4))
+        return -1;
+    std::vector<uint16_t> t((size_t)D0 * 4096);
+    for (int r0 = 0; r0 < n; r0 += 4096) {
+        const int k = std::min(4096, n - r0);
+        for (size_t i = 0; i < (size_t)k * D0; ++i) t[i] = bf16_bits(x[(size_t)r0 * D0 + i]);
+        HIP_OK(hipMemcpy(m->dx + (size_t)r0 * D0, t.data(), (size_t)k * D0 * 2, hipMemcpyHostToDevice));
+    }
+    HIP_OK(hipMemcpy(m->dy, y, (size_t)n * DL * 4, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(m->dmask, mask, (size_t)n * DL * 4, hipMemcpyHostToDevice));
+    m->n_data = n;
+    return 0;
+}
+
+// Minibatch AdamW training on the resident set: `epochs` passes in a seeded
+// shuffled order, batches of `batch` rows (<= max_batch); epoch_loss[e] = mean
+// Huber loss of the epoch's batches (before each update).
+int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_decay, float delta, uint64_t seed,
+                  double* epoch_loss) {
+    if (!m || m->n_data <= 0) return fail("no training set");
+    if (batch <= 0 || batch > m->max_batch || epochs <= 0) return fail("invalid batch or epochs");
+    HIP_OK(hipSetDevice(m->device));
+    const int D0 = m->dims[0], DL = m->dims[m->L];
+    std::vector<int32_t> order(m->n_data);
+    std::iota(order.begin(), order.end(), 0);
+    SplitMix rng{seed};
+    HIP_OK(hipEventRecord(m->ev0, m->stream));
+    for (int e = 0; e < epochs; ++e) {
+        for (int i = m->n_data - 1; i > 0; --i) std::swap(order[i], order[rng.next() % (uint64_t)(i + 1)]);
+        double sum = 0.0;
+        int nb = 0;
+        for (int r0 = 0; r0 + batch <= m->n_data; r0 += batch) {
+            HIP_OK(hipMemcpyAsync(m->drows, order.data() + r0, batch * 4, hipMemcpyHostToDevice, m->stream));
+            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows, batch, D0, DL, m->H[0], m->yb, m->mbuf, m->stream));
+            double l = 0.0;
+            if (m->forward(batch) || m->loss(batch, delta, &l) || m->backward(batch) || m->adamw(lr, weight_decay))
+                return -1;
+            sum += l;
+            ++nb;
+        }
+        if (epoch_loss) epoch_loss[e] = nb ? sum / nb : 0.0;
+    }
+    HIP_OK(hipEventRecord(m->ev1, m->stream));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    HIP_OK(hipEventElapsedTime(&m->last_ms, m->ev0, m->ev1));
+    return 0;
+}
+
+float psr_mlp_last_ms(psr_mlp* m) { return m ? m->last_ms : -1.0f; }
+
+}  // extern "C"

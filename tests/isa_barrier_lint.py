@@ -231,3 +231,8 @@ def count_divergent_loops(asm):
         _, succ = _cfg(blocks)
         res[fname] = len(divergent_latches(blocks, succ))
     return res
+
+
+def count_barriers(asm):
+    """s_barrier instructions per function (sanity check of the parser)."""
+    return {f: sum(1 for x in lines if x.startswith("s_barrier")) for f, lines in _functions(asm).items()}
