@@ -137,15 +137,16 @@ __global__ void colsum_bf16_kernel(const uint16_t* X, int M, int N, float* out) 
     out[n] = s;
 }
 
+constexpr int HUBER_BLOCK = 256;
+
 // Masked Huber loss (delta) of FP32 predictions against targets; writes the
 // gradient d loss / d pred (BF16, scaled by 1 / count) and per-row loss sums.
 __global__ void huber_kernel(const float* pred, const float* target, const float* mask, int rows, int cols, float delta,
                              float inv_count, uint16_t* grad, float* row_loss) {
-    const int r = blockIdx.x;
-    if (r >= rows) return;
-    __shared__ float part[256];
+    const int r = blockIdx.x;   // one workgroup of HUBER_BLOCK threads per row, grid = rows
+    __shared__ float part[HUBER_BLOCK];
     float s = 0.0f;
-    for (int c = threadIdx.x; c < cols; c += blockDim.x) {
+    for (int c = threadIdx.x; c < cols; c += HUBER_BLOCK) {
         const int64_t o = (int64_t)r * cols + c;
         const float w = mask[o];
         const float d = pred[o] - target[o];
@@ -157,7 +158,8 @@ __global__ void huber_kernel(const float* pred, const float* target, const float
     }
     part[threadIdx.x] = s;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+#pragma unroll
+    for (int w = HUBER_BLOCK / 2; w > 0; w >>= 1) {   // compile-time trip count: scalar loop control
         if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
@@ -221,7 +223,8 @@ hipError_t colsum_launch(const uint16_t* X, int M, int N, float* out, hipStream_
 
 hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, float delta,
                         float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream) {
-    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(256), 0, stream, pred, target, mask, rows, cols, delta, inv_count,
+    if (rows <= 0 || cols <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(HUBER_BLOCK), 0, stream, pred, target, mask, rows, cols, delta, inv_count,
                        grad, row_loss);
     return hipGetLastError();
 }

@@ -131,6 +131,19 @@ def load_library(path: Optional[str] = None):
                                                  C.c_int64]),
         "psr_batch_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_batch_normalized_weights": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "psr_mlp_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
+        "psr_mlp_destroy": (None, [vp]),
+        "psr_mlp_num_params": (C.c_int64, [vp]),
+        "psr_mlp_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64]),
+        "psr_mlp_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64]),
+        "psr_mlp_forward": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_float)]),
+        "psr_mlp_gradients": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int,
+                                        C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_int64]),
+        "psr_mlp_set_dataset": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                          C.c_int]),
+        "psr_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint64,
+                                    C.POINTER(C.c_double)]),
+        "psr_mlp_last_ms": (C.c_float, [vp]),
         "psr_batch_expected_values_detail": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_expected_values": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),

@@ -27,8 +27,11 @@ def build():
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-U_FORTIFY_SOURCE", "-ffp-contract=off", "-pthread", "-I" + HERE, "-I" + CSRC,
              "-Wno-unused-function"]
     objs = []
+    mlp_kern = os.path.join(work, "mlp_kernels_emu.cpp")
+    with open(mlp_kern, "w") as f:
+        f.write(open(os.path.join(CSRC, "mlp_kernels.hip")).read())
     for s in [os.path.join(CSRC, "game.cpp"), os.path.join(CSRC, "capi_game.cpp"), os.path.join(CSRC, "gpu_solver.cpp"),
-              kern]:
+              os.path.join(CSRC, "mlp.cpp"), kern, mlp_kern]:
         o = os.path.join(work, os.path.basename(s) + ".o")
         subprocess.check_call([CXX, "-c", *flags, s, "-o", o])
         objs.append(o)

@@ -95,19 +95,25 @@ inline void syncthreads() { wait_for(AT_BARRIER); }
 
 template <class T>
 T exchange(T v, int src) {
-    static_assert(sizeof(T) <= 8, "shuffle of at most 8 bytes");
+    static_assert(sizeof(T) <= 16, "exchange of at most 16 bytes");
     const int me = g_group->current, wave = me / 64;
-    uint64_t bits = 0;
-    std::memcpy(&bits, &v, sizeof(T));
-    g_group->slots[me] = bits;
+    std::memcpy(&g_group->slots[2 * me], &v, sizeof(T));
     wait_for(AT_WAVE);   // every lane of the wave has written its slot
     T r = v;
-    if (src >= 0 && src < 64) {
-        const uint64_t b = g_group->slots[wave * 64 + src];
-        std::memcpy(&r, &b, sizeof(T));
-    }
+    if (src >= 0 && src < 64) std::memcpy(&r, &g_group->slots[2 * (wave * 64 + src)], sizeof(T));
     wait_for(AT_WAVE);   // every lane has read before the slots are reused
     return r;
+}
+
+// all 64 lanes' values of a wave (one exchange)
+template <class T>
+void gather_wave(T v, T* out) {
+    static_assert(sizeof(T) <= 16, "exchange of at most 16 bytes");
+    const int me = g_group->current, wave = me / 64;
+    std::memcpy(&g_group->slots[2 * me], &v, sizeof(T));
+    wait_for(AT_WAVE);
+    for (int l = 0; l < 64; ++l) std::memcpy(&out[l], &g_group->slots[2 * (wave * 64 + l)], sizeof(T));
+    wait_for(AT_WAVE);
 }
 
 inline void lane_entry() {
@@ -137,7 +143,7 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
     Group G;
     G.lanes.resize(nt);
     for (Lane& l : G.lanes) l.stack.reset(new char[STACK]);
-    G.slots.resize(((nt + 63) / 64) * 64);
+    G.slots.resize(2 * ((nt + 63) / 64) * 64);
     std::vector<double> lds((lds_bytes + 7) / 8 + 1);
     G.lds = lds.data();
     G.body = [&]() { kernel(args...); };
@@ -146,8 +152,12 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
     cur_kernarg = first_arg(args...);
     cur_block_dim = block;
     cur_grid = grid;
-    for (unsigned bx = 0; bx < grid.x; ++bx) {
-        cur_block = dim3(bx);
+    const unsigned nblocks = grid.x * grid.y * grid.z;
+    for (unsigned bi = 0; bi < nblocks; ++bi) {
+        const unsigned bx = bi % grid.x;
+        cur_block.x = bx;
+        cur_block.y = (bi / grid.x) % grid.y;
+        cur_block.z = bi / (grid.x * grid.y);
         for (int t = 0; t < nt; ++t) {
             Lane& l = G.lanes[t];
             getcontext(&l.ctx);
@@ -231,6 +241,28 @@ inline const void* __builtin_amdgcn_kernarg_segment_ptr() { return emu::cur_kern
 inline long long __double_as_longlong(double d) { long long i; std::memcpy(&i, &d, 8); return i; }
 inline double __longlong_as_double(long long i) { double d; std::memcpy(&d, &i, 8); return d; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
+// v_mfma_f32_16x16x32_bf16 (wave64): lane l holds A row l % 16 and B column
+// l % 16 with K elements 8 * (l / 16) .. +7; the accumulator holds C rows
+// 4 * (l / 16) + v, column l % 16.  Products are summed in double and added
+// to the f32 accumulator once.
+typedef __bf16 emu_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float emu_floatx4 __attribute__((ext_vector_type(4)));
+inline emu_floatx4 __builtin_amdgcn_mfma_f32_16x16x32_bf16(emu_bf16x8 a, emu_bf16x8 b, emu_floatx4 c, int, int, int) {
+    emu_bf16x8 A[64], B[64];
+    emu::gather_wave(a, A);
+    emu::gather_wave(b, B);
+    const int l = threadIdx.x & 63;
+    emu_floatx4 out = c;
+    for (int v = 0; v < 4; ++v) {
+        const int row = 4 * (l / 16) + v, col = l % 16;
+        double acc = 0.0;
+        for (int k = 0; k < 32; ++k)
+            acc += (double)(float)A[row + 16 * (k / 8)][k % 8] * (double)(float)B[col + 16 * (k / 8)][k % 8];
+        out[v] = (float)((double)c[v] + acc);
+    }
+    return out;
+}
+
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline float __fadd_rn(float a, float b) { return a + b; }
 inline float __fsub_rn(float a, float b) { return a - b; }

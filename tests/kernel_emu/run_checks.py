@@ -234,6 +234,74 @@ def run_expected_values():
             same(b.expected_values(0, p), o.expected_values(p), f"expected values {p}")
 
 
+def bf16(a):
+    """Round to BF16 (nearest even) and back, as the device stores operands."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32)
+
+
+def mlp_case(dims, n, seed):
+    from poker_solver_rust_amd.value_net import ValueNet
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mlp_oracle as M
+    rng = np.random.default_rng(seed)
+    net = ValueNet(dims, max_batch=64, seed=seed)
+    p = net.get_params()
+    x = rng.normal(0, 1, (n, dims[0])).astype(np.float32)
+    y = rng.normal(0, 1, (n, dims[-1])).astype(np.float32)
+    mask = (rng.random((n, dims[-1])) < 0.8).astype(np.float32)
+    return net, M, p, x, y, mask
+
+
+def run_mlp_forward():
+    """Forward pass (MFMA GEMM tiles, edges of ragged 64 x 64 tiles, bias +
+    GELU epilogue) against the oracle on BF16-rounded weights and inputs."""
+    dims = (70, 40, 33, 19)
+    net, M, p, x, y, mask = mlp_case(dims, 37, 3)
+    got = net.forward(x)
+    ref = M.forward(np.concatenate([bf16(a) if k % 2 == 0 else a for k, a in
+                                    enumerate(np.split(p, np.cumsum([s for i in range(len(dims) - 1)
+                                                                     for s in (dims[i] * dims[i + 1], dims[i + 1])])[:-1]))]),
+                    dims, bf16(x).astype(np.float64))
+    err = np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-9)
+    if err > 2e-2:
+        raise AssertionError(f"forward relative error {err}")
+
+
+def run_mlp_gradients():
+    dims = (70, 40, 33, 19)
+    net, M, p, x, y, mask = mlp_case(dims, 37, 4)
+    loss, g = net.gradients(x, y, mask, 1.0)
+    lo, go = M.loss_and_grads(p.astype(np.float64), dims, x.astype(np.float64), y.astype(np.float64),
+                              mask.astype(np.float64), 1.0)
+    if abs(loss - lo) > 2e-2 * abs(lo):
+        raise AssertionError(f"loss {loss} vs oracle {lo}")
+    err = np.abs(g - go).max() / np.abs(go).max()
+    print(f"loss {loss:.6g} oracle {lo:.6g}; max gradient error {err:.3g} of the largest gradient", flush=True)
+    if err > 5e-2:
+        raise AssertionError(f"gradient relative error {err}")
+
+
+def run_mlp_adamw_step():
+    """One training step on a resident batch: every parameter whose oracle
+    gradient is not negligible moves by the oracle's AdamW update (at step 1
+    the update is lr * g / (|g| + eps), weights decayed)."""
+    dims = (70, 40, 33, 19)
+    net, M, p, x, y, mask = mlp_case(dims, 32, 5)
+    net.set_dataset(x, y, mask)
+    net.train(1, 32, lr=1e-2, weight_decay=0.01, delta=1.0, seed=0)
+    p1 = net.get_params().astype(np.float64)
+    _, go = M.loss_and_grads(p.astype(np.float64), dims, x.astype(np.float64), y.astype(np.float64),
+                             mask.astype(np.float64), 1.0)
+    ref = M.AdamW(dims, 1e-2, 0.01).step(p.astype(np.float64), go)
+    big = np.abs(go) > 1e-2 * np.abs(go).max()
+    err = np.abs(p1 - ref)[big].max()
+    print(f"AdamW step: {int(big.sum())} of {len(p)} parameters compared, max deviation {err:.3g} (lr 1e-2)", flush=True)
+    if err > 1e-3:
+        raise AssertionError(f"AdamW step deviates by {err}")
+
+
 CHECKS = {name[4:]: fn for name, fn in globals().items() if name.startswith("run_")}
 
 if __name__ == "__main__":

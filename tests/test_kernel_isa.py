@@ -37,6 +37,19 @@ def kernels_asm(tmp_path_factory):
     return L.compile_to_asm(os.path.join(B.CSRC, "dcfr_kernels.hip"), FLAGS, str(tmp_path_factory.mktemp("dcfr")))
 
 
+@pytest.fixture(scope="module")
+def mlp_asm(tmp_path_factory):
+    return L.compile_to_asm(os.path.join(B.CSRC, "mlp_kernels.hip"), FLAGS + ["-I" + B.CSRC],
+                            str(tmp_path_factory.mktemp("mlp")))
+
+
+def test_mlp_kernels_have_no_barrier_in_divergent_loop(mlp_asm):
+    assert not L.barriers_in_divergent_loops(mlp_asm)
+    bars = L.count_barriers(mlp_asm)
+    assert all(bars[f] >= 2 for f in bars if "gemm_bf16_kernel" in f)
+    assert "v_mfma_f32_16x16x32_bf16" in mlp_asm
+
+
 def test_check_flags_known_bad_shapes(pins_asm):
     bad = {v[0] for v in L.barriers_in_divergent_loops(pins_asm)}
     assert any("old_loop" in f for f in bad)
