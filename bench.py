@@ -107,6 +107,78 @@ def turn_bench(P, W, iters, seed):
                          "frac": round(achieved / HBM_PEAK_GBS, 5)}}
 
 
+def net_solve_bench(args):
+    """The net-backed depth-limited turn solve on the full turn subgame (4
+    seeded board cards, U[0,1] ranges, pot 60, stack 200, 33%/75%/all-in, two
+    raises at 3x).  Steps: a check-down depth-limited solve records the
+    boundary reaches the solve visits; river training solves from the batched
+    GPU solver (half on those reaches, half on synthetic boundary-like ranges,
+    random river cards); value-net training on the GPU; the turn tree cut at
+    the river deal solved with the net at its leaves.  Yardsticks on the same
+    tree: check-down leaf values (the method without the net), exact river
+    solves at the leaves (the fallback) and the exact full turn solve (the
+    judge).  Errors are root expected values (query.rs:639-666) as a fraction
+    of the pot."""
+    import poker_solver_rust_amd as P
+    from poker_solver_rust_amd import net_solve as N
+    from poker_solver_rust_amd import workloads as W
+
+    turn = W.turn_uniform01(args.seed)
+    dl_game = P.PostFlopGame.with_config(turn.card_config(), P.ActionTree(N.depth_limited_tree(turn)))
+    pots = sorted({dl_game.boundary_pot(b) for b in range(dl_game.num_boundary_nodes())})
+    log(f"turn board {turn.board[:4]}, {dl_game.num_boundary_nodes()} river-deal boundaries, pots {pots}")
+    iters = args.turn_solve_iters
+    chk = N.CheckdownLeaves(turn.board[:4])
+    snaps = []
+    t0 = time.time()
+    g_chk, b_chk, leaf_chk, dcfr_chk = N.solve_depth_limited(turn, chk, iters, 10, log, snapshots=snaps)
+    t_chk = time.time() - t0
+    situations = [it for items in snaps for it in items]
+    t0 = time.time()
+    ts = N.river_training_set(turn, pots, args.net_train_solves, seed=args.seed + 1, iterations=args.iterations,
+                              log=log, situations=situations, checkdown=chk)
+    t_data = time.time() - t0
+    t0 = time.time()
+    net, losses, mae = N.train_value_net(ts, epochs=args.net_epochs, seed=args.seed)
+    t_train = time.time() - t0
+    log(f"trained on exact - check-down residuals: loss {losses[0]:.4f} -> {losses[-1]:.4f}, hold-out MAE "
+        f"{mae:.4f} half-pots, {t_train:.1f}s")
+    t0 = time.time()
+    g_net, b_net, leaf_net, dcfr_net = N.solve_depth_limited(turn, N.NetLeaves(net, chk), iters, 10, log)
+    t_net = time.time() - t0
+    t0 = time.time()
+    g_exl, b_exl, leaf_exl, dcfr_exl = N.solve_depth_limited(turn, N.ExactLeaves(turn, args.iterations), iters, 10,
+                                                              log)
+    t_exl = time.time() - t0
+    g_ex = turn.build()
+    b_ex = P.GpuBatchSolver(0)
+    b_ex.prepare_batch([g_ex], iters)
+    t0 = time.time()
+    b_ex.run_iterations(0, iters)
+    t_ex = time.time() - t0
+    err = {name: N.root_ev_error(b_ex, b, g_ex, turn.pot)
+           for name, b in (("net_backed", b_net), ("checkdown", b_chk), ("exact_leaves", b_exl))}
+    for k, v in err.items():
+        log(f"{k}: {v}")
+    gv = {k: max(v["p0_value_diff_pot"], v["p1_value_diff_pot"]) for k, v in err.items()}
+    out = {"metric": "net-backed turn solve: game value error vs the exact turn solve (fraction of pot)",
+           "value": gv["net_backed"], "game_value_error": gv,
+           "gain_over_checkdown": gv["checkdown"] / max(gv["net_backed"], 1e-12),
+           "unit": "pot fraction", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+           "dtype": "bf16 net, f32 solve",
+           "config": {"workload": "turn_uniform01 seed %d, depth_limit 0 at the river deal" % args.seed,
+                      "turn_iterations": iters, "leaf_refresh": 10, "train_solves": ts.solves,
+                      "train_solve_iterations_cap": args.iterations, "epochs": args.net_epochs,
+                      "net": "2706-512x4(GELU)-2652 bf16"},
+           **err,
+           "training": {"loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                        "holdout_mae_half_pots": mae, "data_seconds": t_data, "solve_seconds": ts.solve_seconds,
+                        "river_solves_per_s": ts.solves / max(ts.solve_seconds, 1e-9), "train_seconds": t_train},
+           "seconds": {"net_backed_turn": t_net, "net_leaf_eval": leaf_net, "net_dcfr": dcfr_net,
+                       "checkdown_turn": t_chk, "exact_leaves_turn": t_exl, "exact_turn": t_ex}}
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of `nbytes` (float32 tensors, after a
     warm-up copy), the known-size kernel the FETCH_SIZE / WRITE_SIZE counters
@@ -143,12 +215,20 @@ def main():
     ap.add_argument("--single-sample", type=int, default=64,
                     help="subgames solved one per launch for the single-subgame yardstick (0: skip)")
     ap.add_argument("--turn-iters", type=int, default=200, help="timed turn DCFR iterations (0: skip)")
+    ap.add_argument("--net-solve", action="store_true",
+                    help="run the net-backed depth-limited turn solve instead (river solves -> value net -> turn)")
+    ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
+    ap.add_argument("--net-epochs", type=int, default=40)
+    ap.add_argument("--turn-solve-iters", type=int, default=500)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
 
     if args.copy_bytes > 0:
         copy_probe(args.copy_bytes)
+        return
+    if args.net_solve:
+        net_solve_bench(args)
         return
 
     if args.gpus != 1:

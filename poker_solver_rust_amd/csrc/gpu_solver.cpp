@@ -1175,16 +1175,19 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
             const int v = L.terminals[td.term_base + k];
             if (v < 0 || v >= N || tile_of[v] != t) return fail("terminal outside its tile");
             const uint8_t kind = L.nodes[d.node_base + v].kind;
-            if (kind != KIND_FOLD && kind != KIND_SHOWDOWN) return fail("terminal list holds a non-terminal");
+            if (kind != KIND_FOLD && kind != KIND_SHOWDOWN && kind != KIND_BOUNDARY)
+                return fail("terminal list holds a non-terminal");
         }
         terms += td.num_terminals;
     }
     for (int v = 0; v < N; ++v) {
         if (tile_of[v] < 0) return fail("node in no level");
         const NodeRec& r = L.nodes[d.node_base + v];
-        const bool term = r.kind == KIND_FOLD || r.kind == KIND_SHOWDOWN;
+        const bool term = r.kind == KIND_FOLD || r.kind == KIND_SHOWDOWN || r.kind == KIND_BOUNDARY;
         if (term) {
             if (r.aux < 0 || r.aux * 2 + 1 >= (int)L.runouts.size()) return fail("bad runout table");
+            if (r.kind == KIND_BOUNDARY && (r.state_off < 0 || r.state_off >= d.num_boundaries))
+                return fail("boundary ordinal out of range");
             continue;
         }
         if (r.first_child < 0 || r.first_child + r.num_children > N) return fail("children out of range");

@@ -508,6 +508,11 @@ class GpuBatchSolver:
         _check(load_library().psr_batch_exploitability_f64(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def compute_root_cfvalues(self):
+        """utility.rs:348-372 finalize: the cfvalue pass under the average
+        strategy (also records the reach at every depth-boundary leaf)."""
+        _check(load_library().psr_batch_compute_root_cfvalues(self._h))
+
     def root_cfvalues(self) -> List[Tuple[np.ndarray, np.ndarray]]:
         lib = load_library()
         _check(lib.psr_batch_compute_root_cfvalues(self._h))

@@ -105,4 +105,5 @@ def test_training_reduces_loss():
     net = ValueNet(dims, max_batch=256, seed=6)
     net.set_dataset(x, y, np.ones_like(y))
     losses = net.train(30, 256, lr=1e-3, weight_decay=1e-4, delta=1.0, seed=1)
-    assert losses[-1] < 0.1 * losses[0], losses
+    assert losses[-1] < 0.25 * losses[0], losses
+    assert np.all(np.diff(losses) < 0), losses
