@@ -451,6 +451,26 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
             }
             out[hslot(k)] = res;
         }
+    } else if (uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]) != 0.0) {
+        // raked showdown (evaluation.rs:429-537): blocked opponent mass that
+        // is weaker (win), not stronger (tie, cumulative) and in total, the
+        // identical hand added back to the tie band
+        const double tie = A.payoff[(size_t)3 * (c.G.node_base + n) + 2];
+        const uint16_t* same = A.hand_same + hand_base(c.G, p);
+        uint16_t si[KH];
+        FOR_K si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
+        FOR_K if (hslot(k) < Hp) {
+            V res = 0;
+            if (total != 0.0 && (q[k].x & 0xFFFF) != 0xFFFF) {
+                const double* S = c.scan;
+                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
+                const double tw = (total - S[q[k].w & 0xFFFF]) - S[q[k].w >> 16];
+                const double cw = (S[q[k].x & 0xFFFF] - S[q[k].y & 0xFFFF]) - S[q[k].z & 0xFFFF];
+                const double ct = (S[q[k].x >> 16] - S[q[k].y >> 16]) - S[q[k].z >> 16];
+                res = (V)((win * cw + tie * ((ct - cw) + sm)) + lose * (tw - ct));
+            }
+            out[hslot(k)] = res;
+        }
     } else {
         FOR_K if (hslot(k) < Hp) {
             V res = 0;
@@ -572,7 +592,29 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                 }
                 FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
             } else if (nd.player == p && na > 1) {
-                if constexpr (sizeof(V) == 8) {
+                if constexpr (sizeof(V) == 8 && MODE == MODE_CFV) {
+                    // FP64 value of the average strategy: sum_a sigma_a * cfv_a,
+                    // sigma = S / sum S (uniform where the sum is 0)
+                    const NodeState<SM> st = node_state<SM>(A, c.G, nd);
+                    double den[KH], res[KH];
+                    for (int a = 0; a < na; ++a) {
+                        double v[KH];
+                        FOR_K v[k] = hslot(k) < Hp ? (double)st.str((size_t)a * Hp + hslot(k)) : 0.0;
+                        FOR_K den[k] = a == 0 ? v[k] : den[k] + v[k];
+                    }
+                    for (int a = 0; a < na; ++a) {
+                        double v[KH], cv[KH];
+                        FOR_K {
+                            v[k] = hslot(k) < Hp ? (double)st.str((size_t)a * Hp + hslot(k)) : 0.0;
+                            cv[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0;
+                        }
+                        FOR_K {
+                            const double sig = den[k] == 0.0 ? 1.0 / (double)na : v[k] / den[k];
+                            res[k] = a == 0 ? sig * cv[k] : res[k] + sig * cv[k];
+                        }
+                    }
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                } else if constexpr (sizeof(V) == 8) {
                     // FP64 best response: max over actions
                     double res[KH];
                     FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0;
@@ -709,6 +751,8 @@ __device__ __forceinline__ double root_value(Ctx& c, int p) {
 
 // Exploitability in chips of a single-tile game, utility.rs:380-392
 // (unraked): mean of the two best-response values, each f64 cast to f32.
+// A raked game subtracts the value of the average strategy from each best
+// response first: ((mes0 - cur0) + (mes1 - cur1)) * 0.5.
 template <int SM>
 __device__ __forceinline__ float exploitability(Ctx& c, const TileDesc& T) {
     float ev0 = 0.0f, ev1 = 0.0f;   // scalars: a per-player array would live in scratch
@@ -719,6 +763,15 @@ __device__ __forceinline__ float exploitability(Ctx& c, const TileDesc& T) {
         const float v = (float)root_value(c, p);
         if (p == 0) ev0 = v;
         else ev1 = v;
+    }
+    if (c.G.raked) {
+#pragma unroll 1
+        for (int p = 0; p < 2; ++p) {
+            tile_traverse<MODE_CFV, SM>(c, T, p, none);
+            const float v = (float)root_value(c, p);
+            if (p == 0) ev0 = __fsub_rn(ev0, v);
+            else ev1 = __fsub_rn(ev1, v);
+        }
     }
     return __fmul_rn(__fadd_rn(ev0, ev1), 0.5f);
 }
@@ -735,6 +788,15 @@ __device__ __forceinline__ double exploitability64(Ctx& c, const TileDesc& T) {
         const double v = root_value<double>(c, p);
         if (p == 0) ev0 = v;
         else ev1 = v;
+    }
+    if (c.G.raked) {
+#pragma unroll 1
+        for (int p = 0; p < 2; ++p) {
+            tile_traverse<MODE_CFV, SM, double>(c, T, p, none);
+            const double v = root_value<double>(c, p);
+            if (p == 0) ev0 -= v;
+            else ev1 -= v;
+        }
     }
     return (ev0 + ev1) * 0.5;
 }
@@ -846,12 +908,11 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
             } else {
                 tile_terminals<V>(c, T, p);
                 tile_backward<MODE, SM, V>(c, T, p, disc);
-                if (MODE == MODE_BR) {
+                if (MODE == MODE_BR || MODE == MODE_CFV) {
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
-                } else if (MODE == MODE_CFV) {
-                    store_root_cfvalues(c, T, p);
                 }
+                if (MODE == MODE_CFV && sizeof(V) == 4) store_root_cfvalues(c, T, p);
             }
             __syncthreads();
         }
@@ -909,6 +970,9 @@ static void launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
             break;
         case MODE_BR64:
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_BR, double>), dim3(grid), dim3(BLOCK), lds, stream, args);
+            break;
+        case MODE_CFV64:
+            hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_CFV, double>), dim3(grid), dim3(BLOCK), lds, stream, args);
             break;
         default:
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_CFV, float>), dim3(grid), dim3(BLOCK), lds, stream, args);

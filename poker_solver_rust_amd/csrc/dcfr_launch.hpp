@@ -22,7 +22,7 @@ enum Phase : int {
     PHASE_ROOT_BACKWARD = 2,  // root tile terminals + backward, one block per game
 };
 
-enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3 };
+enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3, MODE_CFV64 = 4 };
 
 // Storage of regrets and strategy sums.
 enum StorageMode : int {

@@ -11,7 +11,8 @@
 //                (level_ranges[level_base + L]); the roots of one chance
 //                node's tiles are contiguous so the chance node addresses
 //                them as its children.  A river game is a single tile.
-//   payoff[]     3 x f64 per node (win, lose, tie amount per combination)
+//   payoff[]     3 x f64 per node (win, lose, tie amount per combination; a
+//                nonzero tie amount selects the raked showdown)
 //   hands[]      per game and player: u16 card pair (c1 | c2 << 8), u16 index
 //                of the same hand in the opponent's list, f32 initial weight
 //   regret[], strategy[]   f32 state, per decision node an A x H_player block
@@ -103,6 +104,8 @@ struct GameDesc {
                             // ordinal b and player q, row (2b + q) of row_stride floats
     int32_t bnd_index;      // index of boundary 0 of the game in the per-boundary mode array
     int32_t num_boundaries;
+    int32_t raked;          // rake_rate > 0 and rake_cap > 0: raked exploitability
+    int32_t pad;
 };
 
 // How a depth-boundary leaf produces its values (single continuation).

@@ -51,6 +51,7 @@ struct HostLayout {
     // per game: decision nodes in arena order as (state offset, A*H, scale index)
     std::vector<std::vector<std::array<int64_t, 3>>> dec_blocks;
     int max_hands = 0;
+    bool any_raked = false;
     std::vector<int> single_games;   // whole-game work queue, most expensive first
     std::vector<int> multi_games;    // games solved by phase launches (turn)
     std::vector<std::pair<double, int>> subtree_cost;
@@ -263,7 +264,6 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
     for (int gi = 0; gi < n; ++gi) {
         const PostFlopGame& g = game_of(games[gi]);
         const TreeConfig& tc = g.tree_config();
-        if (tc.rake_rate > 0.0 && tc.rake_cap > 0.0) return fail("raked games are not supported by the device path");
         const auto& nodes = g.nodes();
         TilePlan plan;
         if (plan_tiles(g, plan) != 0) return -1;
@@ -281,6 +281,8 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         d.state_base = L.state_total;
         d.scratch_base = L.scratch_total;
         d.target = targets ? targets[gi] : 0.0f;
+        d.raked = tc.rake_rate > 0.0 && tc.rake_cap > 0.0;   // game/base.rs is_raked
+        if (d.raked) L.any_raked = true;
         d.num_boundaries = (int32_t)g.boundary_nodes().size();
         d.bnd_index = L.num_boundaries;
         d.bnd_base = L.bnd_rows_total;
@@ -601,14 +603,27 @@ struct psr_batch {
 
     // exploitability of every multi-tile game into host_expl (f32, as
     // utility.rs:380-392: (f32(ev0) + f32(ev1)) * 0.5)
-    int multi_exploitability(std::vector<float>& host_expl) {
+    // Raked games subtract the value of the average strategy from each best
+    // response first: ((mes0 - cur0) + (mes1 - cur1)) * 0.5.
+    int root_values(int mode, std::vector<double>& ev) {
         for (int p = 0; p < 2; ++p)
-            if (traverse(p, MODE_BR, 0)) return -1;
-        std::vector<double> ev(2 * L.games.size());
+            if (traverse(p, mode, 0)) return -1;
+        ev.assign(2 * L.games.size(), 0.0);
         HIP_OK(hipMemcpyAsync(ev.data(), d_ev, ev.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         HIP_OK(hipStreamSynchronize(stream));
+        return 0;
+    }
+    int multi_exploitability(std::vector<float>& host_expl) {
+        std::vector<double> ev, cur;
+        if (root_values(MODE_BR, ev)) return -1;
+        if (L.any_raked && root_values(MODE_CFV, cur)) return -1;
         for (int gi : L.multi_games) {
             volatile float e0 = (float)ev[2 * gi], e1 = (float)ev[2 * gi + 1];
+            if (L.games[gi].raked) {
+                volatile float c0 = (float)cur[2 * gi], c1 = (float)cur[2 * gi + 1];
+                e0 = e0 - c0;
+                e1 = e1 - c1;
+            }
             volatile float sum = e0 + e1;
             host_expl[gi] = sum * 0.5f;
         }
@@ -659,13 +674,16 @@ struct psr_batch {
                 if (multi_exploitability(expl)) return -1;
             }
             if (flags & JOB_EXPL64) {
-                for (int p = 0; p < 2; ++p)
-                    if (traverse(p, MODE_BR64, 0)) return -1;
-                std::vector<double> ev(2 * L.games.size());
-                HIP_OK(hipMemcpyAsync(ev.data(), d_ev, ev.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-                HIP_OK(hipStreamSynchronize(stream));
+                std::vector<double> ev, cur;
+                if (root_values(MODE_BR64, ev)) return -1;
+                if (L.any_raked && root_values(MODE_CFV64, cur)) return -1;
                 for (int gi : L.multi_games) {
-                    const double e = (ev[2 * gi] + ev[2 * gi + 1]) * 0.5;
+                    double e0 = ev[2 * gi], e1 = ev[2 * gi + 1];
+                    if (L.games[gi].raked) {
+                        e0 -= cur[2 * gi];
+                        e1 -= cur[2 * gi + 1];
+                    }
+                    const double e = (e0 + e1) * 0.5;
                     HIP_OK(hipMemcpy(d_expl64 + gi, &e, sizeof(double), hipMemcpyHostToDevice));
                 }
             }

@@ -223,12 +223,13 @@ def train_value_net(ts: TrainingSet, epochs: int = 30, batch: int = 512, lr: flo
 
 
 def river_mix(values: np.ndarray, r_opp: np.ndarray, rivers: Sequence[int]) -> np.ndarray:
-    """Combine per-river values (len(rivers), 1326) of one player into the
-    turn-boundary value: weights R_i(c) = opponent reach unblocked by hand i
-    and by c; combos holding c get no weight from that river."""
+    """Combine per-river values (len(rivers), ..., 1326) of one player into
+    the turn-boundary value (..., 1326): weights R_i(c) = opponent reach
+    (..., 1326) unblocked by hand i and by c; combos holding c get no weight
+    from that river."""
     c1, c2 = _pair_cards()
-    num = np.zeros(NUM_COMBOS)
-    den = np.zeros(NUM_COMBOS)
+    num = np.zeros(values.shape[1:])
+    den = np.zeros(values.shape[1:])
     for k, c in enumerate(rivers):
         w = unblocked_mass(r_opp, c)
         w = np.where((c1 == c) | (c2 == c), 0.0, w)
@@ -249,17 +250,34 @@ class NetLeaves:
         self.checkdown = checkdown
 
     def __call__(self, board4, reach, pot, stack):
-        if stack <= 0:
-            return self.checkdown(board4, reach, pot, stack)
+        return self.values_for([(board4, reach, pot, stack)])[0]
+
+    def values_for(self, items, max_rows: int = 8192):
+        """All boundaries of one turn board at once: one net batch over every
+        (boundary, river card) row, check-down bases in one vectorised pass."""
         rivers = self.checkdown.rivers
-        c1, c2 = _pair_cards()
-        r0 = np.stack([np.where((c1 == c) | (c2 == c), 0.0, reach[0]) for c in rivers])
-        r1 = np.stack([np.where((c1 == c) | (c2 == c), 0.0, reach[1]) for c in rivers])
-        out = self.net.forward(encode(r0, r1, np.array([list(board4) + [c] for c in rivers]), pot, stack))
-        base = self.checkdown.per_river(reach)
-        v0 = river_mix(base[0] + out[:, :NUM_COMBOS], reach[1], rivers)
-        v1 = river_mix(base[1] + out[:, NUM_COMBOS:], reach[0], rivers)
-        return v0, v1
+        K = len(rivers)
+        r0 = np.stack([np.asarray(it[1][0], np.float64) for it in items])
+        r1 = np.stack([np.asarray(it[1][1], np.float64) for it in items])
+        base = self.checkdown.per_river_rows(r0, r1)          # (2, K, B, 1326)
+        live = [k for k, it in enumerate(items) if it[3] > 0]
+        if live:
+            c1, c2 = _pair_cards()
+            held = np.stack([(c1 == c) | (c2 == c) for c in rivers])               # (K, 1326)
+            rows0 = np.where(held[None], 0.0, r0[live][:, None]).reshape(-1, NUM_COMBOS)
+            rows1 = np.where(held[None], 0.0, r1[live][:, None]).reshape(-1, NUM_COMBOS)
+            board4 = list(items[live[0]][0])
+            boards = np.array([board4 + [c] for c in rivers] * len(live))
+            pots = np.repeat([items[k][2] for k in live], K)
+            stacks = np.repeat([items[k][3] for k in live], K)
+            x = encode(rows0, rows1, boards, pots, stacks)
+            out = np.concatenate([self.net.forward(x[i:i + max_rows]) for i in range(0, len(x), max_rows)])
+            out = out.reshape(len(live), K, OUTPUT_DIM).transpose(1, 0, 2)      # (K, live, 2652)
+            base[0][:, live] += out[:, :, :NUM_COMBOS]
+            base[1][:, live] += out[:, :, NUM_COMBOS:]
+        m0 = river_mix(base[0], r1, rivers)
+        m1 = river_mix(base[1], r0, rivers)
+        return [(m0[k], m1[k]) for k in range(len(items))]
 
 
 class CheckdownLeaves:
@@ -317,19 +335,33 @@ class CheckdownLeaves:
         out = np.where(valid & (R > 0), (win - lose) / np.where(R > 0, R, 1.0), 0.0)
         return out
 
-    def per_river(self, reach: List[np.ndarray]):
-        """(2, 48, 1326): check-down values of both players on every river."""
+    def per_river_rows(self, r0: np.ndarray, r1: np.ndarray) -> np.ndarray:
+        """(2, K, B, 1326): check-down values of both players on every river
+        card for B boundaries at once (reach rows r0, r1: (B, 1326))."""
         c1, c2 = _pair_cards()
-        out = np.zeros((2, len(self.rivers), NUM_COMBOS))
+        r0 = np.atleast_2d(r0)
+        r1 = np.atleast_2d(r1)
+        out = np.zeros((2, len(self.rivers), r0.shape[0], NUM_COMBOS))
         for k, c in enumerate(self.rivers):
             held = (c1 == c) | (c2 == c)
-            out[0, k] = self.values(k, np.where(held, 0.0, reach[1]))[0]
-            out[1, k] = self.values(k, np.where(held, 0.0, reach[0]))[0]
+            out[0, k] = self.values(k, np.where(held, 0.0, r1))
+            out[1, k] = self.values(k, np.where(held, 0.0, r0))
         return out
 
+    def per_river(self, reach: List[np.ndarray]):
+        """(2, K, 1326): check-down values of both players on every river."""
+        return self.per_river_rows(reach[0], reach[1])[:, :, 0]
+
+    def values_for(self, items):
+        r0 = np.stack([np.asarray(it[1][0], np.float64) for it in items])
+        r1 = np.stack([np.asarray(it[1][1], np.float64) for it in items])
+        v = self.per_river_rows(r0, r1)
+        m0 = river_mix(v[0], r1, self.rivers)
+        m1 = river_mix(v[1], r0, self.rivers)
+        return [(m0[k], m1[k]) for k in range(len(items))]
+
     def __call__(self, board4, reach, pot, stack):
-        v = self.per_river(reach)
-        return river_mix(v[0], reach[1], self.rivers), river_mix(v[1], reach[0], self.rivers)
+        return self.values_for([(board4, reach, pot, stack)])[0]
 
 
 class ExactLeaves:

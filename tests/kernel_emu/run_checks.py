@@ -22,20 +22,20 @@ from poker_solver_rust_amd import workloads as W  # noqa: E402
 R._LIB = R.load_library(os.path.join(HERE, "libpsr_dcfr_emu.so"))
 
 
-def std_game(bets="50%, a", raises=""):
+def std_game(bets="50%, a", raises="", rake=(0.0, 0.0)):
     cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
                       flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
     tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
-                      river_bet_sizes=[(bets, raises), (bets, raises)])
+                      river_bet_sizes=[(bets, raises), (bets, raises)], rake_rate=rake[0], rake_cap=rake[1])
     return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
 
 
-def turn_game(oop="AA,KK,AKs", ip="QQ-JJ,AQs", board="Qs Jh 2c", turn="8d"):
+def turn_game(oop="AA,KK,AKs", ip="QQ-JJ,AQs", board="Qs Jh 2c", turn="8d", rake=(0.0, 0.0)):
     cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board),
                       turn=P.card_from_str(turn))
     s = [("50%, a", ""), ("50%, a", "")]
     tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200, turn_bet_sizes=s,
-                      river_bet_sizes=s)
+                      river_bet_sizes=s, rake_rate=rake[0], rake_cap=rake[1])
     return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
 
 
@@ -142,6 +142,39 @@ def run_fp64_exploitability():
         eo = o.exploitability()
         if abs(e64 - eo) > 1e-9 * max(1.0, abs(eo)):
             raise AssertionError(f"fp64 exploitability {e64} vs oracle {eo}")
+
+
+def check_fp64(g, iters):
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters)
+    b.run_iterations(0, iters)
+    e64 = float(b.exploitability_f64()[0])
+    o = O.OracleGame(g.serialize(), 64)
+    o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    eo = o.exploitability()
+    if abs(e64 - eo) > 1e-9 * max(1.0, abs(eo)):
+        raise AssertionError(f"fp64 exploitability {e64} vs oracle {eo}")
+
+
+def run_raked():
+    """Raked river (5% capped at 3 chips; the cap binds on the larger pots)
+    and raked turn games: raked showdown values, state after every
+    iteration, the raked exploitability (best response minus the value of the
+    average strategy) in f32 and f64, and the solve schedule."""
+    rake = (0.05, 3.0)
+    check_stepwise(std_game("50%, a", "", rake), 5)
+    check_stepwise(std_game("33%, 75%, a", "2x, a", (0.1, 100.0)), 3)
+    g = std_game("50%, a", "", rake)
+    check_fp64(g, 8)
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 40, [1.0])
+    e, it = b.solve(40, 5)
+    eo, ito = O.OracleGame(g.serialize()).solve(40, 1.0)
+    if int(it[0]) != ito or float(e[0]) != np.float32(eo):
+        raise AssertionError(f"raked solve: ({float(e[0])}, {int(it[0])}) vs oracle ({eo}, {ito})")
+    t = turn_game("AA,AKs", "KK,QQ", rake=rake)
+    check_stepwise(t, 2)
+    check_fp64(t, 2)
 
 
 def run_half_storage():

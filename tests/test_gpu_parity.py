@@ -315,3 +315,67 @@ def test_expected_values_match_oracle(make, iters):
         assert np.array_equal(b.normalized_weights(0, p), o.normalized_weights(p))
         close(b.expected_values_detail(0, p), o.expected_values(p, detail=True))
         close(b.expected_values(0, p), o.expected_values(p))
+
+
+# ----------------------------------------------------------------- raked --
+
+def raked_river(bets="50%, a", raises="", rake=(0.05, 3.0), seed=None):
+    if seed is not None:
+        s = W.river_batch(1, seed=seed, iterations=30)[0]
+        tc = s.tree_config()
+        tc.rake_rate, tc.rake_cap = rake
+        return P.PostFlopGame.with_config(s.card_config(), P.ActionTree(tc))
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
+    tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
+                      river_bet_sizes=[(bets, raises), (bets, raises)], rake_rate=rake[0], rake_cap=rake[1])
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def raked_turn(rake=(0.05, 3.0)):
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200, turn_bet_sizes=s,
+                      river_bet_sizes=s, rake_rate=rake[0], rake_cap=rake[1])
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+@pytest.mark.parametrize("make,iters", [(raked_river, 10), (lambda: raked_river("33%, 75%, a", "2x, a", (0.1, 1e9)), 6),
+                                        (raked_turn, 4)])
+def test_raked_stepwise_matches_oracle(make, iters):
+    """Raked showdown values (evaluation.rs:429-537) and the raked
+    exploitability (utility.rs:380-392: best response minus the value of the
+    average strategy), iteration by iteration; then the f64 best response."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters)
+    o = O.OracleGame(g.serialize())
+    for t in range(iters):
+        b.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, s = o.state()
+        close(b.regrets(0), r)
+        close(b.strategy_sum(0), s)
+    assert abs(float(b.exploitability()[0]) - o.exploitability()) <= RTOL * max(1.0, o.exploitability())
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+    close(cf[1], o.root_cfvalues(1))
+    o64 = O.OracleGame(g.serialize(), 64)
+    o64.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    e64 = float(b.exploitability_f64()[0])
+    assert abs(e64 - o64.exploitability()) <= 1e-9 * max(1.0, abs(o64.exploitability()))
+
+
+def test_raked_and_unraked_batch_solve():
+    """solve() schedule on a batch mixing raked and unraked river and turn
+    games: each game's iterations and exploitability as its own oracle's."""
+    games = [raked_river(), std_game(), raked_river(seed=11), raked_turn(), turn_game()]
+    b = P.GpuBatchSolver(0)
+    targets = [0.5, 0.5, 0.0, 0.5, 0.5]
+    b.prepare_batch(games, 40, targets)
+    e, it = b.solve(40, 5)
+    for i, g in enumerate(games):
+        eo, ito = O.OracleGame(g.serialize()).solve(40, targets[i])
+        assert int(it[i]) == ito, i
+        assert abs(float(e[i]) - eo) <= RTOL * max(1.0, eo), i

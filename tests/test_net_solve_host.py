@@ -83,3 +83,18 @@ def test_checkdown_values_brute_force():
             den += r[j]
             num += r[j] * np.sign(strength[i] - strength[j])
         assert abs(got[i] - (num / den if den > 0 else 0.0)) < 1e-9
+
+
+def test_checkdown_batched_equals_per_boundary():
+    """values_for over several boundaries at once equals one boundary at a
+    time (the vectorised path the depth-limited solve uses)."""
+    rng = np.random.default_rng(5)
+    board4 = [3, 17, 30, 44]
+    leaves = N.CheckdownLeaves(board4)
+    items = [(board4, [rand_reach(rng, board4), rand_reach(rng, board4)], 100.0 + 50 * k, 200.0) for k in range(3)]
+    batched = leaves.values_for(items)
+    for it, (v0, v1) in zip(items, batched):
+        rivers = leaves.rivers
+        per = np.stack([leaves.per_river_rows(it[1][0][None], it[1][1][None])[p, :, 0] for p in range(2)])
+        assert np.allclose(v0, N.river_mix(per[0], it[1][1], rivers))
+        assert np.allclose(v1, N.river_mix(per[1], it[1][0], rivers))

@@ -18,11 +18,11 @@ import poker_solver_rust_amd as P
 from poker_solver_rust_amd import workloads as W
 
 
-def std_game(bets="50%, a"):
+def std_game(bets="50%, a", rake=(0.0, 0.0)):
     cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-JJ,AQs,AJs")),
                       flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"), river=P.card_from_str("3s"))
     tc = P.TreeConfig(initial_state=P.BoardState.River, starting_pot=100, effective_stack=100,
-                      river_bet_sizes=[(bets, ""), (bets, "")])
+                      river_bet_sizes=[(bets, ""), (bets, "")], rake_rate=rake[0], rake_cap=rake[1])
     return cc, P.PostFlopGame.with_config(cc, P.ActionTree(tc))
 
 
@@ -65,13 +65,18 @@ def terminal_nodes(blob):
     return out
 
 
-def brute_force(game, board, node, player, reach, pot):
-    """O(H^2): sum over non-conflicting opponent hands of reach * payoff."""
+def brute_force(game, board, node, player, reach, pot, rake=(0.0, 0.0)):
+    """O(H^2): sum over non-conflicting opponent hands of reach * payoff.
+    With rake, the house takes min(pot * rate, cap) from the pot: the winner
+    of a fold or showdown nets half the pot minus the rake, and a tie splits
+    the raked pot, so each side nets -rake / 2."""
     pc, oc = game.private_cards(player), game.private_cards(player ^ 1)
     ncomb = game.num_combinations()
     half = 0.5 * (pot)
+    rk = min(pot * rake[0], rake[1])
+    win, lose, tie = (half - rk) / ncomb, -half / ncomb, -0.5 * rk / ncomb
     if node[1]:
-        pay = half / ncomb if node[2] != player else -half / ncomb
+        pay = win if node[2] != player else lose
     s_p = [P.hand_evaluate(board + [a, b]) for a, b in pc]
     s_o = [P.hand_evaluate(board + [a, b]) for a, b in oc]
     out = np.zeros(len(pc))
@@ -83,18 +88,24 @@ def brute_force(game, board, node, player, reach, pot):
             if node[1]:
                 acc += pay * reach[j]
             elif s_p[i] > s_o[j]:
-                acc += half / ncomb * reach[j]
+                acc += win * reach[j]
             elif s_p[i] < s_o[j]:
-                acc -= half / ncomb * reach[j]
+                acc += lose * reach[j]
+            else:
+                acc += tie * reach[j]
         out[i] = acc
     return out
 
 
-@pytest.mark.parametrize("which", ["std", "uniform"])
+RAKES = {"std": (0.0, 0.0), "uniform": (0.0, 0.0), "std_raked": (0.05, 3.0), "std_raked_uncapped": (0.1, 1e9)}
+
+
+@pytest.mark.parametrize("which", list(RAKES))
 def test_terminal_values_match_brute_force(which):
     rng = np.random.default_rng(7)
-    if which == "std":
-        cc, g = std_game()
+    rake = RAKES[which]
+    if which.startswith("std"):
+        cc, g = std_game(rake=rake)
         board = list(cc.flop) + [cc.turn, cc.river]
         pot0 = 100
     else:
@@ -111,7 +122,7 @@ def test_terminal_values_match_brute_force(which):
             reach = rng.random(g.num_private_hands(p ^ 1)).astype(np.float32).astype(np.float64)
             reach[rng.random(len(reach)) < 0.2] = 0.0
             got = og.evaluate(t[0], p, reach)
-            ref = brute_force(g, board, t, p, reach, pot0 + 2 * t[3])
+            ref = brute_force(g, board, t, p, reach, pot0 + 2 * t[3], rake)
             scale = np.abs(ref).max()
             assert np.abs(got - ref).max() <= 1e-6 * scale + 1e-12, (t, p)
 
@@ -135,16 +146,25 @@ def test_early_termination():
     assert e <= 1.0 and it < 1000 and it % 5 == 0
 
 
-def test_invariants_along_the_solve():
-    _, g = std_game("33%, 75%, a")
+@pytest.mark.parametrize("rake", [(0.0, 0.0), (0.05, 3.0)])
+def test_invariants_along_the_solve(rake):
+    _, g = std_game("33%, 75%, a", rake)
     og = O.OracleGame(g.serialize())
     for t in range(60):
         og.solve_step(t)
         if t % 7 == 0:
             mes, cur = og.mes_ev(), og.current_ev()
-            assert og.exploitability() >= -1e-5
+            e = og.exploitability()
+            assert e >= -1e-5
             assert np.all(mes >= cur - 1e-4)
-            assert abs(cur[0] + cur[1]) <= 1e-4 * max(1.0, np.abs(cur).max())
+            if rake[0] == 0.0:
+                assert abs(cur[0] + cur[1]) <= 1e-4 * max(1.0, np.abs(cur).max())
+                assert abs(e - 0.5 * (mes[0] + mes[1])) <= 1e-5 * max(1.0, abs(e))
+            else:
+                # the house takes chips: the two values sum to minus the
+                # expected rake, and exploitability is measured from them
+                assert cur[0] + cur[1] < 0.0
+                assert abs(e - 0.5 * ((mes[0] - cur[0]) + (mes[1] - cur[1]))) <= 1e-5 * max(1.0, abs(e))
 
 
 def test_complete_information_value():
