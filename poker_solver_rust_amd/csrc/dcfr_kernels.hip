@@ -316,39 +316,59 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     }
 }
 
-// Block-wide exclusive segmented scan in f64 of the staged reach values named
-// by the index list (SCAN_HEAD starts a segment, SCAN_ZERO adds 0).  Every
-// thread owns a contiguous run of at most SCAN_RUN entries (fully unrolled:
-// the index and reach loads of a run are issued together); runs are combined
-// with a wave scan of (segment-started, sum) pairs and a carry across the
-// four waves.
-constexpr int SCAN_RUN = 16;   // (3 * 1326 + 53) / BLOCK rounded up
+// Barrier for LDS traffic only: waits for this wave's LDS operations, not
+// for its global loads, so a prefetch stays in flight across it.  Global
+// writes that another wave reads need __syncthreads().
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A thread's run of the scan index list: SCAN_RUN u16 entries packed in
+// eight words, one aligned 32-byte load (RunoutTab::scan_runs).
+struct ScanRun {
+    uint32_t w[SCAN_RUN / 2];
+};
+static_assert(SCAN_THREADS == BLOCK, "one scan run per thread");
 static_assert(SCAN_RUN * BLOCK >= 3 * 1326 + 53, "scan runs must cover the longest index list");
 
+__device__ __forceinline__ ScanRun load_scan_run(const KernelArgs& A, const RunoutTab& RT) {
+    const uint4* src = reinterpret_cast<const uint4*>(A.u16pool + RT.scan_runs) + 2 * threadIdx.x;
+    const uint4 a = src[0], b = src[1];
+    return ScanRun{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
+}
+__device__ __forceinline__ uint16_t run_entry(const ScanRun& u, int j) {
+    return (uint16_t)(u.w[j >> 1] >> (16 * (j & 1)));
+}
+
+// Block-wide exclusive segmented scan in f64 of the staged reach values named
+// by the index list (SCAN_HEAD starts a segment, SCAN_ZERO adds 0).  Every
+// thread owns a contiguous run of at most SCAN_RUN entries (its index words
+// already in registers); runs are combined with a wave scan of
+// (segment-started, sum) pairs and a carry across the four waves.  Ends with
+// the scan output visible to the workgroup (LDS barriers only).
 template <typename V>
-__device__ __forceinline__ void segmented_scan(Ctx& c, const uint16_t* idx, int len) {
+__device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len) {
     const V* rs = static_cast<const V*>(c.rs);
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     const int C = (len + BLOCK - 1) / BLOCK;
     const int e0 = threadIdx.x * C;
-    uint16_t u[SCAN_RUN];
-    double x[SCAN_RUN];
-#pragma unroll
-    for (int j = 0; j < SCAN_RUN; ++j) u[j] = (j < C && e0 + j < len) ? idx[e0 + j] : (uint16_t)SCAN_ZERO;
+    V x[SCAN_RUN];   // row values, widened to f64 where summed
 #pragma unroll
     for (int j = 0; j < SCAN_RUN; ++j) {
-        const uint16_t k = u[j] & SCAN_ZERO;
-        x[j] = k == SCAN_ZERO ? 0.0 : (double)rs[k];
+        const uint16_t k = run_entry(u, j) & SCAN_ZERO;
+        x[j] = k == SCAN_ZERO ? V(0) : rs[k];
     }
     bool f = false;
     double s = 0.0;
 #pragma unroll
     for (int j = 0; j < SCAN_RUN; ++j) {
-        if (u[j] & SCAN_HEAD) {
+        if (run_entry(u, j) & SCAN_HEAD) {
             f = true;
-            s = x[j];
+            s = (double)x[j];
         } else {
-            s += x[j];
+            s += (double)x[j];
         }
     }
 #pragma unroll
@@ -370,103 +390,132 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const uint16_t* idx, int 
         c.sh->wave_f[wave] = f;
         c.sh->wave_s[wave] = s;
     }
-    __syncthreads();
+    lds_barrier();
     double cs = 0.0;
     for (int w = 0; w < wave; ++w) cs = c.sh->wave_f[w] ? c.sh->wave_s[w] : cs + c.sh->wave_s[w];
     double run = fe ? se : cs + se;
 #pragma unroll
     for (int j = 0; j < SCAN_RUN; ++j) {
         if (j < C && e0 + j < len) {
-            if (u[j] & SCAN_HEAD) run = 0.0;
+            if (run_entry(u, j) & SCAN_HEAD) run = 0.0;
             c.scan[e0 + j] = run;
-            run += x[j];
+            run += (double)x[j];
         }
     }
-    __syncthreads();
+    lds_barrier();
+}
+
+// The runout tables a terminal's evaluation reads, per thread: its scan run,
+// the scan positions and same-hand index of each of its traversing hands.
+// Every terminal of a tile usually shares one runout, so the terminal loop
+// loads them once per traversal.
+// HOLD_HAND_SCAN: keep the per-hand scan positions in registers across the
+// terminals (24 VGPRs) instead of loading them per terminal (an L2 hit,
+// issued before the scan so it overlaps it).
+#ifndef PSR_HOLD_HAND_SCAN
+#define PSR_HOLD_HAND_SCAN 1
+#endif
+constexpr bool HOLD_HAND_SCAN = PSR_HOLD_HAND_SCAN;
+
+struct RunoutRegs {
+    RunoutTab T;
+    ScanRun u;
+    HandScan q[KH];   // filled only when HOLD_HAND_SCAN
+    uint16_t si[KH];
+};
+
+__device__ __forceinline__ void load_hand_scan(const Ctx& c, const RunoutTab& T, int p, HandScan (&q)[KH]) {
+    const int Hp = num_hands(c.G, p);
+    const HandScan* hs = c.A.hand_scan + T.hand_scan;
+    FOR_K q[k] = hslot(k) < Hp ? hs[hslot(k)] : HandScan{0xFFFFFFFFu, 0, 0, 0};
+}
+
+__device__ __forceinline__ void load_runout(const Ctx& c, int table, int p, RunoutRegs& R) {
+    const KernelArgs& A = c.A;
+    const int Hp = num_hands(c.G, p);
+    R.T = uni_load(A.runouts + table * 2 + p);
+    R.u = load_scan_run(A, R.T);
+    if constexpr (HOLD_HAND_SCAN) load_hand_scan(c, R.T, p, R.q);
+    const uint16_t* same = A.hand_same + hand_base(c.G, p);
+    FOR_K R.si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
 }
 
 // Fold / showdown values of one terminal for traversing player p
-// (game/evaluation.rs:293-425), and depth-boundary leaves
-// (evaluation.rs:87-120, 544-630), whole workgroup.
+// (game/evaluation.rs:293-537), and depth-boundary leaves
+// (evaluation.rs:87-120, 544-630), whole workgroup.  rv holds the opponent
+// reach row of the terminal (slot k = hand hslot(k)); it is staged in LDS,
+// scanned, and the results are written to the terminal's cfv row.  Only LDS
+// barriers inside: the caller publishes the cfv rows with __syncthreads().
 template <typename V>
-__device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p) {
+__device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p, const RunoutRegs& R,
+                                              const V (&rv)[KH]) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p), Ho = num_hands(c.G, p ^ 1);
-    const RunoutTab T = uni_load(A.runouts + nd.aux * 2 + p);
-    const V* r = reach_rows<V>(c) + (size_t)n * c.stride;
     V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
     V* rs = static_cast<V*>(c.rs);
-    {
-        V v[KH];
-        FOR_K v[k] = hslot(k) < Ho ? r[hslot(k)] : V(0);
-        FOR_K if (hslot(k) < Ho) rs[hslot(k)] = v[k];
+    const double win = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]);
+    const double lose = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]);
+    const double tie = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]);
+    HandScan qv[KH];
+    if constexpr (HOLD_HAND_SCAN) {
+        FOR_K qv[k] = R.q[k];
+    } else {
+        load_hand_scan(c, R.T, p, qv);
     }
-    __syncthreads();
-    segmented_scan<V>(c, A.u16pool + T.scan_idx, T.scan_len);
+    lds_barrier();   // the previous terminal's reads of rs and scan are done
+    FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
+    lds_barrier();
+    segmented_scan<V>(c, R.u, R.T.scan_len);
 
-    const HandScan* hs = A.hand_scan + T.hand_scan;
-    const double win = A.payoff[(size_t)3 * (c.G.node_base + n) + 0];
-    const double lose = A.payoff[(size_t)3 * (c.G.node_base + n) + 1];
-    const double total = c.scan[T.num_valid_opp];
-    HandScan q[KH];
-    FOR_K q[k] = hslot(k) < Hp ? hs[hslot(k)] : HandScan{0xFFFFFFFFu, 0, 0, 0};
+    const double total = c.scan[R.T.num_valid_opp];
     if (nd.kind == KIND_BOUNDARY) {
         // record the opponent reach of this visit, then the leaf value of
         // every hand; win holds half_pot / num_combinations
         const int b = nd.state_off;
         float* rec = A.bnd_reach + c.G.bnd_base + (size_t)(2 * b + (p ^ 1)) * c.stride;
-        FOR_K if (hslot(k) < Ho) rec[hslot(k)] = (float)rs[hslot(k)];
+        FOR_K if (hslot(k) < Ho) rec[hslot(k)] = (float)rv[k];
         const int mode = uni(A.bnd_mode[c.G.bnd_index + b]);
         const float* bv = A.bnd_cfv + c.G.bnd_base + (size_t)(2 * b + p) * c.stride;
-        const uint16_t* same = A.hand_same + hand_base(c.G, p);
         float val[KH];
-        uint16_t si[KH];
-        FOR_K {
-            val[k] = hslot(k) < Hp ? bv[hslot(k)] : 0.0f;
-            si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
-        }
+        FOR_K val[k] = hslot(k) < Hp ? bv[hslot(k)] : 0.0f;
         FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            const bool valid = (q[k].x & 0xFFFF) != 0xFFFF;
+            const HandScan& q = qv[k];
+            const bool valid = (q.x & 0xFFFF) != 0xFFFF;
             if (mode == BND_RAW && valid) {
                 res = (V)val[k];
             } else if (mode == BND_SCALED && valid && total != 0.0) {
-                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
-                const double cr = ((total + sm) - c.scan[q[k].w & 0xFFFF]) - c.scan[q[k].w >> 16];
+                const double sm = R.si[k] == 0xFFFF ? 0.0 : (double)rs[R.si[k]];
+                const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
                 res = (V)(((double)val[k] * win) * cr);
             }
             out[hslot(k)] = res;
         }
     } else if (nd.kind == KIND_FOLD) {
         const double payoff = (int)nd.player != p ? win : lose;
-        const uint16_t* same = A.hand_same + hand_base(c.G, p);
-        uint16_t si[KH];
-        FOR_K si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
         FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            if (total != 0.0 && (q[k].x & 0xFFFF) != 0xFFFF) {
-                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
-                const double cr = ((total + sm) - c.scan[q[k].w & 0xFFFF]) - c.scan[q[k].w >> 16];
+            const HandScan& q = qv[k];
+            if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
+                const double sm = R.si[k] == 0xFFFF ? 0.0 : (double)rs[R.si[k]];
+                const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
                 res = (V)(payoff * cr);
             }
             out[hslot(k)] = res;
         }
-    } else if (uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]) != 0.0) {
+    } else if (tie != 0.0) {
         // raked showdown (evaluation.rs:429-537): blocked opponent mass that
         // is weaker (win), not stronger (tie, cumulative) and in total, the
         // identical hand added back to the tie band
-        const double tie = A.payoff[(size_t)3 * (c.G.node_base + n) + 2];
-        const uint16_t* same = A.hand_same + hand_base(c.G, p);
-        uint16_t si[KH];
-        FOR_K si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
         FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            if (total != 0.0 && (q[k].x & 0xFFFF) != 0xFFFF) {
+            const HandScan& q = qv[k];
+            if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
                 const double* S = c.scan;
-                const double sm = si[k] == 0xFFFF ? 0.0 : (double)rs[si[k]];
-                const double tw = (total - S[q[k].w & 0xFFFF]) - S[q[k].w >> 16];
-                const double cw = (S[q[k].x & 0xFFFF] - S[q[k].y & 0xFFFF]) - S[q[k].z & 0xFFFF];
-                const double ct = (S[q[k].x >> 16] - S[q[k].y >> 16]) - S[q[k].z >> 16];
+                const double sm = R.si[k] == 0xFFFF ? 0.0 : (double)rs[R.si[k]];
+                const double tw = (total - S[q.w & 0xFFFF]) - S[q.w >> 16];
+                const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
+                const double ct = (S[q.x >> 16] - S[q.y >> 16]) - S[q.z >> 16];
                 res = (V)((win * cw + tie * ((ct - cw) + sm)) + lose * (tw - ct));
             }
             out[hslot(k)] = res;
@@ -474,18 +523,18 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     } else {
         FOR_K if (hslot(k) < Hp) {
             V res = 0;
-            if ((q[k].x & 0xFFFF) != 0xFFFF) {
+            const HandScan& q = qv[k];
+            if ((q.x & 0xFFFF) != 0xFFFF) {
                 const double* S = c.scan;
-                const double cw = (S[q[k].x & 0xFFFF] - S[q[k].y & 0xFFFF]) - S[q[k].z & 0xFFFF];
-                const double cl = ((total - S[q[k].x >> 16]) - (S[q[k].w & 0xFFFF] - S[q[k].y >> 16])) -
-                                  (S[q[k].w >> 16] - S[q[k].z >> 16]);
+                const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
+                const double cl = ((total - S[q.x >> 16]) - (S[q.w & 0xFFFF] - S[q.y >> 16])) -
+                                  (S[q.w >> 16] - S[q.z >> 16]);
                 if constexpr (sizeof(V) == 8) res = win * cw + lose * cl;
                 else res = __fadd_rn((float)(win * cw), (float)(lose * cl));
             }
             out[hslot(k)] = res;
         }
     }
-    __syncthreads();
 }
 
 // 16-bit storage: the traverser's update computes the new f32 values first
@@ -708,14 +757,40 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
     }
 }
 
+// Every terminal of a tile, in list order.  The reach row of the next
+// terminal is loaded while the current one is scanned (the barriers inside
+// are LDS-only, so the loads stay in flight); the runout tables stay in
+// registers while consecutive terminals share them (always, when the tile
+// has a single runout).  One __syncthreads() at the end publishes the cfv
+// rows to the backward pass.
 template <typename V = float>
 __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p) {
     const int32_t* terms = c.A.terminals + T.term_base;
-    for (int k = 0; k < T.num_terminals; ++k) {
-        const int n = uni(terms[k]);
-        const NodeRec nd = uni_load(c.nodes + n);
-        eval_terminal<V>(c, nd, n, p);
+    const int Ho = num_hands(c.G, p ^ 1);
+    const V* reach = reach_rows<V>(c);
+    if (T.num_terminals == 0) return;
+    int n = uni(terms[0]);
+    NodeRec nd = uni_load(c.nodes + n);
+    RunoutRegs R;
+    int table = nd.aux;
+    load_runout(c, table, p, R);
+    V rv[KH];
+    FOR_K rv[k] = hslot(k) < Ho ? reach[(size_t)n * c.stride + hslot(k)] : V(0);
+    for (int t = 0; t < T.num_terminals; ++t) {
+        const int nn = t + 1 < T.num_terminals ? uni(terms[t + 1]) : n;
+        const NodeRec nnd = uni_load(c.nodes + nn);
+        V rn[KH];
+        FOR_K rn[k] = hslot(k) < Ho ? reach[(size_t)nn * c.stride + hslot(k)] : V(0);
+        eval_terminal<V>(c, nd, n, p, R, rv);
+        if (nnd.aux != table) {
+            table = nnd.aux;
+            load_runout(c, table, p, R);
+        }
+        FOR_K rv[k] = rn[k];
+        n = nn;
+        nd = nnd;
     }
+    __syncthreads();
 }
 
 template <int MODE, int SM, typename V = float>
@@ -839,7 +914,9 @@ __device__ __forceinline__ void store_root_cfvalues(Ctx& c, const TileDesc& T, i
 
 // Whole-game solve of single-tile games (river subgames): one workgroup runs
 // every iteration of one game, games come from the work queue.
-template <int SM>
+// F64 = true is the FP64 best-response job alone (a kernel of its own, so
+// that its f64 rows do not raise the register count of the solve).
+template <int SM, bool F64>
 __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
     KERNEL_SETUP();
     for (;;) {
@@ -852,6 +929,12 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
         Ctx c = make_ctx(A, gi, scan, rs, &sh);
         const TileDesc T = uni_load(A.tiles + c.G.tile_base);
 
+        if constexpr (F64) {
+            const double e = exploitability64<SM>(c, T);
+            if (threadIdx.x == 0) A.expl64_out[gi] = e;
+            __syncthreads();
+            continue;
+        }
         float expl = __int_as_float(0x7f800000);   // +inf: no check done yet
         if (A.flags & JOB_INITIAL_CHECK) expl = exploitability<SM>(c, T);
         uint32_t t = A.iter_begin;
@@ -873,10 +956,6 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
                 tile_traverse<MODE_CFV, SM>(c, T, p, none);
                 store_root_cfvalues(c, T, p);
             }
-        }
-        if (A.flags & JOB_EXPL64) {
-            const double e = exploitability64<SM>(c, T);
-            if (threadIdx.x == 0) A.expl64_out[gi] = e;
         }
         if (threadIdx.x == 0) {
             A.expl_out[gi] = expl;
@@ -954,9 +1033,19 @@ size_t dcfr_lds_bytes(int max_hands) {
     return sizeof(double) * (3 * (size_t)max_hands + 53) + sizeof(double) * (size_t)max_hands;
 }
 
+// The FP64 best response runs as a kernel of its own after the other jobs
+// (each launch takes games from a freshly reset work queue).
 template <int SM>
 static void launch_solve(const KernelArgs& args, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(dcfr_solve_kernel<SM>, dim3(grid), dim3(BLOCK), dcfr_lds_bytes(args.max_hands), stream, args);
+    const size_t lds = dcfr_lds_bytes(args.max_hands);
+    if (args.flags & ~JOB_EXPL64) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, args.queue_counter);
+        hipLaunchKernelGGL((dcfr_solve_kernel<SM, false>), dim3(grid), dim3(BLOCK), lds, stream, args);
+    }
+    if (args.flags & JOB_EXPL64) {
+        hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, args.queue_counter);
+        hipLaunchKernelGGL((dcfr_solve_kernel<SM, true>), dim3(grid), dim3(BLOCK), lds, stream, args);
+    }
 }
 template <int SM>
 static void launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
@@ -981,7 +1070,6 @@ static void launch_phase(const KernelArgs& args, int grid, hipStream_t stream) {
 }
 
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, args.queue_counter);
     switch (args.storage) {
         case SM_F32: launch_solve<SM_F32>(args, grid, stream); break;
         case SM_REF16: launch_solve<SM_REF16>(args, grid, stream); break;
@@ -1004,7 +1092,7 @@ hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t strea
 }
 
 hipError_t dcfr_occupancy(int max_hands, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_REF16>, BLOCK,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_REF16, false>, BLOCK,
                                                         dcfr_lds_bytes(max_hands));
 }
 

@@ -60,11 +60,21 @@ struct HandScan {
     uint32_t x, y, z, w;
 };
 
+// The scan is carried out by SCAN_THREADS threads, each owning a run of at
+// most SCAN_RUN consecutive list entries: thread t owns entries
+// [t * C, t * C + C) with C = ceil(scan_len / SCAN_THREADS).
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_RUN = 16;   // (3 * 1326 + 53) / SCAN_THREADS rounded up
+
 struct RunoutTab {
     int32_t scan_idx;       // u16[scan_len] index list (u16 pool)
     int32_t scan_len;       // 3 * num_valid_opp + 53
     int32_t hand_scan;      // HandScan[H_player] (HandScan pool)
     int32_t num_valid_opp;  // entries of segment 0 before its closing slot
+    int32_t scan_runs;      // u16[SCAN_THREADS * SCAN_RUN], 16-byte aligned: thread t's run
+                            // at [t * SCAN_RUN], padded with SCAN_ZERO (one aligned 32-byte
+                            // load per thread instead of SCAN_RUN scattered ones)
+    int32_t pad[3];
 };
 
 struct ChanceTab {          // chance node: reach scale and isomorphic-card swaps
@@ -87,7 +97,8 @@ struct TileDesc {
     int32_t term_base;      // into terminals (game-local device node ids)
     int32_t num_terminals;
     int32_t flags;          // TILE_GAME_ROOT: level 0 is the game root
-    int32_t pad[2];
+    int32_t runout;         // runout table id shared by every terminal of the tile, or -1
+    int32_t pad;
 };
 
 struct GameDesc {

@@ -155,6 +155,13 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     }
     t.scan_idx = push_u16(L.u16, idx);
     t.scan_len = (int32_t)idx.size();
+    {
+        const int C = (t.scan_len + SCAN_THREADS - 1) / SCAN_THREADS;
+        std::vector<uint16_t> runs((size_t)SCAN_THREADS * SCAN_RUN, SCAN_ZERO);
+        for (int th = 0; th < SCAN_THREADS; ++th)
+            for (int j = 0; j < C && th * C + j < t.scan_len; ++j) runs[(size_t)th * SCAN_RUN + j] = idx[th * C + j];
+        t.scan_runs = push_u16(L.u16, runs, 8);
+    }
     t.num_valid_opp = (int32_t)sorted.size();
     return t;
 }
@@ -394,6 +401,13 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             td.num_terminals = (int32_t)plan.terminals[t].size();
             for (int v : plan.terminals[t]) L.terminals.push_back(v);
             td.flags = t == 0 ? TILE_GAME_ROOT : 0;
+            td.runout = -1;
+            for (size_t k = 0; k < plan.terminals[t].size(); ++k) {
+                const int aux = L.nodes[d.node_base + plan.terminals[t][k]].aux;
+                if (k == 0) td.runout = aux;
+                else if (aux != td.runout) td.runout = -1;
+                if (td.runout < 0) break;
+            }
             if (t > 0) L.subtree_cost.push_back({plan.cost[t] * std::max(d.H[0], d.H[1]), (int)L.tiles.size()});
             L.tiles.push_back(td);
         }
@@ -1195,6 +1209,8 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
             const uint8_t kind = L.nodes[d.node_base + v].kind;
             if (kind != KIND_FOLD && kind != KIND_SHOWDOWN && kind != KIND_BOUNDARY)
                 return fail("terminal list holds a non-terminal");
+            if (td.runout >= 0 && L.nodes[d.node_base + v].aux != td.runout)
+                return fail("tile runout differs from a terminal's");
         }
         terms += td.num_terminals;
     }
@@ -1226,6 +1242,18 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
         const int p = (int)(k & 1), o = p ^ 1;
         if (t.scan_idx < 0 || t.scan_idx + t.scan_len > (int)L.u16.size()) return fail("scan list out of range");
         if (t.scan_len != 3 * t.num_valid_opp + 53) return fail("scan length");
+        if (t.scan_len > SCAN_THREADS * SCAN_RUN) return fail("scan list longer than the scan threads cover");
+        if (t.scan_runs < 0 || t.scan_runs % 8 || t.scan_runs + SCAN_THREADS * SCAN_RUN > (int)L.u16.size())
+            return fail("scan runs out of range or misaligned");
+        {
+            const int C = (t.scan_len + SCAN_THREADS - 1) / SCAN_THREADS;
+            for (int th = 0; th < SCAN_THREADS; ++th)
+                for (int j = 0; j < SCAN_RUN; ++j) {
+                    const bool live = j < C && th * C + j < t.scan_len;
+                    const uint16_t want = live ? L.u16[t.scan_idx + th * C + j] : SCAN_ZERO;
+                    if (L.u16[t.scan_runs + th * SCAN_RUN + j] != want) return fail("scan runs differ from the list");
+                }
+        }
         for (int e = 0; e < t.scan_len; ++e) {
             const uint16_t u = L.u16[t.scan_idx + e];
             const int idx = u & SCAN_ZERO;

@@ -28,6 +28,9 @@ struct dim3 {
     unsigned x = 1, y = 1, z = 1;
     dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
 };
+struct uint4 {
+    unsigned x, y, z, w;
+};
 struct float4 {
     float x, y, z, w;
 };
@@ -213,6 +216,10 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
 #define blockDim (emu::cur_block_dim)
 #define gridDim (emu::cur_grid)
 #define __syncthreads() emu::syncthreads()
+// the LDS-only barrier of the kernels: the fence is a no-op on the CPU (the
+// fibers share one memory), the barrier is the workgroup barrier
+#define __builtin_amdgcn_fence(...) ((void)0)
+#define __builtin_amdgcn_s_barrier() emu::syncthreads()
 #define hipLaunchKernelGGL(k, g, b, s, st, ...) emu::launch(k, dim3(g), dim3(b), (size_t)(s), __VA_ARGS__)
 
 template <class T>
