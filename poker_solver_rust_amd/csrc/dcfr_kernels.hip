@@ -33,6 +33,7 @@ constexpr int BLOCK = 256;
 constexpr int WAVE = 64;
 constexpr int NWAVES = BLOCK / WAVE;
 constexpr int MAXA = 16;   // largest action count of a decision node on the device
+constexpr int NA_REG = 4;  // nodes with up to this many actions keep their rows in registers
 // Hands per thread: a row of H <= 1326 hands is covered by KH strided slots
 // h = threadIdx.x + k * BLOCK.  The per-hand loops are unrolled over k and
 // split into a load phase and a compute/store phase, so every thread has up
@@ -176,6 +177,24 @@ struct NodeState {
     }
 };
 
+// The state of action a of a node: every pointer advanced by a * H (a
+// uniform offset, so the per-lane part of an address stays a 32-bit hand
+// index and loads use a scalar base).
+template <int SM>
+__device__ __forceinline__ NodeState<SM> action_state(const NodeState<SM>& st, int a, int H) {
+    NodeState<SM> s = st;
+    const size_t d = (size_t)a * (size_t)H;
+    if (SM == SM_F32) {
+        s.R += d;
+        s.S += d;
+    } else {
+        s.R16 += d;
+        s.S16 += d;
+    }
+    return s;
+}
+__device__ __forceinline__ unsigned uslot(int k) { return threadIdx.x + (unsigned)k * BLOCK; }
+
 template <int SM>
 __device__ __forceinline__ NodeState<SM> node_state(const KernelArgs& A, const GameDesc& G, const NodeRec& nd) {
     NodeState<SM> v{};
@@ -277,6 +296,30 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                             rc[(size_t)a * c.stride + hslot(k)] = sig * r[k];
                         }
                     }
+                } else if (na <= NA_REG) {
+                    // regret matching / average strategy, solver.rs:600-626,
+                    // utility.rs:229-245: the action rows are loaded once,
+                    // all in flight together; denominator, then sigma
+                    const float def = __fdiv_rn(1.0f, (float)na);
+                    float v[NA_REG][KH], den[KH];
+#pragma unroll
+                    for (int a = 0; a < NA_REG; ++a)
+                        if (a < na) {
+                            const NodeState<SM> sa = action_state(st, a, Ho);
+                            FOR_K v[a][k] = hslot(k) < Ho ? node_value<MODE>(sa, uslot(k)) : 0.0f;
+                        }
+#pragma unroll
+                    for (int a = 0; a < NA_REG; ++a)
+                        if (a < na) FOR_K {
+                            const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
+                            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+                        }
+#pragma unroll
+                    for (int a = 0; a < NA_REG; ++a)
+                        if (a < na) {
+                            V* ra = rc + (size_t)a * c.stride;
+                            FOR_K if (hslot(k) < Ho) ra[uslot(k)] = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), r[k]);
+                        }
                 } else {
                     // regret matching / average strategy, solver.rs:600-626,
                     // utility.rs:229-245: denominator over actions, then sigma
@@ -405,6 +448,11 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len
     lds_barrier();
 }
 
+// Keeps the scheduler from hoisting the LDS reads of every hand slot of a
+// terminal ahead of the first slot's arithmetic: one slot's reads in flight
+// at a time (LDS latency is short) instead of 6 x 6 f64 values.
+__device__ __forceinline__ void slot_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 // The runout tables a terminal's evaluation reads, per thread: its scan run,
 // the scan positions and same-hand index of each of its traversing hands.
 // Every terminal of a tile usually shares one runout, so the terminal loop
@@ -456,16 +504,16 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     const double win = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]);
     const double lose = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]);
     const double tie = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]);
+    lds_barrier();   // the previous terminal's reads of rs and scan are done
+    FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
+    lds_barrier();
+    segmented_scan<V>(c, R.u, R.T.scan_len);
     HandScan qv[KH];
     if constexpr (HOLD_HAND_SCAN) {
         FOR_K qv[k] = R.q[k];
     } else {
         load_hand_scan(c, R.T, p, qv);
     }
-    lds_barrier();   // the previous terminal's reads of rs and scan are done
-    FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
-    lds_barrier();
-    segmented_scan<V>(c, R.u, R.T.scan_len);
 
     const double total = c.scan[R.T.num_valid_opp];
     if (nd.kind == KIND_BOUNDARY) {
@@ -478,7 +526,8 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
         const float* bv = A.bnd_cfv + c.G.bnd_base + (size_t)(2 * b + p) * c.stride;
         float val[KH];
         FOR_K val[k] = hslot(k) < Hp ? bv[hslot(k)] : 0.0f;
-        FOR_K if (hslot(k) < Hp) {
+        FOR_K {
+            if (hslot(k) < Hp) {
             V res = 0;
             const HandScan& q = qv[k];
             const bool valid = (q.x & 0xFFFF) != 0xFFFF;
@@ -490,10 +539,13 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 res = (V)(((double)val[k] * win) * cr);
             }
             out[hslot(k)] = res;
+            }
+            slot_fence();
         }
     } else if (nd.kind == KIND_FOLD) {
         const double payoff = (int)nd.player != p ? win : lose;
-        FOR_K if (hslot(k) < Hp) {
+        FOR_K {
+            if (hslot(k) < Hp) {
             V res = 0;
             const HandScan& q = qv[k];
             if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
@@ -502,12 +554,15 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 res = (V)(payoff * cr);
             }
             out[hslot(k)] = res;
+            }
+            slot_fence();
         }
     } else if (tie != 0.0) {
         // raked showdown (evaluation.rs:429-537): blocked opponent mass that
         // is weaker (win), not stronger (tie, cumulative) and in total, the
         // identical hand added back to the tie band
-        FOR_K if (hslot(k) < Hp) {
+        FOR_K {
+            if (hslot(k) < Hp) {
             V res = 0;
             const HandScan& q = qv[k];
             if (total != 0.0 && (q.x & 0xFFFF) != 0xFFFF) {
@@ -519,9 +574,12 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 res = (V)((win * cw + tie * ((ct - cw) + sm)) + lose * (tw - ct));
             }
             out[hslot(k)] = res;
+            }
+            slot_fence();
         }
     } else {
-        FOR_K if (hslot(k) < Hp) {
+        FOR_K {
+            if (hslot(k) < Hp) {
             V res = 0;
             const HandScan& q = qv[k];
             if ((q.x & 0xFFFF) != 0xFFFF) {
@@ -533,6 +591,8 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 else res = __fadd_rn((float)(win * cw), (float)(lose * cl));
             }
             out[hslot(k)] = res;
+            }
+            slot_fence();
         }
     }
 }
@@ -605,6 +665,70 @@ __device__ __forceinline__ void update_16bit(Ctx& c, const NodeRec& nd, const No
 }
 
 // Backward pass over one tile, deepest level first.
+// Traverser node, f32 rows, at most NA_REG actions, hand slots [K0, K1):
+// sigma of every action (denominator first), the node value
+// sum_a sigma_a * cfv_a (solver.rs:334-383), and in DCFR mode the updates
+// S = S * gamma + sigma, R = (R * (alpha | beta) + cfv_a) - value
+// (solver.rs:556-589).  Every row of the slots is loaded once, all in flight
+// together.
+template <int MODE, int SM, int K0, int K1>
+__device__ __forceinline__ void traverser_rows(const NodeState<SM>& st, const float* cc, int stride, float* out, int Hp,
+                                               int na, float def, const float4 disc) {
+    constexpr int NK = K1 - K0;
+    float v[NA_REG][NK], cv[NA_REG][NK], sv[NA_REG][NK];
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) {
+            const NodeState<SM> sa = action_state(st, a, Hp);
+            const float* ca = cc + (size_t)a * stride;
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const bool live = hslot(K0 + j) < Hp;
+                v[a][j] = live ? node_value<MODE>(sa, uslot(K0 + j)) : 0.0f;
+                cv[a][j] = live ? ca[uslot(K0 + j)] : 0.0f;
+                if (MODE == MODE_DCFR) sv[a][j] = live ? sa.S[uslot(K0 + j)] : 0.0f;
+            }
+        }
+    float den[NK], res[NK];
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) {
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const float x = MODE == MODE_DCFR ? fmax0(v[a][j]) : v[a][j];
+                den[j] = a == 0 ? x : __fadd_rn(den[j], x);
+            }
+        }
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) {
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const float t = __fmul_rn(strat_of<MODE>(v[a][j], den[j], def), cv[a][j]);
+                res[j] = a == 0 ? t : __fadd_rn(res[j], t);
+            }
+        }
+    if (MODE == MODE_DCFR) {
+#pragma unroll
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < na) {
+                const NodeState<SM> sa = action_state(st, a, Hp);
+#pragma unroll
+                for (int j = 0; j < NK; ++j)
+                    if (hslot(K0 + j) < Hp) {
+                        const unsigned h = uslot(K0 + j);
+                        const float sig = strat_of<MODE>(v[a][j], den[j], def);
+                        sa.S[h] = __fadd_rn(__fmul_rn(sv[a][j], disc.z), sig);
+                        const float coef = signbit(v[a][j]) ? disc.y : disc.x;
+                        sa.R[h] = __fsub_rn(__fadd_rn(__fmul_rn(v[a][j], coef), cv[a][j]), res[j]);
+                    }
+            }
+    }
+#pragma unroll
+    for (int j = 0; j < NK; ++j)
+        if (hslot(K0 + j) < Hp) out[uslot(K0 + j)] = res[j];
+}
+
 template <int MODE, int SM, typename V = float>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
@@ -673,6 +797,28 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                         FOR_K res[k] = v[k] > res[k] ? v[k] : res[k];
                     }
                     FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                } else if (MODE == MODE_BR && na <= NA_REG) {
+                    // best response: max over the children, all rows in flight together
+                    float cv[NA_REG][KH];
+#pragma unroll
+                    for (int a = 0; a < NA_REG; ++a)
+                        if (a < na) {
+                            const V* ca = cc + (size_t)a * c.stride;
+                            FOR_K cv[a][k] = hslot(k) < Hp ? ca[uslot(k)] : 0.0f;
+                        }
+                    float res[KH];
+                    FOR_K res[k] = cv[0][k];
+#pragma unroll
+                    for (int a = 1; a < NA_REG; ++a)
+                        if (a < na) FOR_K res[k] = fmaxr(res[k], cv[a][k]);
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                } else if (MODE != MODE_BR && na <= NA_REG && !(MODE == MODE_DCFR && SM != SM_F32)) {
+                    // every row of the node loaded once, all in flight
+                    // together, in two halves of the hand slots (registers)
+                    const NodeState<SM> st = node_state<SM>(A, c.G, nd);
+                    const float def = __fdiv_rn(1.0f, (float)na);
+                    traverser_rows<MODE, SM, 0, KH / 2>(st, cc, c.stride, out, Hp, na, def, disc);
+                    traverser_rows<MODE, SM, KH / 2, KH>(st, cc, c.stride, out, Hp, na, def, disc);
                 } else if (MODE == MODE_BR) {
                     float res[KH];
                     FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0f;
@@ -737,6 +883,28 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                         update_16bit<SM>(c, nd, st, Hp, disc);
                     }
                 }
+            } else if (na <= NA_REG) {
+                // opponent (or single-action) node: sum of the children, all
+                // rows in flight together
+                V cv[NA_REG][KH];
+#pragma unroll
+                for (int a = 0; a < NA_REG; ++a)
+                    if (a < na) {
+                        const V* ca = cc + (size_t)a * c.stride;
+                        FOR_K cv[a][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+                    }
+                V res[KH];
+                FOR_K res[k] = cv[0][k];
+#pragma unroll
+                for (int a = 1; a < NA_REG; ++a)
+                    if (a < na) {
+                        if constexpr (sizeof(V) == 8) {
+                            FOR_K res[k] += cv[a][k];
+                        } else {
+                            FOR_K res[k] = __fadd_rn(res[k], cv[a][k]);
+                        }
+                    }
+                FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
             } else {
                 // opponent (or single-action) node: sum of the children
                 V res[KH];

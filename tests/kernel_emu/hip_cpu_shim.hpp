@@ -220,6 +220,7 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
 // fibers share one memory), the barrier is the workgroup barrier
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_s_barrier() emu::syncthreads()
+#define __builtin_amdgcn_sched_barrier(mask) ((void)(mask))
 #define hipLaunchKernelGGL(k, g, b, s, st, ...) emu::launch(k, dim3(g), dim3(b), (size_t)(s), __VA_ARGS__)
 
 template <class T>
