@@ -29,7 +29,7 @@
 
 namespace psr {
 
-constexpr int BLOCK = 256;
+constexpr int BLOCK = DCFR_BLOCK;
 constexpr int WAVE = 64;
 constexpr int NWAVES = BLOCK / WAVE;
 constexpr int MAXA = 16;   // largest action count of a decision node on the device
@@ -38,7 +38,7 @@ constexpr int NA_REG = 4;  // nodes with up to this many actions keep their rows
 // h = threadIdx.x + k * BLOCK.  The per-hand loops are unrolled over k and
 // split into a load phase and a compute/store phase, so every thread has up
 // to KH independent loads in flight per row instead of one.
-constexpr int KH = 6;
+constexpr int KH = (1326 + BLOCK - 1) / BLOCK;
 static_assert(KH * BLOCK >= 1326, "KH slots must cover the largest range");
 #define FOR_K _Pragma("unroll") for (int k = 0; k < KH; ++k)
 __device__ __forceinline__ int hslot(int k) { return (int)threadIdx.x + k * BLOCK; }
@@ -377,9 +377,18 @@ static_assert(SCAN_THREADS == BLOCK, "one scan run per thread");
 static_assert(SCAN_RUN * BLOCK >= 3 * 1326 + 53, "scan runs must cover the longest index list");
 
 __device__ __forceinline__ ScanRun load_scan_run(const KernelArgs& A, const RunoutTab& RT) {
-    const uint4* src = reinterpret_cast<const uint4*>(A.u16pool + RT.scan_runs) + 2 * threadIdx.x;
-    const uint4 a = src[0], b = src[1];
-    return ScanRun{{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w}};
+    constexpr int NV = SCAN_RUN / 8;   // 16-byte vectors per run
+    const uint4* src = reinterpret_cast<const uint4*>(A.u16pool + RT.scan_runs) + NV * threadIdx.x;
+    ScanRun r;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const uint4 x = src[i];
+        r.w[4 * i + 0] = x.x;
+        r.w[4 * i + 1] = x.y;
+        r.w[4 * i + 2] = x.z;
+        r.w[4 * i + 3] = x.w;
+    }
+    return r;
 }
 __device__ __forceinline__ uint16_t run_entry(const ScanRun& u, int j) {
     return (uint16_t)(u.w[j >> 1] >> (16 * (j & 1)));
@@ -817,8 +826,12 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     // together, in two halves of the hand slots (registers)
                     const NodeState<SM> st = node_state<SM>(A, c.G, nd);
                     const float def = __fdiv_rn(1.0f, (float)na);
-                    traverser_rows<MODE, SM, 0, KH / 2>(st, cc, c.stride, out, Hp, na, def, disc);
-                    traverser_rows<MODE, SM, KH / 2, KH>(st, cc, c.stride, out, Hp, na, def, disc);
+                    if constexpr (KH > 3) {
+                        traverser_rows<MODE, SM, 0, KH / 2>(st, cc, c.stride, out, Hp, na, def, disc);
+                        traverser_rows<MODE, SM, KH / 2, KH>(st, cc, c.stride, out, Hp, na, def, disc);
+                    } else {
+                        traverser_rows<MODE, SM, 0, KH>(st, cc, c.stride, out, Hp, na, def, disc);
+                    }
                 } else if (MODE == MODE_BR) {
                     float res[KH];
                     FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0f;

@@ -63,8 +63,10 @@ struct HandScan {
 // The scan is carried out by SCAN_THREADS threads, each owning a run of at
 // most SCAN_RUN consecutive list entries: thread t owns entries
 // [t * C, t * C + C) with C = ceil(scan_len / SCAN_THREADS).
-constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_RUN = 16;   // (3 * 1326 + 53) / SCAN_THREADS rounded up
+constexpr int DCFR_BLOCK = 512;   // threads of a DCFR workgroup (one game or tile at a time)
+constexpr int SCAN_THREADS = DCFR_BLOCK;
+// (3 * 1326 + 53) / SCAN_THREADS rounded up to a multiple of 8 (16-byte loads)
+constexpr int SCAN_RUN = ((3 * 1326 + 53 + SCAN_THREADS - 1) / SCAN_THREADS + 7) / 8 * 8;
 
 struct RunoutTab {
     int32_t scan_idx;       // u16[scan_len] index list (u16 pool)
