@@ -158,12 +158,22 @@ def net_solve_bench(args):
     t_ex = time.time() - t0
     err = {name: N.root_ev_error(b_ex, b, g_ex, turn.pot)
            for name, b in (("net_backed", b_net), ("checkdown", b_chk), ("exact_leaves", b_exl))}
+    # the net against the same method with exact leaves (isolates the net)
+    err["net_vs_exact_leaves"] = N.root_ev_error(b_exl, b_net, g_exl, turn.pot)
     for k, v in err.items():
         log(f"{k}: {v}")
     gv = {k: max(v["p0_value_diff_pot"], v["p1_value_diff_pot"]) for k, v in err.items()}
+    # the leaf judge: leaf values at the exact-leaf solve's final (average
+    # strategy) reaches, net and check-down against exact river solves
+    items = N.boundary_items(turn, g_exl, b_exl)
+    v_ex = N.ExactLeaves(turn, args.iterations).values_for(items)
+    leaf = {"net": N.leaf_error(items, N.NetLeaves(net, chk).values_for(items), v_ex),
+            "checkdown": N.leaf_error(items, chk.values_for(items), v_ex)}
+    log(f"leaf error (fraction of pot, reach-weighted): {leaf}")
     out = {"metric": "net-backed turn solve: game value error vs the exact turn solve (fraction of pot)",
            "value": gv["net_backed"], "game_value_error": gv,
            "gain_over_checkdown": gv["checkdown"] / max(gv["net_backed"], 1e-12),
+           "leaf_error_pot": leaf,
            "unit": "pot fraction", "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
            "dtype": "bf16 net, f32 solve",
            "config": {"workload": "turn_uniform01 seed %d, depth_limit 0 at the river deal" % args.seed,

@@ -221,6 +221,12 @@ int psr_batch_set_boundary_cfvs(psr_batch* batch, int game, int ordinal, int pla
                                 int raw);
 int psr_batch_update_leaf_cfvs(psr_batch* batch, const float* p0, int64_t n0, const float* p1, int64_t n1);
 int psr_batch_boundary_reach(psr_batch* batch, int game, int ordinal, int player, float* out, int n);
+/* Reach of `player` at arena nodes `nodes` under the average strategy, H_player
+ * values per node into out (n_nodes * H_player): gpu-range-solver
+ * batch.rs:765-797 compute_reach_at_nodes (its result[p] is the reach of the
+ * opponent of p, i.e. this call with player = 1 - p).  Runs the forward pass of
+ * the other player's traversal on the device. */
+int psr_batch_reach_at_nodes(psr_batch* batch, int game, int player, const int64_t* nodes, int n_nodes, float* out);
 /* Wall time of the last device job in milliseconds (HIP events on the batch
  * stream) -- used for roofline accounting. */
 float psr_batch_last_kernel_ms(psr_batch* batch);

@@ -722,6 +722,41 @@ struct Solver {
         cfvalue_rec(out, 0, p, reach.data());
     }
 
+    // Reach of player q at node `target` under the average strategy
+    // (gpu-range-solver batch.rs:765-797 compute_reach_at_nodes: the forward
+    // walk of the other player's traversal, with the reach passed to chance
+    // children scaled as in solve_recursive, solver.rs:256-331).
+    bool reach_rec(uint32_t ni, int q, const T* reach, uint32_t target, T* out) const {
+        const Node& node = g.nodes[ni];
+        const int Hq = g.H(q);
+        if (ni == target) {
+            std::copy(reach, reach + Hq, out);
+            return true;
+        }
+        if (node.player & TERMINAL) return false;
+        const int A = (int)node.num_children;
+        if (node.player & CHANCE) {
+            std::vector<T> upd(Hq);
+            T sc = chance_scale(node);
+            for (int i = 0; i < Hq; ++i) upd[i] = reach[i] * sc;
+            for (int a = 0; a < A; ++a)
+                if (reach_rec(node.first_child + a, q, upd.data(), target, out)) return true;
+            return false;
+        }
+        if ((int)node.player != q || A == 1) {
+            for (int a = 0; a < A; ++a)
+                if (reach_rec(node.first_child + a, q, reach, target, out)) return true;
+            return false;
+        }
+        std::vector<T> sig;
+        node_average(node, A, Hq, sig);
+        for (int a = 0; a < A; ++a) {
+            for (int i = 0; i < Hq; ++i) sig[(size_t)a * Hq + i] *= reach[i];
+            if (reach_rec(node.first_child + a, q, &sig[(size_t)a * Hq], target, out)) return true;
+        }
+        return false;
+    }
+
     // Query interface at the root after back_to_root (query.rs): weights with
     // dead hands zeroed (assign_zero_weights, :886-920), normalized weights
     // (cache_normalized_weights, :498-567), expected_values_detail (:672-760)
@@ -924,6 +959,21 @@ void oracle_root_cfvalues(void* h, int p, double* out) {
         S.root_cfvalues(p, v.data());
         for (size_t i = 0; i < v.size(); ++i) out[i] = (double)v[i];
     });
+}
+
+// reach of player q at arena node `node` under the average strategy
+int oracle_reach_at_node(void* h, uint32_t node, int q, double* out) {
+    int rc = -1;
+    DISPATCH(h, {
+        if (q >= 0 && q <= 1 && node < S.g.nodes.size()) {
+            std::vector<decltype(S.exploitability())> w = S.weights_T(q), v(S.g.H(q));
+            if (S.reach_rec(0, q, w.data(), node, v.data())) {
+                for (size_t i = 0; i < v.size(); ++i) out[i] = (double)v[i];
+                rc = 0;
+            }
+        }
+    });
+    return rc;
 }
 
 // regrets / strategy sums of every decision node in arena order (A*H blocks)

@@ -1130,6 +1130,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
             }
         }
         if (A.flags & JOB_FINAL_CHECK) expl = exploitability<SM>(c, T);
+        if (A.flags & JOB_REACH) tile_forward<MODE_CFV, SM>(c, T, A.player);
         if (A.flags & JOB_ROOT_CFV) {
             const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll 1
@@ -1138,7 +1139,8 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
                 store_root_cfvalues(c, T, p);
             }
         }
-        if (threadIdx.x == 0) {
+        // results of the solve jobs only (a query job leaves them as they are)
+        if (threadIdx.x == 0 && (A.flags & (JOB_INITIAL_CHECK | JOB_ITERATE | JOB_FINAL_CHECK))) {
             A.expl_out[gi] = expl;
             A.iters_out[gi] = t;
         }

@@ -14,6 +14,8 @@ enum JobFlags : int {
     JOB_FINAL_CHECK = 4,     // exploitability after the iterations
     JOB_ROOT_CFV = 8,        // root per-hand cfvalues under the average strategy
     JOB_EXPL64 = 16,         // FP64 best-response exploitability
+    JOB_REACH = 32,          // forward pass of traversal `player` under the average strategy:
+                             // every node's reach row then holds the opponent's reach
 };
 
 enum Phase : int {

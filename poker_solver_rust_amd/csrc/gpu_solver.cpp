@@ -59,6 +59,7 @@ struct HostLayout {
     int32_t num_boundaries = 0;      // depth-boundary leaves over the batch
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
     std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
+    std::vector<std::vector<int32_t>> node_pos;   // per game: device-local node id of every arena node
     int64_t root_act_total = 0;
     // per game: what the expected-value queries need about the root
     struct Root {
@@ -276,6 +277,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         if (plan_tiles(g, plan) != 0) return -1;
         std::vector<int> pos(nodes.size());
         for (size_t k = 0; k < plan.order.size(); ++k) pos[plan.order[k]] = (int)k;
+        L.node_pos.emplace_back(pos.begin(), pos.end());
 
         GameDesc d{};
         d.node_base = (int32_t)L.nodes.size();
@@ -468,6 +470,7 @@ struct psr_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     HostLayout L;
     bool prepared = false;
+    int reach_player = 0;   // traversal whose forward pass JOB_REACH runs
     uint32_t max_iterations = 0;
     int grid = 0, num_cus = 0, blocks_per_cu = 0;
     float last_ms = 0.0f;
@@ -707,8 +710,16 @@ struct psr_batch {
                 for (int p = 0; p < 2; ++p)
                     if (traverse(p, MODE_CFV, 0)) return -1;
             }
-            // publish per-game results next to the single-tile ones
+            if (flags & JOB_REACH) {
+                // root forward, then every river tile (its forward fills the rows;
+                // its terminals and backward only touch cfv rows)
+                if (phase(PHASE_ROOT_FORWARD, reach_player, MODE_CFV, 0)) return -1;
+                if (phase(PHASE_SUBTREES, reach_player, MODE_CFV, 0)) return -1;
+            }
+            // publish per-game results next to the single-tile ones (solve jobs only)
+            const bool solve_job = flags & (JOB_INITIAL_CHECK | JOB_ITERATE | JOB_FINAL_CHECK);
             for (int gi : L.multi_games) {
+                if (!solve_job) break;
                 HIP_OK(hipMemcpyAsync(d_expl + gi, &expl[gi], sizeof(float), hipMemcpyHostToDevice, stream));
                 HIP_OK(hipMemcpyAsync(d_iters + gi, &iters[gi], sizeof(uint32_t), hipMemcpyHostToDevice, stream));
                 HIP_OK(hipStreamSynchronize(stream));
@@ -724,6 +735,7 @@ struct psr_batch {
             a.max_iterations = max_iters;
             a.check_interval = check_interval;
             a.flags = flags;
+            a.player = reach_player;
             HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
         }
         HIP_OK(hipEventRecord(ev1, stream));
@@ -874,6 +886,26 @@ int psr_batch_exploitability_f64(psr_batch* b, double* expl_out) {
     if (int rc = b->run(JOB_EXPL64, 0, 0, 0, b->max_iterations)) return rc;
     if (expl_out)
         HIP_OK(hipMemcpy(expl_out, b->d_expl64, b->L.games.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int psr_batch_reach_at_nodes(psr_batch* b, int game, int player, const int64_t* nodes, int n_nodes, float* out) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (game < 0 || game >= (int)b->L.games.size() || player < 0 || player > 1 || n_nodes < 0 || (n_nodes && (!nodes || !out)))
+        return fail("invalid argument");
+    const GameDesc& d = b->L.games[game];
+    const auto& pos = b->L.node_pos[game];
+    for (int k = 0; k < n_nodes; ++k)
+        if (nodes[k] < 0 || nodes[k] >= (int64_t)pos.size()) return fail("node index out of range");
+    // the traversal of the opponent of `player` propagates player's reach
+    b->reach_player = player ^ 1;
+    if (int rc = b->run(JOB_REACH, 0, 0, 0, b->max_iterations)) return rc;
+    HIP_OK(hipSetDevice(b->device));
+    const int H = d.H[player];
+    for (int k = 0; k < n_nodes; ++k) {
+        const size_t row = (size_t)d.scratch_base + (size_t)pos[nodes[k]] * d.row_stride;
+        HIP_OK(hipMemcpy(out + (size_t)k * H, b->d_reach + row, H * sizeof(float), hipMemcpyDeviceToHost));
+    }
     return 0;
 }
 

@@ -239,6 +239,24 @@ def river_mix(values: np.ndarray, r_opp: np.ndarray, rivers: Sequence[int]) -> n
     return np.where(den > 0, num / np.where(den > 0, den, 1.0), mean)
 
 
+def zero_sum(v0: np.ndarray, v1: np.ndarray, r0: np.ndarray, r1: np.ndarray):
+    """Shift both players' pot-relative values (..., 1326) by a constant each
+    so that the leaf is zero-sum: sum_i r0_i R0_i v0_i + sum_j r1_j R1_j v1_j
+    = 0 (R: unblocked opponent mass), as exact values of an unraked leaf are;
+    half of the violation is taken from each player (the zero-sum output
+    layer of DeepStack-style value nets)."""
+    R0 = unblocked_mass(r1)
+    R1 = unblocked_mass(r0)
+    w0 = r0 * R0
+    w1 = r1 * R1
+    W0 = w0.sum(-1, keepdims=True)
+    W1 = w1.sum(-1, keepdims=True)
+    e = 0.5 * ((w0 * v0).sum(-1, keepdims=True) + (w1 * v1).sum(-1, keepdims=True))
+    d0 = np.where(W0 > 0, e / np.where(W0 > 0, W0, 1.0), 0.0)
+    d1 = np.where(W1 > 0, e / np.where(W1 > 0, W1, 1.0), 0.0)
+    return v0 - d0, v1 - d1
+
+
 class NetLeaves:
     """Leaf values from the value net: on every river card the check-down
     value plus the net's correction (the net is trained on exact-minus-
@@ -273,8 +291,12 @@ class NetLeaves:
             x = encode(rows0, rows1, boards, pots, stacks)
             out = np.concatenate([self.net.forward(x[i:i + max_rows]) for i in range(0, len(x), max_rows)])
             out = out.reshape(len(live), K, OUTPUT_DIM).transpose(1, 0, 2)      # (K, live, 2652)
-            base[0][:, live] += out[:, :, :NUM_COMBOS]
-            base[1][:, live] += out[:, :, NUM_COMBOS:]
+            v0 = base[0][:, live] + out[:, :, :NUM_COMBOS]
+            v1 = base[1][:, live] + out[:, :, NUM_COMBOS:]
+            v0, v1 = zero_sum(v0, v1, rows0.reshape(len(live), K, -1).transpose(1, 0, 2),
+                              rows1.reshape(len(live), K, -1).transpose(1, 0, 2))
+            base[0][:, live] = v0
+            base[1][:, live] = v1
         m0 = river_mix(base[0], r1, rivers)
         m1 = river_mix(base[1], r0, rivers)
         return [(m0[k], m1[k]) for k in range(len(items))]
@@ -507,6 +529,37 @@ def refresh_leaves(turn, game, b, leaf_fn, idx, stacks, snapshots=None):
     for k, v in enumerate(vals):
         for p in range(2):
             b.set_boundary_cfvs(0, k, p, v[p][idx[p]].astype(np.float32), raw=False)
+
+
+def boundary_items(turn: Subgame, game: PostFlopGame, b: GpuBatchSolver) -> list:
+    """(board4, [reach0, reach1], pot, stack) of every boundary of game 0 of b,
+    from the reaches it recorded last (after compute_root_cfvalues: those of
+    the average strategy)."""
+    idx = [hand_index(game, p) for p in range(2)]
+    stacks = boundary_stacks(turn, game)
+    items = []
+    for k in range(game.num_boundary_nodes()):
+        reach = []
+        for p in range(2):
+            full = np.zeros(NUM_COMBOS)
+            full[idx[p]] = b.boundary_reach(0, k, p)
+            reach.append(full)
+        items.append((turn.board[:4], reach, float(game.boundary_pot(k)), stacks[k]))
+    return items
+
+
+def leaf_error(items, got, ref) -> float:
+    """Mean absolute leaf-value error as a fraction of the pot, weighted by
+    how much each hand weighs at its leaf (own reach times unblocked opponent
+    reach), over both players and every leaf: sum |v - v_ref| w / 2 / sum w
+    (v are values per half pot)."""
+    num = den = 0.0
+    for it, g, r in zip(items, got, ref):
+        for p in range(2):
+            w = it[1][p] * unblocked_mass(it[1][p ^ 1])
+            num += float((np.abs(g[p] - r[p]) * w).sum()) * 0.5
+            den += float(w.sum())
+    return num / max(den, 1e-30)
 
 
 def root_ev_error(exact_b: GpuBatchSolver, other_b: GpuBatchSolver, game_exact: PostFlopGame, pot: float) -> dict:

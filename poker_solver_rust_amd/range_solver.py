@@ -120,6 +120,8 @@ def load_library(path: Optional[str] = None):
         "psr_batch_strategy_sum": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_regrets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_last_kernel_ms": (C.c_float, [vp]),
+        "psr_batch_reach_at_nodes": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int,
+                                     C.POINTER(C.c_float)]),
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "psr_game_num_boundaries": (C.c_int, [vp]),
         "psr_game_boundary_node": (C.c_int64, [vp, C.c_int]),
@@ -598,6 +600,18 @@ class GpuBatchSolver:
         visit of the boundary by the opponent."""
         out = np.zeros(self._game(i).num_private_hands(player), np.float32)
         _check(load_library().psr_batch_boundary_reach(self._h, i, ordinal, player, _fptr(out), len(out)))
+        return out
+
+    def reach_at_nodes(self, i: int, player: int, nodes) -> np.ndarray:
+        """batch.rs:765-797 compute_reach_at_nodes: reach of `player` at the
+        given arena nodes under the average strategy, (len(nodes), H_player).
+        The reference's result[p] is reach_at_nodes(i, 1 - p, nodes)."""
+        g = self._game(i)
+        nodes = np.ascontiguousarray(nodes, np.int64)
+        out = np.zeros((len(nodes), g.num_private_hands(player)), np.float32)
+        _check(load_library().psr_batch_reach_at_nodes(self._h, i, player,
+                                                       nodes.ctypes.data_as(C.POINTER(C.c_int64)), len(nodes),
+                                                       _fptr(out)))
         return out
 
     def last_kernel_ms(self) -> float:

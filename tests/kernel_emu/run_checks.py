@@ -177,6 +177,23 @@ def run_raked():
     check_fp64(t, 2)
 
 
+def run_reach_at_nodes():
+    """batch.rs:765-797 compute_reach_at_nodes on the device (river and turn
+    games, every node of each) against the oracle's walk under the average
+    strategy of the same state."""
+    for g, iters in ((std_game("33%, 75%, a", "2x, a"), 6), (turn_game("AA,AKs", "KK,QQ"), 2)):
+        b = P.GpuBatchSolver(0)
+        b.prepare_batch([g], iters)
+        e, it = b.solve(iters, 0)
+        o = O.OracleGame(g.serialize())
+        o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+        nodes = np.arange(g.num_nodes())
+        for p in range(2):
+            got = b.reach_at_nodes(0, p, nodes)
+            for n in nodes:
+                same(got[n], np.float32(o.reach_at_node(int(n), p)), f"reach of {p} at node {n}")
+
+
 def run_half_storage():
     """FP16 / BF16 words with per-node scales: the solve converges like the
     FP64 oracle on the same tree (no reference counterpart)."""

@@ -42,6 +42,7 @@ def load():
         "oracle_state": (None, [vp, dp, dp]),
         "oracle_set_state": (None, [vp, dp, dp]),
         "oracle_evaluate": (C.c_int, [vp, C.c_uint32, C.c_int, dp, dp]),
+        "oracle_reach_at_node": (C.c_int, [vp, C.c_uint32, C.c_int, dp]),
         "oracle_set_boundary_cfvs": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int, C.c_int]),
         "oracle_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "oracle_num_boundaries": (C.c_int, [vp]),
@@ -131,6 +132,12 @@ class OracleGame:
         r = np.ascontiguousarray(regret, np.float64)
         s = np.ascontiguousarray(strategy, np.float64)
         load().oracle_set_state(self._h, _dptr(r), _dptr(s))
+
+    def reach_at_node(self, node, player):
+        out = np.zeros(self.num_hands(player))
+        rc = load().oracle_reach_at_node(self._h, node, player, _dptr(out))
+        assert rc == 0, "node not reached"
+        return out
 
     def evaluate(self, node, player, cfreach):
         cr = np.ascontiguousarray(cfreach, np.float64)

@@ -379,3 +379,27 @@ def test_raked_and_unraked_batch_solve():
         eo, ito = O.OracleGame(g.serialize()).solve(40, targets[i])
         assert int(it[i]) == ito, i
         assert abs(float(e[i]) - eo) <= RTOL * max(1.0, eo), i
+
+
+# ------------------------------------------------------ reach at nodes --
+
+@pytest.mark.parametrize("make,iters", [(lambda: std_game("33%, 75%, a", "2x, a"), 20), (turn_game, 6),
+                                        (lambda: W.river_batch(1, seed=9, iterations=30)[0].build(), 15)])
+def test_reach_at_nodes_matches_oracle(make, iters):
+    """batch.rs:765-797 compute_reach_at_nodes: the reach of each player at
+    every node under the average strategy, against the oracle's walk."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters)
+    b.run_iterations(0, iters)
+    o = O.OracleGame(g.serialize())
+    o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    nodes = np.arange(g.num_nodes())
+    rng = np.random.default_rng(1)
+    sample = nodes if len(nodes) <= 400 else np.sort(rng.choice(nodes, 400, replace=False))
+    for p in range(2):
+        got = b.reach_at_nodes(0, p, sample)
+        for k, n in enumerate(sample):
+            close(got[k], o.reach_at_node(int(n), p))
+    with pytest.raises(P.PsrError):
+        b.reach_at_nodes(0, 0, [g.num_nodes()])

@@ -98,3 +98,22 @@ def test_checkdown_batched_equals_per_boundary():
         per = np.stack([leaves.per_river_rows(it[1][0][None], it[1][1][None])[p, :, 0] for p in range(2)])
         assert np.allclose(v0, N.river_mix(per[0], it[1][1], rivers))
         assert np.allclose(v1, N.river_mix(per[1], it[1][0], rivers))
+
+
+def test_zero_sum_correction():
+    """After the correction the weighted values of the two players cancel;
+    values that already cancel (check-down equities) are left unchanged."""
+    rng = np.random.default_rng(6)
+    board = [3, 17, 30, 44, 9]
+    r0, r1 = rand_reach(rng, board), rand_reach(rng, board)
+    v0, v1 = rng.normal(0, 1, 1326), rng.normal(0, 1, 1326)
+    a0, a1 = N.zero_sum(v0, v1, r0, r1)
+    R0, R1 = N.unblocked_mass(r1), N.unblocked_mass(r0)
+    tot = (r0 * R0 * a0).sum() + (r1 * R1 * a1).sum()
+    assert abs(tot) < 1e-9 * (r0 * R0).sum()
+    assert np.allclose(a0 - v0, (a0 - v0)[0]) and np.allclose(a1 - v1, (a1 - v1)[0])
+    leaves = N.CheckdownLeaves(board[:4])
+    k = leaves.rivers.index(9)
+    c0, c1 = leaves.values(k, r1)[0], leaves.values(k, r0)[0]
+    b0, b1 = N.zero_sum(c0, c1, r0, r1)
+    assert np.allclose(b0, c0, atol=1e-9) and np.allclose(b1, c1, atol=1e-9)
