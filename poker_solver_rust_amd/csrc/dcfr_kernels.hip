@@ -62,7 +62,7 @@ struct Ctx {
     double* reach64;   // f64 rows (FP64 best response)
     double* cfv64;
     int stride;
-    double* scan;      // LDS, 3 * max_hands + 53
+    double* scan;      // LDS, scan_slots(3 * max_hands + 53) (transposed scan output)
     void* rs;          // LDS, max_hands values of the row type (staged reach)
     Shared* sh;
 };
@@ -450,7 +450,7 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len
     for (int j = 0; j < SCAN_RUN; ++j) {
         if (j < C && e0 + j < len) {
             if (run_entry(u, j) & SCAN_HEAD) run = 0.0;
-            c.scan[e0 + j] = run;
+            c.scan[j * BLOCK + threadIdx.x] = run;   // transposed slots (scan_slot)
             run += (double)x[j];
         }
     }
@@ -524,7 +524,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
         load_hand_scan(c, R.T, p, qv);
     }
 
-    const double total = c.scan[R.T.num_valid_opp];
+    const double total = c.scan[R.T.total_pos];
     if (nd.kind == KIND_BOUNDARY) {
         // record the opponent reach of this visit, then the leaf value of
         // every hand; win holds half_pot / num_combinations
@@ -1091,7 +1091,7 @@ __device__ __forceinline__ void store_root_cfvalues(Ctx& c, const TileDesc& T, i
     extern __shared__ double lds[];                                                             \
     __shared__ Shared sh;                                                                       \
     double* scan = lds;                                                                         \
-    void* rs = lds + 3 * A.max_hands + 53
+    void* rs = lds + scan_slots(3 * A.max_hands + 53)
 
 // Whole-game solve of single-tile games (river subgames): one workgroup runs
 // every iteration of one game, games come from the work queue.
@@ -1213,7 +1213,7 @@ __global__ void fill_zero_kernel(float* p, size_t n) {
 // ------------------------------------------------------------------ host --
 
 size_t dcfr_lds_bytes(int max_hands) {
-    return sizeof(double) * (3 * (size_t)max_hands + 53) + sizeof(double) * (size_t)max_hands;
+    return sizeof(double) * ((size_t)scan_slots(3 * max_hands + 53) + (size_t)max_hands);
 }
 
 // The FP64 best response runs as a kernel of its own after the other jobs

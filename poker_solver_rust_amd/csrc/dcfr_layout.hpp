@@ -74,10 +74,21 @@ struct RunoutTab {
     int32_t hand_scan;      // HandScan[H_player] (HandScan pool)
     int32_t num_valid_opp;  // entries of segment 0 before its closing slot
     int32_t scan_runs;      // u16[SCAN_THREADS * SCAN_RUN], 16-byte aligned: thread t's run
-                            // at [t * SCAN_RUN], padded with SCAN_ZERO (one aligned 32-byte
-                            // load per thread instead of SCAN_RUN scattered ones)
-    int32_t pad[3];
+                            // at [t * SCAN_RUN], padded with SCAN_ZERO (one aligned 16-byte
+                            // vector per 8 entries instead of scattered loads)
+    int32_t total_pos;      // scan-output slot of segment 0's closing entry (the total)
+    int32_t pad[2];
 };
+// The scan output is stored transposed: entry j of thread t's run (list
+// position t * C + j, C = ceil(scan_len / SCAN_THREADS)) at slot
+// j * SCAN_THREADS + t, so the threads' writes are consecutive.  HandScan
+// positions and total_pos are slots in that layout.
+constexpr int scan_slot(int pos, int scan_len) {
+    const int C = (scan_len + SCAN_THREADS - 1) / SCAN_THREADS;
+    return (pos % C) * SCAN_THREADS + pos / C;
+}
+// Slots the scan output of a list of up to max_len entries can occupy.
+constexpr int scan_slots(int max_len) { return (max_len + SCAN_THREADS - 1) / SCAN_THREADS * SCAN_THREADS; }
 
 struct ChanceTab {          // chance node: reach scale and isomorphic-card swaps
     float scale;            // 1 / chance factor (f32, as the reference computes it)

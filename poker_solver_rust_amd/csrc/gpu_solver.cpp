@@ -131,7 +131,8 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
         }
         return n;
     };
-    auto pack = [](int lo, int hi) { return (uint32_t)lo | ((uint32_t)hi << 16); };
+    const int len = (int)idx.size();
+    auto pack = [len](int lo, int hi) { return (uint32_t)scan_slot(lo, len) | ((uint32_t)scan_slot(hi, len) << 16); };
     RunoutTab t{};
     t.hand_scan = (int32_t)L.hand_scan.size();
     for (int i = 0; i < Hp; ++i) {
@@ -164,6 +165,7 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
         t.scan_runs = push_u16(L.u16, runs, 8);
     }
     t.num_valid_opp = (int32_t)sorted.size();
+    t.total_pos = scan_slot(t.num_valid_opp, t.scan_len);
     return t;
 }
 
@@ -1275,6 +1277,7 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
         if (t.scan_idx < 0 || t.scan_idx + t.scan_len > (int)L.u16.size()) return fail("scan list out of range");
         if (t.scan_len != 3 * t.num_valid_opp + 53) return fail("scan length");
         if (t.scan_len > SCAN_THREADS * SCAN_RUN) return fail("scan list longer than the scan threads cover");
+        if (t.total_pos != scan_slot(t.num_valid_opp, t.scan_len)) return fail("scan total slot");
         if (t.scan_runs < 0 || t.scan_runs % 8 || t.scan_runs + SCAN_THREADS * SCAN_RUN > (int)L.u16.size())
             return fail("scan runs out of range or misaligned");
         {
@@ -1296,8 +1299,11 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
             if ((h.x & 0xFFFF) == 0xFFFF) continue;
             const uint32_t v[8] = {h.x & 0xFFFF, h.x >> 16, h.y & 0xFFFF, h.y >> 16,
                                    h.z & 0xFFFF, h.z >> 16, h.w & 0xFFFF, h.w >> 16};
-            for (uint32_t x : v)
-                if ((int)x >= t.scan_len) return fail("hand scan position out of range");
+            const int C = (t.scan_len + SCAN_THREADS - 1) / SCAN_THREADS;
+            for (uint32_t x : v) {
+                const int j = (int)x / SCAN_THREADS, th = (int)x % SCAN_THREADS;
+                if (j >= C || th * C + j >= t.scan_len) return fail("hand scan position out of range");
+            }
         }
     }
     out[0] = d.num_tiles;
