@@ -116,6 +116,11 @@ __device__ __forceinline__ T uni_load(const T* p) {
     return v;
 }
 
+// The node whose reach row node n (game-local id) uses in the traversal of p.
+__device__ __forceinline__ int reach_src(const Ctx& c, int n, int p) {
+    return uni(c.A.reach_src[2 * ((size_t)c.G.node_base + n) + p]);
+}
+
 // Sum over the workgroup of one f64 per thread; every thread gets the same
 // value (same association order on every thread).
 __device__ __forceinline__ double block_sum(double v, Shared* sh) {
@@ -271,10 +276,12 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
         bool any = false;
         for (int n = lr.start; n < lr.end; ++n) {
             const NodeRec nd = uni_load(c.nodes + n);
-            if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
-            any = true;
             const int na = nd.num_children;
-            const V* rn = reach + (size_t)n * c.stride;
+            // a node of the traverser or a single-action node passes the reach
+            // through: its children use the same row (KernelArgs::reach_src)
+            if (!(nd.kind == KIND_CHANCE || (nd.kind == KIND_DECISION && nd.player == o && na > 1))) continue;
+            any = true;
+            const V* rn = reach + (size_t)reach_src(c, n, p) * c.stride;
             V* rc = reach + (size_t)nd.first_child * c.stride;
             V r[KH];
             FOR_K r[k] = hslot(k) < Ho ? rn[hslot(k)] : V(0);
@@ -341,7 +348,7 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     }
                 }
             } else {
-                // chance node: reach / chance factor; otherwise the reach passes through
+                // chance node: reach / chance factor into every child's row
                 if (nd.kind == KIND_CHANCE) {
                     const ChanceTab ct = uni_load(A.chance + nd.aux);
                     if constexpr (sizeof(V) == 8) {
@@ -505,7 +512,7 @@ __device__ __forceinline__ void load_runout(const Ctx& c, int table, int p, Runo
 // barriers inside: the caller publishes the cfv rows with __syncthreads().
 template <typename V>
 __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p, const RunoutRegs& R,
-                                              const V (&rv)[KH]) {
+                                              const V (&rv)[KH], bool rescan) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p), Ho = num_hands(c.G, p ^ 1);
     V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
@@ -513,10 +520,12 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
     const double win = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]);
     const double lose = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]);
     const double tie = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]);
-    lds_barrier();   // the previous terminal's reads of rs and scan are done
-    FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
-    lds_barrier();
-    segmented_scan<V>(c, R.u, R.T.scan_len);
+    if (rescan) {   // otherwise rs and the scan already hold this reach row
+        lds_barrier();   // the previous terminal's reads of rs and scan are done
+        FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
+        lds_barrier();
+        segmented_scan<V>(c, R.u, R.T.scan_len);
+    }
     HandScan qv[KH];
     if constexpr (HOLD_HAND_SCAN) {
         FOR_K qv[k] = R.q[k];
@@ -952,17 +961,23 @@ __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p)
     if (T.num_terminals == 0) return;
     int n = uni(terms[0]);
     NodeRec nd = uni_load(c.nodes + n);
+    int src = reach_src(c, n, p);
     RunoutRegs R;
     int table = nd.aux;
     load_runout(c, table, p, R);
     V rv[KH];
-    FOR_K rv[k] = hslot(k) < Ho ? reach[(size_t)n * c.stride + hslot(k)] : V(0);
+    FOR_K rv[k] = hslot(k) < Ho ? reach[(size_t)src * c.stride + hslot(k)] : V(0);
+    bool rescan = true;
     for (int t = 0; t < T.num_terminals; ++t) {
         const int nn = t + 1 < T.num_terminals ? uni(terms[t + 1]) : n;
         const NodeRec nnd = uni_load(c.nodes + nn);
+        const int nsrc = reach_src(c, nn, p);
         V rn[KH];
-        FOR_K rn[k] = hslot(k) < Ho ? reach[(size_t)nn * c.stride + hslot(k)] : V(0);
-        eval_terminal<V>(c, nd, n, p, R, rv);
+        FOR_K rn[k] = hslot(k) < Ho ? reach[(size_t)nsrc * c.stride + hslot(k)] : V(0);
+        eval_terminal<V>(c, nd, n, p, R, rv, rescan);
+        // the next terminal reuses the scan when it reads the same reach row
+        // on the same runout (e.g. fold and call below one of the traverser's nodes)
+        rescan = nsrc != src || nnd.aux != table;
         if (nnd.aux != table) {
             table = nnd.aux;
             load_runout(c, table, p, R);
@@ -970,6 +985,7 @@ __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p)
         FOR_K rv[k] = rn[k];
         n = nn;
         nd = nnd;
+        src = nsrc;
     }
     __syncthreads();
 }

@@ -44,6 +44,7 @@ struct KernelArgs {
     const double* payoff;
     const uint16_t* hand_cards;
     const uint16_t* hand_same;
+    const int32_t* reach_src;     // [2 * (node_base + local) + p]: local node whose reach row it uses
     const float* hand_weight;
     const int32_t* terminals;
     const RunoutTab* runouts;

@@ -60,6 +60,7 @@ struct HostLayout {
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
     std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
     std::vector<std::vector<int32_t>> node_pos;   // per game: device-local node id of every arena node
+    std::vector<int32_t> reach_src;      // per device node and traversing player: node whose reach row it uses
     int64_t root_act_total = 0;
     // per game: what the expected-value queries need about the root
     struct Root {
@@ -392,6 +393,32 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             L.payoff.push_back(tie);
         }
 
+        // Reach rows per traversing player p (the opponent's reach): a node
+        // whose parent passes the reach through unchanged -- a decision node of
+        // p, or a single-action node -- uses its parent's row; the root, the
+        // children of chance nodes and of the opponent's multi-action nodes own
+        // one.  The forward pass writes only owned rows.
+        {
+            std::vector<int> parent(nodes.size(), -1);
+            for (size_t i = 0; i < nodes.size(); ++i)
+                for (int a = 0; a < (int)nodes[i].num_children; ++a) parent[nodes[i].first_child + a] = (int)i;
+            const size_t base = L.reach_src.size();
+            L.reach_src.resize(base + 2 * nodes.size());
+            for (size_t k = 0; k < plan.order.size(); ++k) {   // breadth-first: parents first
+                const int i = plan.order[k];
+                for (int p = 0; p < 2; ++p) {
+                    int src = (int)k;
+                    const int par = parent[i];
+                    if (par >= 0) {
+                        const GameNode& pn = nodes[par];
+                        const bool through = !pn.is_chance() && ((pn.player & PLAYER_MASK) == p || pn.num_children == 1);
+                        if (through) src = L.reach_src[base + 2 * pos[par] + p];
+                    }
+                    L.reach_src[base + 2 * k + p] = src;
+                }
+            }
+        }
+
         // tiles
         d.tile_base = (int32_t)L.tiles.size();
         d.num_tiles = (int32_t)plan.levels.size();
@@ -502,6 +529,7 @@ struct psr_batch {
     int* d_counter = nullptr;
     int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr;
     double* d_ev = nullptr;
+    int32_t* d_reach_src = nullptr;
     float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
     int32_t* d_bnd_mode = nullptr;
     float* d_root_action_cfv = nullptr;
@@ -511,11 +539,13 @@ struct psr_batch {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
-                        d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off};
+                        d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
+                        d_reach_src};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
+        d_reach_src = nullptr;
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
         d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
         d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
@@ -537,6 +567,7 @@ struct psr_batch {
         a.payoff = d_payoff;
         a.hand_cards = d_cards;
         a.hand_same = d_same;
+        a.reach_src = d_reach_src;
         a.hand_weight = d_weights;
         a.terminals = d_terms;
         a.runouts = d_runouts;
@@ -803,7 +834,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         upload(&b->d_runouts, L.runouts) || upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) ||
         upload(&b->d_iso_perm, L.iso_perm) || upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) ||
         upload(&b->d_single, L.single_games) || upload(&b->d_multi, L.multi_games) ||
-        upload(&b->d_subtrees, L.subtree_tiles))
+        upload(&b->d_subtrees, L.subtree_tiles) || upload(&b->d_reach_src, L.reach_src))
         return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
@@ -905,7 +936,8 @@ int psr_batch_reach_at_nodes(psr_batch* b, int game, int player, const int64_t* 
     HIP_OK(hipSetDevice(b->device));
     const int H = d.H[player];
     for (int k = 0; k < n_nodes; ++k) {
-        const size_t row = (size_t)d.scratch_base + (size_t)pos[nodes[k]] * d.row_stride;
+        const int src = b->L.reach_src[2 * ((size_t)d.node_base + pos[nodes[k]]) + (player ^ 1)];
+        const size_t row = (size_t)d.scratch_base + (size_t)src * d.row_stride;
         HIP_OK(hipMemcpy(out + (size_t)k * H, b->d_reach + row, H * sizeof(float), hipMemcpyDeviceToHost));
     }
     return 0;
