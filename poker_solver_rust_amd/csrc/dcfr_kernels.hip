@@ -747,7 +747,10 @@ __device__ __forceinline__ void traverser_rows(const NodeState<SM>& st, const fl
         if (hslot(K0 + j) < Hp) out[uslot(K0 + j)] = res[j];
 }
 
-template <int MODE, int SM, typename V = float>
+// CHANCE: the tile may hold chance nodes (only a turn game's root tile does;
+// river games and river subtree tiles never do, and leaving the chance sums
+// out keeps those kernels' register count down).
+template <int MODE, int SM, typename V = float, bool CHANCE = false>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p);
@@ -764,24 +767,36 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
             V* out = cfv + (size_t)n * c.stride;
             const V* cc = cfv + (size_t)nd.first_child * c.stride;
             if (nd.kind == KIND_CHANCE) {
-                // f64 sum over the dealt cards plus the isomorphic swaps
-                // (solver.rs:256-331)
-                const ChanceTab ct = uni_load(A.chance + nd.aux);
-                double acc[KH];
-                FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
-                for (int a = 1; a < na; ++a) {
-                    V v[KH];
-                    FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : V(0);
-                    FOR_K acc[k] += (double)v[k];
+                if constexpr (CHANCE) {
+                    // f64 sum over the dealt cards plus the isomorphic swaps
+                    // (solver.rs:256-331)
+                    const ChanceTab ct = uni_load(A.chance + nd.aux);
+                    double acc[KH];
+                    FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
+                    // children in groups of CHANCE_GROUP rows in flight together,
+                    // summed in child order
+                    constexpr int CHANCE_GROUP = 8;
+                    for (int a0 = 1; a0 < na; a0 += CHANCE_GROUP) {
+                        V v[CHANCE_GROUP][KH];
+    #pragma unroll
+                        for (int g = 0; g < CHANCE_GROUP; ++g)
+                            if (a0 + g < na) {
+                                const V* ca = cc + (size_t)(a0 + g) * c.stride;
+                                FOR_K v[g][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+                            }
+    #pragma unroll
+                        for (int g = 0; g < CHANCE_GROUP; ++g)
+                            if (a0 + g < na) FOR_K acc[k] += (double)v[g][k];
+                    }
+                    for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
+                        const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
+                        const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
+                        V v[KH];
+                        FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
+                        FOR_K acc[k] += (double)v[k];
+                    }
+                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
                 }
-                for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
-                    const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
-                    const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
-                    V v[KH];
-                    FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
-                    FOR_K acc[k] += (double)v[k];
-                }
-                FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
             } else if (nd.player == p && na > 1) {
                 if constexpr (sizeof(V) == 8 && MODE == MODE_CFV) {
                     // FP64 value of the average strategy: sum_a sigma_a * cfv_a,
@@ -1185,7 +1200,7 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                 tile_forward<MODE, SM, V>(c, T, p);
             } else {
                 tile_terminals<V>(c, T, p);
-                tile_backward<MODE, SM, V>(c, T, p, disc);
+                tile_backward<MODE, SM, V, true>(c, T, p, disc);
                 if (MODE == MODE_BR || MODE == MODE_CFV) {
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;

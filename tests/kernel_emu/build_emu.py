@@ -15,9 +15,28 @@ OUT = os.path.join(HERE, "libpsr_dcfr_emu.so")
 CXX = os.environ.get("EMU_CXX", "/opt/rocm/llvm/bin/clang++")
 
 
+def _inputs():
+    names = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".cpp", ".hpp", ".hip", ".inc"))]
+    names += [os.path.join(HERE, "hip_cpu_shim.hpp"), os.path.join(HERE, "hip", "hip_runtime.h"), __file__]
+    return names
+
+
 def build():
+    """Builds the library unless it is newer than every source; safe to call
+    from concurrent test processes (one builds under a file lock, the others
+    wait and reuse its result)."""
+    import fcntl
+
     work = os.path.join(HERE, "_build")
     os.makedirs(work, exist_ok=True)
+    with open(os.path.join(work, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        if os.path.exists(OUT) and os.path.getmtime(OUT) >= max(os.path.getmtime(f) for f in _inputs()):
+            return OUT
+        return _build(work)
+
+
+def _build(work):
     src = open(os.path.join(CSRC, "dcfr_kernels.hip")).read()
     src, n = re.subn(r"extern __shared__ double (\w+)\[\];", r"double* \1 = emu::dynamic_lds<double>();", src)
     assert n == 1, "dynamic LDS declaration not found"
@@ -35,7 +54,9 @@ def build():
         o = os.path.join(work, os.path.basename(s) + ".o")
         subprocess.check_call([CXX, "-c", *flags, s, "-o", o])
         objs.append(o)
-    subprocess.check_call([CXX, "-shared", "-pthread", *objs, "-o", OUT])
+    tmp = OUT + ".tmp"
+    subprocess.check_call([CXX, "-shared", "-pthread", *objs, "-o", tmp])
+    os.replace(tmp, OUT)
     return OUT
 
 
