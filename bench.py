@@ -82,7 +82,24 @@ def cpu_baseline(subgames, iters, mode, interval, seconds_target=15.0):
                       f"(mean {float(its.mean()):.1f}), {threads} threads on '{cpu_model()}'"}
 
 
-def turn_bench(P, W, iters, seed):
+def cpu_turn_baseline(game, iters=8):
+    """The CPU oracle on the same turn subgame: `iters` solve_step iterations
+    (after one untimed), one thread (the oracle restates the reference's
+    recursion without its rayon split over the turn's children)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as O
+
+    o = O.OracleGame(game.serialize())
+    o.solve_step(0)
+    t = time.time()
+    for k in range(1, iters + 1):
+        o.solve_step(k)
+    secs = time.time() - t
+    return {"value": round(iters / secs, 3), "unit": "DCFR iters/s", "cores": 1, "kind": "port",
+            "sample": f"{iters} solve_step iterations of the same turn subgame, 1 thread on '{cpu_model()}'"}
+
+
+def turn_bench(P, W, iters, seed, cpu=True):
     """Turn DCFR iterations/s on the full turn subgame (4 seeded board cards,
     48 rivers, U[0,1] ranges, pot 60, stack 200, 33%/75%/all-in, 2 raises at
     3x): solve_step iterations only, timed over `iters` iterations after one
@@ -101,7 +118,8 @@ def turn_bench(P, W, iters, seed):
     achieved = abytes / (kms * 1e-3) / 1e9
     log(f"turn: {iters} iterations in {secs * 1e3:.1f} ms ({iters / secs:.1f} it/s), {g.num_nodes()} nodes, "
         f"{edge_hands:.0f} edge-hands")
-    return {"value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
+    extra = {"cpu_baseline": cpu_turn_baseline(g)} if cpu else {}
+    return {**extra, "value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
             "edge_hands": edge_hands, "ms_per_iter": round(secs * 1e3 / iters, 4),
             "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 5)}}
@@ -333,7 +351,7 @@ def main():
         log(f"single-subgame path: {single_rate:.1f} solves/s on {n_single} subgames; batched is "
             f"{value / single_rate:.1f}x")
     if args.turn_iters > 0:
-        out["turn"] = turn_bench(P, W, args.turn_iters, args.seed)
+        out["turn"] = turn_bench(P, W, args.turn_iters, args.seed, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         n_sample = min(len(subs), 512)
         out["cpu_baseline"] = cpu_baseline(subs[:n_sample], args.iterations, args.mode, interval)
