@@ -401,6 +401,22 @@ __device__ __forceinline__ uint16_t run_entry(const ScanRun& u, int j) {
     return (uint16_t)(u.w[j >> 1] >> (16 * (j & 1)));
 }
 
+// One step of an inclusive segmented scan across a wave: the (flag, sum)
+// pair of the DPP source lane is combined into this lane's,
+// (f, s) <- (f | fo, f ? s : so + s).  Lanes whose source lies outside the
+// pattern, or whose row the row mask leaves out, receive (0, 0), the
+// identity of the combination.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ void seg_dpp_step(int& f, double& s) {
+    const int fo = __builtin_amdgcn_update_dpp(0, f, CTRL, ROW_MASK, 0xF, true);
+    const uint64_t sb = (uint64_t)__double_as_longlong(s);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)sb, CTRL, ROW_MASK, 0xF, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(sb >> 32), CTRL, ROW_MASK, 0xF, true);
+    const double so = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    if (!f) s = so + s;
+    f = f | fo;
+}
+
 // Block-wide exclusive segmented scan in f64 of the staged reach values named
 // by the index list (SCAN_HEAD starts a segment, SCAN_ZERO adds 0).  Every
 // thread owns a contiguous run of at most SCAN_RUN entries (its index words
@@ -430,14 +446,18 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len
             s += (double)x[j];
         }
     }
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const int fo = __shfl_up((int)f, d, WAVE);
-        const double so = __shfl_up(s, d, WAVE);
-        if (lane >= d) {
-            if (!f) s = so + s;
-            f = f || fo;
-        }
+    // inclusive wave scan of the (segment-started, sum) pairs with DPP moves:
+    // shifts by 1, 2, 4, 8 inside each row of 16 lanes, then lane 15 of a
+    // row into the next row and lane 31 into rows 2 and 3
+    {
+        int fi = f;
+        seg_dpp_step<0x111, 0xF>(fi, s);   // row_shr:1
+        seg_dpp_step<0x112, 0xF>(fi, s);   // row_shr:2
+        seg_dpp_step<0x114, 0xF>(fi, s);   // row_shr:4
+        seg_dpp_step<0x118, 0xF>(fi, s);   // row_shr:8
+        seg_dpp_step<0x142, 0xA>(fi, s);   // row_bcast:15 into rows 1 and 3
+        seg_dpp_step<0x143, 0xC>(fi, s);   // row_bcast:31 into rows 2 and 3
+        f = fi;
     }
     bool fe = __shfl_up((int)f, 1, WAVE);
     double se = __shfl_up(s, 1, WAVE);
@@ -451,7 +471,19 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len
     }
     lds_barrier();
     double cs = 0.0;
-    for (int w = 0; w < wave; ++w) cs = c.sh->wave_f[w] ? c.sh->wave_s[w] : cs + c.sh->wave_s[w];
+    {
+        // every wave total read at once (the trip count is a compile-time bound)
+        int wf[NWAVES];
+        double ws[NWAVES];
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w) {
+            wf[w] = c.sh->wave_f[w];
+            ws[w] = c.sh->wave_s[w];
+        }
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w)
+            if (w < wave) cs = wf[w] ? ws[w] : cs + ws[w];
+    }
     double run = fe ? se : cs + se;
 #pragma unroll
     for (int j = 0; j < SCAN_RUN; ++j) {
@@ -1305,9 +1337,22 @@ hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t strea
     return hipGetLastError();
 }
 
-hipError_t dcfr_occupancy(int max_hands, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_REF16, false>, BLOCK,
-                                                        dcfr_lds_bytes(max_hands));
+// Workgroups of the storage mode's solve kernel that fit on a CU at once
+// (registers and LDS): the grid of the persistent launch.
+hipError_t dcfr_occupancy(int max_hands, int storage, int* blocks_per_cu) {
+    const size_t lds = dcfr_lds_bytes(max_hands);
+    switch (storage) {
+        case SM_F32:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_F32, false>, BLOCK, lds);
+        case SM_REF16:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_REF16, false>, BLOCK, lds);
+        case SM_FP16:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_FP16, false>, BLOCK, lds);
+        case SM_BF16:
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, dcfr_solve_kernel<SM_BF16, false>, BLOCK, lds);
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 hipError_t fill_zero(float* p, size_t n, hipStream_t stream) {

@@ -95,7 +95,7 @@ struct KernelArgs {
 size_t dcfr_lds_bytes(int max_hands);
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream);
-hipError_t dcfr_occupancy(int max_hands, int* blocks_per_cu);
+hipError_t dcfr_occupancy(int max_hands, int storage, int* blocks_per_cu);
 int dcfr_block_size();
 int dcfr_max_actions();
 hipError_t fill_zero(float* p, size_t n, hipStream_t stream);

@@ -522,6 +522,7 @@ struct psr_batch {
     float *d_rscale = nullptr, *d_sscale = nullptr;
     double *d_reach64 = nullptr, *d_cfv64 = nullptr, *d_expl64 = nullptr;
     int storage = SM_F32;
+    int max_blocks_per_cu = 1;   // persistent workgroups per CU (capped; see psr_batch_prepare)
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
     uint32_t* d_iters = nullptr;
@@ -872,7 +873,8 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_root_cfv, L.cards.size(), b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
-    HIP_OK(dcfr_occupancy(L.max_hands, &b->blocks_per_cu));
+    HIP_OK(dcfr_occupancy(L.max_hands, b->storage, &b->blocks_per_cu));
+    if (b->max_blocks_per_cu > 0) b->blocks_per_cu = std::min(b->blocks_per_cu, b->max_blocks_per_cu);
     if (b->blocks_per_cu < 1) return fail("kernel does not fit on a CU");
     b->grid = b->num_cus * b->blocks_per_cu;
     b->prepared = true;

@@ -246,6 +246,33 @@ inline int __builtin_amdgcn_readfirstlane(int x) {
     return first;
 }
 inline const void* __builtin_amdgcn_kernarg_segment_ptr() { return emu::cur_kernarg; }
+// DPP move (the patterns the kernels use): row_shr:n (0x111-0x11F) inside a
+// row of 16 lanes, row_bcast:15 (0x142, lane 15 of a row into the next row),
+// row_bcast:31 (0x143, lane 31 into rows 2 and 3), wave_shr:1 (0x138).
+// Rows left out by row_mask keep `old`; a source outside the pattern gives 0
+// with bound_ctrl, `old` without.
+inline int __builtin_amdgcn_update_dpp(int old, int src, int ctrl, int row_mask, int bank_mask, bool bound_ctrl) {
+    const int lane = threadIdx.x & 63, row = lane / 16;
+    int from = -1;
+    if (ctrl >= 0x111 && ctrl <= 0x11F) {
+        const int n = ctrl - 0x110;
+        if ((lane & 15) >= n) from = lane - n;
+    } else if (ctrl == 0x142) {
+        if (row >= 1) from = row * 16 - 1;
+    } else if (ctrl == 0x143) {
+        if (row >= 2) from = 31;
+    } else if (ctrl == 0x138) {
+        if (lane >= 1) from = lane - 1;
+    } else {
+        std::fprintf(stderr, "emulated DPP control 0x%x not supported\n", ctrl);
+        std::abort();
+    }
+    const int v = emu::exchange(src, from >= 0 ? from : lane);   // every lane takes part
+    const bool enabled = ((row_mask >> row) & 1) && ((bank_mask >> ((lane & 15) / 4)) & 1);
+    if (!enabled) return old;
+    if (from < 0) return bound_ctrl ? 0 : old;
+    return v;
+}
 inline long long __double_as_longlong(double d) { long long i; std::memcpy(&i, &d, 8); return i; }
 inline double __longlong_as_double(long long i) { double d; std::memcpy(&d, &i, 8); return d; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
