@@ -34,6 +34,7 @@ constexpr int WAVE = 64;
 constexpr int NWAVES = BLOCK / WAVE;
 constexpr int MAXA = 16;   // largest action count of a decision node on the device
 constexpr int NA_REG = 4;  // nodes with up to this many actions keep their rows in registers
+constexpr bool PAIR_NODES = true;   // node passes take two such nodes of a level at a time
 // Hands per thread: a row of H <= 1326 hands is covered by KH strided slots
 // h = threadIdx.x + k * BLOCK.  The per-hand loops are unrolled over k and
 // split into a load phase and a compute/store phase, so every thread has up
@@ -253,6 +254,53 @@ __device__ __forceinline__ float block_max(float v, Shared* sh) {
     return uni(m);
 }
 
+// An opponent's decision node on the register path of the forward pass: its
+// state, the reach row it reads, its children's rows and its action count.
+template <int SM>
+struct FwdNode {
+    NodeState<SM> st;
+    const float* rn;
+    float* rc;
+    int na;
+    float def;
+};
+
+// NN opponent nodes (f32 rows, at most NA_REG actions each): regret matching
+// (DCFR) or the average strategy (best response, cfvalues) -- solver.rs:600-626,
+// utility.rs:229-245 -- denominator over the actions, then child reach =
+// sigma * reach.  Every row of every node is loaded once, all in flight
+// together.
+template <int MODE, int SM, int NN>
+__device__ __forceinline__ void opponent_rows(const FwdNode<SM> (&nd)[NN], int stride, int Ho) {
+    float r[NN][KH], v[NN][NA_REG][KH];
+#pragma unroll
+    for (int i = 0; i < NN; ++i) {
+        FOR_K r[i][k] = hslot(k) < Ho ? nd[i].rn[uslot(k)] : 0.0f;
+#pragma unroll
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < nd[i].na) {
+                const NodeState<SM> sa = action_state(nd[i].st, a, Ho);
+                FOR_K v[i][a][k] = hslot(k) < Ho ? node_value<MODE>(sa, uslot(k)) : 0.0f;
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < NN; ++i) {
+        float den[KH];
+#pragma unroll
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < nd[i].na) FOR_K {
+                const float x = MODE == MODE_DCFR ? fmax0(v[i][a][k]) : v[i][a][k];
+                den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+            }
+#pragma unroll
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < nd[i].na) {
+                float* ra = nd[i].rc + (size_t)a * stride;
+                FOR_K if (hslot(k) < Ho) ra[uslot(k)] = __fmul_rn(strat_of<MODE>(v[i][a][k], den[k], nd[i].def), r[i][k]);
+            }
+    }
+}
+
 // Forward pass over one tile: reach of the opponent of p, level by level.
 // The game-root tile seeds its root row with the opponent's initial weights;
 // a subtree tile finds its root row already written by the chance node above.
@@ -283,6 +331,34 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
             any = true;
             const V* rn = reach + (size_t)reach_src(c, n, p) * c.stride;
             V* rc = reach + (size_t)nd.first_child * c.stride;
+            if constexpr (sizeof(V) == 4) {
+                if (nd.kind == KIND_DECISION && na <= NA_REG) {
+                    // the opponent's node on the register path, paired with the
+                    // next such node of the level when there is one
+                    FwdNode<SM> fn[2] = {{node_state<SM>(A, c.G, nd), rn, rc, na, __fdiv_rn(1.0f, (float)na)},
+                                         {}};
+                    int n2 = n + 1;
+                    NodeRec nd2 = nd;
+                    if (PAIR_NODES) {
+                        for (; n2 < lr.end; ++n2) {
+                            nd2 = uni_load(c.nodes + n2);
+                            if (nd2.kind == KIND_CHANCE || (nd2.kind == KIND_DECISION && nd2.player == o && nd2.num_children > 1))
+                                break;
+                        }
+                    }
+                    if (PAIR_NODES && n2 < lr.end && nd2.kind == KIND_DECISION && nd2.num_children <= NA_REG) {
+                        fn[1] = {node_state<SM>(A, c.G, nd2), reach + (size_t)reach_src(c, n2, p) * c.stride,
+                                 reach + (size_t)nd2.first_child * c.stride, (int)nd2.num_children,
+                                 __fdiv_rn(1.0f, (float)nd2.num_children)};
+                        opponent_rows<MODE, SM, 2>(fn, c.stride, Ho);
+                        n = n2;
+                    } else {
+                        const FwdNode<SM> f1[1] = {fn[0]};
+                        opponent_rows<MODE, SM, 1>(f1, c.stride, Ho);
+                    }
+                    continue;
+                }
+            }
             V r[KH];
             FOR_K r[k] = hslot(k) < Ho ? rn[hslot(k)] : V(0);
             if (nd.kind == KIND_DECISION && nd.player == o && na > 1) {
@@ -303,30 +379,6 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                             rc[(size_t)a * c.stride + hslot(k)] = sig * r[k];
                         }
                     }
-                } else if (na <= NA_REG) {
-                    // regret matching / average strategy, solver.rs:600-626,
-                    // utility.rs:229-245: the action rows are loaded once,
-                    // all in flight together; denominator, then sigma
-                    const float def = __fdiv_rn(1.0f, (float)na);
-                    float v[NA_REG][KH], den[KH];
-#pragma unroll
-                    for (int a = 0; a < NA_REG; ++a)
-                        if (a < na) {
-                            const NodeState<SM> sa = action_state(st, a, Ho);
-                            FOR_K v[a][k] = hslot(k) < Ho ? node_value<MODE>(sa, uslot(k)) : 0.0f;
-                        }
-#pragma unroll
-                    for (int a = 0; a < NA_REG; ++a)
-                        if (a < na) FOR_K {
-                            const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
-                            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
-                        }
-#pragma unroll
-                    for (int a = 0; a < NA_REG; ++a)
-                        if (a < na) {
-                            V* ra = rc + (size_t)a * c.stride;
-                            FOR_K if (hslot(k) < Ho) ra[uslot(k)] = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), r[k]);
-                        }
                 } else {
                     // regret matching / average strategy, solver.rs:600-626,
                     // utility.rs:229-245: denominator over actions, then sigma
@@ -715,68 +767,119 @@ __device__ __forceinline__ void update_16bit(Ctx& c, const NodeRec& nd, const No
 }
 
 // Backward pass over one tile, deepest level first.
-// Traverser node, f32 rows, at most NA_REG actions, hand slots [K0, K1):
-// sigma of every action (denominator first), the node value
+// A decision node on the register path of the backward pass: its state, the
+// cfv rows of its children, its own cfv row and its action count.
+template <int SM>
+struct BwdNode {
+    NodeState<SM> st;
+    const float* cc;
+    float* out;
+    int na;
+    float def;
+};
+
+// NN traverser nodes (f32 rows, at most NA_REG actions each), hand slots
+// [K0, K1): sigma of every action (denominator first), the node value
 // sum_a sigma_a * cfv_a (solver.rs:334-383), and in DCFR mode the updates
 // S = S * gamma + sigma, R = (R * (alpha | beta) + cfv_a) - value
-// (solver.rs:556-589).  Every row of the slots is loaded once, all in flight
-// together.
-template <int MODE, int SM, int K0, int K1>
-__device__ __forceinline__ void traverser_rows(const NodeState<SM>& st, const float* cc, int stride, float* out, int Hp,
-                                               int na, float def, const float4 disc) {
+// (solver.rs:556-589).  Every row of every node is loaded once, all in
+// flight together; then each node is computed and stored.
+template <int MODE, int SM, int NN, int K0, int K1>
+__device__ __forceinline__ void traverser_rows(const BwdNode<SM> (&nd)[NN], int stride, int Hp, const float4 disc) {
     constexpr int NK = K1 - K0;
-    float v[NA_REG][NK], cv[NA_REG][NK], sv[NA_REG][NK];
+    float v[NN][NA_REG][NK], cv[NN][NA_REG][NK], sv[NN][NA_REG][NK];
 #pragma unroll
-    for (int a = 0; a < NA_REG; ++a)
-        if (a < na) {
-            const NodeState<SM> sa = action_state(st, a, Hp);
-            const float* ca = cc + (size_t)a * stride;
+    for (int i = 0; i < NN; ++i)
 #pragma unroll
-            for (int j = 0; j < NK; ++j) {
-                const bool live = hslot(K0 + j) < Hp;
-                v[a][j] = live ? node_value<MODE>(sa, uslot(K0 + j)) : 0.0f;
-                cv[a][j] = live ? ca[uslot(K0 + j)] : 0.0f;
-                if (MODE == MODE_DCFR) sv[a][j] = live ? sa.S[uslot(K0 + j)] : 0.0f;
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < nd[i].na) {
+                const NodeState<SM> sa = action_state(nd[i].st, a, Hp);
+                const float* ca = nd[i].cc + (size_t)a * stride;
+#pragma unroll
+                for (int j = 0; j < NK; ++j) {
+                    const bool live = hslot(K0 + j) < Hp;
+                    v[i][a][j] = live ? node_value<MODE>(sa, uslot(K0 + j)) : 0.0f;
+                    cv[i][a][j] = live ? ca[uslot(K0 + j)] : 0.0f;
+                    if (MODE == MODE_DCFR) sv[i][a][j] = live ? sa.S[uslot(K0 + j)] : 0.0f;
+                }
             }
-        }
-    float den[NK], res[NK];
 #pragma unroll
-    for (int a = 0; a < NA_REG; ++a)
-        if (a < na) {
-#pragma unroll
-            for (int j = 0; j < NK; ++j) {
-                const float x = MODE == MODE_DCFR ? fmax0(v[a][j]) : v[a][j];
-                den[j] = a == 0 ? x : __fadd_rn(den[j], x);
-            }
-        }
-#pragma unroll
-    for (int a = 0; a < NA_REG; ++a)
-        if (a < na) {
-#pragma unroll
-            for (int j = 0; j < NK; ++j) {
-                const float t = __fmul_rn(strat_of<MODE>(v[a][j], den[j], def), cv[a][j]);
-                res[j] = a == 0 ? t : __fadd_rn(res[j], t);
-            }
-        }
-    if (MODE == MODE_DCFR) {
+    for (int i = 0; i < NN; ++i) {
+        const int na = nd[i].na;
+        const float def = nd[i].def;
+        float den[NK], res[NK];
 #pragma unroll
         for (int a = 0; a < NA_REG; ++a)
             if (a < na) {
-                const NodeState<SM> sa = action_state(st, a, Hp);
 #pragma unroll
-                for (int j = 0; j < NK; ++j)
-                    if (hslot(K0 + j) < Hp) {
-                        const unsigned h = uslot(K0 + j);
-                        const float sig = strat_of<MODE>(v[a][j], den[j], def);
-                        sa.S[h] = __fadd_rn(__fmul_rn(sv[a][j], disc.z), sig);
-                        const float coef = signbit(v[a][j]) ? disc.y : disc.x;
-                        sa.R[h] = __fsub_rn(__fadd_rn(__fmul_rn(v[a][j], coef), cv[a][j]), res[j]);
-                    }
+                for (int j = 0; j < NK; ++j) {
+                    const float x = MODE == MODE_DCFR ? fmax0(v[i][a][j]) : v[i][a][j];
+                    den[j] = a == 0 ? x : __fadd_rn(den[j], x);
+                }
             }
-    }
 #pragma unroll
-    for (int j = 0; j < NK; ++j)
-        if (hslot(K0 + j) < Hp) out[uslot(K0 + j)] = res[j];
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < na) {
+#pragma unroll
+                for (int j = 0; j < NK; ++j) {
+                    const float t = __fmul_rn(strat_of<MODE>(v[i][a][j], den[j], def), cv[i][a][j]);
+                    res[j] = a == 0 ? t : __fadd_rn(res[j], t);
+                }
+            }
+        if (MODE == MODE_DCFR) {
+#pragma unroll
+            for (int a = 0; a < NA_REG; ++a)
+                if (a < na) {
+                    const NodeState<SM> sa = action_state(nd[i].st, a, Hp);
+#pragma unroll
+                    for (int j = 0; j < NK; ++j)
+                        if (hslot(K0 + j) < Hp) {
+                            const unsigned h = uslot(K0 + j);
+                            const float sig = strat_of<MODE>(v[i][a][j], den[j], def);
+                            sa.S[h] = __fadd_rn(__fmul_rn(sv[i][a][j], disc.z), sig);
+                            const float coef = signbit(v[i][a][j]) ? disc.y : disc.x;
+                            sa.R[h] = __fsub_rn(__fadd_rn(__fmul_rn(v[i][a][j], coef), cv[i][a][j]), res[j]);
+                        }
+                }
+        }
+#pragma unroll
+        for (int j = 0; j < NK; ++j)
+            if (hslot(K0 + j) < Hp) nd[i].out[uslot(K0 + j)] = res[j];
+    }
+}
+
+// NN nodes whose value is a function of their children's rows only, at most
+// NA_REG children each: the sum (opponent and single-action nodes) or, for
+// the best response, the max (traverser nodes).  All rows in flight together.
+template <bool MAX, int NN, typename V>
+__device__ __forceinline__ void children_rows(const V* const (&cc)[NN], V* const (&out)[NN], const int (&na)[NN],
+                                              int stride, int Hp) {
+    V cv[NN][NA_REG][KH];
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+#pragma unroll
+        for (int a = 0; a < NA_REG; ++a)
+            if (a < na[i]) {
+                const V* ca = cc[i] + (size_t)a * stride;
+                FOR_K cv[i][a][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+            }
+#pragma unroll
+    for (int i = 0; i < NN; ++i) {
+        V res[KH];
+        FOR_K res[k] = cv[i][0][k];
+#pragma unroll
+        for (int a = 1; a < NA_REG; ++a)
+            if (a < na[i]) {
+                if constexpr (MAX) {
+                    FOR_K res[k] = fmaxr(res[k], cv[i][a][k]);
+                } else if constexpr (sizeof(V) == 8) {
+                    FOR_K res[k] += cv[i][a][k];
+                } else {
+                    FOR_K res[k] = __fadd_rn(res[k], cv[i][a][k]);
+                }
+            }
+        FOR_K if (hslot(k) < Hp) out[i][uslot(k)] = res[k];
+    }
 }
 
 // CHANCE: the tile may hold chance nodes (only a turn game's root tile does;
@@ -798,6 +901,41 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
             const int na = nd.num_children;
             V* out = cfv + (size_t)n * c.stride;
             const V* cc = cfv + (size_t)nd.first_child * c.stride;
+            if constexpr (PAIR_NODES && sizeof(V) == 4) {
+                // two decision nodes of the level on the same register path:
+                // both nodes' rows in flight together
+                if (nd.kind == KIND_DECISION && na <= NA_REG) {
+                    int n2 = n + 1;
+                    NodeRec nd2 = nd;
+                    for (; n2 < lr.end; ++n2) {
+                        nd2 = uni_load(c.nodes + n2);
+                        if (nd2.kind == KIND_DECISION || nd2.kind == KIND_CHANCE) break;
+                    }
+                    const bool trav = nd.player == p && na > 1;
+                    const int na2 = nd2.num_children;
+                    if (n2 < lr.end && nd2.kind == KIND_DECISION && na2 <= NA_REG && (nd2.player == p && na2 > 1) == trav) {
+                        V* out2 = cfv + (size_t)n2 * c.stride;
+                        const V* cc2 = cfv + (size_t)nd2.first_child * c.stride;
+                        if (!trav || MODE == MODE_BR) {
+                            const V* ccs[2] = {cc, cc2};
+                            V* outs[2] = {out, out2};
+                            const int nas[2] = {na, na2};
+                            if (trav) children_rows<true, 2, V>(ccs, outs, nas, c.stride, Hp);
+                            else children_rows<false, 2, V>(ccs, outs, nas, c.stride, Hp);
+                            n = n2;
+                            continue;
+                        }
+                        if (!(MODE == MODE_DCFR && SM != SM_F32)) {
+                            const BwdNode<SM> bn[2] = {{node_state<SM>(A, c.G, nd), cc, out, na, __fdiv_rn(1.0f, (float)na)},
+                                                       {node_state<SM>(A, c.G, nd2), cc2, out2, na2,
+                                                        __fdiv_rn(1.0f, (float)na2)}};
+                            traverser_rows<MODE, SM, 2, 0, KH>(bn, c.stride, Hp, disc);
+                            n = n2;
+                            continue;
+                        }
+                    }
+                }
+            }
             if (nd.kind == KIND_CHANCE) {
                 if constexpr (CHANCE) {
                     // f64 sum over the dealt cards plus the isomorphic swaps
@@ -864,29 +1002,19 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
                 } else if (MODE == MODE_BR && na <= NA_REG) {
                     // best response: max over the children, all rows in flight together
-                    float cv[NA_REG][KH];
-#pragma unroll
-                    for (int a = 0; a < NA_REG; ++a)
-                        if (a < na) {
-                            const V* ca = cc + (size_t)a * c.stride;
-                            FOR_K cv[a][k] = hslot(k) < Hp ? ca[uslot(k)] : 0.0f;
-                        }
-                    float res[KH];
-                    FOR_K res[k] = cv[0][k];
-#pragma unroll
-                    for (int a = 1; a < NA_REG; ++a)
-                        if (a < na) FOR_K res[k] = fmaxr(res[k], cv[a][k]);
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                    const V* ccs[1] = {cc};
+                    V* outs[1] = {out};
+                    const int nas[1] = {na};
+                    children_rows<true, 1, V>(ccs, outs, nas, c.stride, Hp);
                 } else if (MODE != MODE_BR && na <= NA_REG && !(MODE == MODE_DCFR && SM != SM_F32)) {
                     // every row of the node loaded once, all in flight
-                    // together, in two halves of the hand slots (registers)
-                    const NodeState<SM> st = node_state<SM>(A, c.G, nd);
-                    const float def = __fdiv_rn(1.0f, (float)na);
+                    // together (registers)
+                    const BwdNode<SM> bn[1] = {{node_state<SM>(A, c.G, nd), cc, out, na, __fdiv_rn(1.0f, (float)na)}};
                     if constexpr (KH > 3) {
-                        traverser_rows<MODE, SM, 0, KH / 2>(st, cc, c.stride, out, Hp, na, def, disc);
-                        traverser_rows<MODE, SM, KH / 2, KH>(st, cc, c.stride, out, Hp, na, def, disc);
+                        traverser_rows<MODE, SM, 1, 0, KH / 2>(bn, c.stride, Hp, disc);
+                        traverser_rows<MODE, SM, 1, KH / 2, KH>(bn, c.stride, Hp, disc);
                     } else {
-                        traverser_rows<MODE, SM, 0, KH>(st, cc, c.stride, out, Hp, na, def, disc);
+                        traverser_rows<MODE, SM, 1, 0, KH>(bn, c.stride, Hp, disc);
                     }
                 } else if (MODE == MODE_BR) {
                     float res[KH];
@@ -955,25 +1083,10 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
             } else if (na <= NA_REG) {
                 // opponent (or single-action) node: sum of the children, all
                 // rows in flight together
-                V cv[NA_REG][KH];
-#pragma unroll
-                for (int a = 0; a < NA_REG; ++a)
-                    if (a < na) {
-                        const V* ca = cc + (size_t)a * c.stride;
-                        FOR_K cv[a][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
-                    }
-                V res[KH];
-                FOR_K res[k] = cv[0][k];
-#pragma unroll
-                for (int a = 1; a < NA_REG; ++a)
-                    if (a < na) {
-                        if constexpr (sizeof(V) == 8) {
-                            FOR_K res[k] += cv[a][k];
-                        } else {
-                            FOR_K res[k] = __fadd_rn(res[k], cv[a][k]);
-                        }
-                    }
-                FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                const V* ccs[1] = {cc};
+                V* outs[1] = {out};
+                const int nas[1] = {na};
+                children_rows<false, 1, V>(ccs, outs, nas, c.stride, Hp);
             } else {
                 // opponent (or single-action) node: sum of the children
                 V res[KH];
