@@ -50,3 +50,22 @@ def test_records_from_batch():
     o.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
     ev0 = np.asarray(o.expected_values(0), np.float64)
     assert np.allclose(b.expected_values(0, 0), ev0, rtol=1e-4, atol=1e-3)
+
+
+def test_river_datagen_end_to_end():
+    """Sampled situations (R(S, p) ranges) -> batched device solve -> records
+    (datagen/domain/solver.rs:33-95 contents): finite values on the valid
+    combos only, game values finite, every situation converged or capped."""
+    from poker_solver_rust_amd import datagen as D
+
+    cfg = D.DatagenConfig(solver_iterations=200)
+    recs, sits = D.generate_river_records(24, cfg, seed=5, chunk=16)
+    assert len(recs) == 48 and len(sits) == 24
+    for k, r in enumerate(recs):
+        s = sits[k // 2]
+        assert r.board == s.board and r.player == k % 2
+        assert np.all(np.isfinite(r.cfvs)) and np.isfinite(r.game_value)
+        assert np.all(r.cfvs[r.valid_mask == 0] == 0)
+        # pot-relative values of a river subgame lie within [-1 - 2 stack / pot, 1 + 2 stack / pot]
+        bound = 1.0 + 2.0 * s.stack / s.pot + 1e-3
+        assert np.all(np.abs(r.cfvs) <= bound)
