@@ -557,18 +557,13 @@ __device__ __forceinline__ void slot_fence() { __builtin_amdgcn_sched_barrier(0)
 // the scan positions and same-hand index of each of its traversing hands.
 // Every terminal of a tile usually shares one runout, so the terminal loop
 // loads them once per traversal.
-// HOLD_HAND_SCAN: keep the per-hand scan positions in registers across the
-// terminals (24 VGPRs) instead of loading them per terminal (an L2 hit,
-// issued before the scan so it overlaps it).
-#ifndef PSR_HOLD_HAND_SCAN
-#define PSR_HOLD_HAND_SCAN 1
-#endif
-constexpr bool HOLD_HAND_SCAN = PSR_HOLD_HAND_SCAN;
-
+// The per-hand scan positions stay in registers across the terminals of a
+// tile (loading them per terminal measured slower: an extra L2 round trip
+// per terminal).
 struct RunoutRegs {
     RunoutTab T;
     ScanRun u;
-    HandScan q[KH];   // filled only when HOLD_HAND_SCAN
+    HandScan q[KH];
     uint16_t si[KH];
 };
 
@@ -583,7 +578,7 @@ __device__ __forceinline__ void load_runout(const Ctx& c, int table, int p, Runo
     const int Hp = num_hands(c.G, p);
     R.T = uni_load(A.runouts + table * 2 + p);
     R.u = load_scan_run(A, R.T);
-    if constexpr (HOLD_HAND_SCAN) load_hand_scan(c, R.T, p, R.q);
+    load_hand_scan(c, R.T, p, R.q);
     const uint16_t* same = A.hand_same + hand_base(c.G, p);
     FOR_K R.si[k] = hslot(k) < Hp ? same[hslot(k)] : (uint16_t)0xFFFF;
 }
@@ -610,12 +605,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
         lds_barrier();
         segmented_scan<V>(c, R.u, R.T.scan_len);
     }
-    HandScan qv[KH];
-    if constexpr (HOLD_HAND_SCAN) {
-        FOR_K qv[k] = R.q[k];
-    } else {
-        load_hand_scan(c, R.T, p, qv);
-    }
+    const HandScan (&qv)[KH] = R.q;
 
     const double total = c.scan[R.T.total_pos];
     if (nd.kind == KIND_BOUNDARY) {
