@@ -1140,11 +1140,11 @@ __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p)
     __syncthreads();
 }
 
-template <int MODE, int SM, typename V = float>
+template <int MODE, int SM, typename V = float, bool CHANCE = false>
 __device__ __forceinline__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     tile_forward<MODE, SM, V>(c, T, p);
     tile_terminals<V>(c, T, p);
-    tile_backward<MODE, SM, V>(c, T, p, disc);
+    tile_backward<MODE, SM, V, CHANCE>(c, T, p, disc);
 }
 
 __device__ __forceinline__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
@@ -1355,7 +1355,8 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         const TileDesc T = uni_load(A.tiles + uni(A.subtree_tiles[q]));
         Ctx c = make_ctx(A, T.game, scan, rs, &sh);
         if (MODE == MODE_DCFR && uni(A.expl_out[T.game]) <= c.G.target) continue;
-        tile_traverse<MODE, SM, V>(c, T, p, disc);
+        if (T.flags & TILE_HAS_CHANCE) tile_traverse<MODE, SM, V, true>(c, T, p, disc);
+        else tile_traverse<MODE, SM, V>(c, T, p, disc);
     }
 }
 

@@ -5,8 +5,9 @@
 // HBM layout (all arrays concatenated over the batch):
 //   nodes[]      NodeRec per node in device order.  A game is cut into tiles:
 //                tile 0 holds the nodes down to (and including) the first
-//                chance level, every child of a chance node roots one more
-//                tile (a river subtree).  Inside a tile nodes are breadth-
+//                chance level, every child of a chance node there roots one
+//                more tile (a river subtree of a turn game; for a flop game a
+//                turn subtree holding its river deals and river subtrees).  Inside a tile nodes are breadth-
 //                first, so each depth level is one contiguous node range
 //                (level_ranges[level_base + L]); the roots of one chance
 //                node's tiles are contiguous so the chance node addresses
@@ -101,7 +102,11 @@ struct LevelRange {
     int32_t start, end;     // game-local device node ids [start, end)
 };
 
-enum TileFlags : int32_t { TILE_GAME_ROOT = 1 };
+enum TileFlags : int32_t {
+    TILE_GAME_ROOT = 1,    // level 0 is the game root
+    TILE_HAS_CHANCE = 2,   // the tile holds chance nodes (a turn game's root tile; a flop game's
+                           // root tile and its turn subtrees, which hold the river deals)
+};
 
 struct TileDesc {
     int32_t game;

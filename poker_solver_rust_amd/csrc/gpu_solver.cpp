@@ -179,6 +179,7 @@ struct TilePlan {
     std::vector<std::vector<std::pair<int, int>>> levels;   // per tile, per level [start,end)
     std::vector<std::vector<int>> terminals;        // per tile, device ids
     std::vector<double> cost;                       // per tile
+    std::vector<char> has_chance;                   // per tile: holds a chance node
 };
 
 int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
@@ -209,6 +210,7 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
     plan.levels.assign(1, {});
     plan.terminals.assign(1, {});
     plan.cost.assign(1, 0.0);
+    plan.has_chance.assign(1, !chance_nodes.empty());
     std::vector<int> pos(N, -1);
     for (size_t k = 0; k < plan.order.size(); ++k) pos[plan.order[k]] = (int)k;
     for (size_t k = 0; k < root_tile.size(); ++k) {
@@ -226,8 +228,6 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
         const GameNode& n = nodes[c];
         for (uint32_t a = 0; a < n.num_children; ++a) {
             const int r = (int)(n.first_child + a);
-            const GameNode& rn = nodes[r];
-            if (rn.is_chance()) return fail("nested chance levels (flop games) are not supported by the device path");
             pos[r] = (int)plan.order.size();
             plan.order.push_back(r);
             roots.push_back(r);
@@ -240,14 +240,18 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
         lv.push_back({pos[r], pos[r] + 1});
         if (nodes[r].is_terminal()) terms.push_back(pos[r]);
         double cost = 1.0;
-        std::vector<int> body;
+        // (the root itself may be a chance node: the river deal right after an
+        // all-in call on the turn deal)
+        bool plan_chance = nodes[r].is_chance();
         std::vector<int> frontier{r};
         while (!frontier.empty()) {
             std::vector<int> next;
             for (int v : frontier) {
                 const GameNode& n = nodes[v];
                 if (n.is_terminal()) continue;
-                if (n.is_chance()) return fail("nested chance levels (flop games) are not supported by the device path");
+                // a chance node inside a subtree (the river deal below a flop
+                // game's turn card) keeps its children in the same tile
+                if (n.is_chance()) plan_chance = true;
                 for (uint32_t a = 0; a < n.num_children; ++a) next.push_back((int)(n.first_child + a));
             }
             if (next.empty()) break;
@@ -264,6 +268,7 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
         plan.levels.push_back(lv);
         plan.terminals.push_back(terms);
         plan.cost.push_back(cost);
+        plan.has_chance.push_back(plan_chance);
     }
     if ((int)plan.order.size() != N) return fail("tile plan does not cover the game");
     return 0;
@@ -431,7 +436,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             td.term_base = (int32_t)L.terminals.size();
             td.num_terminals = (int32_t)plan.terminals[t].size();
             for (int v : plan.terminals[t]) L.terminals.push_back(v);
-            td.flags = t == 0 ? TILE_GAME_ROOT : 0;
+            td.flags = (t == 0 ? TILE_GAME_ROOT : 0) | (plan.has_chance[t] ? TILE_HAS_CHANCE : 0);
             td.runout = -1;
             for (size_t k = 0; k < plan.terminals[t].size(); ++k) {
                 const int aux = L.nodes[d.node_base + plan.terminals[t][k]].aux;
@@ -1295,7 +1300,7 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
         if (r.first_child < 0 || r.first_child + r.num_children > N) return fail("children out of range");
         for (int a = 0; a < r.num_children; ++a) {
             const int ch = r.first_child + a;
-            if (r.kind == KIND_CHANCE) {
+            if (r.kind == KIND_CHANCE && tile_of[v] == 0) {
                 if (tile_of[ch] == tile_of[v] || level_of[ch] != 0) return fail("chance child is not a subtree root");
             } else if (tile_of[ch] != tile_of[v] || level_of[ch] != level_of[v] + 1) {
                 return fail("child not in the next level of its tile");

@@ -194,6 +194,24 @@ def run_reach_at_nodes():
                 same(got[n], np.float32(o.reach_at_node(int(n), p)), f"reach of {p} at node {n}")
 
 
+def flop_game(oop="AA,KK", ip="QQ,AKs", board="Qs Jh 2c", bets="a"):
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board))
+    s = [(bets, ""), (bets, "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def run_flop_stepwise():
+    """A flop game on a monotone flop with suit-symmetric ranges: root tile
+    down to the turn deals (isomorphic turn cards folded in by the swaps),
+    one tile per turn card holding its river deals (isomorphic rivers too)
+    and river subtrees, so chance nodes inside a tile and terminals of many
+    runouts in one tile; state, exploitability and root cfvalues against the
+    oracle."""
+    check_stepwise(flop_game("AA,KK", "QQ,AK", "2s 7s Js"), 1)
+
+
 def run_half_storage():
     """FP16 / BF16 words with per-node scales: the solve converges like the
     FP64 oracle on the same tree (no reference counterpart)."""

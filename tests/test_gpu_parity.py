@@ -403,3 +403,50 @@ def test_reach_at_nodes_matches_oracle(make, iters):
             close(got[k], o.reach_at_node(int(n), p))
     with pytest.raises(P.PsrError):
         b.reach_at_nodes(0, 0, [g.num_nodes()])
+
+
+# ------------------------------------------------------------------ flop --
+
+def flop_game(oop, ip, board, bets="a", raises=""):
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board))
+    s = [(bets, raises), (bets, raises)]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+@pytest.mark.parametrize("make", [lambda: flop_game("AA,KK", "QQ,AKs", "Qs Jh 2c"),
+                                  lambda: flop_game("AA,KK", "QQ,AK", "2s 7s Js"),
+                                  lambda: flop_game("AA,KK,AKs", "QQ-JJ,AQs", "Qs Jh 2c", "50%, a")])
+def test_flop_stepwise_matches_oracle(make):
+    """Flop-root games: turn subtrees as tiles holding their river deals
+    (chance nodes inside a tile; suit-isomorphic turn and river cards on the
+    monotone flop), state after every iteration, exploitability in f32 and
+    f64, root cfvalues."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 4)
+    o = O.OracleGame(g.serialize())
+    for t in range(3):
+        b.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, s = o.state()
+        close(b.regrets(0), r)
+        close(b.strategy_sum(0), s)
+    assert abs(float(b.exploitability()[0]) - o.exploitability()) <= RTOL * max(1.0, o.exploitability())
+    cf = b.root_cfvalues()[0]
+    close(cf[0], o.root_cfvalues(0))
+    close(cf[1], o.root_cfvalues(1))
+    o64 = O.OracleGame(g.serialize(), 64)
+    o64.set_state(b.regrets(0).astype(np.float64), b.strategy_sum(0).astype(np.float64))
+    assert abs(float(b.exploitability_f64()[0]) - o64.exploitability()) <= 1e-9 * max(1.0, abs(o64.exploitability()))
+
+
+def test_flop_solve_schedule_matches_oracle():
+    g = flop_game("AA,KK", "QQ,AK", "2s 7s Js")
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 40, [0.5])
+    e, it = b.solve(40, 5)
+    eo, ito = O.OracleGame(g.serialize()).solve(40, 0.5)
+    assert int(it[0]) == ito
+    assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
