@@ -207,6 +207,83 @@ def net_solve_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def reduced_precision_bench(args):
+    """Config C4: the seeded turn subgame solved on the device with FP32,
+    FP16 and BF16 storage (16-bit words with a per-node FP32 scale, FP32
+    arithmetic) for `--c4-iters` DCFR iterations; the exploitability of each
+    by the device FP64 best response, and the per-hand root cfvalues against
+    the FP64 CPU oracle's solve of the identical tree (same iterations)."""
+    import poker_solver_rust_amd as P
+    from poker_solver_rust_amd import workloads as W
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as O
+
+    s = W.turn_uniform01(args.seed)
+    g = s.build()
+    iters = args.c4_iters
+    t = time.time()
+    o = O.OracleGame(g.serialize(), 64)
+    for k in range(iters):
+        o.solve_step(k)
+        if (k + 1) % 50 == 0:
+            log(f"fp64 oracle: {k + 1}/{iters} iterations")
+    ref = [o.root_cfvalues(p) for p in range(2)]
+    ref_expl = o.exploitability()
+    t_oracle = time.time() - t
+    out = {"metric": "reduced-precision turn solve: FP64 exploitability and root cfv error vs the FP64 oracle",
+           "config": {"workload": "turn_uniform01 seed %d" % args.seed, "iterations": iters, "nodes": g.num_nodes()},
+           "fp64_oracle": {"exploitability": ref_expl, "seconds": t_oracle}, "unit": "chips (pot %d)" % s.pot}
+    for storage in ("f32", "fp16", "bf16"):
+        b = P.GpuBatchSolver(0, storage=storage)
+        b.prepare_batch([g], iters)
+        t = time.perf_counter()
+        b.run_iterations(0, iters)
+        secs = time.perf_counter() - t
+        e64 = float(b.exploitability_f64()[0])
+        cf = b.root_cfvalues()[0]
+        err = max(float(np.abs(cf[p].astype(np.float64) - ref[p]).max()) for p in range(2))
+        scale = max(float(np.abs(ref[p]).max()) for p in range(2))
+        out[storage] = {"exploitability_fp64": e64, "gap_vs_fp64_oracle": e64 - ref_expl,
+                        "root_cfv_max_abs_err": err, "root_cfv_max_abs_err_rel": err / max(scale, 1e-30),
+                        "iters_per_s": iters / secs}
+        log(f"{storage}: exploitability {e64:.5f} (oracle {ref_expl:.5f}), root cfv max err {err:.3e}, "
+            f"{iters / secs:.1f} it/s")
+    print(json.dumps(out), flush=True)
+
+
+def river_uniform_bench(args):
+    """Config C1: one seeded river subgame (uniform ranges, 50%/100%/all-in,
+    one raise), 1000 DCFR iterations with the solve() checks every 5, on the
+    device (single-game path) and by the CPU oracle (one thread)."""
+    import poker_solver_rust_amd as P
+    from poker_solver_rust_amd import workloads as W
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as O
+
+    s = W.river_uniform(args.seed)
+    g = s.build()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 1000)
+    b.solve(1000, 5)
+    b.reset()
+    t = time.perf_counter()
+    e, it = b.solve(1000, 5)
+    secs = time.perf_counter() - t
+    o = O.OracleGame(g.serialize())
+    t = time.time()
+    eo, ito = o.solve(1000, 0.0)
+    t_cpu = time.time() - t
+    out = {"metric": "single river subgame, 1000 DCFR iterations: seconds", "config": {
+               "workload": "river_uniform seed %d" % args.seed, "nodes": g.num_nodes(),
+               "edge_hands": g.num_storage()},
+           "device": {"seconds": secs, "exploitability": float(e[0]), "iterations": int(it[0])},
+           "cpu_oracle_1_thread": {"seconds": t_cpu, "exploitability": eo, "iterations": ito},
+           "higher_is_better": False}
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of `nbytes` (float32 tensors, after a
     warm-up copy), the known-size kernel the FETCH_SIZE / WRITE_SIZE counters
@@ -248,12 +325,21 @@ def main():
     ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--turn-solve-iters", type=int, default=500)
+    ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (config C4) instead")
+    ap.add_argument("--c4-iters", type=int, default=500)
+    ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations (config C1) instead")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
 
     if args.copy_bytes > 0:
         copy_probe(args.copy_bytes)
+        return
+    if args.c4:
+        reduced_precision_bench(args)
+        return
+    if args.c1:
+        river_uniform_bench(args)
         return
     if args.net_solve:
         net_solve_bench(args)
