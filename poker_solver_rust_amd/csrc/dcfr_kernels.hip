@@ -51,6 +51,7 @@ struct Shared {
     int job;
     int wave_f[NWAVES];
     float wave_m[NWAVES];
+    float wave_m2[NWAVES];
     double wave_s[NWAVES];
 };
 
@@ -241,6 +242,31 @@ __device__ __forceinline__ float strat_of(float v, float denom, float def) {
 }
 
 // max over the workgroup of one non-negative f32 per thread (exact, any order)
+// two maxima at once (one pair of barriers): the new regret and strategy-sum
+// scales of a 16-bit node; non-negative inputs, exact in any order
+__device__ __forceinline__ void block_max2(float& a, float& b, Shared* sh) {
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        a = fmaxf(a, __shfl_xor(a, d, WAVE));
+        b = fmaxf(b, __shfl_xor(b, d, WAVE));
+    }
+    if (lane == 0) {
+        sh->wave_m[wave] = a;
+        sh->wave_m2[wave] = b;
+    }
+    __syncthreads();
+    float ma = sh->wave_m[0], mb = sh->wave_m2[0];
+#pragma unroll
+    for (int w = 1; w < NWAVES; ++w) {
+        ma = fmaxf(ma, sh->wave_m[w]);
+        mb = fmaxf(mb, sh->wave_m2[w]);
+    }
+    __syncthreads();
+    a = uni(ma);
+    b = uni(mb);
+}
+
 __device__ __forceinline__ float block_max(float v, Shared* sh) {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
 #pragma unroll
@@ -728,8 +754,7 @@ __device__ __forceinline__ void update_16bit(Ctx& c, const NodeRec& nd, const No
             smax = fmaxf(smax, ns);
         }
     }
-    rmax = block_max(rmax, c.sh);
-    smax = block_max(smax, c.sh);
+    block_max2(rmax, smax, c.sh);
     const float rnz = rmax == 0.0f ? 1.0f : rmax, snz = smax == 0.0f ? 1.0f : smax;
     const float renc = SM == SM_REF16 ? __fdiv_rn(32767.0f, rnz) : __fdiv_rn(1.0f, rnz);
     const float senc = SM == SM_REF16 ? __fdiv_rn(65535.0f, snz) : __fdiv_rn(1.0f, snz);
@@ -835,6 +860,105 @@ __device__ __forceinline__ void traverser_rows(const BwdNode<SM> (&nd)[NN], int 
 #pragma unroll
         for (int j = 0; j < NK; ++j)
             if (hslot(K0 + j) < Hp) nd[i].out[uslot(K0 + j)] = res[j];
+    }
+}
+
+// Traverser node with 16-bit storage (DCFR mode), at most NA_REG actions:
+// the same sigma / node value / updates as traverser_rows, with the new f32
+// regrets and strategy sums kept in registers while the workgroup takes
+// their maxima for the node's new scales, then encoded (utility.rs:142-202
+// for the reference's i16 / u16 words; FP16 / BF16 with round-to-nearest-
+// even) -- no scratch rows, every row of the node in flight together.
+template <int SM>
+__device__ __forceinline__ void traverser_rows16(Ctx& c, const NodeRec& nd, const NodeState<SM>& st, const float* cc,
+                                                 float* out, int Hp, const float4 disc) {
+    const int na = nd.num_children;
+    const float def = __fdiv_rn(1.0f, (float)na);
+    uint16_t rw[NA_REG][KH], sw[NA_REG][KH];
+    float cv[NA_REG][KH];
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) {
+            const NodeState<SM> sa = action_state(st, a, Hp);
+            const float* ca = cc + (size_t)a * c.stride;
+            FOR_K {
+                const bool live = hslot(k) < Hp;
+                rw[a][k] = live ? sa.R16[uslot(k)] : (uint16_t)0;
+                sw[a][k] = live ? sa.S16[uslot(k)] : (uint16_t)0;
+                cv[a][k] = live ? ca[uslot(k)] : 0.0f;
+            }
+        }
+    auto reg = [&](uint16_t w) {
+        if (SM == SM_REF16) return (float)(int16_t)w;
+        if (SM == SM_FP16) return __fmul_rn(fp16_to_f32(w), st.rs);
+        return __fmul_rn(bf16_to_f32(w), st.rs);
+    };
+    auto str = [&](uint16_t w) {
+        if (SM == SM_REF16) return (float)w;
+        if (SM == SM_FP16) return __fmul_rn(fp16_to_f32(w), st.ss);
+        return __fmul_rn(bf16_to_f32(w), st.ss);
+    };
+    float den[KH], res[KH];
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) FOR_K {
+            const float x = fmax0(reg(rw[a][k]));
+            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+        }
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) FOR_K {
+            const float t = __fmul_rn(strat_of<MODE_DCFR>(reg(rw[a][k]), den[k], def), cv[a][k]);
+            res[k] = a == 0 ? t : __fadd_rn(res[k], t);
+        }
+    const float sdec = __fdiv_rn(__fmul_rn(disc.z, st.ss), 65535.0f);
+    const float adec = __fdiv_rn(__fmul_rn(disc.x, st.rs), 32767.0f);
+    const float bdec = __fdiv_rn(__fmul_rn(disc.y, st.rs), 32767.0f);
+    float nr[NA_REG][KH], ns[NA_REG][KH];
+    float rmax = 0.0f, smax = 0.0f;
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) FOR_K if (hslot(k) < Hp) {
+            const float rv = reg(rw[a][k]);
+            const float sig = strat_of<MODE_DCFR>(rv, den[k], def);
+            if (SM == SM_REF16) {
+                const int16_t r = (int16_t)rw[a][k];
+                ns[a][k] = __fadd_rn(sig, __fmul_rn((float)sw[a][k], sdec));
+                nr[a][k] = __fsub_rn(__fadd_rn(cv[a][k], __fmul_rn((float)r, r >= 0 ? adec : bdec)), res[k]);
+            } else {
+                const float coef = signbit(rv) ? disc.y : disc.x;
+                ns[a][k] = __fadd_rn(__fmul_rn(str(sw[a][k]), disc.z), sig);
+                nr[a][k] = __fsub_rn(__fadd_rn(__fmul_rn(rv, coef), cv[a][k]), res[k]);
+            }
+            rmax = fmaxf(rmax, fabsf(nr[a][k]));
+            smax = fmaxf(smax, ns[a][k]);
+        }
+    FOR_K if (hslot(k) < Hp) out[uslot(k)] = res[k];
+    block_max2(rmax, smax, c.sh);
+    const float rnz = rmax == 0.0f ? 1.0f : rmax, snz = smax == 0.0f ? 1.0f : smax;
+    const float renc = SM == SM_REF16 ? __fdiv_rn(32767.0f, rnz) : __fdiv_rn(1.0f, rnz);
+    const float senc = SM == SM_REF16 ? __fdiv_rn(65535.0f, snz) : __fdiv_rn(1.0f, snz);
+#pragma unroll
+    for (int a = 0; a < NA_REG; ++a)
+        if (a < na) {
+            const NodeState<SM> sa = action_state(st, a, Hp);
+            FOR_K if (hslot(k) < Hp) {
+                const unsigned h = uslot(k);
+                if (SM == SM_REF16) {
+                    sa.R16[h] = (uint16_t)(int16_t)(int32_t)roundf(__fmul_rn(nr[a][k], renc));
+                    sa.S16[h] = (uint16_t)(int32_t)__fadd_rn(__fmul_rn(ns[a][k], senc), 0.49999997f);
+                } else if (SM == SM_FP16) {
+                    sa.R16[h] = f32_to_fp16(__fmul_rn(nr[a][k], renc));
+                    sa.S16[h] = f32_to_fp16(__fmul_rn(ns[a][k], senc));
+                } else {
+                    sa.R16[h] = f32_to_bf16(__fmul_rn(nr[a][k], renc));
+                    sa.S16[h] = f32_to_bf16(__fmul_rn(ns[a][k], senc));
+                }
+            }
+        }
+    if (threadIdx.x == 0) {
+        c.A.rscale[nd.aux] = rmax;
+        c.A.sscale[nd.aux] = smax;
     }
 }
 
@@ -996,6 +1120,8 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     V* outs[1] = {out};
                     const int nas[1] = {na};
                     children_rows<true, 1, V>(ccs, outs, nas, c.stride, Hp);
+                } else if (MODE == MODE_DCFR && SM != SM_F32 && na <= NA_REG) {
+                    traverser_rows16<SM>(c, nd, node_state<SM>(A, c.G, nd), cc, out, Hp, disc);
                 } else if (MODE != MODE_BR && na <= NA_REG && !(MODE == MODE_DCFR && SM != SM_F32)) {
                     // every row of the node loaded once, all in flight
                     // together (registers)
