@@ -135,7 +135,7 @@ def net_solve_bench(args):
     the river deal solved with the net at its leaves.  Yardsticks on the same
     tree: check-down leaf values (the method without the net), exact river
     solves at the leaves (the fallback) and the exact full turn solve (the
-    judge).  Errors are root expected values (query.rs:639-666) as a fraction
+    target).  Errors are root expected values (query.rs:639-666) as a fraction
     of the pot."""
     import poker_solver_rust_amd as P
     from poker_solver_rust_amd import net_solve as N
@@ -181,7 +181,7 @@ def net_solve_bench(args):
     for k, v in err.items():
         log(f"{k}: {v}")
     gv = {k: max(v["p0_value_diff_pot"], v["p1_value_diff_pot"]) for k, v in err.items()}
-    # the leaf judge: leaf values at the exact-leaf solve's final (average
+    # leaf yardstick: leaf values at the exact-leaf solve's final (average
     # strategy) reaches, net and check-down against exact river solves
     items = N.boundary_items(turn, g_exl, b_exl)
     v_ex = N.ExactLeaves(turn, args.iterations).values_for(items)
@@ -208,7 +208,7 @@ def net_solve_bench(args):
 
 
 def reduced_precision_bench(args):
-    """Config C4: the seeded turn subgame solved on the device with FP32,
+    """Reduced-precision turn solve: the seeded turn subgame solved on the device with FP32,
     FP16 and BF16 storage (16-bit words with a per-node FP32 scale, FP32
     arithmetic) for `--c4-iters` DCFR iterations; the exploitability of each
     by the device FP64 best response, and the per-hand root cfvalues against
@@ -253,7 +253,7 @@ def reduced_precision_bench(args):
 
 
 def river_uniform_bench(args):
-    """Config C1: one seeded river subgame (uniform ranges, 50%/100%/all-in,
+    """One seeded river subgame (uniform ranges, 50%/100%/all-in,
     one raise), 1000 DCFR iterations with the solve() checks every 5, on the
     device (single-game path) and by the CPU oracle (one thread)."""
     import poker_solver_rust_amd as P
@@ -325,9 +325,9 @@ def main():
     ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--turn-solve-iters", type=int, default=500)
-    ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (config C4) instead")
+    ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")
     ap.add_argument("--c4-iters", type=int, default=500)
-    ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations (config C1) instead")
+    ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations, device and CPU oracle instead")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -346,7 +346,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); see the sharding notes in DESIGN.md",
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharding across GPUs is a design note, not built",
               file=sys.stderr)
         sys.exit(2)
 

@@ -168,7 +168,7 @@ int psr_batch_solve(psr_batch* batch, uint32_t max_iterations, int32_t check_int
 int psr_batch_exploitability(psr_batch* batch, float* expl_out);
 /* compute_exploitability carried out in f64 on the device (average strategy,
  * reach, terminal values and backups all f64): the FP64 exploitability pass
- * used to judge reduced-precision solves. */
+ * used to measure reduced-precision solves. */
 int psr_batch_exploitability_f64(psr_batch* batch, double* expl_out);
 /* utility.rs:407-443 root_cfvalues: per-hand root CFVs of every game under
  * the average strategy; after this call psr_batch_root_cfvalues reads them. */

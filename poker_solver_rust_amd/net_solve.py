@@ -390,7 +390,7 @@ class ExactLeaves:
     """Leaf values from exact river solves: every (boundary, river card)
     subgame with the recorded reaches as ranges, solved in one batched GPU
     launch per refresh (stop at 0.3% of the pot, cap `iterations`) -- the
-    fallback and the error judge of the net."""
+    fallback and the yardstick of the net."""
 
     def __init__(self, turn: Subgame, iterations: int = 300):
         self.turn = turn

@@ -4,7 +4,7 @@ compressed: the reference's i16 regrets / u16 strategy sums with per-node
 scales (allocate_memory(true); utility.rs:142-202, solver.rs:486-553) --
 compared word for word with the oracle's compressed mode.
 fp16 / bf16: value/scale words with a per-node f32 scale and f32 arithmetic
--- no reference counterpart, so they are judged against the FP64 oracle on
+-- no reference counterpart, so they are measured against the FP64 oracle on
 the same tree: exploitability gap and per-hand CFV error in chips.
 """
 import numpy as np
