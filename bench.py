@@ -24,6 +24,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 MFMA
 HBM_PEAK_GBS = 8000.0
 METRIC = "river-subgame solves/sec at <=0.3% pot exploitability; turn DCFR iters/s"
 
@@ -284,6 +285,40 @@ def river_uniform_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def mlp_bench(args):
+    """Value-net throughput on the 2706 -> 4 x 512 -> 2652 net (BF16 MFMA):
+    inference on 8192 rows, and training epochs at batch 512 and 4096 over a
+    32768-row resident set; FLOP counts 2 * sum(d_in * d_out) per row forward
+    and 3x that per training row (forward, input and weight gradients)."""
+    from poker_solver_rust_amd.value_net import DEFAULT_DIMS, ValueNet
+
+    dims = DEFAULT_DIMS
+    fwd_flop = 2.0 * sum(dims[i] * dims[i + 1] for i in range(len(dims) - 1))
+    rng = np.random.default_rng(args.seed)
+    n = 32768
+    x = rng.normal(0, 1, (n, dims[0])).astype(np.float32)
+    y = rng.normal(0, 0.1, (n, dims[-1])).astype(np.float32)
+    out = {"metric": "value-net MFMA throughput", "unit": "TFLOP/s", "peak_bf16_dense": BF16_PEAK_TFLOPS,
+           "flop_per_row_forward": fwd_flop}
+    net = ValueNet(max_batch=8192, seed=1)
+    net.forward(x[:8192])
+    net.forward(x[:8192])
+    ms = net.last_ms()
+    out["forward_8192_rows"] = {"ms": ms, "tflops": fwd_flop * 8192 / (ms * 1e-3) / 1e12}
+    net.set_dataset(x, y, np.ones_like(y))
+    for batch in (512, 4096):
+        net.train(1, batch, seed=2)
+        net.train(2, batch, seed=3)
+        ms = net.last_ms()
+        rows = 2 * (n // batch) * batch
+        tf = 3.0 * fwd_flop * rows / (ms * 1e-3) / 1e12
+        out[f"train_batch_{batch}"] = {"ms_per_epoch": ms / 2, "rows_per_s": rows / (ms * 1e-3), "tflops": tf,
+                                       "frac_of_peak": tf / BF16_PEAK_TFLOPS}
+        log(f"train batch {batch}: {ms / 2:.1f} ms/epoch, {tf:.1f} TFLOP/s")
+    log(f"forward 8192 rows: {out['forward_8192_rows']}")
+    print(json.dumps(out), flush=True)
+
+
 def copy_probe(nbytes):
     """One device-to-device copy of `nbytes` (float32 tensors, after a
     warm-up copy), the known-size kernel the FETCH_SIZE / WRITE_SIZE counters
@@ -327,6 +362,7 @@ def main():
     ap.add_argument("--turn-solve-iters", type=int, default=500)
     ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")
     ap.add_argument("--c4-iters", type=int, default=500)
+    ap.add_argument("--mlp", action="store_true", help="value-net MFMA throughput instead")
     ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations, device and CPU oracle instead")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
@@ -340,6 +376,9 @@ def main():
         return
     if args.c1:
         river_uniform_bench(args)
+        return
+    if args.mlp:
+        mlp_bench(args)
         return
     if args.net_solve:
         net_solve_bench(args)

@@ -69,7 +69,8 @@ struct psr_mlp {
     // resident training set
     uint16_t* dx = nullptr;
     float *dy = nullptr, *dmask = nullptr;
-    int32_t* drows = nullptr;
+    int32_t* drows = nullptr;        // the epoch's shuffled row order (n_data)
+    float* epoch_rows = nullptr;     // per-row losses of an epoch's batches (n_data)
     int n_data = 0;
     int64_t step = 0;
     std::vector<void*> allocs;
@@ -149,10 +150,11 @@ struct psr_mlp {
     }
 
     // mean Huber loss over all outputs of n rows (targets yb, mask mbuf); dZ[L]
-    int loss(int n, float delta, double* out) {
+    // rows_out: where the per-row losses go (row_loss when null)
+    int loss(int n, float delta, double* out, float* rows_out = nullptr) {
         const int D = dims[L];
         const float inv = 1.0f / ((float)n * (float)D);
-        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, delta, inv, dZ[L], row_loss, stream));
+        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, delta, inv, dZ[L], rows_out ? rows_out : row_loss, stream));
         if (out) {
             std::vector<float> rl(n);
             HIP_OK(hipMemcpyAsync(rl.data(), row_loss, n * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -352,7 +354,7 @@ int psr_mlp_gradients(psr_mlp* m, const float* x, const float* y, const float* m
 int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float* mask, int n) {
     if (!m || !x || !y || !mask || n <= 0) return fail("invalid argument");
     HIP_OK(hipSetDevice(m->device));
-    for (void* p : {(void*)m->dx, (void*)m->dy, (void*)m->dmask, (void*)m->drows}) {
+    for (void* p : {(void*)m->dx, (void*)m->dy, (void*)m->dmask, (void*)m->drows, (void*)m->epoch_rows}) {
         if (!p) continue;
         auto it = std::find(m->allocs.begin(), m->allocs.end(), p);
         if (it != m->allocs.end()) m->allocs.erase(it);
@@ -360,7 +362,8 @@ int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float*
     }
     const int D0 = m->dims[0], DL = m->dims[m->L];
     if (m->alloc((void**)&m->dx, (size_t)n * D0 * 2) || m->alloc((void**)&m->dy, (size_t)n * DL * 4) ||
-        m->alloc((void**)&m->dmask, (size_t)n * DL * 4) || m->alloc((void**)&m->drows, (size_t)m->max_batch * 4))
+        m->alloc((void**)&m->dmask, (size_t)n * DL * 4) || m->alloc((void**)&m->drows, (size_t)n * 4) ||
+        m->alloc((void**)&m->epoch_rows, (size_t)n * 4))
         return -1;
     std::vector<uint16_t> t((size_t)D0 * 4096);
     for (int r0 = 0; r0 < n; r0 += 4096) {
@@ -387,20 +390,37 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
     std::iota(order.begin(), order.end(), 0);
     SplitMix rng{seed};
     HIP_OK(hipEventRecord(m->ev0, m->stream));
+    // The epoch's row order goes up once and the batches' losses stay on the
+    // device until the epoch ends: no host round trip between the steps.
+    std::vector<float> rows_loss(m->n_data);
+    const float inv = 1.0f / ((float)batch * (float)DL);
     for (int e = 0; e < epochs; ++e) {
         for (int i = m->n_data - 1; i > 0; --i) std::swap(order[i], order[rng.next() % (uint64_t)(i + 1)]);
-        double sum = 0.0;
+        HIP_OK(hipMemcpyAsync(m->drows, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice, m->stream));
         int nb = 0;
         for (int r0 = 0; r0 + batch <= m->n_data; r0 += batch) {
-            HIP_OK(hipMemcpyAsync(m->drows, order.data() + r0, batch * 4, hipMemcpyHostToDevice, m->stream));
-            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows, batch, D0, DL, m->H[0], m->yb, m->mbuf, m->stream));
-            double l = 0.0;
-            if (m->forward(batch) || m->loss(batch, delta, &l) || m->backward(batch) || m->adamw(lr, weight_decay))
+            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows + r0, batch, D0, DL, m->H[0], m->yb, m->mbuf,
+                                 m->stream));
+            if (m->forward(batch) || m->loss(batch, delta, nullptr, m->epoch_rows + r0) || m->backward(batch) ||
+                m->adamw(lr, weight_decay))
                 return -1;
-            sum += l;
             ++nb;
         }
-        if (epoch_loss) epoch_loss[e] = nb ? sum / nb : 0.0;
+        // one wait per epoch: the host order buffer is reshuffled next
+        if (epoch_loss)
+            HIP_OK(hipMemcpyAsync(rows_loss.data(), m->epoch_rows, (size_t)nb * batch * 4, hipMemcpyDeviceToHost,
+                                  m->stream));
+        HIP_OK(hipStreamSynchronize(m->stream));
+        if (epoch_loss) {
+            // per batch: the mean Huber loss (f64 sum of its rows), then the mean over batches
+            double sum = 0.0;
+            for (int b = 0; b < nb; ++b) {
+                double s = 0.0;
+                for (int i = 0; i < batch; ++i) s += rows_loss[(size_t)b * batch + i];
+                sum += s * (double)inv;
+            }
+            epoch_loss[e] = nb ? sum / nb : 0.0;
+        }
     }
     HIP_OK(hipEventRecord(m->ev1, m->stream));
     HIP_OK(hipStreamSynchronize(m->stream));
