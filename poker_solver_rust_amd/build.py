@@ -38,7 +38,8 @@ def build(verbose=True):
     objs = []
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        run([HIPCC, "-c", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", os.path.join(CSRC, src), "-o", obj])
+        run([HIPCC, "-c", *HOST_FLAGS, f"--offload-arch={ARCH}", "-D__HIP_PLATFORM_AMD__", os.path.join(CSRC, src), "-o",
+             obj])
         objs.append(obj)
     for src in DEVICE_SOURCES:
         obj = os.path.join(BUILD, src + ".o")

@@ -3,10 +3,12 @@
 // mlp_kernels.hip.  C ABI in include/psr_dcfr.h (psr_mlp_*).
 //
 // Layout: layer l (1..L) maps dims[l-1] -> dims[l]; W_l is stored in x out
-// row-major (FP32 master + BF16 working copy), b_l has dims[l] entries.
-// Hidden layers apply GELU, the last layer is linear.  Activations H_l (BF16,
-// the next layer's input) and pre-activations Z_l (FP32) are kept per batch
-// row for the backward pass.
+// row-major (FP32 master, dense; BF16 working copy with rows of ld[l]),
+// b_l has dims[l] entries.  Hidden layers apply GELU, the last layer is
+// linear.  Activations H_l (BF16, the next layer's input), pre-activations
+// Z_l (FP32) and their gradients dZ_l (BF16) are kept per batch row for the
+// backward pass, rows of ld[l] = dims[l] rounded up to a multiple of 8 with
+// zero pads (the GEMM reads 16-byte vectors along K).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,6 +59,7 @@ struct psr_mlp {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     float last_ms = 0.0f;
     std::vector<int> dims;
+    std::vector<int64_t> ld;    // padded row lengths
     int L = 0, max_batch = 0;
     std::vector<float*> W, b, gW, gb, mW, vW, mb, vb;
     std::vector<uint16_t*> Wb;
@@ -64,6 +67,8 @@ struct psr_mlp {
     std::vector<float*> Z;      // Z[l], l >= 1
     std::vector<uint16_t*> dZ;  // dZ[l], l >= 1
     float* row_loss = nullptr;
+    float* ws = nullptr;        // split-K partial products and column-sum partials
+    int64_t ws_floats = 0;
     float* yb = nullptr;        // batch targets / mask
     float* mbuf = nullptr;
     // resident training set
@@ -78,6 +83,12 @@ struct psr_mlp {
     int alloc(void** p, size_t bytes) {
         HIP_OK(hipMalloc(p, std::max<size_t>(bytes, 4)));
         allocs.push_back(*p);
+        return 0;
+    }
+    // zero-filled: row pads of padded matrices must read as zero
+    int alloc_zero(void** p, size_t bytes) {
+        if (alloc(p, bytes)) return -1;
+        HIP_OK(hipMemsetAsync(*p, 0, std::max<size_t>(bytes, 4), stream));
         return 0;
     }
     void release() {
@@ -95,15 +106,15 @@ struct psr_mlp {
         for (int l = 1; l <= L; ++l) {
             GemmArgs g{};
             g.A = H[l - 1];
-            g.sam = dims[l - 1];
-            g.sak = 1;
+            g.lda = ld[l - 1];
+            g.a_kc = true;
             g.B = Wb[l];
-            g.sbk = dims[l];
-            g.sbn = 1;
+            g.ldb = ld[l];
+            g.b_kc = false;
             g.M = n;
             g.N = dims[l];
             g.K = dims[l - 1];
-            g.ldc = dims[l];
+            g.ldc = ld[l];
             g.Cf = Z[l];
             g.Cb = l < L ? H[l] : nullptr;
             g.bias = b[l];
@@ -117,30 +128,32 @@ struct psr_mlp {
         for (int l = L; l >= 1; --l) {
             GemmArgs g{};   // gW_l = H_{l-1}^T dZ_l
             g.A = H[l - 1];
-            g.sam = 1;
-            g.sak = dims[l - 1];
+            g.lda = ld[l - 1];
+            g.a_kc = false;
             g.B = dZ[l];
-            g.sbk = dims[l];
-            g.sbn = 1;
+            g.ldb = ld[l];
+            g.b_kc = false;
             g.M = dims[l - 1];
             g.N = dims[l];
             g.K = n;
             g.ldc = dims[l];
             g.Cf = gW[l];
+            g.ws = ws;
+            g.ws_floats = ws_floats;
             HIP_OK(gemm_launch(g, EPI_NONE, stream));
-            HIP_OK(colsum_launch(dZ[l], n, dims[l], gb[l], stream));
+            HIP_OK(colsum_launch(dZ[l], n, dims[l], ld[l], ws, gb[l], stream));
             if (l > 1) {
                 GemmArgs h{};   // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})
                 h.A = dZ[l];
-                h.sam = dims[l];
-                h.sak = 1;
-                h.B = Wb[l];
-                h.sbk = 1;
-                h.sbn = dims[l];
+                h.lda = ld[l];
+                h.a_kc = true;
+                h.B = Wb[l];        // B(k,n) = W_l[n][k]
+                h.ldb = ld[l];
+                h.b_kc = true;
                 h.M = n;
                 h.N = dims[l - 1];
                 h.K = dims[l];
-                h.ldc = dims[l - 1];
+                h.ldc = ld[l - 1];
                 h.Cb = dZ[l - 1];
                 h.aux = Z[l - 1];
                 HIP_OK(gemm_launch(h, EPI_GELU_GRAD, stream));
@@ -154,7 +167,7 @@ struct psr_mlp {
     int loss(int n, float delta, double* out, float* rows_out = nullptr) {
         const int D = dims[L];
         const float inv = 1.0f / ((float)n * (float)D);
-        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, delta, inv, dZ[L], rows_out ? rows_out : row_loss, stream));
+        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, ld[L], delta, inv, dZ[L], rows_out ? rows_out : row_loss, stream));
         if (out) {
             std::vector<float> rl(n);
             HIP_OK(hipMemcpyAsync(rl.data(), row_loss, n * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -172,17 +185,20 @@ struct psr_mlp {
         const float bc1 = 1.0f - (float)std::pow(0.9, (double)step);
         const float bc2 = 1.0f - (float)std::pow(0.999, (double)step);
         for (int l = 1; l <= L; ++l) {
-            HIP_OK(adamw_launch(W[l], Wb[l], gW[l], mW[l], vW[l], (int64_t)dims[l - 1] * dims[l], lr, b1, b2, eps, wd,
+            HIP_OK(adamw_launch(W[l], Wb[l], gW[l], mW[l], vW[l], dims[l - 1], dims[l], ld[l], lr, b1, b2, eps, wd,
                                 bc1, bc2, stream));
-            HIP_OK(adamw_launch(b[l], nullptr, gb[l], mb[l], vb[l], dims[l], lr, b1, b2, eps, 0.0f, bc1, bc2, stream));
+            HIP_OK(adamw_launch(b[l], nullptr, gb[l], mb[l], vb[l], 1, dims[l], dims[l], lr, b1, b2, eps, 0.0f, bc1, bc2,
+                                stream));
         }
         return 0;
     }
 
-    // host f32 rows -> H[0] (BF16)
+    // host f32 rows -> H[0] (BF16, padded rows)
     int upload_input(const float* x, int n) {
-        std::vector<uint16_t> t((size_t)n * dims[0]);
-        for (size_t i = 0; i < t.size(); ++i) t[i] = bf16_bits(x[i]);
+        const int D0 = dims[0];
+        std::vector<uint16_t> t((size_t)n * ld[0], 0);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < D0; ++c) t[(size_t)r * ld[0] + c] = bf16_bits(x[(size_t)r * D0 + c]);
         HIP_OK(hipMemcpyAsync(H[0], t.data(), t.size() * 2, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return 0;
@@ -198,6 +214,7 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     auto m = new psr_mlp;
     m->device = device;
     m->dims.assign(dims, dims + n_dims);
+    for (int i = 0; i < n_dims; ++i) m->ld.push_back(padded_len(dims[i]));
     m->L = n_dims - 1;
     m->max_batch = max_batch;
     auto bad = [&](int rc) {
@@ -228,13 +245,14 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     SplitMix rng{seed};
     for (int l = 1; l <= L; ++l) {
         const int64_t nw = (int64_t)dims[l - 1] * dims[l];
-        if (bad(m->alloc((void**)&m->W[l], nw * 4)) || bad(m->alloc((void**)&m->Wb[l], nw * 2)) ||
+        if (bad(m->alloc((void**)&m->W[l], nw * 4)) ||
+            bad(m->alloc_zero((void**)&m->Wb[l], (size_t)dims[l - 1] * m->ld[l] * 2)) ||
             bad(m->alloc((void**)&m->gW[l], nw * 4)) || bad(m->alloc((void**)&m->mW[l], nw * 4)) ||
             bad(m->alloc((void**)&m->vW[l], nw * 4)) || bad(m->alloc((void**)&m->b[l], dims[l] * 4)) ||
             bad(m->alloc((void**)&m->gb[l], dims[l] * 4)) || bad(m->alloc((void**)&m->mb[l], dims[l] * 4)) ||
             bad(m->alloc((void**)&m->vb[l], dims[l] * 4)) ||
-            bad(m->alloc((void**)&m->Z[l], (size_t)max_batch * dims[l] * 4)) ||
-            bad(m->alloc((void**)&m->dZ[l], (size_t)max_batch * dims[l] * 2)))
+            bad(m->alloc_zero((void**)&m->Z[l], (size_t)max_batch * m->ld[l] * 4)) ||
+            bad(m->alloc_zero((void**)&m->dZ[l], (size_t)max_batch * m->ld[l] * 2)))
             return -1;
         // U(-1/sqrt(in), 1/sqrt(in)) for weights and biases
         const double bound = 1.0 / std::sqrt((double)dims[l - 1]);
@@ -248,11 +266,17 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
             hipMemcpy(m->vW[l], zero.data(), nw * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(m->mb[l], zero.data(), dims[l] * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(m->vb[l], zero.data(), dims[l] * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            to_bf16_launch(m->W[l], m->Wb[l], nw, m->stream) != hipSuccess)
+            to_bf16_launch(m->W[l], m->Wb[l], dims[l - 1], dims[l], m->ld[l], m->stream) != hipSuccess)
             return bad(fail("parameter upload failed"));
     }
     for (int l = 0; l < L; ++l)
-        if (bad(m->alloc((void**)&m->H[l], (size_t)max_batch * dims[l] * 2))) return -1;
+        if (bad(m->alloc_zero((void**)&m->H[l], (size_t)max_batch * m->ld[l] * 2))) return -1;
+    // workspace: up to 8 K slices of the largest weight gradient, or the
+    // column-sum partials of a batch
+    int64_t wsf = colsum_ws_floats(max_batch, *std::max_element(dims + 1, dims + n_dims));
+    for (int l = 1; l <= L; ++l) wsf = std::max<int64_t>(wsf, 8 * (int64_t)dims[l - 1] * dims[l]);
+    m->ws_floats = wsf;
+    if (bad(m->alloc((void**)&m->ws, (size_t)wsf * 4))) return -1;
     if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||
         bad(m->alloc((void**)&m->yb, (size_t)max_batch * dims[L] * 4)) ||
         bad(m->alloc((void**)&m->mbuf, (size_t)max_batch * dims[L] * 4)))
@@ -296,7 +320,7 @@ int psr_mlp_set_params(psr_mlp* m, const float* in, int64_t n) {
     for (int l = 1; l <= m->L; ++l) {
         const int64_t nw = (int64_t)m->dims[l - 1] * m->dims[l];
         HIP_OK(hipMemcpy(m->W[l], in + off, nw * 4, hipMemcpyHostToDevice));
-        HIP_OK(to_bf16_launch(m->W[l], m->Wb[l], nw, m->stream));
+        HIP_OK(to_bf16_launch(m->W[l], m->Wb[l], m->dims[l - 1], m->dims[l], m->ld[l], m->stream));
         off += nw;
         HIP_OK(hipMemcpy(m->b[l], in + off, m->dims[l] * 4, hipMemcpyHostToDevice));
         off += m->dims[l];
@@ -310,11 +334,16 @@ int psr_mlp_forward(psr_mlp* m, const float* x, int n, float* y) {
     if (!m || !x || !y || n < 0) return fail("invalid argument");
     HIP_OK(hipSetDevice(m->device));
     const int D0 = m->dims[0], DL = m->dims[m->L];
+    const int64_t ldl = m->ld[m->L];
+    std::vector<float> z;
     HIP_OK(hipEventRecord(m->ev0, m->stream));
     for (int r0 = 0; r0 < n; r0 += m->max_batch) {
         const int k = std::min(m->max_batch, n - r0);
         if (m->upload_input(x + (int64_t)r0 * D0, k) || m->forward(k)) return -1;
-        HIP_OK(hipMemcpyAsync(y + (int64_t)r0 * DL, m->Z[m->L], (size_t)k * DL * 4, hipMemcpyDeviceToHost, m->stream));
+        z.resize((size_t)k * ldl);
+        HIP_OK(hipMemcpyAsync(z.data(), m->Z[m->L], z.size() * 4, hipMemcpyDeviceToHost, m->stream));
+        HIP_OK(hipStreamSynchronize(m->stream));
+        for (int r = 0; r < k; ++r) std::memcpy(y + (int64_t)(r0 + r) * DL, z.data() + (size_t)r * ldl, (size_t)DL * 4);
     }
     HIP_OK(hipEventRecord(m->ev1, m->stream));
     HIP_OK(hipStreamSynchronize(m->stream));
@@ -361,15 +390,17 @@ int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float*
         (void)hipFree(p);
     }
     const int D0 = m->dims[0], DL = m->dims[m->L];
-    if (m->alloc((void**)&m->dx, (size_t)n * D0 * 2) || m->alloc((void**)&m->dy, (size_t)n * DL * 4) ||
+    const int64_t ld0 = m->ld[0];
+    if (m->alloc((void**)&m->dx, (size_t)n * ld0 * 2) || m->alloc((void**)&m->dy, (size_t)n * DL * 4) ||
         m->alloc((void**)&m->dmask, (size_t)n * DL * 4) || m->alloc((void**)&m->drows, (size_t)n * 4) ||
         m->alloc((void**)&m->epoch_rows, (size_t)n * 4))
         return -1;
-    std::vector<uint16_t> t((size_t)D0 * 4096);
+    std::vector<uint16_t> t((size_t)ld0 * 4096, 0);   // pads stay zero
     for (int r0 = 0; r0 < n; r0 += 4096) {
         const int k = std::min(4096, n - r0);
-        for (size_t i = 0; i < (size_t)k * D0; ++i) t[i] = bf16_bits(x[(size_t)r0 * D0 + i]);
-        HIP_OK(hipMemcpy(m->dx + (size_t)r0 * D0, t.data(), (size_t)k * D0 * 2, hipMemcpyHostToDevice));
+        for (int r = 0; r < k; ++r)
+            for (int c = 0; c < D0; ++c) t[(size_t)r * ld0 + c] = bf16_bits(x[(size_t)(r0 + r) * D0 + c]);
+        HIP_OK(hipMemcpy(m->dx + (size_t)r0 * ld0, t.data(), (size_t)k * ld0 * 2, hipMemcpyHostToDevice));
     }
     HIP_OK(hipMemcpy(m->dy, y, (size_t)n * DL * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(m->dmask, mask, (size_t)n * DL * 4, hipMemcpyHostToDevice));
@@ -385,7 +416,7 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
     if (!m || m->n_data <= 0) return fail("no training set");
     if (batch <= 0 || batch > m->max_batch || epochs <= 0) return fail("invalid batch or epochs");
     HIP_OK(hipSetDevice(m->device));
-    const int D0 = m->dims[0], DL = m->dims[m->L];
+    const int DL = m->dims[m->L];
     std::vector<int32_t> order(m->n_data);
     std::iota(order.begin(), order.end(), 0);
     SplitMix rng{seed};
@@ -399,7 +430,7 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
         HIP_OK(hipMemcpyAsync(m->drows, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice, m->stream));
         int nb = 0;
         for (int r0 = 0; r0 + batch <= m->n_data; r0 += batch) {
-            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows + r0, batch, D0, DL, m->H[0], m->yb, m->mbuf,
+            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows + r0, batch, m->ld[0], DL, m->H[0], m->yb, m->mbuf,
                                  m->stream));
             if (m->forward(batch) || m->loss(batch, delta, nullptr, m->epoch_rows + r0) || m->backward(batch) ||
                 m->adamw(lr, weight_decay))

@@ -4,20 +4,26 @@
 // pot-relative values out (4 hidden layers of 512, GELU).
 //
 // GEMM: C[M x N] = A[M x K] * B[K x N] with A, B in BF16 and FP32
-// accumulation on v_mfma_f32_16x16x32_bf16.  Operands are addressed through
-// element strides (A(m,k) = A[m * sam + k * sak], B(k,n) = B[k * sbk + n * sbn])
-// so the three products of a dense layer use one kernel:
-//   forward   Z = X W + b        A = X (row-major), B = W (in x out)
-//   input     dX = dZ W^T        B(k,n) = W[n][k]
-//   weight    dW = X^T dZ        A(m,k) = X[k][m]
-// Workgroup 256 threads = 4 waves, tile 64 x 64, K step 32 staged in LDS;
-// each wave owns a 32 x 32 quarter = 2 x 2 MFMA tiles of 16 x 16.
+// accumulation on v_mfma_f32_16x16x32_bf16.  The three products of a dense
+// layer differ in which operand dimension is contiguous in HBM:
+//   forward   Z = X W + b        A = X  (K contiguous), B = W (N contiguous)
+//   input     dX = dZ W^T        A = dZ (K contiguous), B(k,n) = W[n][k] (K contiguous)
+//   weight    dW = X^T dZ        A(m,k) = X[k][m] (M contiguous), B = dZ (N contiguous)
+// Tiles are staged in LDS as rows of K-contiguous BF16 (what an MFMA lane
+// reads: 8 consecutive K of one row or column); an operand whose other
+// dimension is contiguous is loaded as 16-byte vectors along it, two K rows
+// per thread, and interleaved into 32-bit words on the way to LDS.
+// Workgroup 256 threads = 4 waves in 2 x 2, each owning a WM x WN quarter of
+// the tile; K step 32, two LDS buffers so one barrier per step separates the
+// stores of step t + 1 from the fragment reads of step t, and the global
+// loads of step t + 1 are in flight during the MFMAs of step t.
 //
 // MFMA operand layout (16x16x32 bf16, wave64): lane l holds A row (l % 16)
 // and B column (l % 16), K elements 8 * (l / 16) .. +7; the accumulator holds
 // C rows 4 * (l / 16) + v (v = 0..3), column l % 16.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mlp_launch.hpp"
@@ -27,8 +33,9 @@ namespace psr {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int GB = 256;        // threads per GEMM workgroup
-constexpr int TM = 64, TN = 64, TK = 32;
+constexpr int GB = 256;              // threads per GEMM workgroup
+constexpr int TK = 32;               // K step
+constexpr int LDS_ROW = TK + 8;      // LDS row: 80 B, 16-byte aligned; 16 rows of one fragment read hit 64 distinct banks
 
 __device__ __forceinline__ float bf16_bits_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float x) {   // round to nearest even
@@ -45,66 +52,118 @@ __device__ __forceinline__ float gelu_grad(float x) {
     return cdf + x * pdf;
 }
 
-template <int EPI>
-__global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g) {
-    __shared__ uint16_t As[TM][TK + 8];   // row m, k (padded against bank conflicts)
-    __shared__ uint16_t Bs[TN][TK + 8];   // column n, k
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-    floatx4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+// One operand tile: R rows of the output dimension x TK, from HBM into LDS
+// rows S[r][0..TK).  KC: X[r * ld + k]; otherwise X[k * ld + r].
+template <int R, bool KC>
+struct TileStage {
+    // KC: R * TK / 8 vectors (row r = i / 4, K offset 8 * (i % 4)), spread
+    // over the workgroup.  Otherwise R / 8 * TK / 2 pairs of vectors, one per
+    // thread: 8 rows (r0 + 8 * (i / 16)) at K rows 2 * (i % 16) and + 1.
+    static constexpr int NV = KC ? (R * TK / 8 + GB - 1) / GB : 2;
+    static constexpr int PAIRS = R * TK / 16;
+    uint4 v[NV];
 
-    for (int k0 = 0; k0 < g.K; k0 += TK) {
-        // stage A (64 x 32) and B (32 x 64): 2048 elements each, 8 per thread
+    __device__ __forceinline__ void load(const uint16_t* __restrict__ X, int64_t ld, int r0, int rlim, int k0,
+                                         int klim) {
+        const int tid = threadIdx.x;
+        if constexpr (KC) {
 #pragma unroll
-        for (int e = 0; e < (TM * TK) / GB; ++e) {
-            const int idx = tid + e * GB;
-            const int r = idx / TK, kk = idx % TK;
-            const int m = m0 + r, k = k0 + kk;
-            As[r][kk] = (m < g.M && k < g.K) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : (uint16_t)0;
-        }
-#pragma unroll
-        for (int e = 0; e < (TN * TK) / GB; ++e) {
-            const int idx = tid + e * GB;
-            const int cidx = idx / TK, kk = idx % TK;
-            const int n = n0 + cidx, k = k0 + kk;
-            Bs[cidx][kk] = (n < g.N && k < g.K) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : (uint16_t)0;
-        }
-        __syncthreads();
-        const int kq = 8 * (lane >> 4);
-        bf16x8 a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int r = wm + 16 * i + (lane & 15);
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const uint16_t bits = As[r][kq + t];
-                a[i][t] = __builtin_bit_cast(__bf16, bits);
+            for (int e = 0; e < NV; ++e) {
+                const int i = tid + e * GB;
+                const int r = r0 + (i >> 2), k = k0 + 8 * (i & 3);
+                v[e] = make_uint4(0u, 0u, 0u, 0u);
+                if (i < R * TK / 8 && r < rlim && k < klim) v[e] = *(const uint4*)(X + (int64_t)r * ld + k);
+            }
+        } else {
+            const int r = r0 + 8 * (tid >> 4), k = k0 + 2 * (tid & 15);
+            v[0] = make_uint4(0u, 0u, 0u, 0u);
+            v[1] = make_uint4(0u, 0u, 0u, 0u);
+            if (tid < PAIRS && r < rlim) {
+                if (k < klim) v[0] = *(const uint4*)(X + (int64_t)k * ld + r);
+                if (k + 1 < klim) v[1] = *(const uint4*)(X + (int64_t)(k + 1) * ld + r);
             }
         }
+    }
+
+    __device__ __forceinline__ void store(uint16_t (*S)[LDS_ROW]) const {
+        const int tid = threadIdx.x;
+        if constexpr (KC) {
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int cidx = wn + 16 * j + (lane & 15);
+            for (int e = 0; e < NV; ++e) {
+                const int i = tid + e * GB;
+                if (i < R * TK / 8) *(uint4*)&S[i >> 2][8 * (i & 3)] = v[e];
+            }
+        } else {
+            if (tid < PAIRS) {
+                const int rb = 8 * (tid >> 4), kk = 2 * (tid & 15);
+                const uint32_t lo[4] = {v[0].x, v[0].y, v[0].z, v[0].w};
+                const uint32_t hi[4] = {v[1].x, v[1].y, v[1].z, v[1].w};
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const uint16_t bits = Bs[cidx][kq + t];
-                b[j][t] = __builtin_bit_cast(__bf16, bits);
+                for (int q = 0; q < 4; ++q) {
+                    // rows rb + 2q (low halves) and rb + 2q + 1 (high halves)
+                    *(uint32_t*)&S[rb + 2 * q][kk] = (lo[q] & 0xFFFFu) | (hi[q] << 16);
+                    *(uint32_t*)&S[rb + 2 * q + 1][kk] = (lo[q] >> 16) | (hi[q] & 0xFFFF0000u);
+                }
             }
         }
+    }
+};
+
+template <int EPI, int WM, int WN, bool AKC, bool BKC>
+__global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, int64_t cstride) {
+    constexpr int BM = 2 * WM, BN = 2 * WN, FM = WM / 16, FN = WN / 16;
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][LDS_ROW];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN][LDS_ROW];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
+    const int kb = blockIdx.z * kslice, ke = min(g.K, kb + kslice);
+    floatx4 acc[FM][FN];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        for (int j = 0; j < FN; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    TileStage<BM, AKC> sa;
+    TileStage<BN, BKC> sb;
+    sa.load(g.A, g.lda, m0, g.M, kb, ke);
+    sb.load(g.B, g.ldb, n0, g.N, kb, ke);
+    sa.store(As[0]);
+    sb.store(Bs[0]);
+    __syncthreads();
+    const int nk = (ke - kb + TK - 1) / TK;
+    const int kq = 8 * (lane >> 4);
+    for (int t = 0; t < nk; ++t) {
+        const int cur = t & 1;
+        const bool more = t + 1 < nk;
+        if (more) {
+            sa.load(g.A, g.lda, m0, g.M, kb + (t + 1) * TK, ke);
+            sb.load(g.B, g.ldb, n0, g.N, kb + (t + 1) * TK, ke);
+        }
+        bf16x8 a[FM], b[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+            a[i] = __builtin_bit_cast(bf16x8, *(const uint4*)&As[cur][wm + 16 * i + (lane & 15)][kq]);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+            b[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[cur][wn + 16 * j + (lane & 15)][kq]);
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (more) {
+            sa.store(As[cur ^ 1]);
+            sb.store(Bs[cur ^ 1]);
+        }
         __syncthreads();
     }
 
     // epilogue: rows 4 * (lane / 16) + v, column lane % 16 of each 16 x 16 tile
+    float* Cf = g.Cf + blockIdx.z * cstride;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
@@ -114,47 +173,66 @@ __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g) {
                 const int64_t o = (int64_t)m * g.ldc + n;
                 if (EPI == EPI_BIAS_GELU) {
                     z += g.bias[n];
-                    g.Cf[o] = z;                            // pre-activation, kept for the backward pass
+                    Cf[o] = z;                              // pre-activation, kept for the backward pass
                     g.Cb[o] = f32_to_bf16_bits(gelu(z));    // activation, next layer's input
                 } else if (EPI == EPI_BIAS) {
                     z += g.bias[n];
-                    g.Cf[o] = z;
+                    Cf[o] = z;
                 } else if (EPI == EPI_GELU_GRAD) {
                     // dZ of the layer below: dH * gelu'(Z), Z read from aux
                     g.Cb[o] = f32_to_bf16_bits(z * gelu_grad(g.aux[o]));
                 } else {
-                    g.Cf[o] = z;                            // plain FP32 product (weight gradients)
+                    Cf[o] = z;                              // FP32 product or one K slice of it
                 }
             }
 }
 
-// Column sums of a BF16 matrix (bias gradients): out[n] = sum_m X[m][n].
-__global__ void colsum_bf16_kernel(const uint16_t* X, int M, int N, float* out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// Sum of `slices` FP32 partial products (stride n) in slice order.
+__global__ void sum_slices_kernel(const float* ws, int slices, int64_t n, float* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float s = ws[i];
+        for (int z = 1; z < slices; ++z) s += ws[z * n + i];
+        out[i] = s;
+    }
+}
+
+// Column sums of a BF16 matrix (bias gradients), out[n] = sum_m X[m][n], in
+// two passes with a fixed order: workgroup (column strip of 64, row slice)
+// sums its slice with 4 row lanes per column into ws[slice][n], then the
+// slices are added in order.
+constexpr int COLSUM_ROWS = 128;   // rows per slice
+
+__global__ void colsum_bf16_kernel(const uint16_t* X, int M, int N, int64_t ld, float* ws) {
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(M, r0 + COLSUM_ROWS);
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += bf16_bits_to_f32(X[(int64_t)m * N + n]);
-    out[n] = s;
+    if (n < N)
+        for (int m = r0 + rl; m < r1; m += 4) s += bf16_bits_to_f32(X[(int64_t)m * ld + n]);
+    part[rl][c] = s;
+    __syncthreads();
+    if (rl == 0 && n < N) ws[(int64_t)blockIdx.y * N + n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
 }
 
 constexpr int HUBER_BLOCK = 256;
 
 // Masked Huber loss (delta) of FP32 predictions against targets; writes the
 // gradient d loss / d pred (BF16, scaled by 1 / count) and per-row loss sums.
-__global__ void huber_kernel(const float* pred, const float* target, const float* mask, int rows, int cols, float delta,
-                             float inv_count, uint16_t* grad, float* row_loss) {
+__global__ void huber_kernel(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
+                             float delta, float inv_count, uint16_t* grad, float* row_loss) {
     const int r = blockIdx.x;   // one workgroup of HUBER_BLOCK threads per row, grid = rows
     __shared__ float part[HUBER_BLOCK];
     float s = 0.0f;
     for (int c = threadIdx.x; c < cols; c += HUBER_BLOCK) {
-        const int64_t o = (int64_t)r * cols + c;
+        const int64_t o = (int64_t)r * cols + c, op = (int64_t)r * ldp + c;   // targets dense, pred / grad padded
         const float w = mask[o];
-        const float d = pred[o] - target[o];
+        const float d = pred[op] - target[o];
         const float ad = fabsf(d);
         const float l = ad <= delta ? 0.5f * d * d : delta * (ad - 0.5f * delta);
         const float gd = ad <= delta ? d : (d > 0.0f ? delta : -delta);
         s += w * l;
-        grad[o] = f32_to_bf16_bits(w * gd * inv_count);
+        grad[op] = f32_to_bf16_bits(w * gd * inv_count);
     }
     part[threadIdx.x] = s;
     __syncthreads();
@@ -168,8 +246,8 @@ __global__ void huber_kernel(const float* pred, const float* target, const float
 
 // AdamW (decoupled weight decay) on FP32 master parameters, refreshing the
 // BF16 working copy.  bc1 / bc2 are the bias corrections 1 - beta^t.
-__global__ void adamw_kernel(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t n, float lr,
-                             float beta1, float beta2, float eps, float wd, float bc1, float bc2) {
+__global__ void adamw_kernel(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t n, int cols, int64_t ldb,
+                             float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const float gi = g[i];
         const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
@@ -179,22 +257,25 @@ __global__ void adamw_kernel(float* p, uint16_t* pb, const float* g, float* m, f
         const float mh = mi / bc1, vh = vi / bc2;
         const float pi = p[i] - lr * (mh / (sqrtf(vh) + eps) + wd * p[i]);
         p[i] = pi;
-        if (pb) pb[i] = f32_to_bf16_bits(pi);
+        if (pb) pb[(i / cols) * ldb + i % cols] = f32_to_bf16_bits(pi);
     }
 }
 
-__global__ void f32_to_bf16_kernel(const float* x, uint16_t* y, int64_t n) {
+__global__ void f32_to_bf16_kernel(const float* x, uint16_t* y, int64_t n, int cols, int64_t ldy) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        y[i] = f32_to_bf16_bits(x[i]);
+        y[(i / cols) * ldy + i % cols] = f32_to_bf16_bits(x[i]);
 }
 
-// Gather of training rows (a shuffled batch) into contiguous buffers.
+// Gather of training rows (a shuffled batch) into contiguous buffers; input
+// rows move as 16-byte vectors, pads included.
 __global__ void gather_rows_kernel(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n,
-                                   int in_dim, int out_dim, uint16_t* xb, float* yb, float* mb) {
+                                   int64_t ldx, int out_dim, uint16_t* xb, float* yb, float* mb) {
     const int r = blockIdx.x;
     if (r >= n) return;
     const int64_t src = rows[r];
-    for (int c = threadIdx.x; c < in_dim; c += blockDim.x) xb[(int64_t)r * in_dim + c] = x[src * in_dim + c];
+    const uint4* xs = (const uint4*)(x + src * ldx);
+    uint4* xd = (uint4*)(xb + (int64_t)r * ldx);
+    for (int c = threadIdx.x; c < (int)(ldx / 8); c += blockDim.x) xd[c] = xs[c];
     for (int c = threadIdx.x; c < out_dim; c += blockDim.x) {
         yb[(int64_t)r * out_dim + c] = y[src * out_dim + c];
         mb[(int64_t)r * out_dim + c] = mask[src * out_dim + c];
@@ -203,52 +284,102 @@ __global__ void gather_rows_kernel(const uint16_t* x, const float* y, const floa
 
 // ------------------------------------------------------------------ host --
 
+namespace {
+
+template <int EPI, int WM, int WN, bool AKC, bool BKC>
+void launch_tile(const GemmArgs& g, int slices, int kslice, int64_t cstride, hipStream_t stream) {
+    const dim3 grid((g.N + 2 * WN - 1) / (2 * WN), (g.M + 2 * WM - 1) / (2 * WM), slices);
+    hipLaunchKernelGGL((gemm_bf16_kernel<EPI, WM, WN, AKC, BKC>), grid, dim3(GB), 0, stream, g, kslice, cstride);
+}
+
+template <int EPI, bool AKC, bool BKC>
+void launch_shape(const GemmArgs& g, int tile, int slices, int kslice, int64_t cstride, hipStream_t stream) {
+    if (tile == 2) launch_tile<EPI, 64, 64, AKC, BKC>(g, slices, kslice, cstride, stream);
+    else if (tile == 1) launch_tile<EPI, 64, 32, AKC, BKC>(g, slices, kslice, cstride, stream);
+    else launch_tile<EPI, 32, 32, AKC, BKC>(g, slices, kslice, cstride, stream);
+}
+
+int64_t tiles(const GemmArgs& g, int bm, int bn) { return (int64_t)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn); }
+
+}  // namespace
+
+// Tile choice: the largest of 128 x 128, 128 x 64, 64 x 64 that still gives
+// about one workgroup per CU (256); weight gradients (EPI_NONE) split K over
+// up to 8 slices when the output alone has fewer tiles than that.
 hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
-    const dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM);
-    switch (epi) {
-        case EPI_BIAS_GELU: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_BIAS_GELU>, grid, dim3(GB), 0, stream, g); break;
-        case EPI_BIAS: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_BIAS>, grid, dim3(GB), 0, stream, g); break;
-        case EPI_GELU_GRAD: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_GELU_GRAD>, grid, dim3(GB), 0, stream, g); break;
-        case EPI_NONE: hipLaunchKernelGGL(gemm_bf16_kernel<EPI_NONE>, grid, dim3(GB), 0, stream, g); break;
-        default: return hipErrorInvalidValue;
+    if ((g.lda | g.ldb) & 7 || ((uintptr_t)g.A | (uintptr_t)g.B) & 15) return hipErrorInvalidValue;
+    constexpr int64_t FILL = 240;
+    const int tile = tiles(g, 128, 128) >= FILL ? 2 : tiles(g, 128, 64) >= FILL ? 1 : 0;
+    const int64_t nt = tile == 2 ? tiles(g, 128, 128) : tile == 1 ? tiles(g, 128, 64) : tiles(g, 64, 64);
+    int slices = 1, kslice = ((g.K + TK - 1) / TK) * TK;
+    const int64_t out_floats = (int64_t)g.M * g.ldc;
+    if (epi == EPI_NONE && g.ws && nt < FILL) {
+        const int64_t want = std::min<int64_t>({8, (FILL + nt - 1) / nt, g.ws_floats / out_floats, g.K / (4 * TK)});
+        if (want > 1) {
+            kslice = (int)(((g.K + want - 1) / want + TK - 1) / TK) * TK;
+            slices = (g.K + kslice - 1) / kslice;
+        }
+    }
+    GemmArgs h = g;
+    if (slices > 1) h.Cf = g.ws;
+    const int64_t cstride = slices > 1 ? out_floats : 0;
+    if (epi == EPI_BIAS_GELU && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS_GELU, true, false>(h, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_BIAS && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS, true, false>(h, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_GELU_GRAD && g.a_kc && g.b_kc) launch_shape<EPI_GELU_GRAD, true, true>(h, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_NONE && !g.a_kc && !g.b_kc) launch_shape<EPI_NONE, false, false>(h, tile, slices, kslice, cstride, stream);
+    else return hipErrorInvalidValue;   // operand layouts this library does not use
+    if (slices > 1) {
+        int64_t blocks = (out_floats + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const float*)g.ws, slices,
+                           out_floats, g.Cf);
     }
     return hipGetLastError();
 }
 
-hipError_t colsum_launch(const uint16_t* X, int M, int N, float* out, hipStream_t stream) {
-    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, X, M, N, out);
+int64_t colsum_ws_floats(int M, int N) { return (int64_t)((M + COLSUM_ROWS - 1) / COLSUM_ROWS) * N; }
+
+hipError_t colsum_launch(const uint16_t* X, int M, int N, int64_t ld, float* ws, float* out, hipStream_t stream) {
+    if (M <= 0 || N <= 0) return hipErrorInvalidValue;
+    const int slices = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 63) / 64, slices), dim3(256), 0, stream, X, M, N, ld, ws);
+    hipLaunchKernelGGL(sum_slices_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, (const float*)ws, slices,
+                       (int64_t)N, out);
     return hipGetLastError();
 }
 
-hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, float delta,
-                        float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream) {
+hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
+                        float delta, float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream) {
     if (rows <= 0 || cols <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(HUBER_BLOCK), 0, stream, pred, target, mask, rows, cols, delta, inv_count,
-                       grad, row_loss);
+    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(HUBER_BLOCK), 0, stream, pred, target, mask, rows, cols, ldp, delta,
+                       inv_count, grad, row_loss);
     return hipGetLastError();
 }
 
-hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
-                        float beta2, float eps, float wd, float bc1, float bc2, hipStream_t stream) {
+hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t rows, int cols, int64_t ldb,
+                        float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2,
+                        hipStream_t stream) {
+    const int64_t n = rows * cols;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, pb, g, m, v, n, lr, beta1, beta2,
-                       eps, wd, bc1, bc2);
+    hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, pb, g, m, v, n, cols, ldb, lr,
+                       beta1, beta2, eps, wd, bc1, bc2);
     return hipGetLastError();
 }
 
-hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t n, hipStream_t stream) {
+hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, int64_t ldy, hipStream_t stream) {
+    const int64_t n = rows * cols;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, y, n);
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, x, y, n, cols, ldy);
     return hipGetLastError();
 }
 
-hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int in_dim,
+hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int64_t ldx,
                          int out_dim, uint16_t* xb, float* yb, float* mb, hipStream_t stream) {
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(n), dim3(256), 0, stream, x, y, mask, rows, n, in_dim, out_dim, xb, yb,
-                       mb);
+    if ((ldx & 7) || ((uintptr_t)x | (uintptr_t)xb) & 15) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(n), dim3(256), 0, stream, x, y, mask, rows, n, ldx, out_dim, xb, yb, mb);
     return hipGetLastError();
 }
 

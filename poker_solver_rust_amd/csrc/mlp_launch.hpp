@@ -1,10 +1,17 @@
 // Value-net kernel entry points (mlp_kernels.hip), used by mlp.cpp.
+//
+// Every BF16 matrix and every FP32 activation is stored row-major with its
+// row length padded to a multiple of 8 elements (padded_len) so that each row
+// starts on a 16-byte boundary; the pad elements are zero and stay zero (no
+// kernel writes them), which lets the GEMM read whole 16-byte vectors along K.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 namespace psr {
+
+inline int padded_len(int n) { return (n + 7) & ~7; }
 
 enum GemmEpilogue : int {
     EPI_NONE = 0,        // Cf = A B (FP32)
@@ -13,26 +20,45 @@ enum GemmEpilogue : int {
     EPI_GELU_GRAD = 3,   // Cb = bf16((A B) * gelu'(aux))
 };
 
+// C[M x N] = A[M x K] B[K x N], BF16 operands, FP32 accumulation.
+//   a_kc: A(m,k) = A[m * lda + k]  (K contiguous), else A[k * lda + m]
+//   b_kc: B(k,n) = B[n * ldb + k]  (K contiguous), else B[k * ldb + n]
+// Leading dimensions are multiples of 8 and operand pointers 16-byte aligned;
+// along a K-contiguous row the elements from K up to the padded length are zero.
 struct GemmArgs {
-    const uint16_t* A;   // BF16 bits, A(m,k) = A[m * sam + k * sak]
-    const uint16_t* B;   // BF16 bits, B(k,n) = B[k * sbk + n * sbn]
+    const uint16_t* A;
+    const uint16_t* B;
     int M, N, K;
-    int64_t sam, sak, sbk, sbn;
-    int64_t ldc;         // row stride of the outputs
+    int64_t lda, ldb;
+    bool a_kc, b_kc;
+    int64_t ldc;         // row stride of Cf / Cb / aux
     float* Cf;
     uint16_t* Cb;
     const float* bias;
     const float* aux;    // pre-activations for EPI_GELU_GRAD
+    // EPI_NONE only: K may be split over up to ws_floats / (M * ldc) slices
+    // whose partial products go to ws and are summed into Cf in slice order
+    float* ws;
+    int64_t ws_floats;
 };
 
 hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream);
-hipError_t colsum_launch(const uint16_t* X, int M, int N, float* out, hipStream_t stream);
-hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, float delta,
-                        float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream);
-hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t n, float lr, float beta1,
-                        float beta2, float eps, float wd, float bc1, float bc2, hipStream_t stream);
-hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t n, hipStream_t stream);
-hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int in_dim,
+// bias gradients: out[n] = sum over m < M of X[m * ld + n]; ws holds
+// colsum_ws_floats(M, N) floats
+int64_t colsum_ws_floats(int M, int N);
+hipError_t colsum_launch(const uint16_t* X, int M, int N, int64_t ld, float* ws, float* out, hipStream_t stream);
+hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
+                        float delta, float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream);
+// AdamW on n = rows * cols FP32 parameters; pb (optional) is the BF16 copy
+// with row stride ldb
+hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t rows, int cols, int64_t ldb,
+                        float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2,
+                        hipStream_t stream);
+// x [rows x cols] FP32 -> y [rows x ldy] BF16
+hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, int64_t ldy, hipStream_t stream);
+// batch row r <- training row rows[r]: x rows of ldx BF16 (copied whole,
+// pads included), y and mask rows of out_dim FP32
+hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int64_t ldx,
                          int out_dim, uint16_t* xb, float* yb, float* mb, hipStream_t stream);
 
 }  // namespace psr

@@ -309,12 +309,12 @@ def bf16(a):
     return (u.astype(np.uint32) << 16).view(np.float32)
 
 
-def mlp_case(dims, n, seed):
+def mlp_case(dims, n, seed, max_batch=64):
     from poker_solver_rust_amd.value_net import ValueNet
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mlp_oracle as M
     rng = np.random.default_rng(seed)
-    net = ValueNet(dims, max_batch=64, seed=seed)
+    net = ValueNet(dims, max_batch=max_batch, seed=seed)
     p = net.get_params()
     x = rng.normal(0, 1, (n, dims[0])).astype(np.float32)
     y = rng.normal(0, 1, (n, dims[-1])).astype(np.float32)
@@ -337,9 +337,12 @@ def run_mlp_forward():
         raise AssertionError(f"forward relative error {err}")
 
 
-def run_mlp_gradients():
+def run_mlp_gradients(n=37, max_batch=64):
+    """Forward + backward against the oracle: the weight gradients read both
+    operands with rows and columns swapped (X^T dZ), the input gradients
+    read the weights transposed; every width is padded to 8 columns."""
     dims = (70, 40, 33, 19)
-    net, M, p, x, y, mask = mlp_case(dims, 37, 4)
+    net, M, p, x, y, mask = mlp_case(dims, n, 4, max_batch)
     loss, g = net.gradients(x, y, mask, 1.0)
     lo, go = M.loss_and_grads(p.astype(np.float64), dims, x.astype(np.float64), y.astype(np.float64),
                               mask.astype(np.float64), 1.0)
@@ -349,6 +352,13 @@ def run_mlp_gradients():
     print(f"loss {loss:.6g} oracle {lo:.6g}; max gradient error {err:.3g} of the largest gradient", flush=True)
     if err > 5e-2:
         raise AssertionError(f"gradient relative error {err}")
+
+
+def run_mlp_gradients_split_k():
+    """300 rows: the weight gradients are summed over K = 300 rows in two
+    slices of 160 and 140 (the second with a ragged last step) and the bias
+    gradients over three row slices."""
+    run_mlp_gradients(300, 512)
 
 
 def run_mlp_adamw_step():

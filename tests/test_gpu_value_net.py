@@ -37,19 +37,61 @@ def bf16_weights(p, dims):
     return np.concatenate(out)
 
 
-def test_mfma_tile_layout_single_layer():
+@pytest.mark.parametrize("n_in,n_out,n", [(96, 80, 70), (99, 520, 4096), (101, 1000, 4096)])
+def test_mfma_tile_layout_single_layer(n_in, n_out, n):
     """One linear layer (no activation) with exactly representable BF16
     operands: small integers, so every product and sum is exact in FP32 and
     the device result must equal x @ W + b bit for bit -- checks the MFMA
-    operand / accumulator lane layout and the 64 x 64 tiling edges."""
-    dims = (96, 80)
-    net = ValueNet(dims, max_batch=128, seed=1)
+    operand / accumulator lane layout, the tile edges and the K padding
+    (input widths not a multiple of 8).  The shapes select the 64 x 64,
+    128 x 64 and 128 x 128 tiles (the largest that still gives about 256
+    workgroups)."""
+    net = ValueNet((n_in, n_out), max_batch=n, seed=1)
     rng = np.random.default_rng(0)
-    W = rng.integers(-3, 4, (96, 80)).astype(np.float32)
-    b = rng.integers(-3, 4, 80).astype(np.float32)
+    W = rng.integers(-3, 4, (n_in, n_out)).astype(np.float32)
+    b = rng.integers(-3, 4, n_out).astype(np.float32)
     net.set_params(np.concatenate([W.ravel(), b]))
-    x = rng.integers(-3, 4, (70, 96)).astype(np.float32)
+    x = rng.integers(-3, 4, (n, n_in)).astype(np.float32)
     assert np.array_equal(net.forward(x), x @ W + b)
+
+
+def test_weight_gradient_split_k_exact():
+    """Gradients of one linear layer on 4096 rows, summed over K = 4096 in 8
+    slices: with integer inputs and targets one to three units off the
+    prediction, the Huber (delta 1) gradient is -sign(e) / (rows * outputs)
+    = +-2^-22 or 0 (exact in BF16) and every partial sum is an exact FP32
+    multiple of 2^-22, so dW = x^T dZ and db = sum_rows dZ are bit-exact."""
+    n_in, n_out, n = 96, 1024, 4096
+    net = ValueNet((n_in, n_out), max_batch=n, seed=2)
+    rng = np.random.default_rng(5)
+    W = rng.integers(-3, 4, (n_in, n_out)).astype(np.float32)
+    b = rng.integers(-3, 4, n_out).astype(np.float32)
+    net.set_params(np.concatenate([W.ravel(), b]))
+    x = rng.integers(-3, 4, (n, n_in)).astype(np.float32)
+    e = rng.integers(-3, 4, (n, n_out)).astype(np.float32)
+    y = x @ W + b + e
+    _, g = net.gradients(x, y, np.ones_like(y), 1.0)
+    dz = np.clip(-e, -1, 1).astype(np.float64) * 2.0 ** -22
+    assert np.array_equal(g[:n_in * n_out].reshape(n_in, n_out), (x.astype(np.float64).T @ dz).astype(np.float32))
+    assert np.array_equal(g[n_in * n_out:], dz.sum(0).astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [2048])
+def test_gradients_match_oracle_full_width(n):
+    """The 2706 -> 4 x 512 -> 2652 net on 2048 rows: 128 x 128 tiles on the
+    output layer, K split in four for the hidden weight gradients."""
+    dims = DEFAULT_DIMS
+    net = ValueNet(dims, max_batch=n, seed=7)
+    p = net.get_params()
+    rng = np.random.default_rng(8)
+    x = rng.normal(0, 1, (n, dims[0])).astype(np.float32)
+    y = rng.normal(0, 0.3, (n, dims[-1])).astype(np.float32)
+    mask = (rng.random((n, dims[-1])) < 0.8).astype(np.float32)
+    loss, g = net.gradients(x, y, mask, 1.0)
+    lo, go = M.loss_and_grads(p.astype(np.float64), dims, x.astype(np.float64), y.astype(np.float64),
+                              mask.astype(np.float64), 1.0)
+    assert abs(loss - lo) <= 2e-2 * abs(lo)
+    assert np.abs(g - go).max() <= 5e-2 * np.abs(go).max()
 
 
 @pytest.mark.parametrize("dims,n", [((70, 40, 33, 19), 37), (DEFAULT_DIMS, 48)])
