@@ -118,6 +118,8 @@ struct psr_mlp {
             g.Cf = Z[l];
             g.Cb = l < L ? H[l] : nullptr;
             g.bias = b[l];
+            g.ws = ws;
+            g.ws_floats = ws_floats;
             HIP_OK(gemm_launch(g, l < L ? EPI_BIAS_GELU : EPI_BIAS, stream));
         }
         return 0;
@@ -156,6 +158,8 @@ struct psr_mlp {
                 h.ldc = ld[l - 1];
                 h.Cb = dZ[l - 1];
                 h.aux = Z[l - 1];
+                h.ws = ws;
+                h.ws_floats = ws_floats;
                 HIP_OK(gemm_launch(h, EPI_GELU_GRAD, stream));
             }
         }

@@ -14,9 +14,12 @@
 // dimension is contiguous is loaded as 16-byte vectors along it, two K rows
 // per thread, and interleaved into 32-bit words on the way to LDS.
 // Workgroup 256 threads = 4 waves in 2 x 2, each owning a WM x WN quarter of
-// the tile; K step 32, two LDS buffers so one barrier per step separates the
+// the tile; K step 64, two LDS buffers so one barrier per step separates the
 // stores of step t + 1 from the fragment reads of step t, and the global
-// loads of step t + 1 are in flight during the MFMAs of step t.
+// loads of step t + 1 are in flight during the MFMAs of step t.  A product
+// whose output has too few tiles to occupy the chip is split along K; the
+// slices' FP32 partials are summed in slice order by a second kernel that
+// applies the epilogue.
 //
 // MFMA operand layout (16x16x32 bf16, wave64): lane l holds A row (l % 16)
 // and B column (l % 16), K elements 8 * (l / 16) .. +7; the accumulator holds
@@ -34,8 +37,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int GB = 256;              // threads per GEMM workgroup
-constexpr int TK = 32;               // K step
-constexpr int LDS_ROW = TK + 8;      // LDS row: 80 B, 16-byte aligned; 16 rows of one fragment read hit 64 distinct banks
+constexpr int TK = 64;               // K step
+constexpr int LDS_ROW = TK + 8;      // LDS row: 144 B, 16-byte aligned; 16 rows of one fragment read hit 64 distinct banks
 
 __device__ __forceinline__ float bf16_bits_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float x) {   // round to nearest even
@@ -56,12 +59,16 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // rows S[r][0..TK).  KC: X[r * ld + k]; otherwise X[k * ld + r].
 template <int R, bool KC>
 struct TileStage {
-    // KC: R * TK / 8 vectors (row r = i / 4, K offset 8 * (i % 4)), spread
-    // over the workgroup.  Otherwise R / 8 * TK / 2 pairs of vectors, one per
-    // thread: 8 rows (r0 + 8 * (i / 16)) at K rows 2 * (i % 16) and + 1.
-    static constexpr int NV = KC ? (R * TK / 8 + GB - 1) / GB : 2;
+    // KC: R * TK / 8 vectors i = thread + GB * e (row i / 8, K offset
+    // 8 * (i % 8)).  Otherwise R * TK / 16 pairs of vectors i = thread + GB * e,
+    // each 8 rows at two consecutive K rows: pair_row / pair_k below; a wave
+    // covers 16 K pairs x 4 row groups, 64 contiguous bytes of each K row.
+    static constexpr int NV = KC ? R * TK / 8 / GB : 2 * (R * TK / 16 / GB);
     static constexpr int PAIRS = R * TK / 16;
+    static_assert(R * TK / 16 % GB == 0, "tile rows times K step must fill the workgroup's vectors");
     uint4 v[NV];
+    __device__ static __forceinline__ int pair_row(int i) { return 8 * ((i >> 4) % (R / 8)); }
+    __device__ static __forceinline__ int pair_k(int i) { return 2 * ((i & 15) + 16 * ((i >> 4) / (R / 8))); }
 
     __device__ __forceinline__ void load(const uint16_t* __restrict__ X, int64_t ld, int r0, int rlim, int k0,
                                          int klim) {
@@ -70,17 +77,21 @@ struct TileStage {
 #pragma unroll
             for (int e = 0; e < NV; ++e) {
                 const int i = tid + e * GB;
-                const int r = r0 + (i >> 2), k = k0 + 8 * (i & 3);
+                const int r = r0 + (i >> 3), k = k0 + 8 * (i & 7);
                 v[e] = make_uint4(0u, 0u, 0u, 0u);
-                if (i < R * TK / 8 && r < rlim && k < klim) v[e] = *(const uint4*)(X + (int64_t)r * ld + k);
+                if (r < rlim && k < klim) v[e] = *(const uint4*)(X + (int64_t)r * ld + k);
             }
         } else {
-            const int r = r0 + 8 * (tid >> 4), k = k0 + 2 * (tid & 15);
-            v[0] = make_uint4(0u, 0u, 0u, 0u);
-            v[1] = make_uint4(0u, 0u, 0u, 0u);
-            if (tid < PAIRS && r < rlim) {
-                if (k < klim) v[0] = *(const uint4*)(X + (int64_t)k * ld + r);
-                if (k + 1 < klim) v[1] = *(const uint4*)(X + (int64_t)(k + 1) * ld + r);
+#pragma unroll
+            for (int e = 0; e < NV / 2; ++e) {
+                const int i = tid + e * GB;
+                const int r = r0 + pair_row(i), k = k0 + pair_k(i);
+                v[2 * e] = make_uint4(0u, 0u, 0u, 0u);
+                v[2 * e + 1] = make_uint4(0u, 0u, 0u, 0u);
+                if (r < rlim) {
+                    if (k < klim) v[2 * e] = *(const uint4*)(X + (int64_t)k * ld + r);
+                    if (k + 1 < klim) v[2 * e + 1] = *(const uint4*)(X + (int64_t)(k + 1) * ld + r);
+                }
             }
         }
     }
@@ -91,13 +102,16 @@ struct TileStage {
 #pragma unroll
             for (int e = 0; e < NV; ++e) {
                 const int i = tid + e * GB;
-                if (i < R * TK / 8) *(uint4*)&S[i >> 2][8 * (i & 3)] = v[e];
+                *(uint4*)&S[i >> 3][8 * (i & 7)] = v[e];
             }
         } else {
-            if (tid < PAIRS) {
-                const int rb = 8 * (tid >> 4), kk = 2 * (tid & 15);
-                const uint32_t lo[4] = {v[0].x, v[0].y, v[0].z, v[0].w};
-                const uint32_t hi[4] = {v[1].x, v[1].y, v[1].z, v[1].w};
+#pragma unroll
+            for (int e = 0; e < NV / 2; ++e) {
+                const int i = tid + e * GB;
+                const int rb = pair_row(i), kk = pair_k(i);
+                const uint4 a = v[2 * e], b = v[2 * e + 1];
+                const uint32_t lo[4] = {a.x, a.y, a.z, a.w};
+                const uint32_t hi[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
                     // rows rb + 2q (low halves) and rb + 2q + 1 (high halves)
@@ -109,6 +123,26 @@ struct TileStage {
     }
 };
 
+// Output element o = m * ldc + n of the product, z = (A B)(m, n).
+template <int EPI>
+__device__ __forceinline__ void epilogue(const GemmArgs& g, float* Cf, int64_t o, int n, float z) {
+    if (EPI == EPI_BIAS_GELU) {
+        z += g.bias[n];
+        Cf[o] = z;                              // pre-activation, kept for the backward pass
+        g.Cb[o] = f32_to_bf16_bits(gelu(z));    // activation, next layer's input
+    } else if (EPI == EPI_BIAS) {
+        z += g.bias[n];
+        Cf[o] = z;
+    } else if (EPI == EPI_GELU_GRAD) {
+        // dZ of the layer below: dH * gelu'(Z), Z read from aux
+        g.Cb[o] = f32_to_bf16_bits(z * gelu_grad(g.aux[o]));
+    } else {
+        Cf[o] = z;                              // FP32 product
+    }
+}
+
+// EPI applies the epilogue; a K slice of a split product (blockIdx.z, with
+// cstride != 0) writes its raw partials to g.ws + blockIdx.z * cstride.
 template <int EPI, int WM, int WN, bool AKC, bool BKC>
 __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, int64_t cstride) {
     constexpr int BM = 2 * WM, BN = 2 * WN, FM = WM / 16, FN = WN / 16;
@@ -132,7 +166,6 @@ __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, i
     sb.store(Bs[0]);
     __syncthreads();
     const int nk = (ke - kb + TK - 1) / TK;
-    const int kq = 8 * (lane >> 4);
     for (int t = 0; t < nk; ++t) {
         const int cur = t & 1;
         const bool more = t + 1 < nk;
@@ -140,17 +173,22 @@ __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, i
             sa.load(g.A, g.lda, m0, g.M, kb + (t + 1) * TK, ke);
             sb.load(g.B, g.ldb, n0, g.N, kb + (t + 1) * TK, ke);
         }
-        bf16x8 a[FM], b[FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
-            a[i] = __builtin_bit_cast(bf16x8, *(const uint4*)&As[cur][wm + 16 * i + (lane & 15)][kq]);
+        for (int h = 0; h < TK / 32; ++h) {
+            const int kq = 32 * h + 8 * (lane >> 4);
+            bf16x8 a[FM], b[FN];
 #pragma unroll
-        for (int j = 0; j < FN; ++j)
-            b[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[cur][wn + 16 * j + (lane & 15)][kq]);
+            for (int i = 0; i < FM; ++i)
+                a[i] = __builtin_bit_cast(bf16x8, *(const uint4*)&As[cur][wm + 16 * i + (lane & 15)][kq]);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+            for (int j = 0; j < FN; ++j)
+                b[j] = __builtin_bit_cast(bf16x8, *(const uint4*)&Bs[cur][wn + 16 * j + (lane & 15)][kq]);
 #pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         if (more) {
             sa.store(As[cur ^ 1]);
             sb.store(Bs[cur ^ 1]);
@@ -159,7 +197,21 @@ __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, i
     }
 
     // epilogue: rows 4 * (lane / 16) + v, column lane % 16 of each 16 x 16 tile
-    float* Cf = g.Cf + blockIdx.z * cstride;
+    if (cstride) {
+        float* P = g.ws + blockIdx.z * cstride;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
+                    const int n = n0 + wn + 16 * j + (lane & 15);
+                    if (m < g.M && n < g.N) P[(int64_t)m * g.ldc + n] = acc[i][j][v];
+                }
+        return;
+    }
+    float* Cf = g.Cf;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -169,30 +221,21 @@ __global__ void __launch_bounds__(GB) gemm_bf16_kernel(GemmArgs g, int kslice, i
                 const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + v;
                 const int n = n0 + wn + 16 * j + (lane & 15);
                 if (m >= g.M || n >= g.N) continue;
-                float z = acc[i][j][v];
-                const int64_t o = (int64_t)m * g.ldc + n;
-                if (EPI == EPI_BIAS_GELU) {
-                    z += g.bias[n];
-                    Cf[o] = z;                              // pre-activation, kept for the backward pass
-                    g.Cb[o] = f32_to_bf16_bits(gelu(z));    // activation, next layer's input
-                } else if (EPI == EPI_BIAS) {
-                    z += g.bias[n];
-                    Cf[o] = z;
-                } else if (EPI == EPI_GELU_GRAD) {
-                    // dZ of the layer below: dH * gelu'(Z), Z read from aux
-                    g.Cb[o] = f32_to_bf16_bits(z * gelu_grad(g.aux[o]));
-                } else {
-                    Cf[o] = z;                              // FP32 product or one K slice of it
-                }
+                epilogue<EPI>(g, Cf, (int64_t)m * g.ldc + n, n, acc[i][j][v]);
             }
 }
 
-// Sum of `slices` FP32 partial products (stride n) in slice order.
-__global__ void sum_slices_kernel(const float* ws, int slices, int64_t n, float* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float s = ws[i];
-        for (int z = 1; z < slices; ++z) s += ws[z * n + i];
-        out[i] = s;
+// Partials of `slices` K slices (stride cstride in ws) summed in slice order,
+// then the epilogue; one thread per output element (m, n < N).
+template <int EPI>
+__global__ void split_finish_kernel(GemmArgs g, int slices, int64_t cstride) {
+    const int64_t total = (int64_t)g.M * g.N;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int m = (int)(i / g.N), n = (int)(i % g.N);
+        const int64_t o = (int64_t)m * g.ldc + n;
+        float s = g.ws[o];
+        for (int z = 1; z < slices; ++z) s += g.ws[z * cstride + o];
+        epilogue<EPI>(g, g.Cf, o, n, s);
     }
 }
 
@@ -213,6 +256,14 @@ __global__ void colsum_bf16_kernel(const uint16_t* X, int M, int N, int64_t ld, 
     part[rl][c] = s;
     __syncthreads();
     if (rl == 0 && n < N) ws[(int64_t)blockIdx.y * N + n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+__global__ void colsum_finish_kernel(const float* ws, int slices, int N, float* out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = ws[n];
+    for (int z = 1; z < slices; ++z) s += ws[(int64_t)z * N + n];
+    out[n] = s;
 }
 
 constexpr int HUBER_BLOCK = 256;
@@ -297,6 +348,12 @@ void launch_shape(const GemmArgs& g, int tile, int slices, int kslice, int64_t c
     if (tile == 2) launch_tile<EPI, 64, 64, AKC, BKC>(g, slices, kslice, cstride, stream);
     else if (tile == 1) launch_tile<EPI, 64, 32, AKC, BKC>(g, slices, kslice, cstride, stream);
     else launch_tile<EPI, 32, 32, AKC, BKC>(g, slices, kslice, cstride, stream);
+    if (slices > 1) {
+        const int64_t total = (int64_t)g.M * g.N;
+        int64_t blocks = (total + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL(split_finish_kernel<EPI>, dim3((unsigned)blocks), dim3(256), 0, stream, g, slices, cstride);
+    }
 }
 
 int64_t tiles(const GemmArgs& g, int bm, int bn) { return (int64_t)((g.M + bm - 1) / bm) * ((g.N + bn - 1) / bn); }
@@ -304,8 +361,8 @@ int64_t tiles(const GemmArgs& g, int bm, int bn) { return (int64_t)((g.M + bm - 
 }  // namespace
 
 // Tile choice: the largest of 128 x 128, 128 x 64, 64 x 64 that still gives
-// about one workgroup per CU (256); weight gradients (EPI_NONE) split K over
-// up to 8 slices when the output alone has fewer tiles than that.
+// about one workgroup per CU (256); when even 64 x 64 tiles give fewer, K is
+// split over up to 8 slices of at least two K steps (as the workspace allows).
 hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
     if ((g.lda | g.ldb) & 7 || ((uintptr_t)g.A | (uintptr_t)g.B) & 15) return hipErrorInvalidValue;
@@ -314,27 +371,19 @@ hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream) {
     const int64_t nt = tile == 2 ? tiles(g, 128, 128) : tile == 1 ? tiles(g, 128, 64) : tiles(g, 64, 64);
     int slices = 1, kslice = ((g.K + TK - 1) / TK) * TK;
     const int64_t out_floats = (int64_t)g.M * g.ldc;
-    if (epi == EPI_NONE && g.ws && nt < FILL) {
-        const int64_t want = std::min<int64_t>({8, (FILL + nt - 1) / nt, g.ws_floats / out_floats, g.K / (4 * TK)});
+    if (g.ws && nt < FILL) {
+        const int64_t want = std::min<int64_t>({8, (FILL + nt - 1) / nt, g.ws_floats / out_floats, g.K / (2 * TK)});
         if (want > 1) {
             kslice = (int)(((g.K + want - 1) / want + TK - 1) / TK) * TK;
             slices = (g.K + kslice - 1) / kslice;
         }
     }
-    GemmArgs h = g;
-    if (slices > 1) h.Cf = g.ws;
     const int64_t cstride = slices > 1 ? out_floats : 0;
-    if (epi == EPI_BIAS_GELU && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS_GELU, true, false>(h, tile, slices, kslice, cstride, stream);
-    else if (epi == EPI_BIAS && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS, true, false>(h, tile, slices, kslice, cstride, stream);
-    else if (epi == EPI_GELU_GRAD && g.a_kc && g.b_kc) launch_shape<EPI_GELU_GRAD, true, true>(h, tile, slices, kslice, cstride, stream);
-    else if (epi == EPI_NONE && !g.a_kc && !g.b_kc) launch_shape<EPI_NONE, false, false>(h, tile, slices, kslice, cstride, stream);
+    if (epi == EPI_BIAS_GELU && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS_GELU, true, false>(g, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_BIAS && g.a_kc && !g.b_kc) launch_shape<EPI_BIAS, true, false>(g, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_GELU_GRAD && g.a_kc && g.b_kc) launch_shape<EPI_GELU_GRAD, true, true>(g, tile, slices, kslice, cstride, stream);
+    else if (epi == EPI_NONE && !g.a_kc && !g.b_kc) launch_shape<EPI_NONE, false, false>(g, tile, slices, kslice, cstride, stream);
     else return hipErrorInvalidValue;   // operand layouts this library does not use
-    if (slices > 1) {
-        int64_t blocks = (out_floats + 255) / 256;
-        if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, (const float*)g.ws, slices,
-                           out_floats, g.Cf);
-    }
     return hipGetLastError();
 }
 
@@ -344,8 +393,8 @@ hipError_t colsum_launch(const uint16_t* X, int M, int N, int64_t ld, float* ws,
     if (M <= 0 || N <= 0) return hipErrorInvalidValue;
     const int slices = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
     hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 63) / 64, slices), dim3(256), 0, stream, X, M, N, ld, ws);
-    hipLaunchKernelGGL(sum_slices_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, (const float*)ws, slices,
-                       (int64_t)N, out);
+    hipLaunchKernelGGL(colsum_finish_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, (const float*)ws, slices, N,
+                       out);
     return hipGetLastError();
 }
 

@@ -337,11 +337,10 @@ def run_mlp_forward():
         raise AssertionError(f"forward relative error {err}")
 
 
-def run_mlp_gradients(n=37, max_batch=64):
+def run_mlp_gradients(n=37, max_batch=64, dims=(70, 40, 33, 19)):
     """Forward + backward against the oracle: the weight gradients read both
     operands with rows and columns swapped (X^T dZ), the input gradients
     read the weights transposed; every width is padded to 8 columns."""
-    dims = (70, 40, 33, 19)
     net, M, p, x, y, mask = mlp_case(dims, n, 4, max_batch)
     loss, g = net.gradients(x, y, mask, 1.0)
     lo, go = M.loss_and_grads(p.astype(np.float64), dims, x.astype(np.float64), y.astype(np.float64),
@@ -355,10 +354,12 @@ def run_mlp_gradients(n=37, max_batch=64):
 
 
 def run_mlp_gradients_split_k():
-    """300 rows: the weight gradients are summed over K = 300 rows in two
-    slices of 160 and 140 (the second with a ragged last step) and the bias
-    gradients over three row slices."""
-    run_mlp_gradients(300, 512)
+    """300 rows of a 300 -> 40 -> 300 -> 19 net: every product with K = 300
+    (both forward epilogues, the GELU-gradient product of layer 2, the
+    weight gradients over the rows) is split in two K slices of 192 and 108,
+    the second ending in a ragged step; the bias gradients are summed over
+    three row slices."""
+    run_mlp_gradients(300, 512, (300, 40, 300, 19))
 
 
 def run_mlp_adamw_step():
