@@ -7,8 +7,13 @@
 // b_l has dims[l] entries.  Hidden layers apply GELU, the last layer is
 // linear.  Activations H_l (BF16, the next layer's input), pre-activations
 // Z_l (FP32) and their gradients dZ_l (BF16) are kept per batch row for the
-// backward pass, rows of ld[l] = dims[l] rounded up to a multiple of 8 with
-// zero pads (the GEMM reads 16-byte vectors along K).
+// backward pass, in rows of ld[l] elements (a multiple of 8: the GEMM reads
+// 16-byte vectors along K) whose pads are zero -- except column dims[l] of
+// every layer input H_l (l < L), which holds 1: the weight-gradient product
+// H^T dZ over dims[l] + 1 rows then yields the bias gradient (sum over the
+// batch of dZ) as its last row, stored right after the weight gradient.
+// The forward product never reads that column (its K stops at dims[l] and
+// the weight rows past it are masked).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,6 +48,8 @@ struct SplitMix {
     }
     double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
 };
+
+constexpr uint16_t BF16_ONE = 0x3F80;
 
 uint16_t bf16_bits(float x) {
     uint32_t u;
@@ -128,14 +135,14 @@ struct psr_mlp {
     // backward of n rows given dZ[L]; fills gW, gb
     int backward(int n) {
         for (int l = L; l >= 1; --l) {
-            GemmArgs g{};   // gW_l = H_{l-1}^T dZ_l
+            GemmArgs g{};   // [gW_l; gb_l] = [H_{l-1}, 1]^T dZ_l
             g.A = H[l - 1];
             g.lda = ld[l - 1];
             g.a_kc = false;
             g.B = dZ[l];
             g.ldb = ld[l];
             g.b_kc = false;
-            g.M = dims[l - 1];
+            g.M = dims[l - 1] + 1;
             g.N = dims[l];
             g.K = n;
             g.ldc = dims[l];
@@ -143,7 +150,6 @@ struct psr_mlp {
             g.ws = ws;
             g.ws_floats = ws_floats;
             HIP_OK(gemm_launch(g, EPI_NONE, stream));
-            HIP_OK(colsum_launch(dZ[l], n, dims[l], ld[l], ws, gb[l], stream));
             if (l > 1) {
                 GemmArgs h{};   // dZ_{l-1} = (dZ_l W_l^T) * gelu'(Z_{l-1})
                 h.A = dZ[l];
@@ -166,12 +172,16 @@ struct psr_mlp {
         return 0;
     }
 
-    // mean Huber loss over all outputs of n rows (targets yb, mask mbuf); dZ[L]
-    // rows_out: where the per-row losses go (row_loss when null)
-    int loss(int n, float delta, double* out, float* rows_out = nullptr) {
+    // mean Huber loss over all outputs of n rows; dZ[L].  Targets and mask:
+    // rows `rows[r]` of y / mk when given (the resident training set), else
+    // the batch buffers yb / mbuf.  rows_out: where the per-row losses go
+    // (row_loss when null).
+    int loss(int n, float delta, double* out, float* rows_out = nullptr, const float* y = nullptr,
+             const float* mk = nullptr, const int32_t* rows = nullptr) {
         const int D = dims[L];
         const float inv = 1.0f / ((float)n * (float)D);
-        HIP_OK(huber_launch(Z[L], yb, mbuf, n, D, ld[L], delta, inv, dZ[L], rows_out ? rows_out : row_loss, stream));
+        HIP_OK(huber_launch(Z[L], y ? y : yb, y ? mk : mbuf, rows, n, D, ld[L], delta, inv, dZ[L],
+                            rows_out ? rows_out : row_loss, stream));
         if (out) {
             std::vector<float> rl(n);
             HIP_OK(hipMemcpyAsync(rl.data(), row_loss, n * sizeof(float), hipMemcpyDeviceToHost, stream));
@@ -201,8 +211,10 @@ struct psr_mlp {
     int upload_input(const float* x, int n) {
         const int D0 = dims[0];
         std::vector<uint16_t> t((size_t)n * ld[0], 0);
-        for (int r = 0; r < n; ++r)
+        for (int r = 0; r < n; ++r) {
             for (int c = 0; c < D0; ++c) t[(size_t)r * ld[0] + c] = bf16_bits(x[(size_t)r * D0 + c]);
+            t[(size_t)r * ld[0] + D0] = BF16_ONE;
+        }
         HIP_OK(hipMemcpyAsync(H[0], t.data(), t.size() * 2, hipMemcpyHostToDevice, stream));
         HIP_OK(hipStreamSynchronize(stream));
         return 0;
@@ -218,7 +230,7 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     auto m = new psr_mlp;
     m->device = device;
     m->dims.assign(dims, dims + n_dims);
-    for (int i = 0; i < n_dims; ++i) m->ld.push_back(padded_len(dims[i]));
+    for (int i = 0; i < n_dims; ++i) m->ld.push_back(padded_len(dims[i] + (i < n_dims - 1 ? 1 : 0)));
     m->L = n_dims - 1;
     m->max_batch = max_batch;
     auto bad = [&](int rc) {
@@ -251,13 +263,14 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
         const int64_t nw = (int64_t)dims[l - 1] * dims[l];
         if (bad(m->alloc((void**)&m->W[l], nw * 4)) ||
             bad(m->alloc_zero((void**)&m->Wb[l], (size_t)dims[l - 1] * m->ld[l] * 2)) ||
-            bad(m->alloc((void**)&m->gW[l], nw * 4)) || bad(m->alloc((void**)&m->mW[l], nw * 4)) ||
+            bad(m->alloc((void**)&m->gW[l], (nw + dims[l]) * 4)) || bad(m->alloc((void**)&m->mW[l], nw * 4)) ||
             bad(m->alloc((void**)&m->vW[l], nw * 4)) || bad(m->alloc((void**)&m->b[l], dims[l] * 4)) ||
-            bad(m->alloc((void**)&m->gb[l], dims[l] * 4)) || bad(m->alloc((void**)&m->mb[l], dims[l] * 4)) ||
+            bad(m->alloc((void**)&m->mb[l], dims[l] * 4)) ||
             bad(m->alloc((void**)&m->vb[l], dims[l] * 4)) ||
             bad(m->alloc_zero((void**)&m->Z[l], (size_t)max_batch * m->ld[l] * 4)) ||
             bad(m->alloc_zero((void**)&m->dZ[l], (size_t)max_batch * m->ld[l] * 2)))
             return -1;
+        m->gb[l] = m->gW[l] + nw;   // the bias gradient is the product's last row
         // U(-1/sqrt(in), 1/sqrt(in)) for weights and biases
         const double bound = 1.0 / std::sqrt((double)dims[l - 1]);
         std::vector<float> w(nw), bb(dims[l]);
@@ -273,12 +286,17 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
             to_bf16_launch(m->W[l], m->Wb[l], dims[l - 1], dims[l], m->ld[l], m->stream) != hipSuccess)
             return bad(fail("parameter upload failed"));
     }
-    for (int l = 0; l < L; ++l)
-        if (bad(m->alloc_zero((void**)&m->H[l], (size_t)max_batch * m->ld[l] * 2))) return -1;
-    // workspace: up to 8 K slices of the largest weight gradient, or the
-    // column-sum partials of a batch
-    int64_t wsf = colsum_ws_floats(max_batch, *std::max_element(dims + 1, dims + n_dims));
-    for (int l = 1; l <= L; ++l) wsf = std::max<int64_t>(wsf, 8 * (int64_t)dims[l - 1] * dims[l]);
+    for (int l = 0; l < L; ++l) {
+        // layer input: zero pads, column dims[l] = 1 (see the layout above)
+        std::vector<uint16_t> h((size_t)max_batch * m->ld[l], 0);
+        for (int r = 0; r < max_batch; ++r) h[(size_t)r * m->ld[l] + dims[l]] = BF16_ONE;
+        if (bad(m->alloc((void**)&m->H[l], h.size() * 2)) ||
+            hipMemcpy(m->H[l], h.data(), h.size() * 2, hipMemcpyHostToDevice) != hipSuccess)
+            return bad(-1);
+    }
+    // workspace: up to 8 K slices of the largest weight-and-bias gradient
+    int64_t wsf = 0;
+    for (int l = 1; l <= L; ++l) wsf = std::max<int64_t>(wsf, 8 * (int64_t)(dims[l - 1] + 1) * dims[l]);
     m->ws_floats = wsf;
     if (bad(m->alloc((void**)&m->ws, (size_t)wsf * 4))) return -1;
     if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||
@@ -399,11 +417,13 @@ int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float*
         m->alloc((void**)&m->dmask, (size_t)n * DL * 4) || m->alloc((void**)&m->drows, (size_t)n * 4) ||
         m->alloc((void**)&m->epoch_rows, (size_t)n * 4))
         return -1;
-    std::vector<uint16_t> t((size_t)ld0 * 4096, 0);   // pads stay zero
+    std::vector<uint16_t> t((size_t)ld0 * 4096, 0);   // pads zero but for the ones column
     for (int r0 = 0; r0 < n; r0 += 4096) {
         const int k = std::min(4096, n - r0);
-        for (int r = 0; r < k; ++r)
+        for (int r = 0; r < k; ++r) {
             for (int c = 0; c < D0; ++c) t[(size_t)r * ld0 + c] = bf16_bits(x[(size_t)(r0 + r) * D0 + c]);
+            t[(size_t)r * ld0 + D0] = BF16_ONE;
+        }
         HIP_OK(hipMemcpy(m->dx + (size_t)r0 * ld0, t.data(), (size_t)k * ld0 * 2, hipMemcpyHostToDevice));
     }
     HIP_OK(hipMemcpy(m->dy, y, (size_t)n * DL * 4, hipMemcpyHostToDevice));
@@ -434,10 +454,10 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
         HIP_OK(hipMemcpyAsync(m->drows, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice, m->stream));
         int nb = 0;
         for (int r0 = 0; r0 + batch <= m->n_data; r0 += batch) {
-            HIP_OK(gather_launch(m->dx, m->dy, m->dmask, m->drows + r0, batch, m->ld[0], DL, m->H[0], m->yb, m->mbuf,
-                                 m->stream));
-            if (m->forward(batch) || m->loss(batch, delta, nullptr, m->epoch_rows + r0) || m->backward(batch) ||
-                m->adamw(lr, weight_decay))
+            HIP_OK(gather_launch(m->dx, m->drows + r0, batch, m->ld[0], m->H[0], m->stream));
+            if (m->forward(batch) ||
+                m->loss(batch, delta, nullptr, m->epoch_rows + r0, m->dy, m->dmask, m->drows + r0) ||
+                m->backward(batch) || m->adamw(lr, weight_decay))
                 return -1;
             ++nb;
         }

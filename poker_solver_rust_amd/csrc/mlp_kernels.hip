@@ -239,44 +239,19 @@ __global__ void split_finish_kernel(GemmArgs g, int slices, int64_t cstride) {
     }
 }
 
-// Column sums of a BF16 matrix (bias gradients), out[n] = sum_m X[m][n], in
-// two passes with a fixed order: workgroup (column strip of 64, row slice)
-// sums its slice with 4 row lanes per column into ws[slice][n], then the
-// slices are added in order.
-constexpr int COLSUM_ROWS = 128;   // rows per slice
-
-__global__ void colsum_bf16_kernel(const uint16_t* X, int M, int N, int64_t ld, float* ws) {
-    __shared__ float part[4][64];
-    const int c = threadIdx.x & 63, rl = threadIdx.x >> 6;
-    const int n = blockIdx.x * 64 + c;
-    const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(M, r0 + COLSUM_ROWS);
-    float s = 0.0f;
-    if (n < N)
-        for (int m = r0 + rl; m < r1; m += 4) s += bf16_bits_to_f32(X[(int64_t)m * ld + n]);
-    part[rl][c] = s;
-    __syncthreads();
-    if (rl == 0 && n < N) ws[(int64_t)blockIdx.y * N + n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
-}
-
-__global__ void colsum_finish_kernel(const float* ws, int slices, int N, float* out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    float s = ws[n];
-    for (int z = 1; z < slices; ++z) s += ws[(int64_t)z * N + n];
-    out[n] = s;
-}
-
 constexpr int HUBER_BLOCK = 256;
 
 // Masked Huber loss (delta) of FP32 predictions against targets; writes the
 // gradient d loss / d pred (BF16, scaled by 1 / count) and per-row loss sums.
-__global__ void huber_kernel(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
-                             float delta, float inv_count, uint16_t* grad, float* row_loss) {
+__global__ void huber_kernel(const float* pred, const float* target, const float* mask, const int32_t* src_rows,
+                             int rows, int cols, int64_t ldp, float delta, float inv_count, uint16_t* grad,
+                             float* row_loss) {
     const int r = blockIdx.x;   // one workgroup of HUBER_BLOCK threads per row, grid = rows
+    const int64_t sr = src_rows ? src_rows[r] : r;
     __shared__ float part[HUBER_BLOCK];
     float s = 0.0f;
     for (int c = threadIdx.x; c < cols; c += HUBER_BLOCK) {
-        const int64_t o = (int64_t)r * cols + c, op = (int64_t)r * ldp + c;   // targets dense, pred / grad padded
+        const int64_t o = sr * cols + c, op = (int64_t)r * ldp + c;   // targets dense, pred / grad padded
         const float w = mask[o];
         const float d = pred[op] - target[o];
         const float ad = fabsf(d);
@@ -317,20 +292,14 @@ __global__ void f32_to_bf16_kernel(const float* x, uint16_t* y, int64_t n, int c
         y[(i / cols) * ldy + i % cols] = f32_to_bf16_bits(x[i]);
 }
 
-// Gather of training rows (a shuffled batch) into contiguous buffers; input
-// rows move as 16-byte vectors, pads included.
-__global__ void gather_rows_kernel(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n,
-                                   int64_t ldx, int out_dim, uint16_t* xb, float* yb, float* mb) {
+// Gather of training rows (a shuffled batch) into the batch input buffer,
+// as 16-byte vectors (pads and the ones column included).
+__global__ void gather_rows_kernel(const uint16_t* x, const int32_t* rows, int n, int64_t ldx, uint16_t* xb) {
     const int r = blockIdx.x;
     if (r >= n) return;
-    const int64_t src = rows[r];
-    const uint4* xs = (const uint4*)(x + src * ldx);
+    const uint4* xs = (const uint4*)(x + (int64_t)rows[r] * ldx);
     uint4* xd = (uint4*)(xb + (int64_t)r * ldx);
     for (int c = threadIdx.x; c < (int)(ldx / 8); c += blockDim.x) xd[c] = xs[c];
-    for (int c = threadIdx.x; c < out_dim; c += blockDim.x) {
-        yb[(int64_t)r * out_dim + c] = y[src * out_dim + c];
-        mb[(int64_t)r * out_dim + c] = mask[src * out_dim + c];
-    }
 }
 
 // ------------------------------------------------------------------ host --
@@ -387,22 +356,12 @@ hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream) {
     return hipGetLastError();
 }
 
-int64_t colsum_ws_floats(int M, int N) { return (int64_t)((M + COLSUM_ROWS - 1) / COLSUM_ROWS) * N; }
-
-hipError_t colsum_launch(const uint16_t* X, int M, int N, int64_t ld, float* ws, float* out, hipStream_t stream) {
-    if (M <= 0 || N <= 0) return hipErrorInvalidValue;
-    const int slices = (M + COLSUM_ROWS - 1) / COLSUM_ROWS;
-    hipLaunchKernelGGL(colsum_bf16_kernel, dim3((N + 63) / 64, slices), dim3(256), 0, stream, X, M, N, ld, ws);
-    hipLaunchKernelGGL(colsum_finish_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, (const float*)ws, slices, N,
-                       out);
-    return hipGetLastError();
-}
-
-hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
-                        float delta, float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream) {
+hipError_t huber_launch(const float* pred, const float* target, const float* mask, const int32_t* src_rows, int rows,
+                        int cols, int64_t ldp, float delta, float inv_count, uint16_t* grad, float* row_loss,
+                        hipStream_t stream) {
     if (rows <= 0 || cols <= 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(HUBER_BLOCK), 0, stream, pred, target, mask, rows, cols, ldp, delta,
-                       inv_count, grad, row_loss);
+    hipLaunchKernelGGL(huber_kernel, dim3(rows), dim3(HUBER_BLOCK), 0, stream, pred, target, mask, src_rows, rows, cols,
+                       ldp, delta, inv_count, grad, row_loss);
     return hipGetLastError();
 }
 
@@ -425,10 +384,9 @@ hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, i
     return hipGetLastError();
 }
 
-hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int64_t ldx,
-                         int out_dim, uint16_t* xb, float* yb, float* mb, hipStream_t stream) {
+hipError_t gather_launch(const uint16_t* x, const int32_t* rows, int n, int64_t ldx, uint16_t* xb, hipStream_t stream) {
     if ((ldx & 7) || ((uintptr_t)x | (uintptr_t)xb) & 15) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(n), dim3(256), 0, stream, x, y, mask, rows, n, ldx, out_dim, xb, yb, mb);
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(n), dim3(256), 0, stream, x, rows, n, ldx, xb);
     return hipGetLastError();
 }
 

@@ -2,8 +2,10 @@
 //
 // Every BF16 matrix and every FP32 activation is stored row-major with its
 // row length padded to a multiple of 8 elements (padded_len) so that each row
-// starts on a 16-byte boundary; the pad elements are zero and stay zero (no
-// kernel writes them), which lets the GEMM read whole 16-byte vectors along K.
+// starts on a 16-byte boundary, which lets the GEMM read whole 16-byte vectors
+// along K.  No kernel writes the pad elements; a K-contiguous operand's pads
+// must be zero, or the other operand's rows there masked (as N-contiguous
+// operands are, row by row, at K).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,7 +26,8 @@ enum GemmEpilogue : int {
 //   a_kc: A(m,k) = A[m * lda + k]  (K contiguous), else A[k * lda + m]
 //   b_kc: B(k,n) = B[n * ldb + k]  (K contiguous), else B[k * ldb + n]
 // Leading dimensions are multiples of 8 and operand pointers 16-byte aligned;
-// along a K-contiguous row the elements from K up to the padded length are zero.
+// along a K-contiguous row the elements from K up to the padded length are
+// zero unless the other operand is N-contiguous.
 struct GemmArgs {
     const uint16_t* A;
     const uint16_t* B;
@@ -36,19 +39,19 @@ struct GemmArgs {
     uint16_t* Cb;
     const float* bias;
     const float* aux;    // pre-activations for EPI_GELU_GRAD
-    // EPI_NONE only: K may be split over up to ws_floats / (M * ldc) slices
-    // whose partial products go to ws and are summed into Cf in slice order
+    // workspace (optional): K may be split over up to ws_floats / (M * ldc)
+    // slices whose partial products go to ws and are summed in slice order
+    // before the epilogue
     float* ws;
     int64_t ws_floats;
 };
 
 hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream);
-// bias gradients: out[n] = sum over m < M of X[m * ld + n]; ws holds
-// colsum_ws_floats(M, N) floats
-int64_t colsum_ws_floats(int M, int N);
-hipError_t colsum_launch(const uint16_t* X, int M, int N, int64_t ld, float* ws, float* out, hipStream_t stream);
-hipError_t huber_launch(const float* pred, const float* target, const float* mask, int rows, int cols, int64_t ldp,
-                        float delta, float inv_count, uint16_t* grad, float* row_loss, hipStream_t stream);
+// Masked Huber loss of pred (rows of ldp) against target / mask rows
+// src_rows[r] (r when src_rows is null), rows of cols
+hipError_t huber_launch(const float* pred, const float* target, const float* mask, const int32_t* src_rows, int rows,
+                        int cols, int64_t ldp, float delta, float inv_count, uint16_t* grad, float* row_loss,
+                        hipStream_t stream);
 // AdamW on n = rows * cols FP32 parameters; pb (optional) is the BF16 copy
 // with row stride ldb
 hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t rows, int cols, int64_t ldb,
@@ -56,9 +59,7 @@ hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float*
                         hipStream_t stream);
 // x [rows x cols] FP32 -> y [rows x ldy] BF16
 hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, int64_t ldy, hipStream_t stream);
-// batch row r <- training row rows[r]: x rows of ldx BF16 (copied whole,
-// pads included), y and mask rows of out_dim FP32
-hipError_t gather_launch(const uint16_t* x, const float* y, const float* mask, const int32_t* rows, int n, int64_t ldx,
-                         int out_dim, uint16_t* xb, float* yb, float* mb, hipStream_t stream);
+// batch row r <- training row rows[r]: rows of ldx BF16, copied whole
+hipError_t gather_launch(const uint16_t* x, const int32_t* rows, int n, int64_t ldx, uint16_t* xb, hipStream_t stream);
 
 }  // namespace psr
