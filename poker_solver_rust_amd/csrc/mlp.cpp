@@ -74,7 +74,8 @@ struct psr_mlp {
     std::vector<float*> Z;      // Z[l], l >= 1
     std::vector<uint16_t*> dZ;  // dZ[l], l >= 1
     float* row_loss = nullptr;
-    float* ws = nullptr;        // split-K partial products and column-sum partials
+    float* ws = nullptr;        // split-K partial products
+    float* stage = nullptr;     // FP32 input rows on their way to H[0]
     int64_t ws_floats = 0;
     float* yb = nullptr;        // batch targets / mask
     float* mbuf = nullptr;
@@ -207,15 +208,13 @@ struct psr_mlp {
         return 0;
     }
 
-    // host f32 rows -> H[0] (BF16, padded rows)
+    // host f32 rows -> H[0] (BF16, padded rows; the pads and the ones
+    // column set at creation are not touched): FP32 up, converted on the
+    // device.  Returns once the host rows have been read.
     int upload_input(const float* x, int n) {
         const int D0 = dims[0];
-        std::vector<uint16_t> t((size_t)n * ld[0], 0);
-        for (int r = 0; r < n; ++r) {
-            for (int c = 0; c < D0; ++c) t[(size_t)r * ld[0] + c] = bf16_bits(x[(size_t)r * D0 + c]);
-            t[(size_t)r * ld[0] + D0] = BF16_ONE;
-        }
-        HIP_OK(hipMemcpyAsync(H[0], t.data(), t.size() * 2, hipMemcpyHostToDevice, stream));
+        HIP_OK(hipMemcpyAsync(stage, x, (size_t)n * D0 * 4, hipMemcpyHostToDevice, stream));
+        HIP_OK(to_bf16_launch(stage, H[0], n, D0, ld[0], stream));
         HIP_OK(hipStreamSynchronize(stream));
         return 0;
     }
@@ -297,6 +296,7 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     // workspace: up to 8 K slices of the largest weight-and-bias gradient
     int64_t wsf = 0;
     for (int l = 1; l <= L; ++l) wsf = std::max<int64_t>(wsf, 8 * (int64_t)(dims[l - 1] + 1) * dims[l]);
+    if (bad(m->alloc((void**)&m->stage, (size_t)max_batch * dims[0] * 4))) return -1;
     m->ws_floats = wsf;
     if (bad(m->alloc((void**)&m->ws, (size_t)wsf * 4))) return -1;
     if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||
@@ -362,6 +362,8 @@ int psr_mlp_forward(psr_mlp* m, const float* x, int n, float* y) {
     for (int r0 = 0; r0 < n; r0 += m->max_batch) {
         const int k = std::min(m->max_batch, n - r0);
         if (m->upload_input(x + (int64_t)r0 * D0, k) || m->forward(k)) return -1;
+        // one contiguous copy of the padded rows (a pitched copy into pageable
+        // memory is several times slower), unpacked on the host
         z.resize((size_t)k * ldl);
         HIP_OK(hipMemcpyAsync(z.data(), m->Z[m->L], z.size() * 4, hipMemcpyDeviceToHost, m->stream));
         HIP_OK(hipStreamSynchronize(m->stream));
