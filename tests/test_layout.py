@@ -1,5 +1,5 @@
 """Host-side checks of the device layout (no GPU): tiles, levels, child
-placement, state and scan-table bounds for river, turn and batch games."""
+placement, state and scan-table bounds for river, turn, flop and batch games."""
 import pytest
 
 import poker_solver_rust_amd as P
@@ -39,11 +39,15 @@ def test_turn_with_allin_chance_children():
     assert info["tiles"] > 48
 
 
-def test_flop_games_rejected_by_device_layout():
+def test_flop_games_lay_out_one_tile_per_turn_deal():
+    """Flop-root games: the root tile (flop betting) and one tile per turn
+    card below each flop chance node, holding that turn's river deals; on a
+    rainbow flop no two turn cards are suit-isomorphic, so 49 per node."""
     s = [("50%", ""), ("50%", "")]
     cc = P.CardConfig(range=(P.Range.parse("AA"), P.Range.parse("KK")), flop=P.flop_from_str("2c7d9h"))
     tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
                       turn_bet_sizes=s, river_bet_sizes=s)
     g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
-    with pytest.raises(P.PsrError):
-        g.validate_layout()
+    info = g.validate_layout()
+    assert (info["tiles"] - 1) % 49 == 0 and info["tiles"] > 49
+    assert info["nodes"] == g.num_nodes() and info["state"] == g.num_storage()
