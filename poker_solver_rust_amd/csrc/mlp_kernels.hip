@@ -329,14 +329,16 @@ int64_t tiles(const GemmArgs& g, int bm, int bn) { return (int64_t)((g.M + bm - 
 
 }  // namespace
 
-// Tile choice: the largest of 128 x 128, 128 x 64, 64 x 64 that still gives
-// about one workgroup per CU (256); when even 64 x 64 tiles give fewer, K is
-// split over up to 8 slices of at least two K steps (as the workspace allows).
+// Tile choice: 128 x 128 or 128 x 64 only when that still gives more than two
+// workgroups per CU (a 128-row tile holds one workgroup per CU, and with one
+// wave per SIMD its K steps wait on their loads), else 64 x 64; when even those
+// give less than about one workgroup per CU, K is split over up to 8 slices of
+// at least two K steps (as the workspace allows).
 hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream) {
     if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
     if ((g.lda | g.ldb) & 7 || ((uintptr_t)g.A | (uintptr_t)g.B) & 15) return hipErrorInvalidValue;
-    constexpr int64_t FILL = 240;
-    const int tile = tiles(g, 128, 128) >= FILL ? 2 : tiles(g, 128, 64) >= FILL ? 1 : 0;
+    constexpr int64_t TILE_FILL = 600, FILL = 240;
+    const int tile = tiles(g, 128, 128) >= TILE_FILL ? 2 : tiles(g, 128, 64) >= TILE_FILL ? 1 : 0;
     const int64_t nt = tile == 2 ? tiles(g, 128, 128) : tile == 1 ? tiles(g, 128, 64) : tiles(g, 64, 64);
     int slices = 1, kslice = ((g.K + TK - 1) / TK) * TK;
     const int64_t out_floats = (int64_t)g.M * g.ldc;
