@@ -362,6 +362,12 @@ def run_mlp_gradients_split_k():
     run_mlp_gradients(300, 512, (300, 40, 300, 19))
 
 
+def run_mlp_gradients_one_row():
+    """A batch of one row: K = 1 for the weight gradients (a single, ragged
+    K step), one output row per product."""
+    run_mlp_gradients(1, 64)
+
+
 def run_mlp_adamw_step():
     """One training step on a resident batch: every parameter whose oracle
     gradient is not negligible moves by the oracle's AdamW update (at step 1

@@ -94,7 +94,7 @@ def test_gradients_match_oracle_full_width(n):
     assert np.abs(g - go).max() <= 5e-2 * np.abs(go).max()
 
 
-@pytest.mark.parametrize("dims,n", [((70, 40, 33, 19), 37), (DEFAULT_DIMS, 48)])
+@pytest.mark.parametrize("dims,n", [((70, 40, 33, 19), 37), ((70, 40, 33, 19), 1), (DEFAULT_DIMS, 48)])
 def test_forward_matches_oracle(dims, n):
     net = ValueNet(dims, max_batch=64, seed=3)
     p = net.get_params()
@@ -105,7 +105,8 @@ def test_forward_matches_oracle(dims, n):
     assert err <= 2e-2, err
 
 
-@pytest.mark.parametrize("dims,n", [((70, 40, 33, 19), 37), (DEFAULT_DIMS, 64)])
+@pytest.mark.parametrize("dims,n", [((70, 40, 33, 19), 37), ((70, 40, 33, 19), 1), ((70, 40, 33, 19), 2),
+                                    (DEFAULT_DIMS, 64)])
 def test_gradients_match_oracle(dims, n):
     net = ValueNet(dims, max_batch=64, seed=4)
     p = net.get_params()
