@@ -426,18 +426,16 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     }
                 }
             } else {
-                // chance node: reach / chance factor into every child's row
-                if (nd.kind == KIND_CHANCE) {
-                    const ChanceTab ct = uni_load(A.chance + nd.aux);
-                    if constexpr (sizeof(V) == 8) {
-                        const double sc = 1.0 / (double)ct.factor;
-                        FOR_K r[k] = r[k] * sc;
-                    } else {
-                        FOR_K r[k] = __fmul_rn(r[k], ct.scale);
-                    }
+                // chance node: reach / chance factor into the first child's row,
+                // which every child reads (KernelArgs::reach_src)
+                const ChanceTab ct = uni_load(A.chance + nd.aux);
+                if constexpr (sizeof(V) == 8) {
+                    const double sc = 1.0 / (double)ct.factor;
+                    FOR_K r[k] = r[k] * sc;
+                } else {
+                    FOR_K r[k] = __fmul_rn(r[k], ct.scale);
                 }
-                for (int a = 0; a < na; ++a)
-                    FOR_K if (hslot(k) < Ho) rc[(size_t)a * c.stride + hslot(k)] = r[k];
+                FOR_K if (hslot(k) < Ho) rc[hslot(k)] = r[k];
             }
         }
         if (any) __syncthreads();

@@ -400,9 +400,10 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
 
         // Reach rows per traversing player p (the opponent's reach): a node
         // whose parent passes the reach through unchanged -- a decision node of
-        // p, or a single-action node -- uses its parent's row; the root, the
-        // children of chance nodes and of the opponent's multi-action nodes own
-        // one.  The forward pass writes only owned rows.
+        // p, or a single-action node -- uses its parent's row; the children of
+        // a chance node all hold the same scaled reach and use the first
+        // child's row; the root and the children of the opponent's
+        // multi-action nodes own one.  The forward pass writes only owned rows.
         {
             std::vector<int> parent(nodes.size(), -1);
             for (size_t i = 0; i < nodes.size(); ++i)
@@ -418,6 +419,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
                         const GameNode& pn = nodes[par];
                         const bool through = !pn.is_chance() && ((pn.player & PLAYER_MASK) == p || pn.num_children == 1);
                         if (through) src = L.reach_src[base + 2 * pos[par] + p];
+                        else if (pn.is_chance()) src = pos[pn.first_child];
                     }
                     L.reach_src[base + 2 * k + p] = src;
                 }
