@@ -994,9 +994,49 @@ __device__ __forceinline__ void children_rows(const V* const (&cc)[NN], V* const
     }
 }
 
-// CHANCE: the tile may hold chance nodes (only a turn game's root tile does;
-// river games and river subtree tiles never do, and leaving the chance sums
-// out keeps those kernels' register count down).
+// Backward of chance node n (record nd) for traversal p: its cfv row = the
+// f64 sum over the dealt cards' rows, in child order, plus the isomorphic
+// cards' rows (solver.rs:256-331).
+template <typename V>
+__device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int p) {
+    const KernelArgs& A = c.A;
+    const int Hp = num_hands(c.G, p);
+    const int na = nd.num_children;
+    V* cfv = cfv_rows<V>(c);
+    const V* cc = cfv + (size_t)nd.first_child * c.stride;
+    const ChanceTab ct = uni_load(A.chance + nd.aux);
+    double acc[KH];
+    FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
+    // children in groups of CHANCE_GROUP rows in flight together, summed in
+    // child order
+    constexpr int CHANCE_GROUP = 8;
+    for (int a0 = 1; a0 < na; a0 += CHANCE_GROUP) {
+        V v[CHANCE_GROUP][KH];
+#pragma unroll
+        for (int g = 0; g < CHANCE_GROUP; ++g)
+            if (a0 + g < na) {
+                const V* ca = cc + (size_t)(a0 + g) * c.stride;
+                FOR_K v[g][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+            }
+#pragma unroll
+        for (int g = 0; g < CHANCE_GROUP; ++g)
+            if (a0 + g < na) FOR_K acc[k] += (double)v[g][k];
+    }
+    for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
+        const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
+        const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
+        V v[KH];
+        FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
+        FOR_K acc[k] += (double)v[k];
+    }
+    V* out = cfv + (size_t)n * c.stride;
+    FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
+}
+
+// CHANCE: the tile may hold chance nodes whose sums it computes (a flop
+// game's turn tiles); river games and river subtree tiles hold none, and the
+// root tiles' chance nodes are summed by PHASE_CHANCE_SUMS, so leaving the
+// chance sums out keeps those kernels' register count down.
 template <int MODE, int SM, typename V = float, bool CHANCE = false>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
@@ -1049,36 +1089,8 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                 }
             }
             if (nd.kind == KIND_CHANCE) {
-                if constexpr (CHANCE) {
-                    // f64 sum over the dealt cards plus the isomorphic swaps
-                    // (solver.rs:256-331)
-                    const ChanceTab ct = uni_load(A.chance + nd.aux);
-                    double acc[KH];
-                    FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
-                    // children in groups of CHANCE_GROUP rows in flight together,
-                    // summed in child order
-                    constexpr int CHANCE_GROUP = 8;
-                    for (int a0 = 1; a0 < na; a0 += CHANCE_GROUP) {
-                        V v[CHANCE_GROUP][KH];
-    #pragma unroll
-                        for (int g = 0; g < CHANCE_GROUP; ++g)
-                            if (a0 + g < na) {
-                                const V* ca = cc + (size_t)(a0 + g) * c.stride;
-                                FOR_K v[g][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
-                            }
-    #pragma unroll
-                        for (int g = 0; g < CHANCE_GROUP; ++g)
-                            if (a0 + g < na) FOR_K acc[k] += (double)v[g][k];
-                    }
-                    for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
-                        const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
-                        const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
-                        V v[KH];
-                        FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
-                        FOR_K acc[k] += (double)v[k];
-                    }
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
-                }
+                // root tiles' chance nodes are summed by PHASE_CHANCE_SUMS
+                if constexpr (CHANCE) chance_sum<V>(c, nd, n, p);
             } else if (nd.player == p && na > 1) {
                 if constexpr (sizeof(V) == 8 && MODE == MODE_CFV) {
                     // FP64 value of the average strategy: sum_a sigma_a * cfv_a,
@@ -1441,7 +1453,8 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
 }
 
 // One phase of one traversal of multi-tile games (turn subgames).  The host
-// issues ROOT_FORWARD, SUBTREES, ROOT_BACKWARD per traversing player.  In
+// issues ROOT_FORWARD, SUBTREES, CHANCE_SUMS, ROOT_BACKWARD per traversing
+// player.  In
 // DCFR mode a game whose expl_out is already <= target is skipped (the
 // reference solve() stops iterating a converged game).
 template <int MODE, int SM, typename V>
@@ -1459,7 +1472,7 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                 tile_forward<MODE, SM, V>(c, T, p);
             } else {
                 tile_terminals<V>(c, T, p);
-                tile_backward<MODE, SM, V, true>(c, T, p, disc);
+                tile_backward<MODE, SM, V, false>(c, T, p, disc);   // chance rows: PHASE_CHANCE_SUMS
                 if (MODE == MODE_BR || MODE == MODE_CFV) {
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
@@ -1467,6 +1480,18 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                 if (MODE == MODE_CFV && sizeof(V) == 4) store_root_cfvalues(c, T, p);
             }
             __syncthreads();
+        }
+        return;
+    }
+    if (A.phase == PHASE_CHANCE_SUMS) {
+        // every root-tile chance node of every game in parallel; its children
+        // (subtree tile roots) were finished by PHASE_SUBTREES
+        for (int k = blockIdx.x; k < A.num_root_chance; k += gridDim.x) {
+            const int gi = uni(A.root_chance[2 * k]), n = uni(A.root_chance[2 * k + 1]);
+            Ctx c = make_ctx(A, gi, scan, rs, &sh);
+            if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
+            const NodeRec nd = uni_load(c.nodes + n);
+            chance_sum<V>(c, nd, n, p);
         }
         return;
     }
