@@ -1035,7 +1035,7 @@ __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int
 
 // CHANCE: the tile may hold chance nodes whose sums it computes (a flop
 // game's turn tiles); river games and river subtree tiles hold none, and the
-// root tiles' chance nodes are summed by PHASE_CHANCE_SUMS, so leaving the
+// root tiles' chance nodes are summed by PHASE_ROOT_LEAVES, so leaving the
 // chance sums out keeps those kernels' register count down.
 template <int MODE, int SM, typename V = float, bool CHANCE = false>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
@@ -1089,7 +1089,7 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                 }
             }
             if (nd.kind == KIND_CHANCE) {
-                // root tiles' chance nodes are summed by PHASE_CHANCE_SUMS
+                // root tiles' chance nodes are summed by PHASE_ROOT_LEAVES
                 if constexpr (CHANCE) chance_sum<V>(c, nd, n, p);
             } else if (nd.player == p && na > 1) {
                 if constexpr (sizeof(V) == 8 && MODE == MODE_CFV) {
@@ -1453,7 +1453,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
 }
 
 // One phase of one traversal of multi-tile games (turn subgames).  The host
-// issues ROOT_FORWARD, SUBTREES, CHANCE_SUMS, ROOT_BACKWARD per traversing
+// issues ROOT_FORWARD, SUBTREES, ROOT_LEAVES, ROOT_BACKWARD per traversing
 // player.  In
 // DCFR mode a game whose expl_out is already <= target is skipped (the
 // reference solve() stops iterating a converged game).
@@ -1471,8 +1471,8 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
             if (A.phase == PHASE_ROOT_FORWARD) {
                 tile_forward<MODE, SM, V>(c, T, p);
             } else {
-                tile_terminals<V>(c, T, p);
-                tile_backward<MODE, SM, V, false>(c, T, p, disc);   // chance rows: PHASE_CHANCE_SUMS
+                // terminal and chance rows: PHASE_ROOT_LEAVES
+                tile_backward<MODE, SM, V, false>(c, T, p, disc);
                 if (MODE == MODE_BR || MODE == MODE_CFV) {
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
@@ -1483,15 +1483,27 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         }
         return;
     }
-    if (A.phase == PHASE_CHANCE_SUMS) {
-        // every root-tile chance node of every game in parallel; its children
-        // (subtree tile roots) were finished by PHASE_SUBTREES
-        for (int k = blockIdx.x; k < A.num_root_chance; k += gridDim.x) {
-            const int gi = uni(A.root_chance[2 * k]), n = uni(A.root_chance[2 * k + 1]);
+    if (A.phase == PHASE_ROOT_LEAVES) {
+        // every chance node and terminal of every root tile in parallel: a
+        // chance node's children (subtree tile roots) were finished by
+        // PHASE_SUBTREES, a terminal's reach row by PHASE_ROOT_FORWARD
+        for (int k = blockIdx.x; k < A.num_root_leaves; k += gridDim.x) {
+            const int gi = uni(A.root_leaves[2 * k]), n = uni(A.root_leaves[2 * k + 1]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
             if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
             const NodeRec nd = uni_load(c.nodes + n);
-            chance_sum<V>(c, nd, n, p);
+            if (nd.kind == KIND_CHANCE) {
+                chance_sum<V>(c, nd, n, p);
+            } else {
+                const int Ho = num_hands(c.G, p ^ 1);
+                const V* reach = reach_rows<V>(c) + (size_t)reach_src(c, n, p) * c.stride;
+                RunoutRegs R;
+                load_runout(c, nd.aux, p, R);
+                V rv[KH];
+                FOR_K rv[k] = hslot(k) < Ho ? reach[hslot(k)] : V(0);
+                eval_terminal<V>(c, nd, n, p, R, rv, true);
+            }
+            __syncthreads();   // the next node reuses the scan buffers
         }
         return;
     }

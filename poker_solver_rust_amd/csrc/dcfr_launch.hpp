@@ -21,9 +21,9 @@ enum JobFlags : int {
 enum Phase : int {
     PHASE_ROOT_FORWARD = 0,   // root tile forward, one block per multi-tile game
     PHASE_SUBTREES = 1,       // every subtree tile: forward, terminals, backward (work queue)
-    PHASE_ROOT_BACKWARD = 2,  // root tile terminals + backward, one block per game
-    PHASE_CHANCE_SUMS = 3,    // chance nodes of the root tiles, one block per node: the sums
-                              // over their dealt cards, before PHASE_ROOT_BACKWARD
+    PHASE_ROOT_BACKWARD = 2,  // root tile backward, one block per game
+    PHASE_ROOT_LEAVES = 3,    // chance nodes (sums over their dealt cards) and terminals of the
+                              // root tiles, one block per node, before PHASE_ROOT_BACKWARD
 };
 
 enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3, MODE_CFV64 = 4 };
@@ -80,8 +80,8 @@ struct KernelArgs {
     int num_multi_games;
     const int* subtree_tiles; // global tile ids of all subtree tiles, largest first
     int num_subtree_tiles;
-    const int* root_chance;   // (game, local node) pairs: chance nodes of the root tiles
-    int num_root_chance;
+    const int* root_leaves;   // (game, local node) pairs: chance nodes and terminals of the root tiles
+    int num_root_leaves;
     int phase, player, mode;
     uint32_t iteration;
     double* ev_out;           // per game and player: weighted root value (best response)

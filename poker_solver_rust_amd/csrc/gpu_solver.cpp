@@ -56,7 +56,8 @@ struct HostLayout {
     std::vector<int> multi_games;    // games solved by phase launches (turn)
     std::vector<std::pair<double, int>> subtree_cost;
     std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
-    std::vector<int> root_chance;    // (game, local node) of the root tiles' chance nodes, multi-tile games
+    std::vector<int> root_leaves;    // (game, local node): chance nodes and terminals of the root tiles of
+                                     // multi-tile games, chance nodes first
     int32_t num_boundaries = 0;      // depth-boundary leaves over the batch
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
     std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
@@ -455,9 +456,13 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             for (auto& lr : plan.levels[0])
                 for (int k = lr.first; k < lr.second; ++k)
                     if (L.nodes[d.node_base + k].kind == KIND_CHANCE) {
-                        L.root_chance.push_back(gi);
-                        L.root_chance.push_back(k);
+                        L.root_leaves.push_back(gi);
+                        L.root_leaves.push_back(k);
                     }
+            for (int v : plan.terminals[0]) {
+                L.root_leaves.push_back(gi);
+                L.root_leaves.push_back(v);
+            }
         } else {
             L.single_games.push_back(gi);
         }
@@ -545,7 +550,7 @@ struct psr_batch {
     uint32_t* d_iters = nullptr;
     float* d_root_cfv = nullptr;
     int* d_counter = nullptr;
-    int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr, *d_root_chance = nullptr;
+    int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr, *d_root_leaves = nullptr;
     double* d_ev = nullptr;
     int32_t* d_reach_src = nullptr;
     float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
@@ -558,13 +563,13 @@ struct psr_batch {
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
-                        d_reach_src, d_root_chance};
+                        d_reach_src, d_root_leaves};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
         d_reach_src = nullptr;
-        d_root_chance = nullptr;
+        d_root_leaves = nullptr;
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
         d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
         d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
@@ -618,8 +623,8 @@ struct psr_batch {
         a.num_multi_games = (int)L.multi_games.size();
         a.subtree_tiles = d_subtrees;
         a.num_subtree_tiles = (int)L.subtree_tiles.size();
-        a.root_chance = d_root_chance;
-        a.num_root_chance = (int)L.root_chance.size() / 2;
+        a.root_leaves = d_root_leaves;
+        a.num_root_leaves = (int)L.root_leaves.size() / 2;
         a.ev_out = d_ev;
         a.bnd_cfv = d_bnd_cfv;
         a.bnd_reach = d_bnd_reach;
@@ -637,7 +642,7 @@ struct psr_batch {
         a.mode = mode;
         a.iteration = t;
         int g = ph == PHASE_SUBTREES       ? std::min(a.num_subtree_tiles, grid_all())
-                : ph == PHASE_CHANCE_SUMS ? std::min(a.num_root_chance, grid_all())
+                : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves, grid_all())
                                           : a.num_multi_games;
         if (g <= 0) return 0;
         HIP_OK(dcfr_launch_phase(a, g, stream));
@@ -671,7 +676,7 @@ struct psr_batch {
     int traverse(int player, int mode, uint32_t t) {
         if (phase(PHASE_ROOT_FORWARD, player, mode, t)) return -1;
         if (phase(PHASE_SUBTREES, player, mode, t)) return -1;
-        if (phase(PHASE_CHANCE_SUMS, player, mode, t)) return -1;
+        if (phase(PHASE_ROOT_LEAVES, player, mode, t)) return -1;
         return phase(PHASE_ROOT_BACKWARD, player, mode, t);
     }
 
@@ -859,7 +864,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         upload(&b->d_iso_perm, L.iso_perm) || upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) ||
         upload(&b->d_single, L.single_games) || upload(&b->d_multi, L.multi_games) ||
         upload(&b->d_subtrees, L.subtree_tiles) || upload(&b->d_reach_src, L.reach_src) ||
-        upload(&b->d_root_chance, L.root_chance))
+        upload(&b->d_root_leaves, L.root_leaves))
         return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
