@@ -450,3 +450,31 @@ def test_flop_solve_schedule_matches_oracle():
     eo, ito = O.OracleGame(g.serialize()).solve(40, 0.5)
     assert int(it[0]) == ito
     assert abs(float(e[0]) - eo) <= RTOL * max(1.0, eo)
+
+
+def test_several_multi_tile_games_in_one_batch():
+    """Two turn games, a flop game and a river game in one batch: the root
+    tiles' chance nodes and terminals of every multi-tile game share one
+    parallel phase.  State after each iteration, then the solve() schedule
+    with targets that stop the games at different iterations (a stopped
+    game's phase work is skipped)."""
+    games = [turn_game(), turn_game("KK,QQ,AKo", "JJ-99,KQs", "Ks 7d 2h", "4s", "33%, a", "2x"),
+             flop_game("AA,KK", "QQ,AKs", "Qs Jh 2c"), std_game()]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 4)
+    oracles = [O.OracleGame(g.serialize()) for g in games]
+    for t in range(3):
+        b.run_iterations(t, t + 1)
+        for i, o in enumerate(oracles):
+            o.solve_step(t)
+            r, s = o.state()
+            close(b.regrets(i), r)
+            close(b.strategy_sum(i), s)
+    targets = [2.0, 0.5, 1.0, 0.3]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 40, targets)
+    e, it = b.solve(40, 5)
+    for i, g in enumerate(games):
+        eo, ito = O.OracleGame(g.serialize()).solve(40, targets[i])
+        assert int(it[i]) == ito
+        assert abs(float(e[i]) - eo) <= RTOL * max(1.0, eo)
