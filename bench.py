@@ -317,7 +317,29 @@ def mlp_bench(args):
                                        "frac_of_peak": tf / BF16_PEAK_TFLOPS}
         log(f"train batch {batch}: {ms / 2:.1f} ms/epoch, {tf:.1f} TFLOP/s")
     log(f"forward 8192 rows from host memory: {out['forward_8192_rows_from_host']}")
+    out["cpu_baseline"] = mlp_cpu_baseline(net.get_params(), dims, x[:512], y[:512], fwd_flop)
+    log(f"cpu baseline: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
+
+
+def mlp_cpu_baseline(params, dims, x, y, fwd_flop, reps=3):
+    """The numpy oracle of the same net (oracle/mlp_oracle.py) on the host in
+    FP32: forward, masked Huber loss and every gradient for one batch of
+    rows, best of `reps`; numpy's BLAS threads (the box's OMP_NUM_THREADS)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import mlp_oracle as M
+
+    p = np.asarray(params, np.float32)
+    mask = np.ones_like(y)
+    best = float("inf")
+    for _ in range(reps):
+        t = time.perf_counter()
+        M.loss_and_grads(p, dims, x, y, mask, 1.0)
+        best = min(best, time.perf_counter() - t)
+    rows = x.shape[0]
+    return {"value": round(3.0 * fwd_flop * rows / best / 1e12, 4), "unit": "TFLOP/s", "rows_per_s": round(rows / best, 1),
+            "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "port",
+            "sample": f"one FP32 forward + backward (no optimizer step) of {rows} rows, best of {reps}, on '{cpu_model()}'"}
 
 
 def copy_probe(nbytes):
