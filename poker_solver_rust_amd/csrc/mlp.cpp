@@ -76,6 +76,8 @@ struct psr_mlp {
     float* row_loss = nullptr;
     float* ws = nullptr;        // split-K partial products
     float* stage = nullptr;     // FP32 input rows on their way to H[0]
+    AdamSegment* segs = nullptr;  // AdamW segments (W_1, b_1, W_2, ...), device copy
+    int64_t max_seg = 0;
     int64_t ws_floats = 0;
     float* yb = nullptr;        // batch targets / mask
     float* mbuf = nullptr;
@@ -199,12 +201,7 @@ struct psr_mlp {
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float bc1 = 1.0f - (float)std::pow(0.9, (double)step);
         const float bc2 = 1.0f - (float)std::pow(0.999, (double)step);
-        for (int l = 1; l <= L; ++l) {
-            HIP_OK(adamw_launch(W[l], Wb[l], gW[l], mW[l], vW[l], dims[l - 1], dims[l], ld[l], lr, b1, b2, eps, wd,
-                                bc1, bc2, stream));
-            HIP_OK(adamw_launch(b[l], nullptr, gb[l], mb[l], vb[l], 1, dims[l], dims[l], lr, b1, b2, eps, 0.0f, bc1, bc2,
-                                stream));
-        }
+        HIP_OK(adamw_launch(segs, 2 * L, max_seg, lr, b1, b2, eps, wd, bc1, bc2, stream));
         return 0;
     }
 
@@ -297,6 +294,18 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     int64_t wsf = 0;
     for (int l = 1; l <= L; ++l) wsf = std::max<int64_t>(wsf, 8 * (int64_t)(dims[l - 1] + 1) * dims[l]);
     if (bad(m->alloc((void**)&m->stage, (size_t)max_batch * dims[0] * 4))) return -1;
+    {
+        std::vector<AdamSegment> sg;
+        for (int l = 1; l <= L; ++l) {
+            const int64_t nw = (int64_t)dims[l - 1] * dims[l];
+            sg.push_back({m->W[l], m->Wb[l], m->gW[l], m->mW[l], m->vW[l], nw, m->ld[l], dims[l], 1});
+            sg.push_back({m->b[l], nullptr, m->gb[l], m->mb[l], m->vb[l], dims[l], dims[l], dims[l], 0});
+            m->max_seg = std::max(m->max_seg, nw);
+        }
+        if (bad(m->alloc((void**)&m->segs, sg.size() * sizeof(AdamSegment))) ||
+            hipMemcpy(m->segs, sg.data(), sg.size() * sizeof(AdamSegment), hipMemcpyHostToDevice) != hipSuccess)
+            return bad(-1);
+    }
     m->ws_floats = wsf;
     if (bad(m->alloc((void**)&m->ws, (size_t)wsf * 4))) return -1;
     if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||

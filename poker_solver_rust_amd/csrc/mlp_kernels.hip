@@ -272,18 +272,22 @@ __global__ void huber_kernel(const float* pred, const float* target, const float
 
 // AdamW (decoupled weight decay) on FP32 master parameters, refreshing the
 // BF16 working copy.  bc1 / bc2 are the bias corrections 1 - beta^t.
-__global__ void adamw_kernel(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t n, int cols, int64_t ldb,
-                             float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float gi = g[i];
-        const float mi = beta1 * m[i] + (1.0f - beta1) * gi;
-        const float vi = beta2 * v[i] + (1.0f - beta2) * gi * gi;
-        m[i] = mi;
-        v[i] = vi;
+// All parameter segments in one launch: blockIdx.y picks the segment, the
+// x blocks stride over its elements.
+__global__ void adamw_kernel(const AdamSegment* segs, float lr, float beta1, float beta2, float eps, float wd, float bc1,
+                             float bc2) {
+    const AdamSegment sg = segs[blockIdx.y];
+    const float d = sg.decay ? wd : 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sg.n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = sg.g[i];
+        const float mi = beta1 * sg.m[i] + (1.0f - beta1) * gi;
+        const float vi = beta2 * sg.v[i] + (1.0f - beta2) * gi * gi;
+        sg.m[i] = mi;
+        sg.v[i] = vi;
         const float mh = mi / bc1, vh = vi / bc2;
-        const float pi = p[i] - lr * (mh / (sqrtf(vh) + eps) + wd * p[i]);
-        p[i] = pi;
-        if (pb) pb[(i / cols) * ldb + i % cols] = f32_to_bf16_bits(pi);
+        const float pi = sg.p[i] - lr * (mh / (sqrtf(vh) + eps) + d * sg.p[i]);
+        sg.p[i] = pi;
+        if (sg.pb) sg.pb[(i / sg.cols) * sg.ldb + i % sg.cols] = f32_to_bf16_bits(pi);
     }
 }
 
@@ -367,14 +371,13 @@ hipError_t huber_launch(const float* pred, const float* target, const float* mas
     return hipGetLastError();
 }
 
-hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t rows, int cols, int64_t ldb,
-                        float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2,
-                        hipStream_t stream) {
-    const int64_t n = rows * cols;
-    int64_t blocks = (n + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, pb, g, m, v, n, cols, ldb, lr,
-                       beta1, beta2, eps, wd, bc1, bc2);
+hipError_t adamw_launch(const AdamSegment* segs, int num_segs, int64_t max_n, float lr, float beta1, float beta2,
+                        float eps, float wd, float bc1, float bc2, hipStream_t stream) {
+    if (num_segs <= 0 || max_n <= 0) return hipErrorInvalidValue;
+    int64_t blocks = (max_n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)blocks, (unsigned)num_segs), dim3(256), 0, stream, segs, lr, beta1,
+                       beta2, eps, wd, bc1, bc2);
     return hipGetLastError();
 }
 

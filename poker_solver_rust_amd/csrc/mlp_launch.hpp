@@ -52,11 +52,24 @@ hipError_t gemm_launch(const GemmArgs& g, int epi, hipStream_t stream);
 hipError_t huber_launch(const float* pred, const float* target, const float* mask, const int32_t* src_rows, int rows,
                         int cols, int64_t ldp, float delta, float inv_count, uint16_t* grad, float* row_loss,
                         hipStream_t stream);
-// AdamW on n = rows * cols FP32 parameters; pb (optional) is the BF16 copy
-// with row stride ldb
-hipError_t adamw_launch(float* p, uint16_t* pb, const float* g, float* m, float* v, int64_t rows, int cols, int64_t ldb,
-                        float lr, float beta1, float beta2, float eps, float wd, float bc1, float bc2,
-                        hipStream_t stream);
+// One block of FP32 parameters for AdamW: n = rows * cols values with their
+// gradients and moments; pb (optional) is the BF16 copy with row stride ldb;
+// decay: weight decay applies (weights, not biases)
+struct AdamSegment {
+    float* p;
+    uint16_t* pb;
+    const float* g;
+    float* m;
+    float* v;
+    int64_t n;
+    int64_t ldb;
+    int cols;
+    int decay;
+};
+// AdamW over num_segs segments (device array segs) in one launch; max_n =
+// the largest segment
+hipError_t adamw_launch(const AdamSegment* segs, int num_segs, int64_t max_n, float lr, float beta1, float beta2,
+                        float eps, float wd, float bc1, float bc2, hipStream_t stream);
 // x [rows x cols] FP32 -> y [rows x ldy] BF16
 hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, int64_t ldy, hipStream_t stream);
 // batch row r <- training row rows[r]: rows of ldx BF16, copied whole
