@@ -285,6 +285,54 @@ def river_uniform_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def flop_bench(args):
+    """Flop-root subgame (three streets on the device: the flop tile, then
+    one tile per turn card holding its river deals): DCFR iterations/s after
+    one warm-up iteration, the HBM roofline on 20 B per edge-hand per
+    iteration, and the CPU oracle (one thread) on the same tree."""
+    import poker_solver_rust_amd as P
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as O
+
+    oop, ip = "22+,A2s+,K9s+,QTs+,JTs,ATo+,KJo+", "55+,A8s+,KTs+,QJs,AJo+,KQo"
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str("Qs Jh 2c"))
+    sizes = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=200, flop_bet_sizes=sizes,
+                      turn_bet_sizes=sizes, river_bet_sizes=sizes)
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    iters = args.flop_iters
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], iters + 1)
+    b.run_iterations(0, 1)
+    t = time.perf_counter()
+    b.run_iterations(1, iters + 1)
+    secs = time.perf_counter() - t
+    kms = b.last_kernel_ms()
+    edge_hands = float(g.num_storage())
+    achieved = 20.0 * edge_hands * iters / (kms * 1e-3) / 1e9
+    o = O.OracleGame(g.serialize())
+    o.solve_step(0)
+    cpu_iters = 3
+    t = time.perf_counter()
+    for k in range(1, cpu_iters + 1):
+        o.solve_step(k)
+    cpu_secs = time.perf_counter() - t
+    out = {"metric": "flop-root subgame DCFR iterations/s", "value": round(iters / secs, 2), "unit": "DCFR iters/s",
+           "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"flop Qs Jh 2c, OOP {oop}, IP {ip}, pot 100, stack 200, bets 50%/all-in on "
+                                  "every street, no raises", "nodes": g.num_nodes(), "edge_hands": edge_hands,
+                      "tiles": g.validate_layout()["tiles"], "iterations": iters},
+           "ms_per_iter": round(secs * 1e3 / iters, 3),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5)},
+           "cpu_baseline": {"value": round(cpu_iters / cpu_secs, 3), "unit": "DCFR iters/s", "cores": 1, "kind": "port",
+                            "sample": f"{cpu_iters} solve_step iterations of the same tree after one, 1 thread on "
+                                      f"'{cpu_model()}'"}}
+    log(f"flop: {iters} iterations in {secs * 1e3:.1f} ms ({iters / secs:.1f} it/s); CPU oracle "
+        f"{cpu_iters / cpu_secs:.2f} it/s")
+    print(json.dumps(out), flush=True)
+
+
 def mlp_bench(args):
     """Value-net throughput on the 2706 -> 4 x 512 -> 2652 net (BF16 MFMA):
     inference on 8192 rows, and training epochs at batch 512 and 4096 over a
@@ -386,6 +434,8 @@ def main():
     ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")
     ap.add_argument("--c4-iters", type=int, default=500)
     ap.add_argument("--mlp", action="store_true", help="value-net MFMA throughput instead")
+    ap.add_argument("--flop", action="store_true", help="flop-root subgame DCFR iterations/s instead")
+    ap.add_argument("--flop-iters", type=int, default=50)
     ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations, device and CPU oracle instead")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
@@ -402,6 +452,9 @@ def main():
         return
     if args.mlp:
         mlp_bench(args)
+        return
+    if args.flop:
+        flop_bench(args)
         return
     if args.net_solve:
         net_solve_bench(args)
