@@ -1452,9 +1452,10 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
     }
 }
 
-// One phase of one traversal of multi-tile games (turn subgames).  The host
-// issues ROOT_FORWARD, SUBTREES, ROOT_LEAVES, ROOT_BACKWARD per traversing
-// player.  In
+// One phase of one traversal of multi-tile games (turn and flop subgames).
+// The host issues ROOT_FORWARD, MID_FORWARD, SUBTREES, MID_LEAVES,
+// MID_BACKWARD, ROOT_LEAVES, ROOT_BACKWARD per traversing player (the middle
+// phases only when some game has middle tiles).  In
 // DCFR mode a game whose expl_out is already <= target is skipped (the
 // reference solve() stops iterating a converged game).
 template <int MODE, int SM, typename V>
@@ -1483,12 +1484,15 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         }
         return;
     }
-    if (A.phase == PHASE_ROOT_LEAVES) {
-        // every chance node and terminal of every root tile in parallel: a
-        // chance node's children (subtree tile roots) were finished by
-        // PHASE_SUBTREES, a terminal's reach row by PHASE_ROOT_FORWARD
-        for (int k = blockIdx.x; k < A.num_root_leaves; k += gridDim.x) {
-            const int gi = uni(A.root_leaves[2 * k]), n = uni(A.root_leaves[2 * k + 1]);
+    if (A.phase == PHASE_ROOT_LEAVES || A.phase == PHASE_MID_LEAVES) {
+        // every chance node and terminal of every root (middle) tile in
+        // parallel: a chance node's children (tile roots one level down) were
+        // finished before, a terminal's reach row by the forward phases
+        const bool root = A.phase == PHASE_ROOT_LEAVES;
+        const int count = root ? A.num_root_leaves : A.num_mid_leaves;
+        const int* list = root ? A.root_leaves : A.mid_leaves;
+        for (int k = blockIdx.x; k < count; k += gridDim.x) {
+            const int gi = uni(list[2 * k]), n = uni(list[2 * k + 1]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
             if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
             const NodeRec nd = uni_load(c.nodes + n);
@@ -1504,6 +1508,19 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                 eval_terminal<V>(c, nd, n, p, R, rv, true);
             }
             __syncthreads();   // the next node reuses the scan buffers
+        }
+        return;
+    }
+    if (A.phase == PHASE_MID_FORWARD || A.phase == PHASE_MID_BACKWARD) {
+        // middle tiles: the forward, or the decision nodes' backward (their
+        // chance rows and terminals come from PHASE_MID_LEAVES)
+        for (int k = blockIdx.x; k < A.num_mid_tiles; k += gridDim.x) {
+            const TileDesc T = uni_load(A.tiles + uni(A.mid_tiles[k]));
+            Ctx c = make_ctx(A, T.game, scan, rs, &sh);
+            if (MODE == MODE_DCFR && uni(A.expl_out[T.game]) <= c.G.target) continue;
+            if (A.phase == PHASE_MID_FORWARD) tile_forward<MODE, SM, V>(c, T, p);
+            else tile_backward<MODE, SM, V, false>(c, T, p, disc);
+            __syncthreads();
         }
         return;
     }

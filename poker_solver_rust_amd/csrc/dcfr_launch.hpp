@@ -24,6 +24,9 @@ enum Phase : int {
     PHASE_ROOT_BACKWARD = 2,  // root tile backward, one block per game
     PHASE_ROOT_LEAVES = 3,    // chance nodes (sums over their dealt cards) and terminals of the
                               // root tiles, one block per node, before PHASE_ROOT_BACKWARD
+    PHASE_MID_FORWARD = 4,    // middle tiles (a flop game's turn tiles) forward, one block per tile
+    PHASE_MID_LEAVES = 5,     // their chance nodes and terminals, one block per node
+    PHASE_MID_BACKWARD = 6,   // their decision nodes, one block per tile
 };
 
 enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3, MODE_CFV64 = 4 };
@@ -82,6 +85,10 @@ struct KernelArgs {
     int num_subtree_tiles;
     const int* root_leaves;   // (game, local node) pairs: chance nodes and terminals of the root tiles
     int num_root_leaves;
+    const int* mid_tiles;     // global tile ids of the middle tiles (tiles below a root tile that hold
+    int num_mid_tiles;        // chance nodes), and their chance nodes and terminals as (game, node)
+    const int* mid_leaves;
+    int num_mid_leaves;
     int phase, player, mode;
     uint32_t iteration;
     double* ev_out;           // per game and player: weighted root value (best response)

@@ -58,6 +58,8 @@ struct HostLayout {
     std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
     std::vector<int> root_leaves;    // (game, local node): chance nodes and terminals of the root tiles of
                                      // multi-tile games, chance nodes first
+    std::vector<int> mid_tiles;      // tiles below a root tile that hold chance nodes (a flop game's turn tiles)
+    std::vector<int> mid_leaves;     // (game, local node): their chance nodes and terminals
     int32_t num_boundaries = 0;      // depth-boundary leaves over the batch
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
     std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
@@ -172,88 +174,53 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     return t;
 }
 
-// Device node order of one game: tile 0 = nodes reachable from the root
-// without entering a chance node's children (breadth-first), then for each
-// chance node its children (subtree roots, contiguous), then the body of each
-// subtree (breadth-first).  Returns per tile the level ranges and terminals.
+// Device node order of one game, tile by tile: a tile is the part of the
+// tree reachable from its root without entering a chance node's children
+// (breadth-first, so each level is a contiguous range); the children of each
+// of its chance nodes follow it contiguously and are the roots of the tiles of
+// the next tile level.  Tile 0 holds the game root; a turn game has tile
+// levels 0 (turn betting) and 1 (one tile per river card), a flop game levels
+// 0 (flop), 1 (one tile per turn card: its betting down to the river deals)
+// and 2 (one tile per river card).  Returns per tile the level ranges,
+// terminals, cost, whether it holds a chance node, and its tile level.
 struct TilePlan {
     std::vector<int> order;                         // device position -> arena index
     std::vector<std::vector<std::pair<int, int>>> levels;   // per tile, per level [start,end)
     std::vector<std::vector<int>> terminals;        // per tile, device ids
     std::vector<double> cost;                       // per tile
     std::vector<char> has_chance;                   // per tile: holds a chance node
+    std::vector<int> depth;                         // per tile: tile level (0 = game root tile)
 };
 
 int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
     const auto& nodes = g.nodes();
     const int N = (int)nodes.size();
-    std::vector<int> depth(N, 0);
-    for (int i = 0; i < N; ++i)
-        for (uint32_t a = 0; a < nodes[i].num_children; ++a) depth[nodes[i].first_child + a] = depth[i] + 1;
-    // root tile: breadth-first from the root, not entering chance children
-    std::vector<int> root_tile, chance_nodes;
-    std::vector<char> in_root(N, 0);
-    root_tile.push_back(0);
-    in_root[0] = 1;
-    for (size_t k = 0; k < root_tile.size(); ++k) {
-        const GameNode& n = nodes[root_tile[k]];
-        if (n.is_terminal()) continue;
-        if (n.is_chance()) {
-            chance_nodes.push_back(root_tile[k]);
-            continue;
-        }
-        for (uint32_t a = 0; a < n.num_children; ++a) {
-            root_tile.push_back((int)(n.first_child + a));
-            in_root[n.first_child + a] = 1;
-        }
-    }
-    std::stable_sort(root_tile.begin(), root_tile.end(), [&](int a, int b) { return depth[a] < depth[b]; });
-    plan.order = root_tile;
-    plan.levels.assign(1, {});
-    plan.terminals.assign(1, {});
-    plan.cost.assign(1, 0.0);
-    plan.has_chance.assign(1, !chance_nodes.empty());
     std::vector<int> pos(N, -1);
-    for (size_t k = 0; k < plan.order.size(); ++k) pos[plan.order[k]] = (int)k;
-    for (size_t k = 0; k < root_tile.size(); ++k) {
-        const int d = depth[root_tile[k]];
-        if (plan.levels[0].empty() || depth[plan.order[plan.levels[0].back().first]] != d)
-            plan.levels[0].push_back({(int)k, (int)k + 1});
-        else
-            plan.levels[0].back().second = (int)k + 1;
-        if (nodes[root_tile[k]].is_terminal()) plan.terminals[0].push_back((int)k);
-        plan.cost[0] += nodes[root_tile[k]].num_children;
-    }
-    // subtree roots, contiguous per chance node
-    std::vector<int> roots;
-    for (int c : chance_nodes) {
-        const GameNode& n = nodes[c];
-        for (uint32_t a = 0; a < n.num_children; ++a) {
-            const int r = (int)(n.first_child + a);
-            pos[r] = (int)plan.order.size();
-            plan.order.push_back(r);
-            roots.push_back(r);
-        }
-    }
-    // subtree bodies
-    for (int r : roots) {
-        std::vector<std::pair<int, int>> lv;
-        std::vector<int> terms;
-        lv.push_back({pos[r], pos[r] + 1});
-        if (nodes[r].is_terminal()) terms.push_back(pos[r]);
+    plan = TilePlan{};
+    struct Root {
+        int node, depth;
+    };
+    std::vector<Root> roots{{0, 0}};
+    pos[0] = 0;
+    plan.order.push_back(0);
+    for (size_t q = 0; q < roots.size(); ++q) {
+        const int r = roots[q].node;
+        std::vector<std::pair<int, int>> lv{{pos[r], pos[r] + 1}};
+        std::vector<int> terms, chance;
         double cost = 1.0;
-        // (the root itself may be a chance node: the river deal right after an
-        // all-in call on the turn deal)
-        bool plan_chance = nodes[r].is_chance();
         std::vector<int> frontier{r};
-        while (!frontier.empty()) {
+        for (;;) {
             std::vector<int> next;
             for (int v : frontier) {
                 const GameNode& n = nodes[v];
-                if (n.is_terminal()) continue;
-                // a chance node inside a subtree (the river deal below a flop
-                // game's turn card) keeps its children in the same tile
-                if (n.is_chance()) plan_chance = true;
+                if (n.is_terminal()) {
+                    terms.push_back(pos[v]);
+                    continue;
+                }
+                if (n.is_chance()) {
+                    chance.push_back(v);
+                    continue;
+                }
                 for (uint32_t a = 0; a < n.num_children; ++a) next.push_back((int)(n.first_child + a));
             }
             if (next.empty()) break;
@@ -261,16 +228,28 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
             for (int v : next) {
                 pos[v] = (int)plan.order.size();
                 plan.order.push_back(v);
-                if (nodes[v].is_terminal()) terms.push_back(pos[v]);
                 cost += 1.0 + nodes[v].num_children;
             }
             lv.push_back({start, (int)plan.order.size()});
             frontier.swap(next);
         }
+        std::sort(terms.begin(), terms.end());
         plan.levels.push_back(lv);
         plan.terminals.push_back(terms);
         plan.cost.push_back(cost);
-        plan.has_chance.push_back(plan_chance);
+        plan.has_chance.push_back(!chance.empty());
+        plan.depth.push_back(roots[q].depth);
+        if (!chance.empty() && roots[q].depth >= 2) return fail("more than three tile levels");
+        // the chance nodes' children, contiguous per chance node, root the next tile level
+        for (int c : chance) {
+            const GameNode& n = nodes[c];
+            for (uint32_t a = 0; a < n.num_children; ++a) {
+                const int ch = (int)(n.first_child + a);
+                pos[ch] = (int)plan.order.size();
+                plan.order.push_back(ch);
+                roots.push_back({ch, roots[q].depth + 1});
+            }
+        }
     }
     if ((int)plan.order.size() != N) return fail("tile plan does not cover the game");
     return 0;
@@ -448,7 +427,24 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
                 else if (aux != td.runout) td.runout = -1;
                 if (td.runout < 0) break;
             }
-            if (t > 0) L.subtree_cost.push_back({plan.cost[t] * std::max(d.H[0], d.H[1]), (int)L.tiles.size()});
+            // tiles below the root: with chance nodes, middle tiles (forward,
+            // leaves and backward in phases of their own); without, subtree
+            // tiles (traversed whole from the work queue)
+            if (t > 0 && plan.has_chance[t]) {
+                L.mid_tiles.push_back((int)L.tiles.size());
+                for (auto& lr : plan.levels[t])
+                    for (int k = lr.first; k < lr.second; ++k)
+                        if (L.nodes[d.node_base + k].kind == KIND_CHANCE) {
+                            L.mid_leaves.push_back(gi);
+                            L.mid_leaves.push_back(k);
+                        }
+                for (int v : plan.terminals[t]) {
+                    L.mid_leaves.push_back(gi);
+                    L.mid_leaves.push_back(v);
+                }
+            } else if (t > 0) {
+                L.subtree_cost.push_back({plan.cost[t] * std::max(d.H[0], d.H[1]), (int)L.tiles.size()});
+            }
             L.tiles.push_back(td);
         }
         if (d.num_tiles > 1) {
@@ -551,6 +547,7 @@ struct psr_batch {
     float* d_root_cfv = nullptr;
     int* d_counter = nullptr;
     int *d_single = nullptr, *d_multi = nullptr, *d_subtrees = nullptr, *d_root_leaves = nullptr;
+    int *d_mid_tiles = nullptr, *d_mid_leaves = nullptr;
     double* d_ev = nullptr;
     int32_t* d_reach_src = nullptr;
     float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
@@ -563,13 +560,15 @@ struct psr_batch {
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
-                        d_reach_src, d_root_leaves};
+                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
         d_reach_src = nullptr;
         d_root_leaves = nullptr;
+        d_mid_tiles = nullptr;
+        d_mid_leaves = nullptr;
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
         d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
         d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
@@ -625,6 +624,10 @@ struct psr_batch {
         a.num_subtree_tiles = (int)L.subtree_tiles.size();
         a.root_leaves = d_root_leaves;
         a.num_root_leaves = (int)L.root_leaves.size() / 2;
+        a.mid_tiles = d_mid_tiles;
+        a.num_mid_tiles = (int)L.mid_tiles.size();
+        a.mid_leaves = d_mid_leaves;
+        a.num_mid_leaves = (int)L.mid_leaves.size() / 2;
         a.ev_out = d_ev;
         a.bnd_cfv = d_bnd_cfv;
         a.bnd_reach = d_bnd_reach;
@@ -643,6 +646,8 @@ struct psr_batch {
         a.iteration = t;
         int g = ph == PHASE_SUBTREES       ? std::min(a.num_subtree_tiles, grid_all())
                 : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves, grid_all())
+                : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves, grid_all())
+                : ph == PHASE_MID_FORWARD || ph == PHASE_MID_BACKWARD ? std::min(a.num_mid_tiles, grid_all())
                                           : a.num_multi_games;
         if (g <= 0) return 0;
         HIP_OK(dcfr_launch_phase(a, g, stream));
@@ -675,7 +680,10 @@ struct psr_batch {
     }
     int traverse(int player, int mode, uint32_t t) {
         if (phase(PHASE_ROOT_FORWARD, player, mode, t)) return -1;
+        if (phase(PHASE_MID_FORWARD, player, mode, t)) return -1;
         if (phase(PHASE_SUBTREES, player, mode, t)) return -1;
+        if (phase(PHASE_MID_LEAVES, player, mode, t)) return -1;
+        if (phase(PHASE_MID_BACKWARD, player, mode, t)) return -1;
         if (phase(PHASE_ROOT_LEAVES, player, mode, t)) return -1;
         return phase(PHASE_ROOT_BACKWARD, player, mode, t);
     }
@@ -773,9 +781,10 @@ struct psr_batch {
                     if (traverse(p, MODE_CFV, 0)) return -1;
             }
             if (flags & JOB_REACH) {
-                // root forward, then every river tile (its forward fills the rows;
+                // root and middle tile forwards, then every subtree tile (its forward fills the rows;
                 // its terminals and backward only touch cfv rows)
                 if (phase(PHASE_ROOT_FORWARD, reach_player, MODE_CFV, 0)) return -1;
+                if (phase(PHASE_MID_FORWARD, reach_player, MODE_CFV, 0)) return -1;
                 if (phase(PHASE_SUBTREES, reach_player, MODE_CFV, 0)) return -1;
             }
             // publish per-game results next to the single-tile ones (solve jobs only)
@@ -864,7 +873,8 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         upload(&b->d_iso_perm, L.iso_perm) || upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) ||
         upload(&b->d_single, L.single_games) || upload(&b->d_multi, L.multi_games) ||
         upload(&b->d_subtrees, L.subtree_tiles) || upload(&b->d_reach_src, L.reach_src) ||
-        upload(&b->d_root_leaves, L.root_leaves))
+        upload(&b->d_root_leaves, L.root_leaves) || upload(&b->d_mid_tiles, L.mid_tiles) ||
+        upload(&b->d_mid_leaves, L.mid_leaves))
         return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
@@ -1324,8 +1334,8 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
         if (r.first_child < 0 || r.first_child + r.num_children > N) return fail("children out of range");
         for (int a = 0; a < r.num_children; ++a) {
             const int ch = r.first_child + a;
-            if (r.kind == KIND_CHANCE && tile_of[v] == 0) {
-                if (tile_of[ch] == tile_of[v] || level_of[ch] != 0) return fail("chance child is not a subtree root");
+            if (r.kind == KIND_CHANCE) {
+                if (tile_of[ch] == tile_of[v] || level_of[ch] != 0) return fail("chance child is not a tile root");
             } else if (tile_of[ch] != tile_of[v] || level_of[ch] != level_of[v] + 1) {
                 return fail("child not in the next level of its tile");
             }

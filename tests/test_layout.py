@@ -39,15 +39,16 @@ def test_turn_with_allin_chance_children():
     assert info["tiles"] > 48
 
 
-def test_flop_games_lay_out_one_tile_per_turn_deal():
-    """Flop-root games: the root tile (flop betting) and one tile per turn
-    card below each flop chance node, holding that turn's river deals; on a
-    rainbow flop no two turn cards are suit-isomorphic, so 49 per node."""
+def test_flop_games_lay_out_in_three_tile_levels():
+    """Flop-root games: the root tile (flop betting), one tile per turn card
+    below each of the 3 flop chance nodes (the turn betting down to its 3
+    river deals), one tile per river card below each of those; on a rainbow
+    flop no two cards are suit-isomorphic: 1 + 3 * 49 + 3 * 49 * 3 * 48."""
     s = [("50%", ""), ("50%", "")]
     cc = P.CardConfig(range=(P.Range.parse("AA"), P.Range.parse("KK")), flop=P.flop_from_str("2c7d9h"))
     tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
                       turn_bet_sizes=s, river_bet_sizes=s)
     g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
     info = g.validate_layout()
-    assert (info["tiles"] - 1) % 49 == 0 and info["tiles"] > 49
+    assert info["tiles"] == 1 + 3 * 49 + 3 * 49 * 3 * 48
     assert info["nodes"] == g.num_nodes() and info["state"] == g.num_storage()
