@@ -300,7 +300,8 @@ def test_mixed_river_and_turn_batch():
 
 
 @pytest.mark.parametrize("make,iters", [(lambda: std_game("33%, 75%, a", "2x, a"), 40),
-                                        (lambda: W.river_uniform(2).build(), 60), (turn_game, 10)])
+                                        (lambda: W.river_uniform(2).build(), 60), (turn_game, 10),
+                                        (lambda: flop_game("AA,KK", "QQ,AKs", "Qs Jh 2c", "50%, a"), 3)])
 def test_expected_values_match_oracle(make, iters):
     """query.rs:498-760 at the root: normalized weights, per-action detail of
     the acting player and expected values of both players."""
@@ -384,7 +385,8 @@ def test_raked_and_unraked_batch_solve():
 # ------------------------------------------------------ reach at nodes --
 
 @pytest.mark.parametrize("make,iters", [(lambda: std_game("33%, 75%, a", "2x, a"), 20), (turn_game, 6),
-                                        (lambda: W.river_batch(1, seed=9, iterations=30)[0].build(), 15)])
+                                        (lambda: W.river_batch(1, seed=9, iterations=30)[0].build(), 15),
+                                        (lambda: flop_game("AA,KK", "QQ,AKs", "Qs Jh 2c", "50%, a"), 3)])
 def test_reach_at_nodes_matches_oracle(make, iters):
     """batch.rs:765-797 compute_reach_at_nodes: the reach of each player at
     every node under the average strategy, against the oracle's walk."""
