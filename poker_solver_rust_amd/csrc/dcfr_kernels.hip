@@ -1530,9 +1530,12 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         const int q = uni(sh.job);
         __syncthreads();
         if (q >= A.num_subtree_tiles) break;
-        const TileDesc T = uni_load(A.tiles + uni(A.subtree_tiles[q]));
-        Ctx c = make_ctx(A, T.game, scan, rs, &sh);
-        if (MODE == MODE_DCFR && uni(A.expl_out[T.game]) <= c.G.target) continue;
+        // the job names tile and game, so their records and the game's
+        // convergence flag are fetched together
+        const int t = uni(A.subtree_tiles[2 * q]), gi = uni(A.subtree_tiles[2 * q + 1]);
+        Ctx c = make_ctx(A, gi, scan, rs, &sh);
+        const TileDesc T = uni_load(A.tiles + t);
+        if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
         if (T.flags & TILE_HAS_CHANCE) tile_traverse<MODE, SM, V, true>(c, T, p, disc);
         else tile_traverse<MODE, SM, V>(c, T, p, disc);
     }

@@ -81,7 +81,7 @@ struct KernelArgs {
     // phase launches (multi-tile games)
     const int* multi_games;   // game ids of multi-tile games
     int num_multi_games;
-    const int* subtree_tiles; // global tile ids of all subtree tiles, largest first
+    const int* subtree_tiles; // (global tile id, game) of all subtree tiles, largest first
     int num_subtree_tiles;
     const int* root_leaves;   // (game, local node) pairs: chance nodes and terminals of the root tiles
     int num_root_leaves;

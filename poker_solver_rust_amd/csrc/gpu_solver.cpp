@@ -55,7 +55,7 @@ struct HostLayout {
     std::vector<int> single_games;   // whole-game work queue, most expensive first
     std::vector<int> multi_games;    // games solved by phase launches (turn)
     std::vector<std::pair<double, int>> subtree_cost;
-    std::vector<int> subtree_tiles;  // subtree work queue, most expensive first
+    std::vector<int> subtree_tiles;  // subtree work queue as (tile, game) pairs, most expensive first
     std::vector<int> root_leaves;    // (game, local node): chance nodes and terminals of the root tiles of
                                      // multi-tile games, chance nodes first
     std::vector<int> mid_tiles;      // tiles below a root tile that hold chance nodes (a flop game's turn tiles)
@@ -497,7 +497,10 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
     std::stable_sort(L.single_games.begin(), L.single_games.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     std::stable_sort(L.subtree_cost.begin(), L.subtree_cost.end(),
                      [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first > b.first; });
-    for (auto& sc : L.subtree_cost) L.subtree_tiles.push_back(sc.second);
+    for (auto& sc : L.subtree_cost) {
+        L.subtree_tiles.push_back(sc.second);
+        L.subtree_tiles.push_back(L.tiles[sc.second].game);
+    }
     return 0;
 }
 
@@ -621,7 +624,7 @@ struct psr_batch {
         a.multi_games = d_multi;
         a.num_multi_games = (int)L.multi_games.size();
         a.subtree_tiles = d_subtrees;
-        a.num_subtree_tiles = (int)L.subtree_tiles.size();
+        a.num_subtree_tiles = (int)L.subtree_tiles.size() / 2;
         a.root_leaves = d_root_leaves;
         a.num_root_leaves = (int)L.root_leaves.size() / 2;
         a.mid_tiles = d_mid_tiles;
