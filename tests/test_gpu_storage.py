@@ -34,8 +34,19 @@ def turn_game():
     return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
 
 
-@pytest.mark.parametrize("make", [std_game, turn_game, lambda: W.river_uniform(1).build()])
+def flop_game():
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK"), P.Range.parse("QQ,AKs")), flop=P.flop_from_str("Qs Jh 2c"))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+@pytest.mark.parametrize("make", [std_game, turn_game, lambda: W.river_uniform(1).build(), flop_game])
 def test_compressed_words_match_oracle(make):
+    """Word for word against the oracle's compressed mode after 12
+    iterations; the flop game runs every phase (root, turn and river tiles)
+    through the 16-bit update."""
     g = make()
     b = P.GpuBatchSolver(0, storage="compressed")
     b.prepare_batch([g], 40)
