@@ -327,6 +327,25 @@ __device__ __forceinline__ void opponent_rows(const FwdNode<SM> (&nd)[NN], int s
     }
 }
 
+// Opponent nodes of a level on the register path, `cnt` (1..FWD_GROUP,
+// wave-uniform) at a time.
+constexpr int FWD_GROUP = 4;
+template <int MODE, int SM>
+__device__ __forceinline__ void fwd_group(const FwdNode<SM> (&fn)[FWD_GROUP], int cnt, int stride, int Ho) {
+    if (cnt == 4) {
+        opponent_rows<MODE, SM, 4>(fn, stride, Ho);
+    } else if (cnt == 3) {
+        const FwdNode<SM> f[3] = {fn[0], fn[1], fn[2]};
+        opponent_rows<MODE, SM, 3>(f, stride, Ho);
+    } else if (cnt == 2) {
+        const FwdNode<SM> f[2] = {fn[0], fn[1]};
+        opponent_rows<MODE, SM, 2>(f, stride, Ho);
+    } else {
+        const FwdNode<SM> f[1] = {fn[0]};
+        opponent_rows<MODE, SM, 1>(f, stride, Ho);
+    }
+}
+
 // Forward pass over one tile: reach of the opponent of p, level by level.
 // The game-root tile seeds its root row with the opponent's initial weights;
 // a subtree tile finds its root row already written by the chance node above.
@@ -359,29 +378,34 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
             V* rc = reach + (size_t)nd.first_child * c.stride;
             if constexpr (sizeof(V) == 4) {
                 if (nd.kind == KIND_DECISION && na <= NA_REG) {
-                    // the opponent's node on the register path, paired with the
-                    // next such node of the level when there is one
-                    FwdNode<SM> fn[2] = {{node_state<SM>(A, c.G, nd), rn, rc, na, __fdiv_rn(1.0f, (float)na)},
-                                         {}};
-                    int n2 = n + 1;
-                    NodeRec nd2 = nd;
-                    if (PAIR_NODES) {
-                        for (; n2 < lr.end; ++n2) {
-                            nd2 = uni_load(c.nodes + n2);
-                            if (nd2.kind == KIND_CHANCE || (nd2.kind == KIND_DECISION && nd2.player == o && nd2.num_children > 1))
-                                break;
+                    // the opponent's node on the register path, grouped with up
+                    // to FWD_GROUP - 1 further such nodes of the level (all
+                    // their rows in flight together); the group stops at a
+                    // chance node or a node off the register path, which the
+                    // loop then takes on its own
+                    FwdNode<SM> fn[FWD_GROUP];
+                    fn[0] = {node_state<SM>(A, c.G, nd), rn, rc, na, __fdiv_rn(1.0f, (float)na)};
+                    int cnt = 1, m = n + 1;
+#pragma unroll
+                    for (int slot = 1; slot < FWD_GROUP; ++slot) {   // compile-time slots: fn stays in registers
+                        bool found = false;
+                        for (; PAIR_NODES && m < lr.end; ++m) {
+                            const NodeRec nd2 = uni_load(c.nodes + m);
+                            const bool opp = nd2.kind == KIND_DECISION && nd2.player == o && nd2.num_children > 1;
+                            if (!(nd2.kind == KIND_CHANCE || opp)) continue;   // passes the reach through
+                            if (opp && nd2.num_children <= NA_REG) {
+                                fn[slot] = {node_state<SM>(A, c.G, nd2), reach + (size_t)reach_src(c, m, p) * c.stride,
+                                            reach + (size_t)nd2.first_child * c.stride, (int)nd2.num_children,
+                                            __fdiv_rn(1.0f, (float)nd2.num_children)};
+                                found = true;
+                            }
+                            break;
                         }
+                        if (!found) break;
+                        cnt = slot + 1;
+                        n = m++;
                     }
-                    if (PAIR_NODES && n2 < lr.end && nd2.kind == KIND_DECISION && nd2.num_children <= NA_REG) {
-                        fn[1] = {node_state<SM>(A, c.G, nd2), reach + (size_t)reach_src(c, n2, p) * c.stride,
-                                 reach + (size_t)nd2.first_child * c.stride, (int)nd2.num_children,
-                                 __fdiv_rn(1.0f, (float)nd2.num_children)};
-                        opponent_rows<MODE, SM, 2>(fn, c.stride, Ho);
-                        n = n2;
-                    } else {
-                        const FwdNode<SM> f1[1] = {fn[0]};
-                        opponent_rows<MODE, SM, 1>(f1, c.stride, Ho);
-                    }
+                    fwd_group<MODE, SM>(fn, cnt, c.stride, Ho);
                     continue;
                 }
             }
