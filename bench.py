@@ -211,7 +211,7 @@ def net_solve_bench(args):
 def reduced_precision_bench(args):
     """Reduced-precision turn solve: the seeded turn subgame solved on the device with FP32,
     FP16 and BF16 storage (16-bit words with a per-node FP32 scale, FP32
-    arithmetic) for `--c4-iters` DCFR iterations; the exploitability of each
+    arithmetic) for `--half-storage-iters` DCFR iterations; the exploitability of each
     by the device FP64 best response, and the per-hand root cfvalues against
     the FP64 CPU oracle's solve of the identical tree (same iterations)."""
     import poker_solver_rust_amd as P
@@ -222,7 +222,7 @@ def reduced_precision_bench(args):
 
     s = W.turn_uniform01(args.seed)
     g = s.build()
-    iters = args.c4_iters
+    iters = args.half_storage_iters
     t = time.time()
     o = O.OracleGame(g.serialize(), 64)
     for k in range(iters):
@@ -431,12 +431,14 @@ def main():
     ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
     ap.add_argument("--net-epochs", type=int, default=40)
     ap.add_argument("--turn-solve-iters", type=int, default=500)
-    ap.add_argument("--c4", action="store_true", help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")
-    ap.add_argument("--c4-iters", type=int, default=500)
+    ap.add_argument("--half-storage", action="store_true",
+                    help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")
+    ap.add_argument("--half-storage-iters", type=int, default=500)
     ap.add_argument("--mlp", action="store_true", help="value-net MFMA throughput instead")
     ap.add_argument("--flop", action="store_true", help="flop-root subgame DCFR iterations/s instead")
     ap.add_argument("--flop-iters", type=int, default=50)
-    ap.add_argument("--c1", action="store_true", help="single river subgame, 1000 iterations, device and CPU oracle instead")
+    ap.add_argument("--river-single", action="store_true",
+                    help="single river subgame, 1000 iterations, device and CPU oracle instead")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run one plain device copy of this many bytes (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -444,10 +446,10 @@ def main():
     if args.copy_bytes > 0:
         copy_probe(args.copy_bytes)
         return
-    if args.c4:
+    if args.half_storage:
         reduced_precision_bench(args)
         return
-    if args.c1:
+    if args.river_single:
         river_uniform_bench(args)
         return
     if args.mlp:
