@@ -82,6 +82,14 @@ int psr_action_tree_actions(const psr_action_tree* tree, const psr_action* histo
                             psr_action* out, int cap);
 /* action_tree.rs:1171-1195 count_num_action_nodes: nodes per street. */
 int psr_action_tree_count_nodes(const psr_action_tree* tree, uint64_t out[3]);
+/* The node a history reaches: info[0] = is_terminal_node, info[1] =
+ * is_chance_node, info[2..3] = total_bet_amount [OOP, IP]
+ * (action_tree.rs:438-449, 523-527).  < 0 if an action is not available. */
+int psr_action_tree_node_info(const psr_action_tree* tree, const psr_action* history, int n_history,
+                              int32_t info[4]);
+/* action_tree.rs:356-362 invalid_terminals: the number of non-terminal
+ * nodes without children. */
+int psr_action_tree_invalid_terminals(const psr_action_tree* tree);
 
 /* ---------------------------------------------------------------- game */
 
@@ -98,6 +106,13 @@ typedef struct psr_game psr_game;
 /* game/interpreter.rs:128-166 PostFlopGame::with_config (+ allocate_memory:
  * storage is device-side and allocated by a batch). */
 int psr_game_build(const psr_card_config* cards, const psr_action_tree* tree, psr_game** out);
+/* psr_game_build with flags.  PSR_GAME_NO_ISOMORPHISM deals every card at
+ * every chance node instead of skipping suit-isomorphic ones
+ * (isomorphism.rs:33-237 is then not applied): no reference counterpart --
+ * the reference always applies the isomorphism; a game built without it has
+ * the same values, which is how the isomorphic chance sums are checked. */
+enum { PSR_GAME_NO_ISOMORPHISM = 1 };
+int psr_game_build_ex(const psr_card_config* cards, const psr_action_tree* tree, uint32_t flags, psr_game** out);
 void psr_game_free(psr_game* game);
 /* interface.rs num_private_hands / private_cards / initial_weights. */
 int psr_game_num_hands(const psr_game* game, int player);

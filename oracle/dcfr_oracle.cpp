@@ -193,9 +193,59 @@ bool load(const uint8_t* blob, size_t len, Game& g) {
     return true;
 }
 
-// ------------------------------------------------------------- scalar ops --
+// ------------------------------------------------------------- slice ops --
 // sliceop.rs:5-11: max without NaN propagation
 template <class T> inline T max_(T x, T y) { return x > y ? x : y; }
+
+// sliceop.rs:24-39 sub_slice / mul_slice / div_slice (element-wise over n)
+template <class T> void sub_slice(T* lhs, const T* rhs, size_t n) {
+    for (size_t i = 0; i < n; ++i) lhs[i] -= rhs[i];
+}
+template <class T> void mul_slice(T* lhs, const T* rhs, size_t n) {
+    for (size_t i = 0; i < n; ++i) lhs[i] *= rhs[i];
+}
+template <class T> void div_slice(T* lhs, const T* rhs, size_t n, T def) {
+    for (size_t i = 0; i < n; ++i) lhs[i] = rhs[i] == T(0) ? def : lhs[i] / rhs[i];
+}
+// sliceop.rs:62-78, 100-124, 127-142: the rows of src (total values, rows of
+// len) reduced into dst in row order: sum, sum of products with src2, max
+template <class T> void sum_slices(T* dst, const T* src, size_t len, size_t total) {
+    for (size_t i = 0; i < len; ++i) dst[i] = src[i];
+    for (size_t r = len; r < total; r += len)
+        for (size_t i = 0; i < len; ++i) dst[i] += src[r + i];
+}
+template <class T> void fma_slices(T* dst, const T* src1, const T* src2, size_t len, size_t total) {
+    for (size_t i = 0; i < len; ++i) dst[i] = src1[i] * src2[i];
+    for (size_t r = len; r < total; r += len)
+        for (size_t i = 0; i < len; ++i) dst[i] += src1[r + i] * src2[r + i];
+}
+template <class T> void max_slices(T* dst, const T* src, size_t len, size_t total) {
+    for (size_t i = 0; i < len; ++i) dst[i] = src[i];
+    for (size_t r = len; r < total; r += len)
+        for (size_t i = 0; i < len; ++i) dst[i] = max_(dst[i], src[r + i]);
+}
+
+// solver.rs:600-626 regret_matching_into: max(R, 0) normalised per hand over
+// the A actions (rows of n / A hands), uniform 1/A where the sum is 0
+template <class T> void regret_matching(const T* regret, size_t n, int A, std::vector<T>& s) {
+    s.resize(n);
+    for (size_t k = 0; k < n; ++k) s[k] = max_<T>(regret[k], T(0));
+    const size_t H = n / (size_t)A;
+    std::vector<T> denom(H);
+    sum_slices(denom.data(), s.data(), H, n);
+    const T def = T(1) / (T)A;
+    for (int a = 0; a < A; ++a) div_slice(&s[(size_t)a * H], denom.data(), H, def);
+}
+// solver.rs:631-652 regret_matching_compressed_into: the same on i16 regrets
+template <class T> void regret_matching_compressed(const int16_t* regret, size_t n, int A, std::vector<T>& s) {
+    s.resize(n);
+    for (size_t k = 0; k < n; ++k) s[k] = (T)(float)std::max<int16_t>(regret[k], 0);
+    const size_t H = n / (size_t)A;
+    std::vector<T> denom(H);
+    sum_slices(denom.data(), s.data(), H, n);
+    const T def = T(1) / (T)A;
+    for (int a = 0; a < A; ++a) div_slice(&s[(size_t)a * H], denom.data(), H, def);
+}
 
 struct DiscountParams {
     float alpha, beta, gamma;
@@ -245,6 +295,13 @@ struct Solver {
     // compressed storage (solver.rs:486-553): i16 regrets, u16 strategy sums,
     // one f32 scale each per decision node
     bool compressed = false;
+    // > 1: the children of every node before the river deal (river card
+    // not dealt) run as parallel tasks on this many threads, as the
+    // reference's for_each_child does with rayon (utility.rs:103-115,
+    // game/node.rs:347-349).  Each child writes its own rows and the parent
+    // combines them in child order afterwards, so results do not depend on
+    // the thread count.
+    int threads = 1;
     std::vector<int16_t> regret16;
     std::vector<uint16_t> strategy16;
     std::vector<float> rscale, sscale;
@@ -260,16 +317,34 @@ struct Solver {
         }
     }
 
+    template <class F>
+    void for_each_child(const Node& node, int A, F&& f) const {
+        if (threads > 1 && node.river == 0xFF && A > 1) {
+            for (int a = 0; a < A; ++a) {
+#pragma omp task firstprivate(a) shared(f)
+                f(a);
+            }
+#pragma omp taskwait
+        } else {
+            for (int a = 0; a < A; ++a) f(a);
+        }
+    }
+    // runs f (a traversal from the root) inside a team of `threads` threads
+    template <class F>
+    void at_root(F&& f) const {
+        if (threads > 1) {
+#pragma omp parallel num_threads(threads)
+#pragma omp single
+            f();
+        } else {
+            f();
+        }
+    }
+
     // regret matching from the node's storage (solver.rs:600-654)
     void node_regret_match(const Node& nd, int A, int H, std::vector<T>& s) const {
-        if (!compressed) {
-            regret_match(&regret[nd.storage], A, H, s);
-            return;
-        }
-        s.resize((size_t)A * H);
-        const int16_t* r = &regret16[nd.storage];
-        for (int k = 0; k < A * H; ++k) s[k] = (T)(float)std::max<int16_t>(r[k], 0);
-        normalize_rows(s, A, H);
+        if (!compressed) regret_matching(&regret[nd.storage], (size_t)A * H, A, s);
+        else regret_matching_compressed(&regret16[nd.storage], (size_t)A * H, A, s);
     }
     // normalised average strategy from storage (utility.rs:229-327)
     void node_average(const Node& nd, int A, int H, std::vector<T>& s) const {
@@ -283,15 +358,10 @@ struct Solver {
         normalize_rows(s, A, H);
     }
     void normalize_rows(std::vector<T>& s, int A, int H) const {
-        std::vector<T> denom(s.begin(), s.begin() + H);
-        for (int a = 1; a < A; ++a)
-            for (int h = 0; h < H; ++h) denom[h] += s[(size_t)a * H + h];
-        T def = T(1) / (T)A;
-        for (int a = 0; a < A; ++a)
-            for (int h = 0; h < H; ++h) {
-                T& x = s[(size_t)a * H + h];
-                x = denom[h] == T(0) ? def : x / denom[h];
-            }
+        std::vector<T> denom(H);
+        sum_slices(denom.data(), s.data(), (size_t)H, (size_t)A * H);
+        const T def = T(1) / (T)A;
+        for (int a = 0; a < A; ++a) div_slice(&s[(size_t)a * H], denom.data(), (size_t)H, def);
     }
 
     // ------------------------------------------------ terminal evaluation --
@@ -471,30 +541,10 @@ struct Solver {
     }
 
     // ------------------------------------------------------------ helpers --
-    // solver.rs:600-626
-    void regret_match(const T* R, int A, int H, std::vector<T>& s) const {
-        s.resize((size_t)A * H);
-        for (int k = 0; k < A * H; ++k) s[k] = max_<T>(R[k], T(0));
-        std::vector<T> denom(s.begin(), s.begin() + H);
-        for (int a = 1; a < A; ++a)
-            for (int h = 0; h < H; ++h) denom[h] += s[(size_t)a * H + h];
-        T def = T(1) / (T)A;
-        for (int a = 0; a < A; ++a)
-            for (int h = 0; h < H; ++h) {
-                T& x = s[(size_t)a * H + h];
-                x = denom[h] == T(0) ? def : x / denom[h];
-            }
-    }
-    // utility.rs:229-245 (normalized_strategy)
+    // utility.rs:229-245 (normalized_strategy): S / sum_a S, uniform where 0
     void normalized(const T* S, int A, int H, std::vector<T>& s) const {
-        s.resize((size_t)A * H);
-        std::vector<T> denom(S, S + H);
-        for (int a = 1; a < A; ++a)
-            for (int h = 0; h < H; ++h) denom[h] += S[(size_t)a * H + h];
-        T def = T(1) / (T)A;
-        for (int a = 0; a < A; ++a)
-            for (int h = 0; h < H; ++h)
-                s[(size_t)a * H + h] = denom[h] == T(0) ? def : S[(size_t)a * H + h] / denom[h];
+        s.assign(S, S + (size_t)A * H);
+        normalize_rows(s, A, H);
     }
     void chance_sum(T* result, const Node& node, int player, std::vector<T>& cfv, int H) const {
         int A = (int)node.num_children;
@@ -530,8 +580,8 @@ struct Solver {
         const float adec = dp.alpha * rscale[node.dec] / 32767.0f;
         const float bdec = dp.beta * rscale[node.dec] / 32767.0f;
         for (size_t k = 0; k < n; ++k) cp[k] += (float)R[k] * (R[k] >= 0 ? adec : bdec);
-        for (int a = 0; a < A; ++a)
-            for (int h = 0; h < H; ++h) cp[(size_t)a * H + h] -= (float)result[h];
+        std::vector<float> res(result, result + H);
+        for (int a = 0; a < A; ++a) sub_slice(&cp[(size_t)a * H], res.data(), (size_t)H);
         rscale[node.dec] = encode_signed(R, cp, n);
     }
 
@@ -549,15 +599,13 @@ struct Solver {
             std::vector<T> upd(Ho);
             T sc = chance_scale(node);
             for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
-            for (int a = 0; a < A; ++a) solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data(), dp);
+            for_each_child(node, A, [&](int a) { solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data(), dp); });
             chance_sum(result, node, player, cfv, H);
         } else if (node.player == player) {
-            for (int a = 0; a < A; ++a) solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach, dp);
+            for_each_child(node, A, [&](int a) { solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach, dp); });
             std::vector<T> sig;
             node_regret_match(node, A, H, sig);
-            for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
+            fma_slices(result, sig.data(), cfv.data(), (size_t)H, (size_t)A * H);
             if (compressed) {
                 update_compressed(node, A, H, sig, cfv, result, dp);
                 return;
@@ -570,19 +618,16 @@ struct Solver {
                 T coef = std::signbit(R[k]) ? beta : alpha;
                 R[k] = R[k] * coef + cfv[k];
             }
-            for (int a = 0; a < A; ++a)
-                for (int h = 0; h < H; ++h) R[(size_t)a * H + h] -= result[h];
+            for (int a = 0; a < A; ++a) sub_slice(&R[(size_t)a * H], result, (size_t)H);
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
             node_regret_match(node, A, Ho, sig);
-            for (int a = 0; a < A; ++a)
-                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
-            for (int a = 0; a < A; ++a)
+            for (int a = 0; a < A; ++a) mul_slice(&sig[(size_t)a * Ho], cfreach, (size_t)Ho);
+            for_each_child(node, A, [&](int a) {
                 solve_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho], dp);
-            for (int h = 0; h < H; ++h) result[h] = cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+            });
+            sum_slices(result, cfv.data(), (size_t)H, (size_t)A * H);
         }
     }
 
@@ -600,25 +645,21 @@ struct Solver {
             std::vector<T> upd(Ho);
             T sc = chance_scale(node);
             for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
-            for (int a = 0; a < A; ++a) best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data());
+            for_each_child(node, A, [&](int a) { best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data()); });
             std::vector<T> tmp(cfv);
             chance_sum(result, node, player, tmp, H);
         } else if (node.player == player) {
-            for (int a = 0; a < A; ++a) best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach);
-            for (int h = 0; h < H; ++h) result[h] = cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] = max_<T>(result[h], cfv[(size_t)a * H + h]);
+            for_each_child(node, A, [&](int a) { best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach); });
+            max_slices(result, cfv.data(), (size_t)H, (size_t)A * H);
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
             node_average(node, A, Ho, sig);
-            for (int a = 0; a < A; ++a)
-                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
-            for (int a = 0; a < A; ++a)
+            for (int a = 0; a < A; ++a) mul_slice(&sig[(size_t)a * Ho], cfreach, (size_t)Ho);
+            for_each_child(node, A, [&](int a) {
                 best_cfv_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho]);
-            for (int h = 0; h < H; ++h) result[h] = cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+            });
+            sum_slices(result, cfv.data(), (size_t)H, (size_t)A * H);
         }
     }
 
@@ -639,26 +680,22 @@ struct Solver {
             std::vector<T> upd(Ho);
             T sc = chance_scale(node);
             for (int i = 0; i < Ho; ++i) upd[i] = cfreach[i] * sc;
-            for (int a = 0; a < A; ++a) cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data());
+            for_each_child(node, A, [&](int a) { cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, upd.data()); });
             chance_sum(result, node, player, cfv, H);
         } else if (node.player == player) {
-            for (int a = 0; a < A; ++a) cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach);
+            for_each_child(node, A, [&](int a) { cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, cfreach); });
             std::vector<T> sig;
             node_average(node, A, H, sig);
-            for (int h = 0; h < H; ++h) result[h] = sig[h] * cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] += sig[(size_t)a * H + h] * cfv[(size_t)a * H + h];
+            fma_slices(result, sig.data(), cfv.data(), (size_t)H, (size_t)A * H);
         } else {
             const int Ho = g.H(player ^ 1);
             std::vector<T> sig;
             node_average(node, A, Ho, sig);
-            for (int a = 0; a < A; ++a)
-                for (int i = 0; i < Ho; ++i) sig[(size_t)a * Ho + i] *= cfreach[i];
-            for (int a = 0; a < A; ++a)
+            for (int a = 0; a < A; ++a) mul_slice(&sig[(size_t)a * Ho], cfreach, (size_t)Ho);
+            for_each_child(node, A, [&](int a) {
                 cfvalue_rec(&cfv[(size_t)a * H], node.first_child + a, player, &sig[(size_t)a * Ho]);
-            for (int h = 0; h < H; ++h) result[h] = cfv[h];
-            for (int a = 1; a < A; ++a)
-                for (int h = 0; h < H; ++h) result[h] += cfv[(size_t)a * H + h];
+            });
+            sum_slices(result, cfv.data(), (size_t)H, (size_t)A * H);
         }
     }
 
@@ -674,14 +711,14 @@ struct Solver {
     void mes_ev(T out[2]) const {
         for (int p = 0; p < 2; ++p) {
             std::vector<T> cfv(g.H(p)), reach = weights_T(p ^ 1);
-            best_cfv_rec(cfv.data(), 0, p, reach.data());
+            at_root([&] { best_cfv_rec(cfv.data(), 0, p, reach.data()); });
             out[p] = weighted_sum(cfv, p);
         }
     }
     void current_ev(T out[2]) const {
         for (int p = 0; p < 2; ++p) {
             std::vector<T> cfv(g.H(p)), reach = weights_T(p ^ 1);
-            cfvalue_rec(cfv.data(), 0, p, reach.data());
+            at_root([&] { cfvalue_rec(cfv.data(), 0, p, reach.data()); });
             out[p] = weighted_sum(cfv, p);
         }
     }
@@ -699,7 +736,7 @@ struct Solver {
         DiscountParams dp = discount(t);
         for (int p = 0; p < 2; ++p) {
             std::vector<T> res(g.H(p)), reach = weights_T(p ^ 1);
-            solve_rec(res.data(), 0, p, reach.data(), dp);
+            at_root([&] { solve_rec(res.data(), 0, p, reach.data(), dp); });
         }
     }
 
@@ -719,7 +756,7 @@ struct Solver {
 
     void root_cfvalues(int p, T* out) const {
         std::vector<T> reach = weights_T(p ^ 1);
-        cfvalue_rec(out, 0, p, reach.data());
+        at_root([&] { cfvalue_rec(out, 0, p, reach.data()); });
     }
 
     // Reach of player q at node `target` under the average strategy
@@ -922,6 +959,10 @@ void oracle_discount(uint32_t t, float out[3]) {
 
 void oracle_solve_step(void* h, uint32_t t) { DISPATCH(h, S.step(t)); }
 
+// threads > 1: children of the nodes before the river deal in parallel
+// (the reference's rayon policy); 1 = sequential
+void oracle_set_threads(void* h, int threads) { DISPATCH(h, S.threads = std::max(threads, 1)); }
+
 double oracle_exploitability(void* h) {
     double r = 0;
     DISPATCH(h, r = (double)S.exploitability());
@@ -1081,7 +1122,7 @@ int oracle_num_boundaries(void* hv) { return (int)static_cast<Handle*>(hv)->game
 // solve per thread at a time (the reference runs river subgames single-
 // threaded, node.rs:347-349).  Returns wall seconds; writes exploitability.
 double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n, uint32_t iters, int threads,
-                         const double* targets, double* expl_out, uint32_t* iters_out) {
+                         const double* targets, double* expl_out, uint32_t* iters_out, int finalize) {
     std::atomic<int> next{0};
     auto t0 = std::chrono::steady_clock::now();
     auto work = [&]() {
@@ -1094,12 +1135,92 @@ double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n,
             uint32_t run = 0;
             expl_out[i] = (double)s.solve(iters, targets ? (float)targets[i] : 0.0f, &run);
             if (iters_out) iters_out[i] = run;
+            if (finalize) {   // solver.rs:171 finalize: the root cfvalues of both players
+                std::vector<float> cf0(g.H(0)), cf1(g.H(1));
+                s.root_cfvalues(0, cf0.data());
+                s.root_cfvalues(1, cf1.data());
+            }
         }
     };
     std::vector<std::thread> pool;
     for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(work);
     for (auto& th : pool) th.join();
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// FP64 solves of n games with a given number of DCFR iterations each (no
+// early stop), `threads` games at a time: the FP64 oracle's exploitability
+// after exactly the iterations a device solve ran.
+double oracle_fp64_expl_many(const uint8_t* const* blobs, const size_t* lens, int n, const uint32_t* iters,
+                             int threads, double* expl_out) {
+    std::atomic<int> next{0};
+    auto t0 = std::chrono::steady_clock::now();
+    auto work = [&]() {
+        for (;;) {
+            int i = next.fetch_add(1);
+            if (i >= n) break;
+            Game g;
+            if (!load(blobs[i], lens[i], g)) { expl_out[i] = NAN; continue; }
+            Solver<double> s(g);
+            for (uint32_t t = 0; t < iters[i]; ++t) s.step(t);
+            expl_out[i] = s.exploitability();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < std::max(threads, 1); ++t) pool.emplace_back(work);
+    for (auto& th : pool) th.join();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// solver.rs:600-652 regret matching of n regrets (A rows of n / A hands)
+void oracle_regret_matching(const float* regret, size_t n, int A, float* out) {
+    std::vector<float> s;
+    regret_matching(regret, n, A, s);
+    std::copy(s.begin(), s.end(), out);
+}
+void oracle_regret_matching_compressed(const int16_t* regret, size_t n, int A, float* out) {
+    std::vector<float> s;
+    regret_matching_compressed<float>(regret, n, A, s);
+    std::copy(s.begin(), s.end(), out);
+}
+
+// The slice operations (sliceop.rs) the oracle's recursion is written with:
+// op 0 sub_slice, 1 mul_slice, 2 div_slice (default def), each on len values
+// of lhs; 3 sum_slices, 4 fma_slices (with src2), 5 max_slices of total
+// values in rows of len into lhs; 6 max (lhs[0] = max(lhs[0], rhs[0])).
+int oracle_slice_op(int op, float* lhs, const float* rhs, const float* src2, size_t len, size_t total, float def) {
+    switch (op) {
+        case 0: sub_slice(lhs, rhs, len); return 0;
+        case 1: mul_slice(lhs, rhs, len); return 0;
+        case 2: div_slice(lhs, rhs, len, def); return 0;
+        case 3: sum_slices(lhs, rhs, len, total); return 0;
+        case 4: fma_slices(lhs, rhs, src2, len, total); return 0;
+        case 5: max_slices(lhs, rhs, len, total); return 0;
+        case 6: lhs[0] = max_(lhs[0], rhs[0]); return 0;
+        default: return -1;
+    }
+}
+
+// Test hook: plants a mistake in the isomorphic chance sums of a loaded game
+// so that a test can show its check catches it.  mode 1: the isomorphic
+// cards' rows are added without their hand swaps; mode 2: the isomorphic
+// cards are left out of the sums.  Returns the number of entries changed.
+int oracle_debug_break_isomorphism(void* hv, int mode) {
+    Handle* h = static_cast<Handle*>(hv);
+    int changed = 0;
+    for (auto& v : h->game.iso) {
+        if (mode == 1) {
+            for (Iso& is : v) {
+                changed += !is.swap[0].empty() || !is.swap[1].empty();
+                is.swap[0].clear();
+                is.swap[1].clear();
+            }
+        } else if (mode == 2) {
+            changed += (int)v.size();
+            v.clear();
+        }
+    }
+    return changed;
 }
 
 }  // extern "C"

@@ -10,6 +10,14 @@ the device path is unavailable).
 """
 from .range_solver import (  # noqa: F401
     NOT_DEALT,
+    PLAYER_CHANCE,
+    PLAYER_CHANCE_FLAG,
+    PLAYER_DEPTH_BOUNDARY_FLAG,
+    PLAYER_FOLD_FLAG,
+    PLAYER_IP,
+    PLAYER_MASK,
+    PLAYER_OOP,
+    PLAYER_TERMINAL_FLAG,
     Action,
     ActionTree,
     BoardState,
@@ -21,7 +29,9 @@ from .range_solver import (  # noqa: F401
     PsrError,
     Range,
     TreeConfig,
+    action_to_string,
     card_from_str,
+    card_to_string,
     card_pair_to_index,
     flop_from_str,
     gpu_solve_hand_parallel,

@@ -128,8 +128,33 @@ int psr_action_tree_count_nodes(const psr_action_tree* t, uint64_t out[3]) {
     return 0;
 }
 
+int psr_action_tree_node_info(const psr_action_tree* t, const psr_action* hist, int n, int32_t info[4]) {
+    if (!t || !info || n < 0 || (n > 0 && !hist)) return fail("null argument");
+    std::vector<Action> h;
+    for (int i = 0; i < n; ++i) h.push_back({(ActKind)hist[i].kind, hist[i].amount});
+    bool terminal = false, chance = false;
+    int32_t bet[2] = {0, 0};
+    std::string err;
+    if (!t->tree.node_info(h, terminal, chance, bet, err)) return fail(err);
+    info[0] = terminal;
+    info[1] = chance;
+    info[2] = bet[0];
+    info[3] = bet[1];
+    return 0;
+}
+
+int psr_action_tree_invalid_terminals(const psr_action_tree* t) {
+    if (!t) return fail("null tree");
+    return (int)t->tree.invalid_terminals().size();
+}
+
 int psr_game_build(const psr_card_config* c, const psr_action_tree* t, psr_game** out) {
+    return psr_game_build_ex(c, t, 0u, out);
+}
+
+int psr_game_build_ex(const psr_card_config* c, const psr_action_tree* t, uint32_t flags, psr_game** out) {
     if (!c || !t || !out || !c->range[0] || !c->range[1]) return fail("null argument");
+    if (flags & ~(uint32_t)PSR_GAME_NO_ISOMORPHISM) return fail("unknown game build flag");
     CardConfig cc;
     std::string err;
     for (int p = 0; p < 2; ++p)
@@ -138,7 +163,7 @@ int psr_game_build(const psr_card_config* c, const psr_action_tree* t, psr_game*
     cc.turn = c->turn;
     cc.river = c->river;
     auto g = new psr_game;
-    if (!PostFlopGame::with_config(cc, t->tree, g->game, err)) {
+    if (!PostFlopGame::with_config(cc, t->tree, g->game, err, !(flags & PSR_GAME_NO_ISOMORPHISM))) {
         delete g;
         return fail(err);
     }

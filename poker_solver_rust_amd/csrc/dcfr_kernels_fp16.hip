@@ -1,0 +1,6 @@
+// DCFR kernels of the SM_FP16 storage mode (device code in dcfr_kernels.hpp).
+#include "dcfr_kernels.hpp"
+
+namespace psr {
+PSR_INSTANTIATE_STORAGE(SM_FP16)
+}  // namespace psr

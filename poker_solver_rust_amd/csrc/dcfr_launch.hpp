@@ -104,6 +104,15 @@ struct KernelArgs {
 };
 
 size_t dcfr_lds_bytes(int max_hands);
+// Per storage mode (instantiated by dcfr_kernels_<mode>.hip).
+template <int SM>
+hipError_t dcfr_launch_sm(const KernelArgs& args, int grid, hipStream_t stream);
+template <int SM>
+hipError_t dcfr_launch_phase_sm(const KernelArgs& args, int grid, hipStream_t stream);
+template <int SM>
+hipError_t dcfr_occupancy_sm(int max_hands, int* blocks_per_cu);
+// Zeroes the work-queue counter on the stream.
+hipError_t reset_counter(int* counter, hipStream_t stream);
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_occupancy(int max_hands, int storage, int* blocks_per_cu);

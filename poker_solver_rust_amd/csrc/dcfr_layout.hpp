@@ -65,6 +65,7 @@ struct HandScan {
 // most SCAN_RUN consecutive list entries: thread t owns entries
 // [t * C, t * C + C) with C = ceil(scan_len / SCAN_THREADS).
 constexpr int DCFR_BLOCK = 512;   // threads of a DCFR workgroup (one game or tile at a time)
+constexpr int DCFR_MAX_ACTIONS = 16;   // largest action count of a decision node on the device
 constexpr int SCAN_THREADS = DCFR_BLOCK;
 // (3 * 1326 + 53) / SCAN_THREADS rounded up to a multiple of 8 (16-byte loads)
 constexpr int SCAN_RUN = ((3 * 1326 + 53 + SCAN_THREADS - 1) / SCAN_THREADS + 7) / 8 * 8;

@@ -841,6 +841,29 @@ std::vector<std::vector<Action>> ActionTree::invalid_terminals() const {
     return out;
 }
 
+bool ActionTree::node_info(const std::vector<Action>& history, bool& terminal, bool& chance, int32_t total_bet[2],
+                           std::string& err) const {
+    // stacks along the history as the tree build tracks them (BetState);
+    // a chance node is stepped over without an action, as
+    // total_bet_amount_recursive does
+    BetState info;
+    info.stack[0] = info.stack[1] = cfg_.effective_stack;
+    const ActionTreeNode* n = root_.get();
+    for (const Action& a : history) {
+        while (n->is_chance() && !n->children.empty()) n = n->children[0].get();
+        auto it = std::find(n->actions.begin(), n->actions.end(), a);
+        if (it == n->actions.end()) { err = "Action `" + a.to_string() + "` is not available"; return false; }
+        info = info.next(n->player, a);
+        n = n->children[it - n->actions.begin()].get();
+    }
+    const ActionTreeNode* s = n;
+    while (s->is_chance() && !s->children.empty()) s = s->children[0].get();
+    terminal = s->is_terminal();
+    chance = n->is_chance() && !terminal;
+    for (int p = 0; p < 2; ++p) total_bet[p] = cfg_.effective_stack - info.stack[p];
+    return true;
+}
+
 void ActionTree::count_nodes(uint64_t out[3]) const {
     out[0] = out[1] = out[2] = 0;
     count_rec(*root_, (int)cfg_.initial_state, out);
@@ -849,9 +872,10 @@ void ActionTree::count_nodes(uint64_t out[3]) const {
 // ====================================================================== game
 
 bool PostFlopGame::with_config(const CardConfig& cards, const ActionTree& tree, PostFlopGame& out,
-                               std::string& err) {
+                               std::string& err, bool isomorphism) {
     if (!tree.invalid_terminals().empty()) { err = "Invalid terminal found in action tree"; return false; }
     PostFlopGame g;
+    g.isomorphism_ = isomorphism;
     g.ccfg_ = cards;
     g.tcfg_ = tree.config();
     if (!g.check_card_config(err)) return false;
@@ -958,7 +982,7 @@ void PostFlopGame::compute_isomorphism() {
     int next_class = 1;
     for (int s2 = 1; s2 < 4; ++s2) {
         bool found = false;
-        for (int s1 = 0; s1 < s2 && !found; ++s1) {
+        for (int s1 = 0; s1 < s2 && !found && isomorphism_; ++s1) {
             if (ccfg_.range[0].is_suit_isomorphic(s1, s2) && ccfg_.range[1].is_suit_isomorphic(s1, s2)) {
                 suit_class[s2] = suit_class[s1];
                 found = true;

@@ -131,6 +131,11 @@ public:
     std::vector<std::vector<Action>> invalid_terminals() const;
     // [decision/terminal/chance count per street] action_tree.rs:1171-1195
     void count_nodes(uint64_t out[3]) const;
+    // The node a history reaches (action_tree.rs:438-449, 523-527,
+    // 1142-1166): whether it is terminal (after leading chance nodes), whether
+    // it is a chance node, and each player's total bet along the history.
+    bool node_info(const std::vector<Action>& history, bool& terminal, bool& chance, int32_t total_bet[2],
+                   std::string& err) const;
 
 private:
     TreeConfig cfg_;
@@ -186,8 +191,10 @@ struct IsoChance {
 class PostFlopGame {
 public:
     // game/interpreter.rs:128-166
+    // isomorphism = false deals every card at every chance node (no suit
+    // isomorphism); the reference always applies it.
     static bool with_config(const CardConfig& cards, const ActionTree& tree, PostFlopGame& out,
-                            std::string& err);
+                            std::string& err, bool isomorphism = true);
 
     const TreeConfig& tree_config() const { return tcfg_; }
     const CardConfig& card_config() const { return ccfg_; }
@@ -228,6 +235,7 @@ private:
 
     CardConfig ccfg_;
     TreeConfig tcfg_;
+    bool isomorphism_ = true;
     std::vector<std::pair<Card, Card>> hands_[2];
     std::vector<float> weights_[2];
     std::vector<uint16_t> same_hand_[2];

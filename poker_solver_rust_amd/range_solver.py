@@ -98,7 +98,10 @@ def load_library(path: Optional[str] = None):
         "psr_action_tree_free": (None, [vp]),
         "psr_action_tree_actions": (C.c_int, [vp, C.POINTER(_ActionC), C.c_int, C.POINTER(_ActionC), C.c_int]),
         "psr_action_tree_count_nodes": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+        "psr_action_tree_node_info": (C.c_int, [vp, C.POINTER(_ActionC), C.c_int, C.POINTER(C.c_int32)]),
+        "psr_action_tree_invalid_terminals": (C.c_int, [vp]),
         "psr_game_build": (C.c_int, [C.POINTER(_CardConfigC), vp, C.POINTER(vp)]),
+        "psr_game_build_ex": (C.c_int, [C.POINTER(_CardConfigC), vp, C.c_uint32, C.POINTER(vp)]),
         "psr_game_free": (None, [vp]),
         "psr_game_num_hands": (C.c_int, [vp, C.c_int]),
         "psr_game_private_cards": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint8)]),
@@ -232,12 +235,19 @@ class Range:
 # ------------------------------------------------------------------- tree --
 
 class BoardState(enum.IntEnum):
+    """action_tree.rs:78-100: Flop < Turn < River, default Flop."""
     Flop = 0
     Turn = 1
     River = 2
 
+    def __str__(self):
+        return self.name
+
 
 class Action(enum.IntEnum):
+    """Action kinds of action_tree.rs:37-51 in their Ord order.  An action is
+    the pair (kind, amount); pairs compare as the reference's derived Ord
+    (variant first, then amount) and hash by value."""
     None_ = 0
     Fold = 1
     Check = 2
@@ -246,6 +256,36 @@ class Action(enum.IntEnum):
     Raise = 5
     AllIn = 6
     Chance = 7
+
+
+# action_tree.rs:10-33 player field flags
+PLAYER_OOP = 0
+PLAYER_IP = 1
+PLAYER_CHANCE = 2
+PLAYER_MASK = 3
+PLAYER_CHANCE_FLAG = 4
+PLAYER_TERMINAL_FLAG = 8
+PLAYER_FOLD_FLAG = 24
+PLAYER_DEPTH_BOUNDARY_FLAG = 40
+
+_RANKS, _SUITS = "23456789TJQKA", "cdhs"
+
+
+def card_to_string(card: int) -> str:
+    """card.rs card_to_string: "2c" .. "As"; raises for cards outside 0..51."""
+    if not 0 <= int(card) < 52:
+        raise PsrError(f"Invalid card: {card}")
+    return _RANKS[card >> 2] + _SUITS[card & 3]
+
+
+def action_to_string(action: Tuple[int, int]) -> str:
+    """action_tree.rs:53-72 Display for Action."""
+    kind, amount = Action(action[0]), int(action[1])
+    if kind in (Action.None_, Action.Fold, Action.Check, Action.Call):
+        return "None" if kind == Action.None_ else kind.name
+    if kind == Action.Chance:
+        return f"Chance({card_to_string(amount)})" if 0 <= amount < 52 else f"Chance({amount})"
+    return f"{kind.name}({amount})"
 
 
 def _act(kind, amount=0):
@@ -300,8 +340,15 @@ class TreeConfig:
         return c, keep
 
 
+def _history_c(history):
+    return (_ActionC * max(len(history), 1))(*[_ActionC(int(k), int(a)) for k, a in history])
+
+
 class ActionTree:
-    """action_tree.rs:326-334 ActionTree::new and its query helpers."""
+    """action_tree.rs:326-334 ActionTree::new, its navigation
+    (play / undo / back_to_root / apply_history / history,
+    action_tree.rs:416-481) and query helpers.  Actions are (kind, amount)
+    pairs; chance actions are left out of a history, as in the reference."""
 
     def __init__(self, config: TreeConfig):
         lib = load_library()
@@ -310,18 +357,64 @@ class ActionTree:
         _check(lib.psr_action_tree_build(C.byref(c), C.byref(h)))
         self._h = h
         self.config = config
+        self._history: List[Tuple[Action, int]] = []
 
     def __del__(self):
         if getattr(self, "_h", None):
             load_library().psr_action_tree_free(self._h)
             self._h = None
 
-    def available_actions(self, history: Sequence[Tuple[int, int]] = ()) -> List[Tuple[Action, int]]:
+    def available_actions(self, history: Optional[Sequence[Tuple[int, int]]] = None) -> List[Tuple[Action, int]]:
+        """Actions at the node `history` reaches (the current node when None),
+        after any chance node there (action_tree.rs:452-455)."""
         lib = load_library()
-        hist = (_ActionC * max(len(history), 1))(*[_ActionC(int(k), int(a)) for k, a in history])
+        history = self._history if history is None else history
         out = (_ActionC * 64)()
-        n = _check(lib.psr_action_tree_actions(self._h, hist, len(history), out, 64))
+        n = _check(lib.psr_action_tree_actions(self._h, _history_c(history), len(history), out, 64))
         return [(Action(out[i].kind), out[i].amount) for i in range(n)]
+
+    def _node_info(self, history):
+        info = (C.c_int32 * 4)()
+        _check(load_library().psr_action_tree_node_info(self._h, _history_c(history), len(history), info))
+        return list(info)
+
+    def history(self) -> List[Tuple[Action, int]]:
+        return list(self._history)
+
+    def play(self, action: Tuple[int, int]):
+        """action_tree.rs:462-470: the action must be available here."""
+        action = (Action(action[0]), int(action[1]))
+        if action not in self.available_actions():
+            raise PsrError(f"Action `{action_to_string(action)}` is not available")
+        self._history.append(action)
+
+    def undo(self):
+        if not self._history:
+            raise PsrError("No action to undo")
+        self._history.pop()
+
+    def back_to_root(self):
+        self._history = []
+
+    def apply_history(self, history: Sequence[Tuple[int, int]]):
+        self.back_to_root()
+        for a in history:
+            self.play(a)
+
+    def is_terminal_node(self) -> bool:
+        return bool(self._node_info(self._history)[0])
+
+    def is_chance_node(self) -> bool:
+        return bool(self._node_info(self._history)[1])
+
+    def total_bet_amount(self) -> List[int]:
+        """action_tree.rs:523-527: chips each player has put in along the
+        current history."""
+        return self._node_info(self._history)[2:4]
+
+    def num_invalid_terminals(self) -> int:
+        """len(invalid_terminals()), action_tree.rs:356-362."""
+        return _check(load_library().psr_action_tree_invalid_terminals(self._h))
 
     def count_nodes(self) -> List[int]:
         out = (C.c_uint64 * 3)()
@@ -345,7 +438,9 @@ class PostFlopGame:
         self._h = handle
 
     @classmethod
-    def with_config(cls, card_config: CardConfig, action_tree: ActionTree) -> "PostFlopGame":
+    def with_config(cls, card_config: CardConfig, action_tree: ActionTree, isomorphism: bool = True) -> "PostFlopGame":
+        """isomorphism=False deals every card at every chance node (no suit
+        isomorphism; the reference always applies it)."""
         lib = load_library()
         r0 = np.ascontiguousarray(card_config.range[0].data, np.float32)
         r1 = np.ascontiguousarray(card_config.range[1].data, np.float32)
@@ -357,7 +452,7 @@ class PostFlopGame:
         cc.turn = card_config.turn
         cc.river = card_config.river
         h = C.c_void_p()
-        _check(lib.psr_game_build(C.byref(cc), action_tree._h, C.byref(h)))
+        _check(lib.psr_game_build_ex(C.byref(cc), action_tree._h, 0 if isomorphism else 1, C.byref(h)))
         return cls(h)
 
     def __del__(self):

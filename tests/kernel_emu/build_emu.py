@@ -1,12 +1,12 @@
 """Builds tests/kernel_emu/libpsr_dcfr_emu.so: the library's own sources
 (host game build, device layout, C ABI and the HIP kernels) compiled for the
-CPU against hip_cpu_shim.hpp.  The kernel source is used unchanged except
-that its dynamic LDS declaration is pointed at the emulated block's buffer.
+CPU against hip_cpu_shim.hpp, unchanged (the shim points the kernels'
+dynamic LDS block at the emulated workgroup's buffer).
 """
 import os
-import re
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -37,23 +37,25 @@ def build():
 
 
 def _build(work):
-    src = open(os.path.join(CSRC, "dcfr_kernels.hip")).read()
-    src, n = re.subn(r"extern __shared__ double (\w+)\[\];", r"double* \1 = emu::dynamic_lds<double>();", src)
-    assert n == 1, "dynamic LDS declaration not found"
-    kern = os.path.join(work, "dcfr_kernels_emu.cpp")
-    with open(kern, "w") as f:
-        f.write(src)
     flags = ["-std=c++20", "-O1", "-g", "-fPIC", "-U_FORTIFY_SOURCE", "-ffp-contract=off", "-pthread", "-I" + HERE, "-I" + CSRC,
              "-Wno-unused-function"]
-    objs = []
-    mlp_kern = os.path.join(work, "mlp_kernels_emu.cpp")
-    with open(mlp_kern, "w") as f:
-        f.write(open(os.path.join(CSRC, "mlp_kernels.hip")).read())
-    for s in [os.path.join(CSRC, "game.cpp"), os.path.join(CSRC, "capi_game.cpp"), os.path.join(CSRC, "gpu_solver.cpp"),
-              os.path.join(CSRC, "mlp.cpp"), kern, mlp_kern]:
+    sys.path.insert(0, ROOT)
+    from poker_solver_rust_amd import build as B
+
+    srcs = [os.path.join(CSRC, f) for f in B.HOST_SOURCES]
+    for f in B.DEVICE_SOURCES:   # .hip sources compiled as C++ for the CPU
+        cpp = os.path.join(work, f.replace(".hip", "_emu.cpp"))
+        with open(cpp, "w") as out:
+            out.write(f'#include "{f}"\n')
+        srcs.append(cpp)
+
+    def compile_one(s):
         o = os.path.join(work, os.path.basename(s) + ".o")
         subprocess.check_call([CXX, "-c", *flags, s, "-o", o])
-        objs.append(o)
+        return o
+
+    with ThreadPoolExecutor(max_workers=max(1, min(len(srcs), os.cpu_count() or 1))) as pool:
+        objs = list(pool.map(compile_one, srcs))
     tmp = OUT + ".tmp"
     subprocess.check_call([CXX, "-shared", "-pthread", *objs, "-o", tmp])
     os.replace(tmp, OUT)

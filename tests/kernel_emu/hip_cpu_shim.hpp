@@ -358,3 +358,6 @@ inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int* n, K, int, s
     *n = 1;
     return hipSuccess;
 }
+
+// The kernels' dynamic LDS block is the emulated workgroup's buffer.
+#define PSR_DYNAMIC_LDS(T, name) T* name = emu::dynamic_lds<T>()

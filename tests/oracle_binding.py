@@ -53,8 +53,17 @@ def load():
         "oracle_num_decisions": (C.c_int, [vp]),
         "oracle_encode_signed": (C.c_float, [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_int16)]),
         "oracle_encode_unsigned": (C.c_float, [C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_uint16)]),
+        "oracle_regret_matching": (None, [C.POINTER(C.c_float), C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
+        "oracle_regret_matching_compressed": (None, [C.POINTER(C.c_int16), C.c_size_t, C.c_int,
+                                                     C.POINTER(C.c_float)]),
+        "oracle_slice_op": (C.c_int, [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.c_size_t, C.c_size_t, C.c_float]),
+        "oracle_debug_break_isomorphism": (C.c_int, [vp, C.c_int]),
+        "oracle_set_threads": (None, [vp, C.c_int]),
         "oracle_solve_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
-                                           C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32)]),
+                                           C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32), C.c_int]),
+        "oracle_fp64_expl_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
+                                               C.POINTER(C.c_uint32), C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -98,6 +107,11 @@ class OracleGame:
 
     def solve_step(self, t):
         load().oracle_solve_step(self._h, t)
+
+    def set_threads(self, threads):
+        """threads > 1: the children of nodes before the river deal run in
+        parallel (the reference's rayon policy); results are unchanged."""
+        load().oracle_set_threads(self._h, threads)
 
     def exploitability(self):
         return load().oracle_exploitability(self._h)
@@ -189,6 +203,11 @@ class OracleGame:
     def num_boundaries(self):
         return load().oracle_num_boundaries(self._h)
 
+    def break_isomorphism(self, mode):
+        """Plants a mistake in the isomorphic chance sums (1: no hand swaps,
+        2: isomorphic cards left out); returns the entries changed."""
+        return load().oracle_debug_break_isomorphism(self._h, mode)
+
     def boundary_reach(self, ordinal, player):
         """Opponent reach recorded at the last visit of boundary `ordinal`,
         indexed like the reference's boundary_reach(ordinal, player)."""
@@ -197,6 +216,53 @@ class OracleGame:
         if n < 0:
             raise ValueError("bad boundary ordinal")
         return out[:n].copy()
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def regret_matching(regret, num_actions):
+    """solver.rs:657-662 regret_matching (f32)."""
+    r = np.ascontiguousarray(regret, np.float32)
+    out = np.zeros(len(r), np.float32)
+    load().oracle_regret_matching(_fp(r), len(r), num_actions, _fp(out))
+    return out
+
+
+def regret_matching_compressed(regret, num_actions):
+    """solver.rs:667-677 regret_matching_compressed (i16 regrets)."""
+    r = np.ascontiguousarray(regret, np.int16)
+    out = np.zeros(len(r), np.float32)
+    load().oracle_regret_matching_compressed(r.ctypes.data_as(C.POINTER(C.c_int16)), len(r), num_actions, _fp(out))
+    return out
+
+
+_SLICE_OPS = {"sub": 0, "mul": 1, "div": 2, "sum": 3, "fma": 4, "max_slices": 5, "max": 6}
+
+
+def slice_op(op, lhs, rhs=None, src2=None, row=None, default=0.0):
+    """One of the oracle's slice operations (sliceop.rs); returns the result.
+    sub / mul / div act element-wise on lhs; sum / fma / max_slices reduce
+    the rows (of `row` values) of rhs (and src2) into a new row; max is the
+    scalar max(lhs, rhs)."""
+    code = _SLICE_OPS[op]
+    if op in ("sum", "fma", "max_slices"):
+        src = np.ascontiguousarray(rhs, np.float32)
+        s2 = np.ascontiguousarray(src if src2 is None else src2, np.float32)
+        dst = np.zeros(row, np.float32)
+        rc = load().oracle_slice_op(code, _fp(dst), _fp(src), _fp(s2), row, len(src), 0.0)
+        assert rc == 0
+        return dst
+    if op == "max":
+        a = np.array([lhs], np.float32)
+        b = np.array([rhs], np.float32)
+        load().oracle_slice_op(code, _fp(a), _fp(b), _fp(b), 1, 1, 0.0)
+        return float(a[0])
+    a = np.array(lhs, np.float32)
+    b = np.ascontiguousarray(rhs, np.float32)
+    load().oracle_slice_op(code, _fp(a), _fp(b), _fp(b), len(a), len(a), default)
+    return a
 
 
 def encode_signed(values):
@@ -215,18 +281,40 @@ def encode_unsigned(values):
     return sc, out
 
 
-def solve_many(blobs, iterations, threads, targets=None):
-    """Solves serialized games with the reference solve() schedule using
-    `threads` worker threads (one game per thread at a time).  Returns
-    (wall seconds, exploitability array, iterations array)."""
-    lib = load()
+def _blob_array(blobs):
     n = len(blobs)
     bufs = [(C.c_uint8 * len(b)).from_buffer_copy(b) for b in blobs]
     ptrs = (C.POINTER(C.c_uint8) * n)(*[C.cast(b, C.POINTER(C.c_uint8)) for b in bufs])
     lens = (C.c_size_t * n)(*[len(b) for b in blobs])
+    return bufs, ptrs, lens
+
+
+def solve_many(blobs, iterations, threads, targets=None, finalize=False):
+    """Solves serialized games with the reference solve() schedule using
+    `threads` worker threads (one game per thread at a time); finalize adds
+    the root cfvalue pass of both players (solver.rs:171).  Returns (wall
+    seconds, exploitability array, iterations array)."""
+    lib = load()
+    n = len(blobs)
+    bufs, ptrs, lens = _blob_array(blobs)
     expl = np.zeros(n)
     its = np.zeros(n, np.uint32)
     tg = None if targets is None else np.ascontiguousarray(targets, np.float64)
     secs = lib.oracle_solve_many(ptrs, lens, n, iterations, threads, None if tg is None else _dptr(tg), _dptr(expl),
-                                 its.ctypes.data_as(C.POINTER(C.c_uint32)))
+                                 its.ctypes.data_as(C.POINTER(C.c_uint32)), 1 if finalize else 0)
+    del bufs
     return secs, expl, its
+
+
+def fp64_exploitability_many(blobs, iterations, threads):
+    """The FP64 oracle's exploitability of each game after exactly
+    iterations[i] DCFR iterations (no early stop), `threads` games at a time.
+    Returns (wall seconds, exploitability array)."""
+    lib = load()
+    n = len(blobs)
+    bufs, ptrs, lens = _blob_array(blobs)
+    its = np.ascontiguousarray(iterations, np.uint32)
+    expl = np.zeros(n)
+    secs = lib.oracle_fp64_expl_many(ptrs, lens, n, its.ctypes.data_as(C.POINTER(C.c_uint32)), threads, _dptr(expl))
+    del bufs
+    return secs, expl

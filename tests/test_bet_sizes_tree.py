@@ -170,3 +170,283 @@ def test_bench_river_tree_shape():
     assert acts(t) == [(A.Check, 0), (A.Bet, 50), (A.AllIn, 100)]
     assert acts(t, [(A.Bet, 50)]) == [(A.Fold, 0), (A.Call, 0), (A.AllIn, 100)]
     assert t.count_nodes() == [0, 0, 21]
+
+
+# --------------------------------------------------------------------------
+# action_tree.rs:1238-2363: Action / BoardState / constants, navigation
+# (play, undo, back_to_root, apply_history, is_terminal_node,
+# is_chance_node, total_bet_amount), invalid terminals, depth limits.
+
+def ref_river(bet, raise_, pot, stack):
+    """action_tree.rs:1543-1557 river_config."""
+    return river_tree(bet, raise_, pot, stack)
+
+
+def ref_three_street(stack, depth=None):
+    s = [("50%", ""), ("50%", "")]
+    return P.ActionTree(P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=stack,
+                                     flop_bet_sizes=s, turn_bet_sizes=s, river_bet_sizes=s, add_allin_threshold=1.5,
+                                     force_allin_threshold=0.15, merging_threshold=0.12, depth_limit=depth))
+
+
+def ref_turn(depth):
+    s = [("50%", ""), ("50%", "")]
+    return P.ActionTree(P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200,
+                                     turn_bet_sizes=s, river_bet_sizes=s, add_allin_threshold=1.5,
+                                     force_allin_threshold=0.15, merging_threshold=0.12, depth_limit=depth))
+
+
+def test_action_display_simple():
+    assert P.action_to_string((A.None_, 0)) == "None"
+    assert P.action_to_string((A.Fold, 0)) == "Fold"
+    assert P.action_to_string((A.Check, 0)) == "Check"
+    assert P.action_to_string((A.Call, 0)) == "Call"
+
+
+def test_action_display_with_amount():
+    assert P.action_to_string((A.Bet, 100)) == "Bet(100)"
+    assert P.action_to_string((A.Raise, 250)) == "Raise(250)"
+    assert P.action_to_string((A.AllIn, 1000)) == "AllIn(1000)"
+
+
+def test_action_display_chance():
+    assert P.action_to_string((A.Chance, 0)) == "Chance(2c)"
+    assert P.action_to_string((A.Chance, 51)) == "Chance(As)"
+
+
+def test_action_display_chance_invalid_card():
+    assert P.action_to_string((A.Chance, 255)) == "Chance(255)"
+
+
+def test_action_ordering():
+    chain = [(A.None_, 0), (A.Fold, 0), (A.Check, 0), (A.Call, 0), (A.Bet, 1), (A.Bet, 2), (A.Bet, 100),
+             (A.Raise, 1), (A.Raise, 100), (A.AllIn, 1), (A.AllIn, 100), (A.Chance, 0)]
+    assert all(a < b for a, b in zip(chain, chain[1:]))
+
+
+def test_action_eq_and_hash():
+    assert len({(A.Bet, 100), (A.Bet, 100), (A.Raise, 100)}) == 2
+
+
+def test_board_state_display_order_repr_default():
+    assert [str(s) for s in (P.BoardState.Flop, P.BoardState.Turn, P.BoardState.River)] == ["Flop", "Turn", "River"]
+    assert P.BoardState.Flop < P.BoardState.Turn < P.BoardState.River
+    assert [int(s) for s in P.BoardState] == [0, 1, 2]
+    assert P.TreeConfig().initial_state == P.BoardState.Flop
+
+
+def test_player_constants():
+    assert (P.PLAYER_OOP, P.PLAYER_IP, P.PLAYER_CHANCE, P.PLAYER_MASK) == (0, 1, 2, 3)
+    assert (P.PLAYER_CHANCE_FLAG, P.PLAYER_TERMINAL_FLAG, P.PLAYER_FOLD_FLAG) == (4, 8, 24)
+    assert P.PLAYER_FOLD_FLAG & P.PLAYER_TERMINAL_FLAG != 0
+
+
+def test_depth_boundary_flag_constant():
+    assert P.PLAYER_DEPTH_BOUNDARY_FLAG & P.PLAYER_TERMINAL_FLAG != 0
+    assert P.PLAYER_DEPTH_BOUNDARY_FLAG & P.PLAYER_CHANCE_FLAG == 0
+    assert P.PLAYER_DEPTH_BOUNDARY_FLAG != P.PLAYER_FOLD_FLAG
+
+
+def test_tree_config_default():
+    c = P.TreeConfig()
+    assert (c.starting_pot, c.effective_stack, c.rake_rate, c.rake_cap) == (0, 0, 0.0, 0.0)
+    assert (c.add_allin_threshold, c.force_allin_threshold, c.merging_threshold) == (0.0, 0.0, 0.0)
+    assert c.turn_donk_sizes is None and c.river_donk_sizes is None and c.depth_limit is None
+
+
+def test_tree_config_custom():
+    s = [("60%, e, a", "2.5x"), ("60%, e, a", "2.5x")]
+    c = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=200, effective_stack=900, rake_rate=0.05,
+                     rake_cap=30.0, turn_bet_sizes=s, river_bet_sizes=s, river_donk_sizes="50%",
+                     add_allin_threshold=1.5, force_allin_threshold=0.15, merging_threshold=0.1)
+    t = P.ActionTree(c)
+    assert t.config.starting_pot == 200 and t.config.effective_stack == 900
+    assert abs(t.config.rake_rate - 0.05) < 1e-15 and abs(t.config.rake_cap - 30.0) < 1e-15
+    assert t.config.river_donk_sizes is not None
+
+
+def test_action_tree_default_accessors():
+    t = ref_river("50%", "60%", 100, 100)
+    assert t.history() == [] and t.num_invalid_terminals() == 0
+
+
+def test_check_config_rejects_negative_pot():
+    with pytest.raises(P.PsrError):
+        P.ActionTree(P.TreeConfig(starting_pot=-10, effective_stack=100))
+
+
+def test_river_check_check_is_terminal():
+    t = ref_river("50%", "60%", 100, 100)
+    t.play((A.Check, 0))
+    assert (A.Check, 0) in t.available_actions()
+    t.play((A.Check, 0))
+    assert t.is_terminal_node()
+
+
+def test_river_bet_fold_is_terminal():
+    t = ref_river("50%", "60%", 100, 100)
+    t.play((A.Bet, 50))
+    a = t.available_actions()
+    assert (A.Fold, 0) in a and (A.Call, 0) in a
+    t.play((A.Fold, 0))
+    assert t.is_terminal_node()
+
+
+def test_river_bet_call_is_terminal():
+    t = ref_river("50%", "60%", 100, 100)
+    t.play((A.Bet, 50))
+    t.play((A.Call, 0))
+    assert t.is_terminal_node()
+
+
+def test_play_and_undo():
+    t = ref_river("50%", "60%", 100, 100)
+    assert t.history() == []
+    t.play((A.Check, 0))
+    assert t.history() == [(A.Check, 0)]
+    t.undo()
+    assert t.history() == []
+
+
+def test_play_invalid_action():
+    with pytest.raises(P.PsrError):
+        ref_river("50%", "60%", 100, 100).play((A.Fold, 0))
+
+
+def test_undo_at_root_is_err():
+    with pytest.raises(P.PsrError):
+        ref_river("50%", "60%", 100, 100).undo()
+
+
+def test_back_to_root():
+    t = ref_river("50%", "60%", 100, 100)
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    t.back_to_root()
+    assert t.history() == []
+
+
+def test_apply_history():
+    t = ref_river("50%", "60%", 100, 100)
+    t.apply_history([(A.Check, 0), (A.Check, 0)])
+    assert t.history() == [(A.Check, 0), (A.Check, 0)]
+    assert t.is_terminal_node()
+
+
+def test_apply_history_invalid():
+    with pytest.raises(P.PsrError):
+        ref_river("50%", "60%", 100, 100).apply_history([(A.Fold, 0)])
+
+
+def test_total_bet_amount_at_root():
+    t = ref_river("50%", "60%", 100, 100)
+    assert t.total_bet_amount() == [0, 0]
+    t.play((A.Bet, 50))
+    assert t.total_bet_amount() == [50, 0]
+    t.play((A.Call, 0))
+    assert t.total_bet_amount() == [50, 50]
+
+
+def test_total_bet_amount_raise_and_streets():
+    """Raises pay the difference to the new amount plus the call; amounts
+    accumulate over streets (chance nodes are stepped over)."""
+    t = river_tree("50%", "3x", 100, 1000, add=0.0, force=0.0, merge=0.0)
+    t.apply_history([(A.Bet, 50), (A.Raise, 150)])
+    assert t.total_bet_amount() == [50, 150]
+    t.play((A.Call, 0))
+    assert t.total_bet_amount() == [150, 150]
+    f = ref_three_street(1000)
+    f.apply_history([(A.Bet, 50), (A.Call, 0), (A.Check, 0), (A.Bet, 100), (A.Call, 0)])
+    assert f.total_bet_amount() == [150, 150]
+
+
+def test_flop_tree_has_chance_nodes():
+    t = ref_three_street(200)
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_chance_node()
+    assert (A.Check, 0) in t.available_actions()
+
+
+def test_flop_through_all_streets():
+    t = ref_three_street(1000)
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_chance_node() and not t.is_terminal_node()
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_chance_node()
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_terminal_node()
+
+
+def test_river_raise_actions():
+    t = ref_river("50%", "60%", 100, 200)
+    t.play((A.Bet, 50))
+    a = t.available_actions()
+    assert (A.Fold, 0) in a and (A.Call, 0) in a and (A.AllIn, 200) in a
+
+
+def test_prev_bet_relative_raise_navigation():
+    t = ref_river("50%", "3x", 100, 500)
+    t.play((A.Bet, 50))
+    assert (A.Raise, 150) in t.available_actions()
+
+
+def test_turn_donk_bet_navigation():
+    s = [("50%", ""), ("50%", "")]
+    t = P.ActionTree(P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=1000,
+                                  flop_bet_sizes=s, turn_bet_sizes=s, turn_donk_sizes="75%", add_allin_threshold=1.5,
+                                  force_allin_threshold=0.15, merging_threshold=0.12))
+    t.play((A.Check, 0))
+    assert (A.Bet, 50) in t.available_actions()
+    t.back_to_root()
+    t.play((A.Check, 0))
+    t.play((A.Bet, 50))
+    t.play((A.Call, 0))
+    assert t.is_chance_node()
+    a = t.available_actions()
+    assert (A.Check, 0) in a and (A.Bet, 150) in a
+
+
+def test_no_invalid_terminals():
+    assert ref_river("50%", "60%", 100, 100).num_invalid_terminals() == 0
+    assert ref_three_street(200).num_invalid_terminals() == 0
+
+
+def test_count_action_nodes_river_exact():
+    c = ref_river("50%", "60%", 100, 200).count_nodes()
+    assert c[0] == 0 and c[1] == 0 and c[2] > 0
+
+
+def test_depth_limit_none_does_not_affect_tree():
+    t = ref_turn(None)
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_chance_node()
+
+
+def test_depth_limit_zero_blocks_all_transitions():
+    t = ref_turn(0)
+    assert not t.is_terminal_node() and not t.is_chance_node()
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_terminal_node() and not t.is_chance_node()
+
+
+def test_depth_limit_zero_bet_call_is_terminal():
+    t = ref_turn(0)
+    t.play((A.Bet, 50))
+    t.play((A.Call, 0))
+    assert t.is_terminal_node()
+
+
+def test_depth_limit_one_allows_one_transition():
+    t = ref_three_street(1000, depth=1)
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_chance_node()
+    t.play((A.Check, 0))
+    t.play((A.Check, 0))
+    assert t.is_terminal_node() and not t.is_chance_node()

@@ -116,3 +116,45 @@ def test_fp64_exploitability_matches_oracle(make):
     # and the f32 pass agrees to f32 precision
     e32 = float(b.exploitability()[0])
     assert abs(e32 - eo) <= 1e-4 * max(1.0, abs(eo))
+
+
+@pytest.fixture(scope="module")
+def fp64_turn_oracle():
+    """The FP64 oracle's solve of the seeded turn subgame (turn_uniform01:
+    U[0,1] ranges, 48 river cards, ~17k nodes) for C4_ITERS iterations, the
+    turn's children on the host's cores as the reference's rayon policy."""
+    import os
+    s = W.turn_uniform01(20260)
+    g = s.build()
+    o = O.OracleGame(g.serialize(), 64)
+    o.set_threads(max(1, min(16, os.cpu_count() or 1)))
+    for t in range(C4_ITERS):
+        o.solve_step(t)
+    return s, g, o.exploitability(), [o.root_cfvalues(p) for p in range(2)]
+
+
+C4_ITERS = 100
+
+
+@pytest.mark.parametrize("storage,gap_pot,cfv_rel", [("f32", 0.002, 0.01), ("fp16", 0.002, 0.01), ("bf16", 0.01, 0.05)])
+def test_reduced_precision_turn_solve_against_fp64_oracle(fp64_turn_oracle, storage, gap_pot, cfv_rel):
+    """Config C4 on its own workload: the turn tree of C2 with regrets and
+    strategy sums in FP16 / BF16 words with per-node FP32 scales (f32 for
+    reference), C4_ITERS DCFR iterations on the device, then the device's
+    FP64 best response (utility.rs:380-392 in f64).  The exploitability gap
+    to the FP64 oracle's own solve of the same tree (gap_pot: fraction of the
+    pot) and the per-hand root CFV error (cfv_rel: fraction of the largest
+    oracle CFV) stay within the stated bounds."""
+    s, g, eo, ref = fp64_turn_oracle
+    b = P.GpuBatchSolver(0, storage=storage)
+    b.prepare_batch([g], C4_ITERS)
+    b.run_iterations(0, C4_ITERS)
+    e64 = float(b.exploitability_f64()[0])
+    cf = b.root_cfvalues()[0]
+    scale = max(float(np.abs(r).max()) for r in ref)
+    err = max(float(np.abs(cf[p].astype(np.float64) - ref[p]).max()) for p in range(2))
+    print(f"C4 {storage}: fp64 exploitability {e64:.6f} vs oracle {eo:.6f} (gap {e64 - eo:+.6f} chips, pot {s.pot}), "
+          f"root cfv max abs err {err:.3e} (max |cfv| {scale:.3e})")
+    assert e64 >= 0.0
+    assert abs(e64 - eo) <= gap_pot * s.pot
+    assert err <= cfv_rel * scale

@@ -5,11 +5,14 @@ barrier structure without a GPU; the compiled gfx950 code is checked by
 test_kernel_isa.py and, on the device, by the gpu-marked tests.
 
 Each check runs in its own process (tests/kernel_emu/run_checks.py) so the
-emulated library never shares a process with the HIP build.
+emulated library never shares a process with the HIP build; the checks are
+started together when the module's first test runs (one per CPU core at a
+time) and each test waits for its own.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 import pytest
 
@@ -33,13 +36,29 @@ CHECKS = [("std_stepwise", True), ("std_raises_stepwise", True), ("std_solve", T
           ("expected_values", False), ("raked", True), ("reach_at_nodes", False), ("flop_stepwise", False), ("mlp_forward", True), ("mlp_gradients", True), ("mlp_gradients_split_k", True), ("mlp_gradients_one_row", True), ("mlp_adamw_step", True)]
 
 
-@pytest.mark.parametrize("check,uniform", CHECKS)
-def test_emulated_kernels_match_oracle(emu_lib, check, uniform):
+def _run_check(check, uniform):
     env = dict(os.environ)
     if not uniform:
         env["EMU_NO_UNIFORM_CHECK"] = "1"
-    r = subprocess.run([sys.executable, os.path.join(HERE, "kernel_emu", "run_checks.py"), check],
-                       capture_output=True, text=True, timeout=600, env=env)
+    return subprocess.run([sys.executable, os.path.join(HERE, "kernel_emu", "run_checks.py"), check],
+                          capture_output=True, text=True, timeout=900, env=env)
+
+
+@pytest.fixture(scope="module")
+def emu_runs(emu_lib):
+    """Every check started at once on a pool of one process per core, the
+    longest first; {check: future}."""
+    order = ["raked", "turn_stepwise", "flop_stepwise", "mixed_batch", "expected_values"]
+    todo = sorted(CHECKS, key=lambda c: order.index(c[0]) if c[0] in order else len(order))
+    pool = ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1)))
+    futures = {check: pool.submit(_run_check, check, uniform) for check, uniform in todo}
+    yield futures
+    pool.shutdown(wait=True, cancel_futures=True)
+
+
+@pytest.mark.parametrize("check,uniform", CHECKS)
+def test_emulated_kernels_match_oracle(emu_runs, check, uniform):
+    r = emu_runs[check].result()
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
 
 

@@ -34,7 +34,13 @@ def pins_asm(tmp_path_factory):
 
 @pytest.fixture(scope="module")
 def kernels_asm(tmp_path_factory):
-    return L.compile_to_asm(os.path.join(B.CSRC, "dcfr_kernels.hip"), FLAGS, str(tmp_path_factory.mktemp("dcfr")))
+    """Assembly of every storage mode's DCFR kernels (compiled concurrently)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    out = str(tmp_path_factory.mktemp("dcfr"))
+    srcs = [os.path.join(B.CSRC, f) for f in B.DCFR_SOURCES]
+    with ThreadPoolExecutor(max_workers=len(srcs)) as pool:
+        return "\n".join(pool.map(lambda s: L.compile_to_asm(s, FLAGS, out), srcs))
 
 
 @pytest.fixture(scope="module")

@@ -218,3 +218,27 @@ def test_compressed_oracle_tracks_uncompressed():
     s32 = o32.state()[1]
     s16 = o16.state()[1]
     assert np.abs(s16 - s32).max() <= 0.05 * np.abs(s32).max()
+
+
+def test_parallel_children_give_identical_state():
+    """The oracle's parallel traversal of the nodes before the river deal
+    (the reference's rayon policy, utility.rs:103-115) leaves every regret
+    and strategy sum bit-identical to the sequential one."""
+    import poker_solver_rust_amd as P
+
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs,AQo"), P.Range.parse("QQ-99,AJs,KQs")),
+                      flop=P.flop_from_str("Qs Jh 2c"), turn=P.card_from_str("8d"))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200, turn_bet_sizes=s,
+                      river_bet_sizes=s)
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    a, b = O.OracleGame(g.serialize()), O.OracleGame(g.serialize())
+    b.set_threads(4)
+    for t in range(6):
+        a.solve_step(t)
+        b.solve_step(t)
+    ra, sa = a.state()
+    rb, sb = b.state()
+    assert np.array_equal(ra, rb) and np.array_equal(sa, sb)
+    assert a.exploitability() == b.exploitability()
+    assert np.array_equal(a.root_cfvalues(0), b.root_cfvalues(0))
