@@ -43,12 +43,13 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def algorithmic_bytes(storage, iters, checks):
+def algorithmic_bytes(storage, iters, checks, finalize=True):
     """20 B per edge-hand per DCFR iteration (regret read as the opponent's
     strategy, read + written at the traverser's update; strategy sum read +
-    written) and 4 B per edge-hand per exploitability pass (strategy sums
-    read once as the opponent's average strategy)."""
-    return float(np.sum(storage * (20.0 * iters + 4.0 * checks)))
+    written), 4 B per edge-hand per exploitability pass (strategy sums read
+    once as the opponent's average strategy) and 8 B per edge-hand for
+    finalize (strategy sums read in both players' cfvalue traversals)."""
+    return float(np.sum(storage * (20.0 * iters + 4.0 * checks + (8.0 if finalize else 0.0))))
 
 
 def num_checks(iters, max_iters, interval):
@@ -61,14 +62,19 @@ def num_checks(iters, max_iters, interval):
     return c + extra.astype(iters.dtype)
 
 
+def box_threads():
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
 def cpu_baseline(subgames, iters, mode, interval, seconds_target=15.0):
     """The CPU oracle (restatement of the reference range-solver) timed on
     this host: independent subgames on all cores, one subgame per thread at
-    a time, as the reference runs river subgames single-threaded."""
+    a time, as the reference runs river subgames single-threaded; each solve
+    ends with finalize (the root cfvalues), as solve() does."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_binding as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    threads = box_threads()
     blobs = [s.build().serialize() for s in subgames]
     targets = [s.target for s in subgames] if mode == "solve" else None
     # calibrate on a few subgames, then size the sample to ~seconds_target
@@ -80,38 +86,74 @@ def cpu_baseline(subgames, iters, mode, interval, seconds_target=15.0):
     sched = "to 0.3% pot (checked every 5 iterations), cap" if mode == "solve" else "fixed"
     return {"value": round(n / secs, 3), "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"first {n} of the seeded river subgames, solve() {sched} {iters} iterations "
-                      f"(mean {float(its.mean()):.1f}), {threads} threads on '{cpu_model()}'"}
+                      f"(mean {float(its.mean()):.1f}) + finalize, {threads} threads on '{cpu_model()}'"}
 
 
-def cpu_turn_baseline(game, iters=8):
-    """The CPU oracle on the same turn subgame: `iters` solve_step iterations
-    (after one untimed), one thread (the oracle restates the reference's
-    recursion without its rayon split over the turn's children)."""
+def cpu_turn_baseline(game, seconds_target=8.0):
+    """The CPU oracle on the same turn subgame on all cores (the turn's
+    children in parallel, the reference's rayon policy): DCFR iterations/s of
+    plain solve_step iterations, and of the solve() schedule (exploitability
+    before the first iteration and every 5), each over a bounded number of
+    iterations sized to ~seconds_target."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_binding as O
 
+    threads = box_threads()
     o = O.OracleGame(game.serialize())
-    o.solve_step(0)
+    o.set_threads(threads)
     t = time.time()
-    for k in range(1, iters + 1):
+    o.solve_step(0)
+    one = max(time.time() - t, 1e-3)
+    n = int(max(5, min(500, seconds_target / one)) // 5 * 5)
+    t = time.time()
+    for k in range(1, n + 1):
         o.solve_step(k)
-    secs = time.time() - t
-    return {"value": round(iters / secs, 3), "unit": "DCFR iters/s", "cores": 1, "kind": "port",
-            "sample": f"{iters} solve_step iterations of the same turn subgame, 1 thread on '{cpu_model()}'"}
+    step_rate = n / (time.time() - t)
+    o2 = O.OracleGame(game.serialize())
+    o2.set_threads(threads)
+    t = time.time()
+    o2.solve(n, 0.0)
+    solve_rate = n / (time.time() - t)
+    return {"value": round(step_rate, 3), "unit": "DCFR iters/s", "cores": threads, "kind": "port",
+            "with_solve_checks": round(solve_rate, 3),
+            "sample": f"{n} solve_step iterations of the same turn subgame ({n} more under the solve() schedule "
+                      f"for with_solve_checks), {threads} threads on '{cpu_model()}'"}
+
+
+def fp64_gap_river(P, games, pots, iters, expl64, n_sample):
+    """FP64 exploitability gap on a sample of the batch: the device's FP64
+    best response of its own f32 solution against the FP64 oracle's solve of
+    the same tree with the same number of iterations."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as O
+
+    n = min(n_sample, len(games))
+    blobs = [g.serialize() for g in games[:n]]
+    secs, eo = O.fp64_exploitability_many(blobs, iters[:n], box_threads())
+    gap = (expl64[:n] - eo) / pots[:n]
+    return {"sample": n, "max_abs_gap_pot_pct": round(float(np.abs(gap).max()) * 100, 5),
+            "mean_gap_pot_pct": round(float(gap.mean()) * 100, 6),
+            "device_fp64_max_expl_pot_pct": round(float((expl64 / pots).max()) * 100, 5),
+            "oracle_seconds": round(secs, 2)}
 
 
 def turn_bench(P, W, iters, seed, cpu=True):
     """Turn DCFR iterations/s on the full turn subgame (4 seeded board cards,
     48 rivers, U[0,1] ranges, pot 60, stack 200, 33%/75%/all-in, 2 raises at
     3x): solve_step iterations only, timed over `iters` iterations after one
-    warm-up iteration."""
+    warm-up iteration (a fresh solve of `iters` iterations, reset first);
+    then the FP64 exploitability gap of that solve against the FP64 oracle's
+    solve of the same tree."""
     s = W.turn_uniform01(seed)
     g = s.build()
     b = P.GpuBatchSolver(0)
-    b.prepare_batch([g], iters + 1)
-    b.run_iterations(0, 1)
     t = time.perf_counter()
-    b.run_iterations(1, iters + 1)
+    b.prepare_batch([g], iters + 1)
+    prep = time.perf_counter() - t
+    b.run_iterations(0, 1)
+    b.reset()
+    t = time.perf_counter()
+    b.run_iterations(0, iters)
     secs = time.perf_counter() - t
     kms = b.last_kernel_ms()
     edge_hands = float(g.num_storage())
@@ -119,11 +161,27 @@ def turn_bench(P, W, iters, seed, cpu=True):
     achieved = abytes / (kms * 1e-3) / 1e9
     log(f"turn: {iters} iterations in {secs * 1e3:.1f} ms ({iters / secs:.1f} it/s), {g.num_nodes()} nodes, "
         f"{edge_hands:.0f} edge-hands")
-    extra = {"cpu_baseline": cpu_turn_baseline(g)} if cpu else {}
-    return {**extra, "value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
-            "edge_hands": edge_hands, "ms_per_iter": round(secs * 1e3 / iters, 4),
-            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 5)}}
+    out = {"value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
+           "edge_hands": edge_hands, "ms_per_iter": round(secs * 1e3 / iters, 4), "prepare_s": round(prep, 3),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5)}}
+    if cpu:
+        out["cpu_baseline"] = cpu_turn_baseline(g)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_binding as O
+
+        e64 = float(b.exploitability_f64()[0])
+        t = time.time()
+        o = O.OracleGame(g.serialize(), 64)
+        o.set_threads(box_threads())
+        for k in range(iters):
+            o.solve_step(k)
+        eo = o.exploitability()
+        out["fp64_gap"] = {"device_fp64_exploitability": e64, "oracle_fp64_exploitability": eo,
+                           "gap_chips": e64 - eo, "gap_pot_pct": (e64 - eo) / s.pot * 100,
+                           "oracle_seconds": round(time.time() - t, 2)}
+        log(f"turn fp64 gap: device {e64:.6f} vs oracle {eo:.6f} chips (pot {s.pot})")
+    return out
 
 
 def net_solve_bench(args):
@@ -255,8 +313,10 @@ def reduced_precision_bench(args):
 
 def river_uniform_bench(args):
     """One seeded river subgame (uniform ranges, 50%/100%/all-in,
-    one raise), 1000 DCFR iterations with the solve() checks every 5, on the
-    device (single-game path) and by the CPU oracle (one thread)."""
+    one raise), 1000 DCFR iterations with the solve() checks every 5 and
+    finalize, on the device (single-game path) and by the CPU oracle (one
+    thread); the FP64 exploitability gap of the device's solution against the
+    FP64 oracle's solve of the same tree."""
     import poker_solver_rust_amd as P
     from poker_solver_rust_amd import workloads as W
 
@@ -274,13 +334,20 @@ def river_uniform_bench(args):
     secs = time.perf_counter() - t
     o = O.OracleGame(g.serialize())
     t = time.time()
-    eo, ito = o.solve(1000, 0.0)
+    eo, ito = o.solve(1000, 0.0)   # ends with finalize
     t_cpu = time.time() - t
+    e64 = float(b.exploitability_f64()[0])
+    o64 = O.OracleGame(g.serialize(), 64)
+    for k in range(int(it[0])):
+        o64.solve_step(k)
+    eo64 = o64.exploitability()
     out = {"metric": "single river subgame, 1000 DCFR iterations: seconds", "config": {
                "workload": "river_uniform seed %d" % args.seed, "nodes": g.num_nodes(),
                "edge_hands": g.num_storage()},
            "device": {"seconds": secs, "exploitability": float(e[0]), "iterations": int(it[0])},
            "cpu_oracle_1_thread": {"seconds": t_cpu, "exploitability": eo, "iterations": ito},
+           "fp64_gap": {"device_fp64_exploitability": e64, "oracle_fp64_exploitability": eo64,
+                        "gap_chips": e64 - eo64, "gap_pot_pct": (e64 - eo64) / s.pot * 100},
            "higher_is_better": False}
     print(json.dumps(out), flush=True)
 
@@ -425,7 +492,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--single-sample", type=int, default=64,
                     help="subgames solved one per launch for the single-subgame yardstick (0: skip)")
-    ap.add_argument("--turn-iters", type=int, default=200, help="timed turn DCFR iterations (0: skip)")
+    ap.add_argument("--turn-iters", type=int, default=500, help="timed turn DCFR iterations (0: skip)")
+    ap.add_argument("--fp64-sample", type=int, default=128,
+                    help="river subgames whose FP64 exploitability gap to the FP64 oracle is measured (0: skip)")
     ap.add_argument("--net-solve", action="store_true",
                     help="run the net-backed depth-limited turn solve instead (river solves -> value net -> turn)")
     ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
@@ -481,7 +550,8 @@ def main():
     solver = P.GpuBatchSolver(0)
     t0 = time.time()
     solver.prepare_batch(games, args.iterations, targets if args.mode == "solve" else None)
-    log(f"prepared in {time.time() - t0:.1f}s; launch {solver.launch_info()}")
+    prepare_s = time.time() - t0
+    log(f"prepared in {prepare_s:.1f}s; launch {solver.launch_info()}")
     interval = 5 if args.mode == "solve" else 0
 
     def step():
@@ -494,6 +564,7 @@ def main():
     for _ in range(args.steps):
         solver.reset()
         t = time.perf_counter()
+        # one launch: solve() schedule, then finalize (root cfvalues)
         expl, iters = solver.solve(args.iterations, interval)
         times.append(time.perf_counter() - t)
         kernel_ms.append(solver.last_kernel_ms())
@@ -502,7 +573,7 @@ def main():
     iters = iters.astype(np.int64)
     checks = num_checks(iters, args.iterations, interval)
     if args.mode == "fixed":
-        checks = np.ones_like(iters)
+        checks = np.ones_like(iters) * 2
     abytes = algorithmic_bytes(storage, iters, checks)
     kms = float(np.mean(kernel_ms))
     achieved = abytes / (kms * 1e-3) / 1e9
@@ -529,7 +600,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"river_batch: {len(games)} seeded river subgames, "
                                f"{'solve() to 0.3% pot, cap' if args.mode == 'solve' else 'fixed'} "
-                               f"{args.iterations} DCFR iterations",
+                               f"{args.iterations} DCFR iterations, then finalize",
                    "subgames": len(games), "iterations": args.iterations, "mode": args.mode, "seed": args.seed,
                    "mean_edge_hands": round(float(storage.mean()), 1),
                    "mean_iterations": round(float(iters.mean()), 2),
@@ -538,7 +609,31 @@ def main():
         "roofline": roofline,
         "kernel_ms": round(kms, 3),
         "algorithmic_bytes_per_launch": abytes,
+        "prepare_s": round(prepare_s, 3),
+        "end_to_end_solves_per_s": round(len(games) / (prepare_s + step_s), 3),
     }
+    if traffic:
+        out["traffic_measured_on"] = traffic.get("measured_on")
+    if args.fp64_sample > 0:
+        e64 = solver.exploitability_f64()
+        out["fp64_gap"] = fp64_gap_river(P, games, pots, iters, e64, args.fp64_sample)
+        log(f"fp64 gap: {out['fp64_gap']}")
+    if args.mode == "solve":
+        # the same batch at exactly 300 iterations (no early stop), then one
+        # exploitability pass and finalize
+        fixed = 300
+        solver.set_targets(None)
+        solver.reset()
+        t = time.perf_counter()
+        fe, fi = solver.solve(fixed, 0)
+        fsecs = time.perf_counter() - t
+        fkms = solver.last_kernel_ms()
+        fbytes = algorithmic_bytes(storage, np.full(len(games), fixed), np.full(len(games), 2))
+        out["fixed_300"] = {"value": round(len(games) / fsecs, 3), "unit": "solves/s", "kernel_ms": round(fkms, 3),
+                            "roofline_frac": round(fbytes / (fkms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                            "max_expl_pot_pct": round(float((fe / pots).max()) * 100, 4)}
+        log(f"fixed 300 iterations: {out['fixed_300']}")
+        solver.set_targets(targets)
     # the plain single-subgame path on the same subgames: one game per batch,
     # one launch each (the batched launch is the product; this is its yardstick)
     n_single = min(args.single_sample, len(games))

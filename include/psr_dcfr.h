@@ -131,6 +131,12 @@ int64_t psr_game_num_storage(const psr_game* game);
 int psr_game_num_boundaries(const psr_game* game);
 int64_t psr_game_boundary_node(const psr_game* game, int ordinal);
 int psr_game_boundary_pot(const psr_game* game, int ordinal);
+/* Parent and depth of every node (arena index; parent -1 at the root).  The
+ * arena is breadth-first with the children of a node contiguous and in
+ * action order, so arena indices are the node ids of gpu-range-solver's
+ * extract_topology (extract.rs:55-191: node_depth, edge_parent) and edge e of
+ * that topology leads to node e + 1. */
+int psr_game_topology(const psr_game* game, int32_t* parent, int32_t* depth, int64_t n);
 /* Flat description of the built game (format in game.cpp) -- the input of the
  * CPU oracle and of the device layout.  Returns the byte size; writes it when
  * cap is large enough.  Replaces extract_topology/extract_terminal_data
@@ -169,14 +175,20 @@ void psr_batch_destroy(psr_batch* batch);
  * table.  targets (chips, per game) may be NULL = 0. */
 int psr_batch_prepare(psr_batch* batch, const psr_game* const* games, int n, uint32_t max_iterations,
                       const float* targets);
+/* New early-stop targets (chips, one per game; NULL = 0) for the next
+ * psr_batch_solve: the target_exploitability argument of solve()
+ * (solver.rs:108-114), per game. */
+int psr_batch_set_targets(psr_batch* batch, const float* targets);
 /* Zero regrets and strategy sums of every game (restart the solve). */
 int psr_batch_reset(psr_batch* batch);
 /* batch.rs:414-534 run_iterations(start, end): DCFR iterations [start, end)
  * of every game, no exploitability checks. */
 int psr_batch_run_iterations(psr_batch* batch, uint32_t start, uint32_t end);
 /* solver.rs:108-176 solve(): initial check, iterations with a check every
- * check_interval and at max_iterations, early stop at the per-game target.
- * expl_out / iters_out: one entry per game (may be NULL). */
+ * check_interval and at max_iterations, early stop at the per-game target,
+ * then finalize (solver.rs:171: the root cfvalues under the average
+ * strategy, read with psr_batch_root_cfvalues).  expl_out / iters_out: one
+ * entry per game (may be NULL). */
 int psr_batch_solve(psr_batch* batch, uint32_t max_iterations, int32_t check_interval, float* expl_out,
                     uint32_t* iters_out);
 /* utility.rs:380-392 compute_exploitability of every game (chips). */
@@ -189,8 +201,10 @@ int psr_batch_exploitability_f64(psr_batch* batch, double* expl_out);
  * the average strategy; after this call psr_batch_root_cfvalues reads them. */
 int psr_batch_compute_root_cfvalues(psr_batch* batch);
 int psr_batch_root_cfvalues(psr_batch* batch, int game, int player, float* out /* H */);
-/* batch.rs:587-609 extract_results: regret / strategy-sum arrays of one game,
- * psr_game_num_storage() floats each, decision nodes in game arena order. */
+/* Regret / strategy-sum arrays of one game in this library's storage order:
+ * psr_game_num_storage() floats each, per decision node in arena order an
+ * A x H_player block (psr_batch_strategy_sum_edges gives the reference's
+ * [E x H] edge layout of batch.rs:587-609). */
 int psr_batch_strategy_sum(psr_batch* batch, int game, float* out, int64_t cap);
 int psr_batch_regrets(psr_batch* batch, int game, float* out, int64_t cap);
 /* Storage of regrets and strategy sums, set before psr_batch_prepare:
@@ -236,6 +250,28 @@ int psr_batch_set_boundary_cfvs(psr_batch* batch, int game, int ordinal, int pla
                                 int raw);
 int psr_batch_update_leaf_cfvs(psr_batch* batch, const float* p0, int64_t n0, const float* p1, int64_t n1);
 int psr_batch_boundary_reach(psr_batch* batch, int game, int ordinal, int player, float* out, int n);
+/* Hand strides of the next three calls: 1326 = values indexed by
+ * card_pair_to_index (the reference's canonical hand layout, blocked combos
+ * 0); any other stride >= every game's hand count = values in the game's own
+ * hand order, rows of `hand_stride` values. */
+/* batch.rs:279-300 set_leaf_injection(node_ids, depths): the leaves (arena
+ * indices = the reference's node ids, with their depths, checked) whose
+ * values psr_batch_update_leaf_cfvs then takes in the reference's layout
+ * [game][leaf, in the order given][hand_stride] per player (batch.rs:557-585),
+ * written raw.  Every game must have a depth-boundary leaf at each node id;
+ * may be called before or after psr_batch_prepare (checked at both). */
+int psr_batch_set_leaf_injection(psr_batch* batch, const int32_t* node_ids, const int32_t* depths, int n_leaves,
+                                 int32_t hand_stride);
+/* batch.rs:537-546 download_reach: the reach rows of the last traversal
+ * (player 1's, so player 0's reach under the current strategy, chance
+ * scaling included) at every node: [game][node][hand_stride], games in batch
+ * order, zeros before any iteration. */
+int psr_batch_download_reach(psr_batch* batch, float* out, int64_t cap, int32_t hand_stride);
+/* batch.rs:587-609 extract_results: one game's strategy sums in the
+ * reference's edge layout [edge][hand_stride] (edge e = the action leading to
+ * node e + 1; chance edges 0), the input of compute_evs_from_strategy_sum
+ * (batch.rs:799-933). */
+int psr_batch_strategy_sum_edges(psr_batch* batch, int game, float* out, int64_t cap, int32_t hand_stride);
 /* Reach of `player` at arena nodes `nodes` under the average strategy, H_player
  * values per node into out (n_nodes * H_player): gpu-range-solver
  * batch.rs:765-797 compute_reach_at_nodes (its result[p] is the reach of the

@@ -740,8 +740,9 @@ struct Solver {
         }
     }
 
-    // solver.rs:108-176 (finalize's cfvalue pass is read back on demand via
-    // root_cfvalues, which is the same recursion)
+    // solver.rs:108-176, ending with finalize (utility.rs:348-372: the
+    // cfvalue pass of player 0, then player 1; it also records the reach at
+    // depth-boundary leaves; root_cfvalues reruns the same recursion)
     T solve(uint32_t max_iter, T target, uint32_t* iters_run) {
         T expl = exploitability();
         uint32_t t = 0;
@@ -751,6 +752,10 @@ struct Solver {
             if ((t + 1) % 5 == 0 || t + 1 == max_iter) expl = exploitability();
         }
         if (iters_run) *iters_run = t;
+        for (int p = 0; p < 2; ++p) {
+            std::vector<T> cf(g.H(p));
+            root_cfvalues(p, cf.data());
+        }
         return expl;
     }
 
@@ -937,6 +942,13 @@ float oracle_encode_unsigned(const float* src, size_t n, uint16_t* dst) { return
 
 int oracle_num_decisions(void* hv) { return static_cast<Handle*>(hv)->game.num_decisions; }
 
+// player field of a node (flags TERMINAL / CHANCE / FOLD / BOUNDARY and the
+// acting player), -1 past the last node
+int oracle_node_player(void* hv, uint32_t node) {
+    const Game& g = static_cast<Handle*>(hv)->game;
+    return node < g.nodes.size() ? (int)g.nodes[node].player : -1;
+}
+
 void oracle_game_free(void* h) { delete static_cast<Handle*>(h); }
 
 int64_t oracle_storage_size(void* hv) { return static_cast<Handle*>(hv)->game.storage_size; }
@@ -1122,7 +1134,7 @@ int oracle_num_boundaries(void* hv) { return (int)static_cast<Handle*>(hv)->game
 // solve per thread at a time (the reference runs river subgames single-
 // threaded, node.rs:347-349).  Returns wall seconds; writes exploitability.
 double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n, uint32_t iters, int threads,
-                         const double* targets, double* expl_out, uint32_t* iters_out, int finalize) {
+                         const double* targets, double* expl_out, uint32_t* iters_out) {
     std::atomic<int> next{0};
     auto t0 = std::chrono::steady_clock::now();
     auto work = [&]() {
@@ -1135,11 +1147,6 @@ double oracle_solve_many(const uint8_t* const* blobs, const size_t* lens, int n,
             uint32_t run = 0;
             expl_out[i] = (double)s.solve(iters, targets ? (float)targets[i] : 0.0f, &run);
             if (iters_out) iters_out[i] = run;
-            if (finalize) {   // solver.rs:171 finalize: the root cfvalues of both players
-                std::vector<float> cf0(g.H(0)), cf1(g.H(1));
-                s.root_cfvalues(0, cf0.data());
-                s.root_cfvalues(1, cf1.data());
-            }
         }
     };
     std::vector<std::thread> pool;

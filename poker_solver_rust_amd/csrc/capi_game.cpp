@@ -217,6 +217,21 @@ int psr_game_boundary_pot(const psr_game* g, int ordinal) {
     return g->game.boundary_pot(ordinal);
 }
 
+int psr_game_topology(const psr_game* g, int32_t* parent, int32_t* depth, int64_t n) {
+    if (!g) return fail("null game");
+    const auto& nodes = g->game.nodes();
+    if (n != (int64_t)nodes.size() || !parent || !depth) return fail("parent and depth must hold one value per node");
+    parent[0] = -1;
+    depth[0] = 0;
+    for (size_t i = 0; i < nodes.size(); ++i)   // breadth-first: a parent precedes its children
+        for (uint32_t a = 0; a < nodes[i].num_children; ++a) {
+            const size_t c = nodes[i].first_child + a;
+            parent[c] = (int32_t)i;
+            depth[c] = depth[i] + 1;
+        }
+    return 0;
+}
+
 int64_t psr_game_serialize(const psr_game* g, uint8_t* buf, int64_t cap) {
     if (!g) return fail("null game");
     std::vector<uint8_t> b = g->game.serialize();

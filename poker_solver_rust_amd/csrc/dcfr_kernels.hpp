@@ -1064,11 +1064,10 @@ __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int
     FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
 }
 
-// CHANCE: the tile may hold chance nodes whose sums it computes (a flop
-// game's turn tiles); river games and river subtree tiles hold none, and the
-// root tiles' chance nodes are summed by PHASE_ROOT_LEAVES, so leaving the
-// chance sums out keeps those kernels' register count down.
-template <int MODE, int SM, typename V = float, bool CHANCE = false>
+// A tile's chance nodes are not summed here: the root and middle tiles'
+// chance sums run in PHASE_ROOT_LEAVES / PHASE_MID_LEAVES, and river games
+// and subtree tiles hold none (build_layout checks it).
+template <int MODE, int SM, typename V = float>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p);
@@ -1120,8 +1119,7 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                 }
             }
             if (nd.kind == KIND_CHANCE) {
-                // root tiles' chance nodes are summed by PHASE_ROOT_LEAVES
-                if constexpr (CHANCE) chance_sum<V>(c, nd, n, p);
+                // summed by PHASE_ROOT_LEAVES / PHASE_MID_LEAVES
             } else if (nd.player == p && na > 1) {
                 if constexpr (sizeof(V) == 8 && MODE == MODE_CFV) {
                     // FP64 value of the average strategy: sum_a sigma_a * cfv_a,
@@ -1307,11 +1305,11 @@ __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p)
     __syncthreads();
 }
 
-template <int MODE, int SM, typename V = float, bool CHANCE = false>
+template <int MODE, int SM, typename V = float>
 __device__ __forceinline__ void tile_traverse(Ctx& c, const TileDesc& T, int p, const float4 disc) {
     tile_forward<MODE, SM, V>(c, T, p);
     tile_terminals<V>(c, T, p);
-    tile_backward<MODE, SM, V, CHANCE>(c, T, p, disc);
+    tile_backward<MODE, SM, V>(c, T, p, disc);
 }
 
 __device__ __forceinline__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
@@ -1487,7 +1485,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
 // The host issues ROOT_FORWARD, MID_FORWARD, SUBTREES, MID_LEAVES,
 // MID_BACKWARD, ROOT_LEAVES, ROOT_BACKWARD per traversing player (the middle
 // phases only when some game has middle tiles).  In
-// DCFR mode a game whose expl_out is already <= target is skipped (the
+// DCFR mode a game whose phase_expl is already <= target is skipped (the
 // reference solve() stops iterating a converged game).
 template <int MODE, int SM, typename V>
 __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& sh) {
@@ -1498,13 +1496,13 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {
             const int gi = uni(A.multi_games[k]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
-            if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
+            if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
             const TileDesc T = uni_load(A.tiles + c.G.tile_base);
             if (A.phase == PHASE_ROOT_FORWARD) {
                 tile_forward<MODE, SM, V>(c, T, p);
             } else {
                 // terminal and chance rows: PHASE_ROOT_LEAVES
-                tile_backward<MODE, SM, V, false>(c, T, p, disc);
+                tile_backward<MODE, SM, V>(c, T, p, disc);
                 if (MODE == MODE_BR || MODE == MODE_CFV) {
                     const double v = root_value<V>(c, p);
                     if (threadIdx.x == 0) A.ev_out[2 * gi + p] = v;
@@ -1525,7 +1523,7 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         for (int k = blockIdx.x; k < count; k += gridDim.x) {
             const int gi = uni(list[2 * k]), n = uni(list[2 * k + 1]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
-            if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
+            if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
             const NodeRec nd = uni_load(c.nodes + n);
             if (nd.kind == KIND_CHANCE) {
                 chance_sum<V>(c, nd, n, p);
@@ -1548,9 +1546,9 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         for (int k = blockIdx.x; k < A.num_mid_tiles; k += gridDim.x) {
             const TileDesc T = uni_load(A.tiles + uni(A.mid_tiles[k]));
             Ctx c = make_ctx(A, T.game, scan, rs, &sh);
-            if (MODE == MODE_DCFR && uni(A.expl_out[T.game]) <= c.G.target) continue;
+            if (MODE == MODE_DCFR && uni(A.phase_expl[T.game]) <= c.G.target) continue;
             if (A.phase == PHASE_MID_FORWARD) tile_forward<MODE, SM, V>(c, T, p);
-            else tile_backward<MODE, SM, V, false>(c, T, p, disc);
+            else tile_backward<MODE, SM, V>(c, T, p, disc);
             __syncthreads();
         }
         return;
@@ -1566,9 +1564,8 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         const int t = uni(A.subtree_tiles[2 * q]), gi = uni(A.subtree_tiles[2 * q + 1]);
         Ctx c = make_ctx(A, gi, scan, rs, &sh);
         const TileDesc T = uni_load(A.tiles + t);
-        if (MODE == MODE_DCFR && uni(A.expl_out[gi]) <= c.G.target) continue;
-        if (T.flags & TILE_HAS_CHANCE) tile_traverse<MODE, SM, V, true>(c, T, p, disc);
-        else tile_traverse<MODE, SM, V>(c, T, p, disc);
+        if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
+        tile_traverse<MODE, SM, V>(c, T, p, disc);
     }
 }
 

@@ -72,6 +72,8 @@ struct KernelArgs {
     const float4* discount;   // per iteration: alpha, beta, gamma, 0
     float* expl_out;
     uint32_t* iters_out;
+    const float* phase_expl;  // multi-tile games: exploitability the phase launches compare with the
+                              // target (+inf = still iterating); kept apart from expl_out
     float* root_cfv;          // indexed like the hand arrays
     int* queue_counter;
     const int* queue_order;   // single-tile games (whole-game solve), largest first

@@ -64,6 +64,10 @@ struct HostLayout {
     int64_t bnd_rows_total = 0;      // floats of the boundary value / reach rows
     std::vector<int64_t> root_act_off;   // per game: root per-action cfvalues (-1: root not a decision)
     std::vector<std::vector<int32_t>> node_pos;   // per game: device-local node id of every arena node
+    // per game and arena node: parent and first child (arena indices, -1 if none) -- the
+    // arena is breadth-first with children contiguous, so it numbers nodes as the
+    // reference's extract_topology BFS does and edge e leads to node e + 1
+    std::vector<std::vector<int32_t>> arena_parent, arena_first;
     std::vector<int32_t> reach_src;      // per device node and traversing player: node whose reach row it uses
     int64_t root_act_total = 0;
     // per game: what the expected-value queries need about the root
@@ -389,6 +393,11 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
             std::vector<int> parent(nodes.size(), -1);
             for (size_t i = 0; i < nodes.size(); ++i)
                 for (int a = 0; a < (int)nodes[i].num_children; ++a) parent[nodes[i].first_child + a] = (int)i;
+            L.arena_parent.emplace_back(parent.begin(), parent.end());
+            std::vector<int32_t> first(nodes.size(), -1);
+            for (size_t i = 0; i < nodes.size(); ++i)
+                if (nodes[i].num_children) first[i] = (int32_t)nodes[i].first_child;
+            L.arena_first.push_back(std::move(first));
             const size_t base = L.reach_src.size();
             L.reach_src.resize(base + 2 * nodes.size());
             for (size_t k = 0; k < plan.order.size(); ++k) {   // breadth-first: parents first
@@ -494,6 +503,12 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         cost[gi] = (double)off + (double)nodes.size() * d.row_stride;
         L.games.push_back(d);
     }
+    // the whole-game kernel and the subtree work queue sum no chance nodes
+    // (their chance sums would read rows other workgroups are writing)
+    for (int gi : L.single_games)
+        if (L.tiles[L.games[gi].tile_base].flags & TILE_HAS_CHANCE) return fail("single-tile game holds a chance node");
+    for (auto& sc : L.subtree_cost)
+        if (L.tiles[sc.second].flags & TILE_HAS_CHANCE) return fail("subtree tile holds a chance node");
     std::stable_sort(L.single_games.begin(), L.single_games.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     std::stable_sort(L.subtree_cost.begin(), L.subtree_cost.end(),
                      [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first > b.first; });
@@ -504,19 +519,36 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
     return 0;
 }
 
+// One device buffer of a prepared batch: its pointer slot, its size and the
+// host data uploaded into it (none: zero-filled or written by the kernels).
+struct DevBuf {
+    void** slot;
+    size_t bytes;
+    const void* host;
+    size_t host_bytes;
+};
 template <class T>
-int upload(T** dptr, const std::vector<T>& v) {
-    size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
-    HIP_OK(hipMalloc((void**)dptr, bytes));
-    if (!v.empty()) HIP_OK(hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return 0;
+DevBuf dev_buf(T** slot, const std::vector<T>& v) {
+    return {(void**)slot, std::max<size_t>(v.size(), 1) * sizeof(T), v.data(), v.size() * sizeof(T)};
 }
+template <class T>
+DevBuf dev_buf_n(T** slot, size_t n) {
+    return {(void**)slot, std::max<size_t>(n, 1) * sizeof(T), nullptr, 0};
+}
+// bytes a buffer takes in an arena (every buffer 256-byte aligned)
+size_t arena_bytes(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
 }  // namespace
 
 struct psr_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    // device memory given from outside (the allocation-free entry point):
+    // buffers are carved from it instead of hipMalloc'd, and never freed
+    struct Arena {
+        char* base = nullptr;
+        size_t cap = 0, used = 0;
+    } arena;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     HostLayout L;
     bool prepared = false;
@@ -546,6 +578,7 @@ struct psr_batch {
     int max_blocks_per_cu = 1;   // persistent workgroups per CU (capped; see psr_batch_prepare)
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
+    float* d_phase_expl = nullptr;   // multi-tile games: skip flags of the phase launches
     uint32_t* d_iters = nullptr;
     float* d_root_cfv = nullptr;
     int* d_counter = nullptr;
@@ -553,6 +586,13 @@ struct psr_batch {
     int *d_mid_tiles = nullptr, *d_mid_leaves = nullptr;
     double* d_ev = nullptr;
     int32_t* d_reach_src = nullptr;
+    // leaf injection (psr_batch_set_leaf_injection): leaf nodes (arena indices), their
+    // depths and the hand stride of update_leaf_cfvs' value arrays
+    std::vector<int32_t> inject_nodes, inject_depths;
+    int32_t inject_stride = 0;
+    // host mirror of the boundary value rows and modes (one upload per update)
+    std::vector<float> h_bnd_cfv;
+    std::vector<int32_t> h_bnd_mode;
     float *d_bnd_cfv = nullptr, *d_bnd_reach = nullptr;
     int32_t* d_bnd_mode = nullptr;
     float* d_root_action_cfv = nullptr;
@@ -561,11 +601,12 @@ struct psr_batch {
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
-                        d_reach, d_cfv, d_discount, d_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
+                        d_reach, d_cfv, d_discount, d_expl, d_phase_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
                         d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves};
         for (void* p : ptrs)
-            if (p) (void)hipFree(p);
+            if (p && !arena.base) (void)hipFree(p);
+        arena.used = 0;
         d_games = nullptr; d_tiles = nullptr; d_levels = nullptr; d_nodes = nullptr; d_payoff = nullptr;
         d_cards = nullptr; d_same = nullptr; d_weights = nullptr; d_terms = nullptr; d_runouts = nullptr;
         d_reach_src = nullptr;
@@ -575,11 +616,22 @@ struct psr_batch {
         d_chance = nullptr; d_iso_ref = nullptr; d_iso_perm = nullptr; d_u16 = nullptr; d_hand_scan = nullptr;
         d_regret = nullptr; d_strategy = nullptr; d_regret16 = nullptr; d_strategy16 = nullptr; d_rscale = nullptr;
         d_sscale = nullptr; d_reach64 = nullptr; d_cfv64 = nullptr; d_expl64 = nullptr; d_reach = nullptr; d_cfv = nullptr; d_discount = nullptr;
-        d_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
+        d_expl = nullptr; d_phase_expl = nullptr; d_iters = nullptr; d_root_cfv = nullptr; d_counter = nullptr; d_single = nullptr;
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         d_bnd_cfv = nullptr; d_bnd_reach = nullptr; d_bnd_mode = nullptr;
         d_root_action_cfv = nullptr; d_root_act_off = nullptr;
         prepared = false;
+    }
+
+    int dalloc(void** p, size_t bytes) {
+        if (!arena.base) {
+            HIP_OK(hipMalloc(p, bytes));
+            return 0;
+        }
+        if (arena.used + arena_bytes(bytes) > arena.cap) return fail("allocation-free entry point: device memory exhausted");
+        *p = arena.base + arena.used;
+        arena.used += arena_bytes(bytes);
+        return 0;
     }
 
     KernelArgs args() const {
@@ -617,6 +669,7 @@ struct psr_batch {
         a.discount = d_discount;
         a.expl_out = d_expl;
         a.iters_out = d_iters;
+        a.phase_expl = d_phase_expl;
         a.root_cfv = d_root_cfv;
         a.queue_counter = d_counter;
         a.queue_order = d_single;
@@ -662,9 +715,9 @@ struct psr_batch {
     int ensure64() {
         if (d_reach64) return 0;
         const size_t rows = std::max<int64_t>(L.scratch_total, 1);
-        HIP_OK(hipMalloc((void**)&d_reach64, rows * sizeof(double)));
-        HIP_OK(hipMalloc((void**)&d_cfv64, rows * sizeof(double)));
-        HIP_OK(hipMalloc((void**)&d_expl64, std::max<size_t>(L.games.size(), 1) * sizeof(double)));
+        if (dalloc((void**)&d_reach64, rows * sizeof(double)) || dalloc((void**)&d_cfv64, rows * sizeof(double)) ||
+            dalloc((void**)&d_expl64, std::max<size_t>(L.games.size(), 1) * sizeof(double)))
+            return -1;
         return 0;
     }
 
@@ -720,8 +773,9 @@ struct psr_batch {
         return 0;
     }
 
+    // the phase launches' skip flags (the published d_expl is left alone)
     int set_expl(const std::vector<float>& host_expl) {
-        HIP_OK(hipMemcpyAsync(d_expl, host_expl.data(), host_expl.size() * sizeof(float), hipMemcpyHostToDevice,
+        HIP_OK(hipMemcpyAsync(d_phase_expl, host_expl.data(), host_expl.size() * sizeof(float), hipMemcpyHostToDevice,
                               stream));
         return 0;
     }
@@ -732,7 +786,7 @@ struct psr_batch {
     int run(int flags, uint32_t begin, uint32_t end, int check_interval, uint32_t max_iters) {
         if (!prepared) return fail("psr_batch_prepare has not been called");
         HIP_OK(hipSetDevice(device));
-        HIP_OK(hipEventRecord(ev0, stream));
+        if (ev0) HIP_OK(hipEventRecord(ev0, stream));
         if (!L.multi_games.empty()) {
             const float inf = INFINITY;
             std::vector<float> expl(L.games.size(), inf);
@@ -812,9 +866,9 @@ struct psr_batch {
             a.player = reach_player;
             HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
         }
-        HIP_OK(hipEventRecord(ev1, stream));
+        if (ev1) HIP_OK(hipEventRecord(ev1, stream));
         HIP_OK(hipStreamSynchronize(stream));
-        HIP_OK(hipEventElapsedTime(&last_ms, ev0, ev1));
+        if (ev0 && ev1) HIP_OK(hipEventElapsedTime(&last_ms, ev0, ev1));
         return 0;
     }
 };
@@ -827,6 +881,60 @@ static float4 discount_params(uint32_t t) {
     double pa = ta * std::sqrt(ta);
     double x = tg / (tg + 1.0);
     return make_float4((float)(pa / (pa + 1.0)), 0.5f, (float)(x * x * x), 0.0f);
+}
+
+// Position of hand h of player p of game gi in a caller's value array of
+// `stride` values per row: the hand index itself, or its card_pair_to_index
+// combo when stride is 1326 (every game has fewer hands than that).
+static int hand_slot(const HostLayout& L, int gi, int p, int h, int stride) {
+    if (stride != NUM_COMBOS) return h;
+    const uint16_t c = L.cards[L.games[gi].hand_base[p] + h];
+    return card_pair_to_index((Card)(c & 0xFF), (Card)(c >> 8));
+}
+
+static int check_stride(const HostLayout& L, int32_t stride) {
+    if (stride == NUM_COMBOS) return 0;
+    if (stride < L.max_hands) return fail("hand stride must be 1326 (combo space) or at least the largest range");
+    return 0;
+}
+
+// Arena depth of every node of game gi (root 0).
+static std::vector<int32_t> node_depths(const HostLayout& L, int gi) {
+    const auto& par = L.arena_parent[gi];
+    std::vector<int32_t> d(par.size(), 0);
+    for (size_t i = 1; i < par.size(); ++i) d[i] = d[par[i]] + 1;   // parents precede children
+    return d;
+}
+
+// Every injected leaf must be a depth-boundary leaf of every game, at the
+// depth given for it.
+static int check_injection(psr_batch* b) {
+    if (!b->prepared || b->inject_nodes.empty()) return 0;
+    const HostLayout& L = b->L;
+    if (check_stride(L, b->inject_stride)) return -1;
+    for (size_t gi = 0; gi < L.games.size(); ++gi) {
+        const std::vector<int32_t> depth = node_depths(L, (int)gi);
+        for (size_t li = 0; li < b->inject_nodes.size(); ++li) {
+            const int32_t n = b->inject_nodes[li];
+            if (n < 0 || n >= (int32_t)depth.size()) return fail("leaf node id out of range");
+            const NodeRec& r = L.nodes[L.games[gi].node_base + L.node_pos[gi][n]];
+            if (r.kind != KIND_BOUNDARY) return fail("leaf node " + std::to_string(n) + " is not a depth-boundary leaf");
+            if (depth[n] != b->inject_depths[li])
+                return fail("leaf node " + std::to_string(n) + " lies at depth " + std::to_string(depth[n]) + ", not " +
+                            std::to_string(b->inject_depths[li]));
+        }
+    }
+    return 0;
+}
+
+static int upload_boundaries(psr_batch* b) {
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipMemcpyAsync(b->d_bnd_cfv, b->h_bnd_cfv.data(), b->h_bnd_cfv.size() * sizeof(float),
+                          hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_bnd_mode, b->h_bnd_mode.data(), b->h_bnd_mode.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return 0;
 }
 
 extern "C" {
@@ -869,46 +977,57 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     b->L = HostLayout{};
     if (build_layout(games, n, targets, b->L) != 0) return -1;
     HostLayout& L = b->L;
-    if (upload(&b->d_games, L.games) || upload(&b->d_tiles, L.tiles) || upload(&b->d_levels, L.level_ranges) ||
-        upload(&b->d_nodes, L.nodes) || upload(&b->d_payoff, L.payoff) || upload(&b->d_cards, L.cards) ||
-        upload(&b->d_same, L.same) || upload(&b->d_weights, L.weights) || upload(&b->d_terms, L.terminals) ||
-        upload(&b->d_runouts, L.runouts) || upload(&b->d_chance, L.chance) || upload(&b->d_iso_ref, L.iso_ref) ||
-        upload(&b->d_iso_perm, L.iso_perm) || upload(&b->d_u16, L.u16) || upload(&b->d_hand_scan, L.hand_scan) ||
-        upload(&b->d_single, L.single_games) || upload(&b->d_multi, L.multi_games) ||
-        upload(&b->d_subtrees, L.subtree_tiles) || upload(&b->d_reach_src, L.reach_src) ||
-        upload(&b->d_root_leaves, L.root_leaves) || upload(&b->d_mid_tiles, L.mid_tiles) ||
-        upload(&b->d_mid_leaves, L.mid_leaves))
-        return -1;
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
-    if (upload(&b->d_discount, disc)) return -1;
     b->max_iterations = max_iterations;
+    // every device buffer of the batch, with the host data uploaded into it
+    const size_t nd = (size_t)L.num_decisions;
+    const size_t state = (size_t)L.state_total, rows = (size_t)L.scratch_total;
+    const size_t brows = std::max<int64_t>(L.bnd_rows_total, 1), nbnd = std::max<int32_t>(L.num_boundaries, 1);
+    std::vector<DevBuf> bufs = {
+        dev_buf(&b->d_games, L.games), dev_buf(&b->d_tiles, L.tiles), dev_buf(&b->d_levels, L.level_ranges),
+        dev_buf(&b->d_nodes, L.nodes), dev_buf(&b->d_payoff, L.payoff), dev_buf(&b->d_cards, L.cards),
+        dev_buf(&b->d_same, L.same), dev_buf(&b->d_weights, L.weights), dev_buf(&b->d_terms, L.terminals),
+        dev_buf(&b->d_runouts, L.runouts), dev_buf(&b->d_chance, L.chance), dev_buf(&b->d_iso_ref, L.iso_ref),
+        dev_buf(&b->d_iso_perm, L.iso_perm), dev_buf(&b->d_u16, L.u16), dev_buf(&b->d_hand_scan, L.hand_scan),
+        dev_buf(&b->d_single, L.single_games), dev_buf(&b->d_multi, L.multi_games),
+        dev_buf(&b->d_subtrees, L.subtree_tiles), dev_buf(&b->d_reach_src, L.reach_src),
+        dev_buf(&b->d_root_leaves, L.root_leaves), dev_buf(&b->d_mid_tiles, L.mid_tiles),
+        dev_buf(&b->d_mid_leaves, L.mid_leaves), dev_buf(&b->d_discount, disc),
+        dev_buf(&b->d_root_act_off, L.root_act_off),
+        dev_buf_n(&b->d_reach, rows), dev_buf_n(&b->d_cfv, rows), dev_buf_n(&b->d_expl, (size_t)n),
+        dev_buf_n(&b->d_phase_expl, (size_t)n), dev_buf_n(&b->d_iters, (size_t)n),
+        dev_buf_n(&b->d_root_cfv, L.cards.size()), dev_buf_n(&b->d_counter, 2), dev_buf_n(&b->d_ev, 2 * (size_t)n),
+        dev_buf_n(&b->d_bnd_cfv, brows), dev_buf_n(&b->d_bnd_reach, brows), dev_buf_n(&b->d_bnd_mode, nbnd),
+        dev_buf_n(&b->d_root_action_cfv, (size_t)std::max<int64_t>(L.root_act_total, 1)),
+    };
     if (b->storage == SM_F32) {
-        HIP_OK(hipMalloc((void**)&b->d_regret, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
-        HIP_OK(hipMalloc((void**)&b->d_strategy, std::max<int64_t>(L.state_total, 1) * sizeof(float)));
+        bufs.push_back(dev_buf_n(&b->d_regret, state));
+        bufs.push_back(dev_buf_n(&b->d_strategy, state));
     } else {
-        HIP_OK(hipMalloc((void**)&b->d_regret16, std::max<int64_t>(L.state_total, 2) * sizeof(uint16_t)));
-        HIP_OK(hipMalloc((void**)&b->d_strategy16, std::max<int64_t>(L.state_total, 2) * sizeof(uint16_t)));
-        HIP_OK(hipMalloc((void**)&b->d_rscale, std::max<int32_t>(L.num_decisions, 1) * sizeof(float)));
-        HIP_OK(hipMalloc((void**)&b->d_sscale, std::max<int32_t>(L.num_decisions, 1) * sizeof(float)));
+        bufs.push_back(dev_buf_n(&b->d_regret16, std::max<size_t>(state, 2)));
+        bufs.push_back(dev_buf_n(&b->d_strategy16, std::max<size_t>(state, 2)));
+        bufs.push_back(dev_buf_n(&b->d_rscale, nd));
+        bufs.push_back(dev_buf_n(&b->d_sscale, nd));
     }
-    HIP_OK(hipMalloc((void**)&b->d_reach, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_cfv, std::max<int64_t>(L.scratch_total, 1) * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_expl, n * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_iters, n * sizeof(uint32_t)));
-    HIP_OK(hipMalloc((void**)&b->d_root_cfv, std::max<size_t>(L.cards.size(), 1) * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_counter, 2 * sizeof(int)));
-    HIP_OK(hipMalloc((void**)&b->d_ev, 2 * n * sizeof(double)));
-    const int64_t brows = std::max<int64_t>(L.bnd_rows_total, 1);
-    const int64_t nbnd = std::max<int32_t>(L.num_boundaries, 1);
-    HIP_OK(hipMalloc((void**)&b->d_bnd_cfv, brows * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_bnd_reach, brows * sizeof(float)));
-    HIP_OK(hipMalloc((void**)&b->d_bnd_mode, nbnd * sizeof(int32_t)));
+    if (b->arena.base) {
+        size_t need = 0;
+        for (const DevBuf& d : bufs) need += arena_bytes(d.bytes);
+        if (need > b->arena.cap)
+            return fail("the game needs " + std::to_string(need) + " bytes of device memory; the allocation-free "
+                        "entry point holds at most " + std::to_string(b->arena.cap));
+    }
+    for (const DevBuf& d : bufs) {
+        if (b->dalloc(d.slot, d.bytes)) return -1;
+        if (d.host_bytes)
+            HIP_OK(hipMemcpyAsync(*d.slot, d.host, d.host_bytes, hipMemcpyHostToDevice, b->stream));
+    }
+    HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_counter), 2, b->stream));
+    b->h_bnd_cfv.assign(brows, 0.0f);
+    b->h_bnd_mode.assign(nbnd, BND_UNSET);
     HIP_OK(fill_zero(b->d_bnd_cfv, brows, b->stream));
     HIP_OK(fill_zero(b->d_bnd_reach, brows, b->stream));
     HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_bnd_mode), nbnd, b->stream));   // BND_UNSET
-    if (upload(&b->d_root_act_off, L.root_act_off)) return -1;
-    HIP_OK(hipMalloc((void**)&b->d_root_action_cfv, std::max<int64_t>(L.root_act_total, 1) * sizeof(float)));
     HIP_OK(fill_zero(b->d_root_action_cfv, std::max<int64_t>(L.root_act_total, 1), b->stream));
     if (b->zero_state()) return -1;
     HIP_OK(fill_zero(b->d_reach, L.scratch_total, b->stream));
@@ -920,6 +1039,14 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     if (b->blocks_per_cu < 1) return fail("kernel does not fit on a CU");
     b->grid = b->num_cus * b->blocks_per_cu;
     b->prepared = true;
+    return check_injection(b);
+}
+
+int psr_batch_set_targets(psr_batch* b, const float* targets) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    HIP_OK(hipSetDevice(b->device));
+    for (size_t gi = 0; gi < b->L.games.size(); ++gi) b->L.games[gi].target = targets ? targets[gi] : 0.0f;
+    HIP_OK(hipMemcpy(b->d_games, b->L.games.data(), b->L.games.size() * sizeof(GameDesc), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -942,7 +1069,9 @@ int psr_batch_solve(psr_batch* b, uint32_t max_iterations, int32_t check_interva
                     uint32_t* iters_out) {
     if (!b || !b->prepared) return fail("batch not prepared");
     if (max_iterations > b->max_iterations) return fail("max_iterations exceeds the prepared discount table");
-    const int flags = JOB_INITIAL_CHECK | JOB_ITERATE | (check_interval > 0 ? 0 : JOB_FINAL_CHECK);
+    // solve() ends with finalize (solver.rs:171): the root cfvalues under the
+    // average strategy, computed in the same launch
+    const int flags = JOB_INITIAL_CHECK | JOB_ITERATE | (check_interval > 0 ? 0 : JOB_FINAL_CHECK) | JOB_ROOT_CFV;
     if (int rc = b->run(flags, 0, max_iterations, check_interval, max_iterations)) return rc;
     const size_t n = b->L.games.size();
     if (expl_out) HIP_OK(hipMemcpy(expl_out, b->d_expl, n * sizeof(float), hipMemcpyDeviceToHost));
@@ -1233,31 +1362,118 @@ int psr_batch_set_boundary_cfvs(psr_batch* b, int game, int ordinal, int player,
     if (player < 0 || player > 1) return fail("player must be 0 or 1");
     if (!cfvs || n != d.H[player]) return fail("cfvs must hold one value per private hand of the player");
     HIP_OK(hipSetDevice(b->device));
-    float* row = b->d_bnd_cfv + d.bnd_base + (int64_t)(2 * ordinal + player) * d.row_stride;
-    const int32_t mode = raw ? BND_RAW : BND_SCALED;
-    HIP_OK(hipMemcpyAsync(row, cfvs, (size_t)n * sizeof(float), hipMemcpyHostToDevice, b->stream));
-    HIP_OK(hipMemcpyAsync(b->d_bnd_mode + d.bnd_index + ordinal, &mode, sizeof(int32_t), hipMemcpyHostToDevice,
-                          b->stream));
+    const int64_t off = d.bnd_base + (int64_t)(2 * ordinal + player) * d.row_stride;
+    std::copy(cfvs, cfvs + n, b->h_bnd_cfv.begin() + off);
+    b->h_bnd_mode[d.bnd_index + ordinal] = raw ? BND_RAW : BND_SCALED;
+    HIP_OK(hipMemcpyAsync(b->d_bnd_cfv + off, cfvs, (size_t)n * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipMemcpyAsync(b->d_bnd_mode + d.bnd_index + ordinal, &b->h_bnd_mode[d.bnd_index + ordinal],
+                          sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     return 0;
 }
 
 int psr_batch_update_leaf_cfvs(psr_batch* b, const float* p0, int64_t n0, const float* p1, int64_t n1) {
     if (!b || !b->prepared) return fail("batch not prepared");
+    const HostLayout& L = b->L;
+    const float* src[2] = {p0, p1};
+    if (!b->inject_nodes.empty()) {
+        // batch.rs:557-585: [game][leaf][hand stride] values of each player for the leaves of
+        // set_leaf_injection, written raw (batch order, leaves in the order given)
+        const int64_t nl = (int64_t)b->inject_nodes.size(), st = b->inject_stride;
+        const int64_t need = (int64_t)L.games.size() * nl * st;
+        if (!p0 || !p1 || n0 != need || n1 != need)
+            return fail("leaf cfvs must hold games x injected leaves x hand stride values per player");
+        for (size_t gi = 0; gi < L.games.size(); ++gi) {
+            const GameDesc& d = L.games[gi];
+            for (int64_t li = 0; li < nl; ++li) {
+                const int ord = L.nodes[d.node_base + L.node_pos[gi][b->inject_nodes[li]]].state_off;
+                for (int p = 0; p < 2; ++p) {
+                    const float* v = src[p] + ((int64_t)gi * nl + li) * st;
+                    float* row = b->h_bnd_cfv.data() + d.bnd_base + (int64_t)(2 * ord + p) * d.row_stride;
+                    for (int h = 0; h < d.H[p]; ++h) row[h] = v[hand_slot(L, (int)gi, p, h, (int)st)];
+                }
+                b->h_bnd_mode[d.bnd_index + ord] = BND_RAW;
+            }
+        }
+        return upload_boundaries(b);
+    }
     int64_t need[2] = {0, 0};
-    for (const GameDesc& d : b->L.games)
+    for (const GameDesc& d : L.games)
         for (int p = 0; p < 2; ++p) need[p] += (int64_t)d.num_boundaries * d.H[p];
     if (!p0 || !p1 || n0 != need[0] || n1 != need[1])
         return fail("leaf cfvs must hold, per game in batch order, per boundary ordinal, one value per hand");
     int64_t off[2] = {0, 0};
-    const float* src[2] = {p0, p1};
-    for (size_t gi = 0; gi < b->L.games.size(); ++gi) {
-        const GameDesc& d = b->L.games[gi];
-        for (int o = 0; o < d.num_boundaries; ++o)
+    for (const GameDesc& d : L.games)
+        for (int o = 0; o < d.num_boundaries; ++o) {
             for (int p = 0; p < 2; ++p) {
-                if (psr_batch_set_boundary_cfvs(b, (int)gi, o, p, src[p] + off[p], d.H[p], 1)) return -1;
+                std::copy(src[p] + off[p], src[p] + off[p] + d.H[p],
+                          b->h_bnd_cfv.begin() + d.bnd_base + (int64_t)(2 * o + p) * d.row_stride);
                 off[p] += d.H[p];
             }
+            b->h_bnd_mode[d.bnd_index + o] = BND_RAW;
+        }
+    return upload_boundaries(b);
+}
+
+int psr_batch_set_leaf_injection(psr_batch* b, const int32_t* node_ids, const int32_t* depths, int n,
+                                 int32_t hand_stride) {
+    if (!b || n < 0 || (n > 0 && (!node_ids || !depths))) return fail("invalid argument");
+    if (hand_stride <= 0) return fail("hand stride must be positive");
+    b->inject_nodes.assign(node_ids, node_ids + n);
+    b->inject_depths.assign(depths, depths + n);
+    b->inject_stride = hand_stride;
+    if (int rc = check_injection(b)) {
+        b->inject_nodes.clear();
+        b->inject_depths.clear();
+        return rc;
+    }
+    return 0;
+}
+
+int psr_batch_download_reach(psr_batch* b, float* out, int64_t cap, int32_t hand_stride) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    const HostLayout& L = b->L;
+    if (check_stride(L, hand_stride)) return -1;
+    int64_t need = 0;
+    for (const GameDesc& d : L.games) need += (int64_t)d.num_nodes * hand_stride;
+    if (!out || cap < need) return fail("out must hold sum over games of nodes x hand stride values");
+    HIP_OK(hipSetDevice(b->device));
+    std::vector<float> rows(std::max<int64_t>(L.scratch_total, 1));
+    HIP_OK(hipMemcpy(rows.data(), b->d_reach, (size_t)L.scratch_total * sizeof(float), hipMemcpyDeviceToHost));
+    std::fill(out, out + need, 0.0f);
+    int64_t base = 0;
+    for (size_t gi = 0; gi < L.games.size(); ++gi) {
+        const GameDesc& d = L.games[gi];
+        for (int n = 0; n < d.num_nodes; ++n) {
+            // the row the last traversal (player 1's) used at node n: player 0's reach
+            const int src = L.reach_src[2 * ((size_t)d.node_base + L.node_pos[gi][n]) + 1];
+            const float* row = rows.data() + d.scratch_base + (int64_t)src * d.row_stride;
+            float* dst = out + base + (int64_t)n * hand_stride;
+            for (int h = 0; h < d.H[0]; ++h) dst[hand_slot(L, (int)gi, 0, h, hand_stride)] = row[h];
+        }
+        base += (int64_t)d.num_nodes * hand_stride;
+    }
+    return 0;
+}
+
+int psr_batch_strategy_sum_edges(psr_batch* b, int game, float* out, int64_t cap, int32_t hand_stride) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    const HostLayout& L = b->L;
+    if (game < 0 || game >= (int)L.games.size()) return fail("game index out of range");
+    if (check_stride(L, hand_stride)) return -1;
+    const GameDesc& d = L.games[game];
+    const int64_t edges = d.num_nodes - 1;
+    if (!out || cap < edges * hand_stride) return fail("out must hold (nodes - 1) x hand stride values");
+    std::vector<float> S((size_t)std::max<int64_t>(L.storage_size[game], 1));
+    if (int rc = psr_batch_strategy_sum(b, game, S.data(), (int64_t)S.size())) return rc;
+    std::fill(out, out + edges * hand_stride, 0.0f);
+    for (int64_t e = 0; e < edges; ++e) {
+        const int c = (int)e + 1, par = L.arena_parent[game][c];
+        const NodeRec& r = L.nodes[d.node_base + L.node_pos[game][par]];
+        if (r.kind != KIND_DECISION) continue;   // chance edges carry no strategy
+        const int p = r.player, a = c - L.arena_first[game][par];
+        const float* blk = S.data() + r.state_off + (int64_t)a * d.H[p];
+        for (int h = 0; h < d.H[p]; ++h) out[e * hand_stride + hand_slot(L, game, p, h, hand_stride)] = blk[h];
     }
     return 0;
 }
@@ -1324,8 +1540,15 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
         }
         terms += td.num_terminals;
     }
+    for (size_t k = 0; k < L.subtree_tiles.size(); k += 2)
+        if (L.tiles[L.subtree_tiles[k]].flags & TILE_HAS_CHANCE) return fail("subtree tile holds a chance node");
     for (int v = 0; v < N; ++v) {
         if (tile_of[v] < 0) return fail("node in no level");
+        // a chance node lies in the root tile or a middle tile, never in a
+        // tile of the subtree work queue
+        if (L.nodes[d.node_base + v].kind == KIND_CHANCE &&
+            !(L.tiles[d.tile_base + tile_of[v]].flags & TILE_HAS_CHANCE))
+            return fail("chance node in a tile without the chance flag");
         const NodeRec& r = L.nodes[d.node_base + v];
         const bool term = r.kind == KIND_FOLD || r.kind == KIND_SHOWDOWN || r.kind == KIND_BOUNDARY;
         if (term) {
@@ -1391,41 +1614,65 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]) {
     return 0;
 }
 
-int psr_gpu_solve(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result, float* root_strategy,
-                  int64_t cap) {
-    if (!game || !cfg || !result) return fail("null argument");
-    psr_batch* b = nullptr;
-    if (psr_batch_create(0, &b)) return -1;
+}  // extern "C"
+
+// The one-game solve of psr_gpu_solve on a batch the caller has set up.
+static int solve_one(psr_batch* b, const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result,
+                     float* root_strategy, int64_t cap) {
     float target = cfg->target_exploitability;
     int rc = psr_batch_prepare(b, &game, 1, std::max<uint32_t>(cfg->max_iterations, 1), &target);
     float expl = 0.0f;
     uint32_t iters = 0;
     if (rc == 0) rc = psr_batch_solve(b, cfg->max_iterations, cfg->check_interval > 0 ? cfg->check_interval : 5, &expl, &iters);
-    if (rc == 0) {
-        result->exploitability = expl;
-        result->iterations_run = iters;
-        if (root_strategy) {
-            const PostFlopGame& g = game_of(game);
-            const GameNode& root = g.nodes()[0];
-            int64_t n = (int64_t)root.num_children * g.num_private_hands(root.player & 1);
-            if (root.is_terminal() || root.is_chance()) n = 0;
-            if (cap < n) rc = fail("root_strategy buffer too small");
-            else if (n > 0) {
-                std::vector<float> s(b->L.storage_size[0]);
-                rc = psr_batch_strategy_sum(b, 0, s.data(), (int64_t)s.size());
-                if (rc == 0) {
-                    // normalised average strategy at the root (utility.rs:229-245)
-                    const int A = root.num_children, H = g.num_private_hands(root.player & 1);
-                    for (int h = 0; h < H; ++h) {
-                        float denom = s[h];
-                        for (int a = 1; a < A; ++a) denom += s[(size_t)a * H + h];
-                        for (int a = 0; a < A; ++a)
-                            root_strategy[(size_t)a * H + h] = denom == 0.0f ? 1.0f / (float)A : s[(size_t)a * H + h] / denom;
-                    }
-                }
-            }
-        }
+    if (rc != 0) return rc;
+    result->exploitability = expl;
+    result->iterations_run = iters;
+    if (!root_strategy) return 0;
+    const PostFlopGame& g = game_of(game);
+    const GameNode& root = g.nodes()[0];
+    int64_t n = (int64_t)root.num_children * g.num_private_hands(root.player & 1);
+    if (root.is_terminal() || root.is_chance()) n = 0;
+    if (cap < n) return fail("root_strategy buffer too small");
+    if (n == 0) return 0;
+    std::vector<float> s(b->L.storage_size[0]);
+    if ((rc = psr_batch_strategy_sum(b, 0, s.data(), (int64_t)s.size()))) return rc;
+    // normalised average strategy at the root (utility.rs:229-245)
+    const int A = root.num_children, H = g.num_private_hands(root.player & 1);
+    for (int h = 0; h < H; ++h) {
+        float denom = s[h];
+        for (int a = 1; a < A; ++a) denom += s[(size_t)a * H + h];
+        for (int a = 0; a < A; ++a)
+            root_strategy[(size_t)a * H + h] = denom == 0.0f ? 1.0f / (float)A : s[(size_t)a * H + h] / denom;
     }
+    return 0;
+}
+
+namespace psr {
+// psr_gpu_solve on `stream` of the current device with every device buffer
+// carved from [base, base + cap): no allocation, no stream or event created.
+int solve_in_arena(void* base, size_t cap, void* stream, const psr_game* game, const psr_solver_config* cfg,
+                   psr_solve_result* result, float* root_strategy, int64_t root_cap) {
+    if (!game || !cfg || !result) return fail("null argument");
+    psr_batch b;
+    HIP_OK(hipGetDevice(&b.device));
+    b.stream = static_cast<hipStream_t>(stream);
+    b.arena.base = static_cast<char*>(base);
+    b.arena.cap = cap;
+    HIP_OK(hipDeviceGetAttribute(&b.num_cus, hipDeviceAttributeMultiprocessorCount, b.device));
+    const int rc = solve_one(&b, game, cfg, result, root_strategy, root_cap);
+    b.release();
+    return rc;
+}
+}  // namespace psr
+
+extern "C" {
+
+int psr_gpu_solve(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result, float* root_strategy,
+                  int64_t cap) {
+    if (!game || !cfg || !result) return fail("null argument");
+    psr_batch* b = nullptr;
+    if (psr_batch_create(0, &b)) return -1;
+    const int rc = solve_one(b, game, cfg, result, root_strategy, cap);
     psr_batch_destroy(b);
     return rc;
 }

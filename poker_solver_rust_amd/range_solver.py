@@ -115,6 +115,7 @@ def load_library(path: Optional[str] = None):
         "psr_batch_destroy": (None, [vp]),
         "psr_batch_prepare": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.c_uint32, C.POINTER(C.c_float)]),
         "psr_batch_reset": (C.c_int, [vp]),
+        "psr_batch_set_targets": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "psr_batch_run_iterations": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
         "psr_batch_solve": (C.c_int, [vp, C.c_uint32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),
         "psr_batch_exploitability": (C.c_int, [vp, C.POINTER(C.c_float)]),
@@ -136,6 +137,11 @@ def load_library(path: Optional[str] = None):
                                                  C.c_int64]),
         "psr_batch_boundary_reach": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_batch_normalized_weights": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
+        "psr_game_topology": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64]),
+        "psr_batch_set_leaf_injection": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int,
+                                                   C.c_int32]),
+        "psr_batch_download_reach": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int64, C.c_int32]),
+        "psr_batch_strategy_sum_edges": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64, C.c_int32]),
         "psr_mlp_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint64, C.POINTER(vp)]),
         "psr_mlp_destroy": (None, [vp]),
         "psr_mlp_num_params": (C.c_int64, [vp]),
@@ -495,6 +501,17 @@ class PostFlopGame:
         """interpreter.rs:354-358."""
         return _check(load_library().psr_game_boundary_pot(self._h, ordinal))
 
+    def topology(self):
+        """(parent, depth) of every node: arena indices are the node ids of
+        gpu-range-solver's extract_topology (extract.rs:55-191) and edge e
+        leads to node e + 1."""
+        n = self.num_nodes()
+        par = np.zeros(n, np.int32)
+        dep = np.zeros(n, np.int32)
+        _check(load_library().psr_game_topology(self._h, par.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                dep.ctypes.data_as(C.POINTER(C.c_int32)), n))
+        return par, dep
+
     def validate_layout(self) -> dict:
         """Host-only check of the device layout (no GPU needed)."""
         out = (C.c_int64 * 6)()
@@ -583,6 +600,14 @@ class GpuBatchSolver:
     def reset(self):
         _check(load_library().psr_batch_reset(self._h))
 
+    def set_targets(self, targets: Optional[Sequence[float]]):
+        """Early-stop targets (chips per game; None = 0) for the next solve."""
+        t = None
+        if targets is not None:
+            self._targets = np.ascontiguousarray(targets, np.float32)
+            t = _fptr(self._targets)
+        _check(load_library().psr_batch_set_targets(self._h, t))
+
     def run_iterations(self, start: int, end: int):
         _check(load_library().psr_batch_run_iterations(self._h, start, end))
 
@@ -610,9 +635,12 @@ class GpuBatchSolver:
         strategy (also records the reach at every depth-boundary leaf)."""
         _check(load_library().psr_batch_compute_root_cfvalues(self._h))
 
-    def root_cfvalues(self) -> List[Tuple[np.ndarray, np.ndarray]]:
+    def root_cfvalues(self, compute: bool = True) -> List[Tuple[np.ndarray, np.ndarray]]:
+        """Per-hand root cfvalues of every game; compute=False reads the ones
+        the last solve() (which ends with finalize) or cfvalue pass left."""
         lib = load_library()
-        _check(lib.psr_batch_compute_root_cfvalues(self._h))
+        if compute:
+            _check(lib.psr_batch_compute_root_cfvalues(self._h))
         out = []
         for i, g in enumerate(self.games):
             pair = []
@@ -684,11 +712,40 @@ class GpuBatchSolver:
                                                           1 if raw else 0))
 
     def update_leaf_cfvs(self, p0, p1):
-        """batch.rs:557-585: raw leaf values of every boundary of every game
-        (per game in batch order, per ordinal, H_player values)."""
+        """batch.rs:557-585: raw leaf values, per player: after
+        set_leaf_injection [game][injected leaf][num_hands] as the reference
+        lays them out; otherwise every boundary of every game (per game in
+        batch order, per ordinal, H_player values)."""
         a = np.ascontiguousarray(p0, np.float32)
         b = np.ascontiguousarray(p1, np.float32)
         _check(load_library().psr_batch_update_leaf_cfvs(self._h, _fptr(a), len(a), _fptr(b), len(b)))
+
+    def set_leaf_injection(self, node_ids, depths, num_hands: int):
+        """batch.rs:279-300: leaves whose values update_leaf_cfvs then takes
+        as [game][leaf][num_hands] arrays (num_hands 1326 = combo space)."""
+        ids = np.ascontiguousarray(node_ids, np.int32)
+        dep = np.ascontiguousarray(depths, np.int32)
+        _check(load_library().psr_batch_set_leaf_injection(self._h, ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                                           dep.ctypes.data_as(C.POINTER(C.c_int32)), len(ids),
+                                                           num_hands))
+
+    def download_reach(self, num_hands: int) -> np.ndarray:
+        """batch.rs:537-546: [sum of nodes over games, num_hands] reach of
+        player 0 from the last traversal."""
+        n = sum(g.num_nodes() for g in self.games)
+        out = np.zeros(n * num_hands, np.float32)
+        _check(load_library().psr_batch_download_reach(self._h, _fptr(out), len(out), num_hands))
+        return out.reshape(n, num_hands)
+
+    def extract_results(self, num_hands: int) -> List[np.ndarray]:
+        """batch.rs:587-609: per game the strategy sums as [edges, num_hands]."""
+        lib = load_library()
+        res = []
+        for i, g in enumerate(self.games):
+            out = np.zeros((g.num_nodes() - 1) * num_hands, np.float32)
+            _check(lib.psr_batch_strategy_sum_edges(self._h, i, _fptr(out), len(out), num_hands))
+            res.append(out.reshape(-1, num_hands))
+        return res
 
     def boundary_reach(self, i: int, ordinal: int, player: int) -> np.ndarray:
         """interpreter.rs:396-403: reach of `player` recorded at the last

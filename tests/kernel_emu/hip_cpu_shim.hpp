@@ -51,6 +51,8 @@ enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 #define __shared__ static
+#define __align__(n) __attribute__((aligned(n)))
+#define HIP_SYMBOL(x) (x)
 
 namespace emu {
 // Every lane of a workgroup is a fiber (ucontext) on the calling thread.  A
@@ -315,6 +317,10 @@ using std::signbit;
 inline const char* hipGetErrorString(hipError_t e) { return e == 0 ? "hipSuccess" : "emulated HIP error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
+inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
+// a __device__ variable is an ordinary host object here
+template <class T>
+inline hipError_t hipGetSymbolAddress(void** p, T& symbol) { *p = (void*)&symbol; return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
 template <class T>

@@ -60,8 +60,9 @@ def load():
                                       C.c_size_t, C.c_size_t, C.c_float]),
         "oracle_debug_break_isomorphism": (C.c_int, [vp, C.c_int]),
         "oracle_set_threads": (None, [vp, C.c_int]),
+        "oracle_node_player": (C.c_int, [vp, C.c_uint32]),
         "oracle_solve_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
-                                           C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32), C.c_int]),
+                                           C.c_uint32, C.c_int, dp, dp, C.POINTER(C.c_uint32)]),
         "oracle_fp64_expl_many": (C.c_double, [C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_size_t), C.c_int,
                                                C.POINTER(C.c_uint32), C.c_int, dp]),
     }
@@ -203,6 +204,22 @@ class OracleGame:
     def num_boundaries(self):
         return load().oracle_num_boundaries(self._h)
 
+    def node_kinds(self):
+        """Per node (arena order): (kind, player) with kind one of
+        "decision", "chance", "fold", "showdown", "boundary"."""
+        out = []
+        n = 0
+        while True:
+            f = load().oracle_node_player(self._h, n)
+            if f < 0:
+                return out
+            if f & 8:
+                kind = "boundary" if (f & 40) == 40 else "fold" if (f & 24) == 24 else "showdown"
+            else:
+                kind = "chance" if f & 4 else "decision"
+            out.append((kind, f & 3))
+            n += 1
+
     def break_isomorphism(self, mode):
         """Plants a mistake in the isomorphic chance sums (1: no hand swaps,
         2: isomorphic cards left out); returns the entries changed."""
@@ -289,11 +306,11 @@ def _blob_array(blobs):
     return bufs, ptrs, lens
 
 
-def solve_many(blobs, iterations, threads, targets=None, finalize=False):
-    """Solves serialized games with the reference solve() schedule using
-    `threads` worker threads (one game per thread at a time); finalize adds
-    the root cfvalue pass of both players (solver.rs:171).  Returns (wall
-    seconds, exploitability array, iterations array)."""
+def solve_many(blobs, iterations, threads, targets=None):
+    """Solves serialized games with the reference solve() schedule (which
+    ends with finalize) using `threads` worker threads (one game per thread
+    at a time).  Returns (wall seconds, exploitability array, iterations
+    array)."""
     lib = load()
     n = len(blobs)
     bufs, ptrs, lens = _blob_array(blobs)
@@ -301,7 +318,7 @@ def solve_many(blobs, iterations, threads, targets=None, finalize=False):
     its = np.zeros(n, np.uint32)
     tg = None if targets is None else np.ascontiguousarray(targets, np.float64)
     secs = lib.oracle_solve_many(ptrs, lens, n, iterations, threads, None if tg is None else _dptr(tg), _dptr(expl),
-                                 its.ctypes.data_as(C.POINTER(C.c_uint32)), 1 if finalize else 0)
+                                 its.ctypes.data_as(C.POINTER(C.c_uint32)))
     del bufs
     return secs, expl, its
 

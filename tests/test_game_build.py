@@ -116,3 +116,20 @@ def test_turn_game_has_48_rivers():
     g = W.turn_uniform01(0).build()
     assert g.num_private_hands(0) == 1128
     assert g.num_nodes() > 10000
+
+
+def test_topology_parents_and_depths():
+    """psr_game_topology: breadth-first arena, children contiguous, depth =
+    parent depth + 1, so edge e of the reference topology leads to node
+    e + 1 and parents are non-decreasing along the arena."""
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK"), P.Range.parse("QQ,AKs")), flop=P.flop_from_str("Qs Jh 2c"),
+                      turn=P.card_from_str("8d"))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Turn, starting_pot=100, effective_stack=200, turn_bet_sizes=s,
+                      river_bet_sizes=s)
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    par, dep = g.topology()
+    assert par[0] == -1 and dep[0] == 0
+    assert np.all(np.diff(par[1:]) >= 0)
+    assert np.all(dep[1:] == dep[par[1:]] + 1)
+    assert dep.max() >= 4   # turn betting, chance, river betting

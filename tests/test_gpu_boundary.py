@@ -12,6 +12,7 @@ import pytest
 
 import oracle_binding as O
 import poker_solver_rust_amd as P
+from poker_solver_rust_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -94,3 +95,139 @@ def test_boundary_errors():
         b.set_boundary_cfvs(0, 0, 0, np.zeros(g.num_private_hands(0) + 1, np.float32))
     with pytest.raises(P.PsrError):
         b.boundary_reach(1, 0, 0)
+
+
+def _combo_rows(g, q, values_by_hand, stride=1326):
+    """Values in game q's hand order placed at their card_pair_to_index slots."""
+    out = np.zeros(stride, np.float32)
+    for h, (c1, c2) in enumerate(g.private_cards(q)):
+        out[P.card_pair_to_index(c1, c2)] = values_by_hand[h]
+    return out
+
+
+@pytest.mark.parametrize("stride", [1326, 0])
+def test_leaf_injection_layout_matches_per_boundary_values(stride):
+    """set_leaf_injection(node_ids, depths) + update_leaf_cfvs in the
+    reference's [game][leaf][hand] layout (batch.rs:279-300, 557-585; leaves
+    in a shuffled order, hands in the 1326-combo space or the game's own
+    order) solves exactly like the same raw values given per boundary."""
+    games = [boundary_game(), boundary_game(oop="AA,KK,AKs,AQs", ip="QQ-TT,AJs")]
+    nb = games[0].num_boundary_nodes()
+    assert games[1].num_boundary_nodes() == nb
+    nodes = np.array(games[0].boundary_node_indices(), np.int32)
+    assert nodes.tolist() == games[1].boundary_node_indices()   # the same tree
+    depth = games[0].topology()[1]
+    order = np.random.default_rng(3).permutation(nb)
+    st = stride or max(g.num_private_hands(q) for g in games for q in range(2)) + 5
+    vals = [values(g, 11 + i) for i, g in enumerate(games)]
+    a = P.GpuBatchSolver(0)
+    a.set_leaf_injection(nodes[order], depth[nodes[order]], st)
+    a.prepare_batch(games, 10)
+    flat = [np.zeros((len(games), nb, st), np.float32) for _ in range(2)]
+    for gi, g in enumerate(games):
+        for li, k in enumerate(order):
+            for q in range(2):
+                row = _combo_rows(g, q, vals[gi][k][q]) if stride == 1326 else np.pad(
+                    vals[gi][k][q], (0, st - g.num_private_hands(q)))
+                flat[q][gi, li] = row
+    a.update_leaf_cfvs(flat[0].ravel(), flat[1].ravel())
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch(games, 10)
+    for gi in range(len(games)):
+        for k in range(nb):
+            for q in range(2):
+                b.set_boundary_cfvs(gi, k, q, vals[gi][k][q], raw=True)
+    a.run_iterations(0, 10)
+    b.run_iterations(0, 10)
+    for gi in range(len(games)):
+        assert np.array_equal(a.regrets(gi), b.regrets(gi))
+        assert np.array_equal(a.strategy_sum(gi), b.strategy_sum(gi))
+
+
+def test_leaf_injection_rejects_wrong_nodes_and_depths():
+    g = boundary_game()
+    nodes = np.array(g.boundary_node_indices(), np.int32)
+    depth = g.topology()[1]
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 5)
+    with pytest.raises(P.PsrError, match="depth"):
+        b.set_leaf_injection(nodes, depth[nodes] + 1, 1326)
+    with pytest.raises(P.PsrError, match="not a depth-boundary leaf"):
+        b.set_leaf_injection([0], [0], 1326)
+    with pytest.raises(P.PsrError, match="hand stride"):
+        b.set_leaf_injection(nodes, depth[nodes], 3)
+
+
+@pytest.mark.parametrize("make", [boundary_game, lambda: W.river_uniform(5).build()])
+def test_download_reach_is_the_last_traversals_reach(make):
+    """download_reach (batch.rs:537-546) after run_iterations: player 0's
+    reach of player 1's last traversal at every node.  The root holds the
+    initial weights; a chance node's children hold its reach times 1/44; a
+    player-1 node passes its reach through; at a player-0 node the children's
+    reaches are sigma * reach, so they sum to the node's reach; the
+    1326-combo layout holds the same values."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 6)
+    b.run_iterations(0, 6)
+    H0 = g.num_private_hands(0)
+    r = b.download_reach(H0 + 3)
+    assert np.all(r[:, H0:] == 0.0)
+    r = r[:, :H0]
+    par, _ = g.topology()
+    nodes = list(range(g.num_nodes()))
+    kids = {}
+    for c in nodes[1:]:
+        kids.setdefault(int(par[c]), []).append(c)
+    assert np.array_equal(r[0], g.initial_weights(0))
+    blob_nodes = O.OracleGame(g.serialize()).node_kinds()
+    for n, cs in kids.items():
+        kind, player = blob_nodes[n]
+        if kind == "chance":
+            for c in cs:
+                assert np.array_equal(r[c], (r[n] * np.float32(1.0 / 44.0)).astype(np.float32))
+        elif player == 1 or len(cs) == 1:
+            for c in cs:
+                assert np.array_equal(r[c], r[n])
+        else:
+            tot = np.sum([r[c] for c in cs], axis=0)
+            assert np.allclose(tot, r[n], rtol=2e-6, atol=1e-30)
+    rc = b.download_reach(1326)
+    for n in (0, len(nodes) - 1):
+        assert np.array_equal(rc[n], _combo_rows(g, 0, r[n]))
+
+
+@pytest.mark.parametrize("make", [boundary_game, lambda: W.river_uniform(6).build()])
+def test_strategy_sum_edges_layout(make):
+    """extract_results' [edge][hand] layout (batch.rs:587-609): edge e is the
+    action leading to node e + 1; a decision node's actions carry its A x H
+    block of the storage-order strategy sums, chance edges are zero."""
+    g = make()
+    b = P.GpuBatchSolver(0)
+    b.prepare_batch([g], 8)
+    b.run_iterations(0, 8)
+    S = b.strategy_sum(0)
+    st = 1200
+    E = b.extract_results(st)[0]
+    par, _ = g.topology()
+    assert E.shape == (g.num_nodes() - 1, st)
+    blob_nodes = O.OracleGame(g.serialize()).node_kinds()
+    off, seen = 0, set()
+    for n in range(g.num_nodes()):
+        kind, p = blob_nodes[n]
+        if kind != "decision":
+            continue
+        cs = [c for c in range(1, g.num_nodes()) if par[c] == n]
+        H = g.num_private_hands(p)
+        for a, c in enumerate(cs):
+            assert np.array_equal(E[c - 1, :H], S[off + a * H: off + (a + 1) * H])
+            assert np.all(E[c - 1, H:] == 0.0)
+            seen.add(c - 1)
+        off += len(cs) * H
+    assert off == len(S)
+    for e in set(range(g.num_nodes() - 1)) - seen:
+        assert np.all(E[e] == 0.0)
+    Ec = b.extract_results(1326)[0]
+    for e in list(seen)[:5]:
+        kind, p = blob_nodes[int(par[e + 1])]
+        assert np.array_equal(Ec[e], _combo_rows(g, p, E[e, :g.num_private_hands(p)]))

@@ -136,7 +136,9 @@ def fp64_turn_oracle():
 C4_ITERS = 100
 
 
-@pytest.mark.parametrize("storage,gap_pot,cfv_rel", [("f32", 0.002, 0.01), ("fp16", 0.002, 0.01), ("bf16", 0.01, 0.05)])
+# measured on the device (100 iterations, pot 60): gaps +0.0006 / +0.0081 /
+# +0.0081 chips, root cfv errors 0.16% / 0.22% / 0.12% of the largest CFV
+@pytest.mark.parametrize("storage,gap_pot,cfv_rel", [("f32", 0.0005, 0.01), ("fp16", 0.0005, 0.01), ("bf16", 0.0005, 0.01)])
 def test_reduced_precision_turn_solve_against_fp64_oracle(fp64_turn_oracle, storage, gap_pot, cfv_rel):
     """Config C4 on its own workload: the turn tree of C2 with regrets and
     strategy sums in FP16 / BF16 words with per-node FP32 scales (f32 for
