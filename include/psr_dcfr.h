@@ -164,6 +164,20 @@ typedef struct psr_solve_result {
 int psr_gpu_solve(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result,
                   float* root_strategy, int64_t cap);
 
+/* No counterpart in the reference: psr_gpu_solve for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The solve runs on `stream` (a
+ * hipStream_t of the current device; NULL = the default stream), and every
+ * device buffer it uses is carved from one static device array of 32 MiB per
+ * GPU, held until the process ends: no hipMalloc / hipFree, no stream or
+ * event created.  A game whose device footprint exceeds the array is
+ * rejected with an error naming both sizes (a river subgame needs a few MiB;
+ * a full turn tree does not fit).  The array keeps the last call's contents
+ * (each call re-initialises every buffer it reads); calls that overlap on
+ * one GPU race on it.  Results are identical to psr_gpu_solve's. */
+int psr_gpu_solve_noalloc(const psr_game* game, const psr_solver_config* cfg, psr_solve_result* result,
+                          float* root_strategy, int64_t cap, void* stream);
+
 typedef struct psr_batch psr_batch;
 
 /* batch.rs:146-253 GpuBatchSolver::new: device, stream and work queue. */

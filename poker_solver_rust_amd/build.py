@@ -28,7 +28,7 @@ HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-
 HOST_SOURCES = ["game.cpp", "capi_game.cpp", "gpu_solver.cpp", "mlp.cpp"]
 # the DCFR kernels: one translation unit per storage mode plus the dispatch
 DCFR_SOURCES = ["dcfr_kernels_f32.hip", "dcfr_kernels_ref16.hip", "dcfr_kernels_fp16.hip", "dcfr_kernels_bf16.hip"]
-DEVICE_SOURCES = [*DCFR_SOURCES, "dcfr_kernels.hip", "mlp_kernels.hip"]
+DEVICE_SOURCES = [*DCFR_SOURCES, "dcfr_kernels.hip", "mlp_kernels.hip", "solve_noalloc.hip"]
 
 
 def run(cmd):
