@@ -150,6 +150,15 @@ def turn_bench(P, W, iters, seed, cpu=True):
     t = time.perf_counter()
     b.prepare_batch([g], iters + 1)
     prep = time.perf_counter() - t
+    # the same iterations with every phase launch issued from the host, for comparison
+    b.set_launch(0, 1, 1)
+    b.run_iterations(0, 1)
+    b.reset()
+    t = time.perf_counter()
+    b.run_iterations(0, iters)
+    eager = iters / (time.perf_counter() - t)
+    shape = b.autotune(10)
+    log(f"turn: autotuned launch {shape}; eager launches {eager:.1f} it/s")
     b.run_iterations(0, 1)
     b.reset()
     t = time.perf_counter()
@@ -163,6 +172,7 @@ def turn_bench(P, W, iters, seed, cpu=True):
         f"{edge_hands:.0f} edge-hands")
     out = {"value": round(iters / secs, 2), "unit": "DCFR iters/s", "iterations": iters, "nodes": g.num_nodes(),
            "edge_hands": edge_hands, "ms_per_iter": round(secs * 1e3 / iters, 4), "prepare_s": round(prep, 3),
+           "launch": shape, "eager_launches_it_per_s": round(eager, 2),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5)}}
     if cpu:
@@ -370,6 +380,8 @@ def flop_bench(args):
     iters = args.flop_iters
     b = P.GpuBatchSolver(0)
     b.prepare_batch([g], iters + 1)
+    shape = b.autotune(3)
+    log(f"flop: autotuned launch {shape}")
     b.run_iterations(0, 1)
     t = time.perf_counter()
     b.run_iterations(1, iters + 1)
@@ -388,7 +400,7 @@ def flop_bench(args):
            "higher_is_better": True, "n_gpus": 1, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"flop Qs Jh 2c, OOP {oop}, IP {ip}, pot 100, stack 200, bets 50%/all-in on "
                                   "every street, no raises", "nodes": g.num_nodes(), "edge_hands": edge_hands,
-                      "tiles": g.validate_layout()["tiles"], "iterations": iters},
+                      "tiles": g.validate_layout()["tiles"], "iterations": iters, "launch": shape},
            "ms_per_iter": round(secs * 1e3 / iters, 3),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(achieved / HBM_PEAK_GBS, 5)},
@@ -551,7 +563,8 @@ def main():
     t0 = time.time()
     solver.prepare_batch(games, args.iterations, targets if args.mode == "solve" else None)
     prepare_s = time.time() - t0
-    log(f"prepared in {prepare_s:.1f}s; launch {solver.launch_info()}")
+    shape = solver.autotune(5)
+    log(f"prepared in {prepare_s:.1f}s; launch {solver.launch_info()}, autotuned {shape}")
     interval = 5 if args.mode == "solve" else 0
 
     def step():
@@ -602,6 +615,7 @@ def main():
                                f"{'solve() to 0.3% pot, cap' if args.mode == 'solve' else 'fixed'} "
                                f"{args.iterations} DCFR iterations, then finalize",
                    "subgames": len(games), "iterations": args.iterations, "mode": args.mode, "seed": args.seed,
+                   "launch": {**solver.launch_info(), **shape},
                    "mean_edge_hands": round(float(storage.mean()), 1),
                    "mean_iterations": round(float(iters.mean()), 2),
                    "converged_frac": round(converged, 5),

@@ -302,6 +302,23 @@ float psr_batch_last_kernel_ms(psr_batch* batch);
 int psr_layout_validate(const psr_game* game, int64_t out[6]);
 /* Device properties: number of CUs and the workgroups launched. */
 int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
+/* Launch shape of a prepared batch (no reference counterpart: the reference
+ * fixes its launch shapes by hand, batch.rs:414-534, gpu.rs:379-385):
+ * graph_iterations = DCFR iterations of the multi-tile games (turn, flop)
+ * captured into one replayed hipGraph (0 = phase launches issued one by one
+ * from the host; a graph never spans an exploitability check);
+ * subtree_per_cu = workgroups per CU of their subtree work queue;
+ * blocks_per_cu = persistent whole-game workgroups per CU (1..what the
+ * kernel's registers and LDS allow).  Results do not depend on the shape. */
+int psr_batch_set_launch(psr_batch* batch, int32_t graph_iterations, int32_t subtree_per_cu, int32_t blocks_per_cu);
+/* Autotuner: times every candidate launch shape on the prepared batch over
+ * `iterations` DCFR iterations, applies the fastest and caches it per
+ * workload (batch shape, storage, device) for the process, so a later batch
+ * of the same shape gets it without timing.  Leaves the batch reset (regrets
+ * and strategy sums zero).  chosen (may be NULL) receives graph_iterations,
+ * subtree_per_cu, blocks_per_cu.  Returns 0 when tuned, 1 when the cached
+ * shape was applied. */
+int psr_batch_autotune(psr_batch* batch, uint32_t iterations, int32_t* chosen);
 
 /* ------------------------------------------------------------ value net */
 

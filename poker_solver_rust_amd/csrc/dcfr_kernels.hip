@@ -20,6 +20,24 @@ __global__ void fill_zero_kernel(float* p, size_t n) {
         p[i] = 0.0f;
 }
 
+__global__ void set_iteration_kernel(uint32_t* it, uint32_t value) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it = value;
+}
+
+__global__ void advance_iteration_kernel(uint32_t* it) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *it += 1u;
+}
+
+hipError_t set_iteration(uint32_t* it, uint32_t value, hipStream_t stream) {
+    hipLaunchKernelGGL(set_iteration_kernel, dim3(1), dim3(64), 0, stream, it, value);
+    return hipGetLastError();
+}
+
+hipError_t advance_iteration(uint32_t* it, hipStream_t stream) {
+    hipLaunchKernelGGL(advance_iteration_kernel, dim3(1), dim3(64), 0, stream, it);
+    return hipGetLastError();
+}
+
 hipError_t reset_counter(int* counter, hipStream_t stream) {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, counter);
     return hipGetLastError();

@@ -1490,7 +1490,8 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
 template <int MODE, int SM, typename V>
 __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, void* rs, Shared& sh) {
     const int p = A.player;
-    const float4 disc = MODE == MODE_DCFR ? A.discount[A.iteration] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t it = A.iteration_dev ? uni((int)*A.iteration_dev) : A.iteration;
+    const float4 disc = MODE == MODE_DCFR ? A.discount[it] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (A.phase == PHASE_ROOT_FORWARD || A.phase == PHASE_ROOT_BACKWARD) {
         if (A.phase == PHASE_ROOT_FORWARD && blockIdx.x == 0 && threadIdx.x == 0) A.queue_counter[1] = 0;
         for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {

@@ -93,6 +93,8 @@ struct KernelArgs {
     int num_mid_leaves;
     int phase, player, mode;
     uint32_t iteration;
+    const uint32_t* iteration_dev;   // non-null: the iteration index is read here instead (replayed
+                                     // graphs of phase launches advance it on the device)
     double* ev_out;           // per game and player: weighted root value (best response)
     double* expl64_out;       // per game: FP64 exploitability
     // depth-boundary leaves (GameDesc::bnd_base / bnd_index)
@@ -115,6 +117,9 @@ template <int SM>
 hipError_t dcfr_occupancy_sm(int max_hands, int* blocks_per_cu);
 // Zeroes the work-queue counter on the stream.
 hipError_t reset_counter(int* counter, hipStream_t stream);
+// Sets / advances by one the device iteration index of replayed phase graphs.
+hipError_t set_iteration(uint32_t* it, uint32_t value, hipStream_t stream);
+hipError_t advance_iteration(uint32_t* it, hipStream_t stream);
 hipError_t dcfr_launch(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_launch_phase(const KernelArgs& args, int grid, hipStream_t stream);
 hipError_t dcfr_occupancy(int max_hands, int storage, int* blocks_per_cu);

@@ -10,6 +10,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -555,6 +557,7 @@ struct psr_batch {
     int reach_player = 0;   // traversal whose forward pass JOB_REACH runs
     uint32_t max_iterations = 0;
     int grid = 0, num_cus = 0, blocks_per_cu = 0;
+    int occupancy = 0;   // whole-game workgroups per CU the kernel's registers and LDS allow
     float last_ms = 0.0f;
     // device buffers
     GameDesc* d_games = nullptr;
@@ -576,6 +579,14 @@ struct psr_batch {
     double *d_reach64 = nullptr, *d_cfv64 = nullptr, *d_expl64 = nullptr;
     int storage = SM_F32;
     int max_blocks_per_cu = 1;   // persistent workgroups per CU (capped; see psr_batch_prepare)
+    // launch shape (psr_batch_set_launch / psr_batch_autotune): DCFR iterations of multi-tile
+    // games per replayed graph (0: launches issued one by one from the host), workgroups per
+    // CU of the subtree work queue
+    int graph_iters = 5;
+    int subtree_per_cu = 1;
+    uint32_t* d_iter = nullptr;   // device iteration index of the replayed graphs
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};   // [0]: graph_iters iterations, [1]: one
+    int graph_len[2] = {0, 0};
     float4* d_discount = nullptr;
     float* d_expl = nullptr;
     float* d_phase_expl = nullptr;   // multi-tile games: skip flags of the phase launches
@@ -603,7 +614,7 @@ struct psr_batch {
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_phase_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
-                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves};
+                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves, d_iter};
         for (void* p : ptrs)
             if (p && !arena.base) (void)hipFree(p);
         arena.used = 0;
@@ -620,6 +631,8 @@ struct psr_batch {
         d_multi = nullptr; d_subtrees = nullptr; d_ev = nullptr;
         d_bnd_cfv = nullptr; d_bnd_reach = nullptr; d_bnd_mode = nullptr;
         d_root_action_cfv = nullptr; d_root_act_off = nullptr;
+        d_iter = nullptr;
+        drop_graphs();
         prepared = false;
     }
 
@@ -694,13 +707,24 @@ struct psr_batch {
     }
 
     // one phase launch of a multi-tile traversal
-    int phase(int ph, int player, int mode, uint32_t t) {
+    void drop_graphs() {
+        for (int k = 0; k < 2; ++k) {
+            if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
+            graph_exec[k] = nullptr;
+            graph_len[k] = 0;
+        }
+    }
+
+    // one phase launch of a multi-tile traversal; dev_iter: the iteration
+    // index is read from d_iter (graph capture) instead of t
+    int phase(int ph, int player, int mode, uint32_t t, bool dev_iter = false) {
         KernelArgs a = args();
         a.phase = ph;
         a.player = player;
         a.mode = mode;
         a.iteration = t;
-        int g = ph == PHASE_SUBTREES       ? std::min(a.num_subtree_tiles, grid_all())
+        a.iteration_dev = dev_iter ? d_iter : nullptr;
+        int g = ph == PHASE_SUBTREES       ? std::min(a.num_subtree_tiles, num_cus * std::max(subtree_per_cu, 1))
                 : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves, grid_all())
                 : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves, grid_all())
                 : ph == PHASE_MID_FORWARD || ph == PHASE_MID_BACKWARD ? std::min(a.num_mid_tiles, grid_all())
@@ -734,14 +758,63 @@ struct psr_batch {
         }
         return 0;
     }
-    int traverse(int player, int mode, uint32_t t) {
-        if (phase(PHASE_ROOT_FORWARD, player, mode, t)) return -1;
-        if (phase(PHASE_MID_FORWARD, player, mode, t)) return -1;
-        if (phase(PHASE_SUBTREES, player, mode, t)) return -1;
-        if (phase(PHASE_MID_LEAVES, player, mode, t)) return -1;
-        if (phase(PHASE_MID_BACKWARD, player, mode, t)) return -1;
-        if (phase(PHASE_ROOT_LEAVES, player, mode, t)) return -1;
-        return phase(PHASE_ROOT_BACKWARD, player, mode, t);
+    int traverse(int player, int mode, uint32_t t, bool dev_iter = false) {
+        if (phase(PHASE_ROOT_FORWARD, player, mode, t, dev_iter)) return -1;
+        if (phase(PHASE_MID_FORWARD, player, mode, t, dev_iter)) return -1;
+        if (phase(PHASE_SUBTREES, player, mode, t, dev_iter)) return -1;
+        if (phase(PHASE_MID_LEAVES, player, mode, t, dev_iter)) return -1;
+        if (phase(PHASE_MID_BACKWARD, player, mode, t, dev_iter)) return -1;
+        if (phase(PHASE_ROOT_LEAVES, player, mode, t, dev_iter)) return -1;
+        return phase(PHASE_ROOT_BACKWARD, player, mode, t, dev_iter);
+    }
+
+    // The phase launches of k DCFR iterations of every multi-tile game
+    // captured into one graph: the iteration index comes from d_iter, which
+    // the graph advances after each iteration (no host data, no memset nodes).
+    int build_graph(int slot, int k) {
+        if (graph_exec[slot] && graph_len[slot] == k) return 0;
+        if (graph_exec[slot]) (void)hipGraphExecDestroy(graph_exec[slot]);
+        graph_exec[slot] = nullptr;
+        hipGraph_t g = nullptr;
+        HIP_OK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (int i = 0; i < k && rc == 0; ++i) {
+            rc = traverse(0, MODE_DCFR, 0, true) || traverse(1, MODE_DCFR, 0, true);
+            if (rc == 0 && advance_iteration(d_iter, stream) != hipSuccess) rc = fail("advance_iteration launch failed");
+        }
+        const hipError_t e = hipStreamEndCapture(stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return -1;
+        }
+        if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiate(&graph_exec[slot], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        graph_len[slot] = k;
+        return 0;
+    }
+
+    // DCFR iterations [t, t + n) of every multi-tile game: replayed graphs of
+    // graph_iters iterations (then single-iteration ones), or launches issued
+    // one by one
+    int iterate_multi(uint32_t t, uint32_t n) {
+        if (graph_iters <= 0) {
+            for (uint32_t i = 0; i < n; ++i)
+                if (traverse(0, MODE_DCFR, t + i) || traverse(1, MODE_DCFR, t + i)) return -1;
+            return 0;
+        }
+        HIP_OK(set_iteration(d_iter, t, stream));
+        uint32_t left = n;
+        if (left >= (uint32_t)graph_iters) {
+            if (build_graph(0, graph_iters)) return -1;
+            for (; left >= (uint32_t)graph_iters; left -= graph_iters) HIP_OK(hipGraphLaunch(graph_exec[0], stream));
+        }
+        if (left > 0) {
+            if (build_graph(1, 1)) return -1;
+            for (; left > 0; --left) HIP_OK(hipGraphLaunch(graph_exec[1], stream));
+        }
+        return 0;
     }
 
     // exploitability of every multi-tile game into host_expl (f32, as
@@ -796,16 +869,24 @@ struct psr_batch {
                 if (multi_exploitability(expl) || set_expl(expl)) return -1;
             }
             if (flags & JOB_ITERATE) {
-                for (uint32_t t = begin; t < end; ++t) {
+                // segments between exploitability checks: the active games (and so
+                // the launches) stay the same inside one
+                for (uint32_t t = begin; t < end;) {
+                    uint32_t stop = end;
+                    if (check_interval > 0) {
+                        const uint32_t next = (t / check_interval + 1) * check_interval;   // (t'+1) % ci == 0
+                        stop = std::min({end, next, std::max(max_iters, t + 1)});
+                    }
                     bool active = false;
                     for (int gi : L.multi_games)
                         if (!(expl[gi] <= L.games[gi].target)) {
                             active = true;
-                            iters[gi] = t + 1;
+                            iters[gi] = stop;
                         }
                     if (!active) break;
-                    if (traverse(0, MODE_DCFR, t) || traverse(1, MODE_DCFR, t)) return -1;
-                    if (check_interval > 0 && ((t + 1) % check_interval == 0 || t + 1 == max_iters)) {
+                    if (iterate_multi(t, stop - t)) return -1;
+                    t = stop;
+                    if (check_interval > 0 && (t % check_interval == 0 || t == max_iters)) {
                         std::vector<float> fresh(expl);
                         if (multi_exploitability(fresh)) return -1;
                         for (int gi : L.multi_games)
@@ -1000,6 +1081,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         dev_buf_n(&b->d_root_cfv, L.cards.size()), dev_buf_n(&b->d_counter, 2), dev_buf_n(&b->d_ev, 2 * (size_t)n),
         dev_buf_n(&b->d_bnd_cfv, brows), dev_buf_n(&b->d_bnd_reach, brows), dev_buf_n(&b->d_bnd_mode, nbnd),
         dev_buf_n(&b->d_root_action_cfv, (size_t)std::max<int64_t>(L.root_act_total, 1)),
+        dev_buf_n(&b->d_iter, 1),
     };
     if (b->storage == SM_F32) {
         bufs.push_back(dev_buf_n(&b->d_regret, state));
@@ -1034,7 +1116,8 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     HIP_OK(fill_zero(b->d_cfv, L.scratch_total, b->stream));
     HIP_OK(fill_zero(b->d_root_cfv, L.cards.size(), b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
-    HIP_OK(dcfr_occupancy(L.max_hands, b->storage, &b->blocks_per_cu));
+    HIP_OK(dcfr_occupancy(L.max_hands, b->storage, &b->occupancy));
+    b->blocks_per_cu = b->occupancy;
     if (b->max_blocks_per_cu > 0) b->blocks_per_cu = std::min(b->blocks_per_cu, b->max_blocks_per_cu);
     if (b->blocks_per_cu < 1) return fail("kernel does not fit on a CU");
     b->grid = b->num_cus * b->blocks_per_cu;
@@ -1490,6 +1573,80 @@ int psr_batch_boundary_reach(psr_batch* b, int game, int ordinal, int player, fl
     HIP_OK(hipMemcpyAsync(out, row, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     return 0;
+}
+
+int psr_batch_set_launch(psr_batch* b, int32_t graph_iterations, int32_t subtree_per_cu, int32_t blocks_per_cu) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (graph_iterations < 0 || graph_iterations > 64) return fail("graph iterations must be 0..64");
+    if (subtree_per_cu < 1 || subtree_per_cu > 4) return fail("subtree workgroups per CU must be 1..4");
+    if (blocks_per_cu < 1 || blocks_per_cu > b->occupancy)
+        return fail("whole-game workgroups per CU must be 1.." + std::to_string(b->occupancy) +
+                    " (registers and LDS of the kernel)");
+    if (graph_iterations != b->graph_iters) b->drop_graphs();
+    b->graph_iters = graph_iterations;
+    b->subtree_per_cu = subtree_per_cu;
+    b->blocks_per_cu = blocks_per_cu;
+    b->grid = b->num_cus * b->blocks_per_cu;
+    return 0;
+}
+
+// Launch shapes picked by the autotuner, per workload: the shape of the
+// batch (games, multi-tile games, subtree tiles, nodes, widest range,
+// storage) -> graph iterations, subtree and whole-game workgroups per CU.
+static std::mutex g_tune_lock;
+static std::map<std::string, std::array<int32_t, 3>> g_tuned;
+
+static std::string workload_key(const psr_batch* b) {
+    const HostLayout& L = b->L;
+    int64_t nodes = 0;
+    for (const GameDesc& d : L.games) nodes += d.num_nodes;
+    return std::to_string(L.games.size()) + "/" + std::to_string(L.multi_games.size()) + "/" +
+           std::to_string(L.subtree_tiles.size() / 2) + "/" + std::to_string(nodes) + "/" + std::to_string(L.max_hands) +
+           "/" + std::to_string(b->storage) + "/" + std::to_string(b->num_cus);
+}
+
+int psr_batch_autotune(psr_batch* b, uint32_t iterations, int32_t* chosen) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (iterations == 0 || iterations > b->max_iterations)
+        return fail("tuning iterations must be 1..max_iterations of psr_batch_prepare");
+    const std::string key = workload_key(b);
+    std::array<int32_t, 3> best{b->graph_iters, b->subtree_per_cu, b->blocks_per_cu};
+    bool cached = false;
+    {
+        std::lock_guard<std::mutex> guard(g_tune_lock);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        std::vector<std::array<int32_t, 3>> cand;
+        const bool multi = !b->L.multi_games.empty(), single = !b->L.single_games.empty();
+        for (int32_t gi : multi ? std::vector<int32_t>{0, 1, 5} : std::vector<int32_t>{b->graph_iters})
+            for (int32_t sp : multi ? std::vector<int32_t>{1, 2} : std::vector<int32_t>{b->subtree_per_cu})
+                for (int32_t bc = 1; bc <= (single ? b->occupancy : 1); ++bc) cand.push_back({gi, sp, single ? bc : b->blocks_per_cu});
+        float best_ms = INFINITY;
+        for (const auto& c : cand) {
+            if (psr_batch_set_launch(b, c[0], c[1], c[2])) return -1;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; ++rep) {   // the first run builds the graphs
+                if (b->zero_state()) return -1;
+                if (b->run(JOB_ITERATE, 0, iterations, 0, b->max_iterations)) return -1;
+                ms = b->last_ms;
+            }
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        std::lock_guard<std::mutex> guard(g_tune_lock);
+        g_tuned[key] = best;
+    }
+    if (psr_batch_set_launch(b, best[0], best[1], best[2]) || b->zero_state()) return -1;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    if (chosen) std::copy(best.begin(), best.end(), chosen);
+    return cached ? 1 : 0;
 }
 
 int psr_batch_launch_info(psr_batch* b, int* num_cus, int* grid, int* blocks_per_cu) {

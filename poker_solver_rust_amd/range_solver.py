@@ -127,6 +127,8 @@ def load_library(path: Optional[str] = None):
         "psr_batch_reach_at_nodes": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_int64), C.c_int,
                                      C.POINTER(C.c_float)]),
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        "psr_batch_set_launch": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+        "psr_batch_autotune": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int32)]),
         "psr_game_num_boundaries": (C.c_int, [vp]),
         "psr_game_boundary_node": (C.c_int64, [vp, C.c_int]),
         "psr_game_boundary_pot": (C.c_int, [vp, C.c_int]),
@@ -768,6 +770,20 @@ class GpuBatchSolver:
 
     def last_kernel_ms(self) -> float:
         return float(load_library().psr_batch_last_kernel_ms(self._h))
+
+    def set_launch(self, graph_iterations: int, subtree_per_cu: int, blocks_per_cu: int):
+        """Launch shape: DCFR iterations per replayed graph of the multi-tile
+        games (0 = eager launches), subtree workgroups per CU, whole-game
+        workgroups per CU."""
+        _check(load_library().psr_batch_set_launch(self._h, graph_iterations, subtree_per_cu, blocks_per_cu))
+
+    def autotune(self, iterations: int = 10) -> dict:
+        """Times the candidate launch shapes over `iterations` DCFR
+        iterations, applies the fastest (cached per workload) and resets the
+        solve; returns the shape and whether it came from the cache."""
+        out = (C.c_int32 * 3)()
+        rc = _check(load_library().psr_batch_autotune(self._h, iterations, out))
+        return {"graph_iterations": out[0], "subtree_per_cu": out[1], "blocks_per_cu": out[2], "cached": rc == 1}
 
     def launch_info(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()

@@ -212,6 +212,23 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
     }
     g_group = saved;
 }
+
+// Stream capture: launches on the capturing stream are recorded (arguments
+// copied) instead of run; a graph launch runs them in order.
+struct Graph {
+    std::vector<std::function<void()>> nodes;
+};
+inline Graph* g_capture = nullptr;
+inline void* g_capture_stream = nullptr;
+
+template <class K, class... Args>
+void launch_or_record(void* stream, K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
+    if (g_capture && stream == g_capture_stream) {
+        g_capture->nodes.push_back([=]() { launch(kernel, grid, block, lds_bytes, args...); });
+        return;
+    }
+    launch(kernel, grid, block, lds_bytes, args...);
+}
 }  // namespace emu
 
 #define threadIdx (emu::cur_thread)
@@ -224,7 +241,8 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
 #define __builtin_amdgcn_fence(...) ((void)0)
 #define __builtin_amdgcn_s_barrier() emu::syncthreads()
 #define __builtin_amdgcn_sched_barrier(mask) ((void)(mask))
-#define hipLaunchKernelGGL(k, g, b, s, st, ...) emu::launch(k, dim3(g), dim3(b), (size_t)(s), __VA_ARGS__)
+#define hipLaunchKernelGGL(k, g, b, s, st, ...) \
+    emu::launch_or_record((void*)(st), k, dim3(g), dim3(b), (size_t)(s), __VA_ARGS__)
 
 template <class T>
 inline T __shfl_up(T v, unsigned d, int = 64) {
@@ -339,6 +357,32 @@ inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind
 }
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t) { std::memset(d, v, n); return hipSuccess; }
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = (void*)1; return hipSuccess; }
+typedef emu::Graph* hipGraph_t;
+typedef emu::Graph* hipGraphExec_t;
+typedef struct hipGraphNode* hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1, hipStreamCaptureModeRelaxed = 2 };
+inline hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+    if (emu::g_capture) return (hipError_t)1;
+    emu::g_capture = new emu::Graph;
+    emu::g_capture_stream = s;
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+    if (!emu::g_capture || s != emu::g_capture_stream) return (hipError_t)1;
+    *g = emu::g_capture;
+    emu::g_capture = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+    *e = new emu::Graph(*g);
+    return hipSuccess;
+}
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    for (auto& n : e->nodes) n();
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
 inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
 inline hipError_t hipEventCreate(hipEvent_t* e) {
