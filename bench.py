@@ -553,7 +553,7 @@ def main():
 
     t0 = time.time()
     subs = W.river_batch(args.subgames, seed=args.seed, iterations=args.iterations)
-    games = [s.build() for s in subs]
+    games = W.build_all(subs)
     storage = np.array([g.num_storage() for g in games], np.float64)
     pots = np.array([s.pot for s in subs], np.float64)
     targets = (0.003 * pots).astype(np.float32)

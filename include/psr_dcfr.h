@@ -215,6 +215,15 @@ int psr_batch_exploitability_f64(psr_batch* batch, double* expl_out);
  * the average strategy; after this call psr_batch_root_cfvalues reads them. */
 int psr_batch_compute_root_cfvalues(psr_batch* batch);
 int psr_batch_root_cfvalues(psr_batch* batch, int game, int player, float* out /* H */);
+/* The root cfvalues of `player` of every game, concatenated in batch order. */
+int psr_batch_root_cfvalues_all(psr_batch* batch, int player, float* out, int64_t cap);
+/* New initial weights (the root reach, [0, 1] per private hand of the built
+ * game) of one game, or of every game (game = -1, concatenated in batch
+ * order), on the prepared batch: the initial_weights of a new SubgameSpec on
+ * the same tree (batch.rs:301-409).  num_combinations, and so the payoff
+ * scale, stays that of the built game. */
+int psr_batch_set_initial_weights(psr_batch* batch, int game, const float* w0, int64_t n0, const float* w1,
+                                  int64_t n1);
 /* Regret / strategy-sum arrays of one game in this library's storage order:
  * psr_game_num_storage() floats each, per decision node in arena order an
  * A x H_player block (psr_batch_strategy_sum_edges gives the reference's

@@ -132,15 +132,14 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     } else {
         for (uint16_t i : ro.valid[p]) pstrength[i] = 0;
     }
-    // number of list entries with strength < s (strict) or <= s
+    // number of list entries with strength < s (strict) or <= s; every list
+    // is in ascending strength, so a binary search
     auto bound = [&](const std::vector<uint16_t>& list, int s, bool inclusive) {
-        int n = 0;
-        for (uint16_t oi : list) {
-            int os = opp_strength[oi];
-            if (os < s || (inclusive && os == s)) ++n;
-            else break;
-        }
-        return n;
+        auto it = inclusive ? std::upper_bound(list.begin(), list.end(), s,
+                                               [&](int v, uint16_t oi) { return v < opp_strength[oi]; })
+                            : std::lower_bound(list.begin(), list.end(), s,
+                                               [&](uint16_t oi, int v) { return opp_strength[oi] < v; });
+        return (int)(it - list.begin());
     };
     const int len = (int)idx.size();
     auto pack = [len](int lo, int hi) { return (uint32_t)scan_slot(lo, len) | ((uint32_t)scan_slot(hi, len) << 16); };
@@ -1202,6 +1201,51 @@ int psr_batch_reach_at_nodes(psr_batch* b, int game, int player, const int64_t* 
 int psr_batch_compute_root_cfvalues(psr_batch* b) {
     if (!b || !b->prepared) return fail("batch not prepared");
     return b->run(JOB_ROOT_CFV, 0, 0, 0, b->max_iterations);
+}
+
+int psr_batch_root_cfvalues_all(psr_batch* b, int player, float* out, int64_t cap) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    if (player < 0 || player > 1) return fail("player must be 0 or 1");
+    const HostLayout& L = b->L;
+    int64_t need = 0;
+    for (const GameDesc& d : L.games) need += d.H[player];
+    if (!out || cap < need) return fail("out must hold the hands of the player of every game");
+    std::vector<float> all(std::max<size_t>(L.cards.size(), 1));
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipMemcpy(all.data(), b->d_root_cfv, L.cards.size() * sizeof(float), hipMemcpyDeviceToHost));
+    int64_t off = 0;
+    for (const GameDesc& d : L.games) {
+        std::copy(all.begin() + d.hand_base[player], all.begin() + d.hand_base[player] + d.H[player], out + off);
+        off += d.H[player];
+    }
+    return 0;
+}
+
+int psr_batch_set_initial_weights(psr_batch* b, int game, const float* w0, int64_t n0, const float* w1, int64_t n1) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    HostLayout& L = b->L;
+    if (game < -1 || game >= (int)L.games.size()) return fail("game index out of range");
+    const float* src[2] = {w0, w1};
+    const int64_t n[2] = {n0, n1};
+    const int g0 = game < 0 ? 0 : game, g1 = game < 0 ? (int)L.games.size() : game + 1;
+    for (int p = 0; p < 2; ++p) {
+        int64_t need = 0;
+        for (int gi = g0; gi < g1; ++gi) need += L.games[gi].H[p];
+        if (!src[p] || n[p] != need) return fail("weights must hold one value per private hand (per game in batch order)");
+        int64_t off = 0;
+        for (int gi = g0; gi < g1; ++gi) {
+            const GameDesc& d = L.games[gi];
+            for (int h = 0; h < d.H[p]; ++h) {
+                const float w = src[p][off + h];
+                if (!(w >= 0.0f && w <= 1.0f)) return fail("weights must lie in [0, 1]");
+                L.weights[d.hand_base[p] + h] = w;
+            }
+            off += d.H[p];
+        }
+    }
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipMemcpy(b->d_weights, L.weights.data(), L.weights.size() * sizeof(float), hipMemcpyHostToDevice));
+    return 0;
 }
 
 int psr_batch_root_cfvalues(psr_batch* b, int game, int player, float* out) {

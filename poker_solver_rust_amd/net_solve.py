@@ -34,7 +34,7 @@ import numpy as np
 
 from .range_solver import ActionTree, GpuBatchSolver, PostFlopGame, TreeConfig, hand_evaluate
 from .value_net import INPUT_DIM, NUM_COMBOS, OUTPUT_DIM, ValueNet
-from .workloads import Subgame, _pair_cards, blocked
+from .workloads import Subgame, _pair_cards, blocked, build_all
 
 # -------------------------------------------------------------- encoding --
 
@@ -165,7 +165,7 @@ def river_training_set(turn: Subgame, tree_pots: Sequence[int], n: int, seed: in
             subs.append(Subgame(board, (boundary_like_ranges(rng, bm), boundary_like_ranges(rng, bm)), pot, stack,
                                 turn.bet, turn.raise_, turn.max_raises, iterations))
         k = len(subs)
-        games = [s.build() for s in subs]
+        games = build_all(subs)
         t_build += time.time() - t
         t = time.time()
         b = GpuBatchSolver(0)
@@ -261,11 +261,20 @@ class NetLeaves:
     """Leaf values from the value net: on every river card the check-down
     value plus the net's correction (the net is trained on exact-minus-
     check-down targets), mixed over the rivers.  Boundaries with no chips
-    behind (all-in) are river showdowns: the check-down value is exact there."""
+    behind (all-in) are river showdowns: the check-down value is exact there.
+    `net` may be a list of nets trained with different seeds: their
+    corrections are averaged (an ensemble, which cuts the seed-to-seed spread
+    of the error)."""
 
-    def __init__(self, net: ValueNet, checkdown: "CheckdownLeaves"):
-        self.net = net
+    def __init__(self, net, checkdown: "CheckdownLeaves"):
+        self.nets = list(net) if isinstance(net, (list, tuple)) else [net]
         self.checkdown = checkdown
+
+    def _forward(self, x):
+        out = self.nets[0].forward(x)
+        for n in self.nets[1:]:
+            out = out + n.forward(x)
+        return out / len(self.nets)
 
     def __call__(self, board4, reach, pot, stack):
         return self.values_for([(board4, reach, pot, stack)])[0]
@@ -289,7 +298,7 @@ class NetLeaves:
             pots = np.repeat([items[k][2] for k in live], K)
             stacks = np.repeat([items[k][3] for k in live], K)
             x = encode(rows0, rows1, boards, pots, stacks)
-            out = np.concatenate([self.net.forward(x[i:i + max_rows]) for i in range(0, len(x), max_rows)])
+            out = np.concatenate([self._forward(x[i:i + max_rows]) for i in range(0, len(x), max_rows)])
             out = out.reshape(len(live), K, OUTPUT_DIM).transpose(1, 0, 2)      # (K, live, 2652)
             v0 = base[0][:, live] + out[:, :, :NUM_COMBOS]
             v1 = base[1][:, live] + out[:, :, NUM_COMBOS:]
@@ -386,6 +395,91 @@ class CheckdownLeaves:
         return self.values_for([(board4, reach, pot, stack)])[0]
 
 
+class DeviceCheckdownLeaves:
+    """The check-down values of CheckdownLeaves computed by the device
+    showdown evaluator (game/evaluation.rs:357-425): for each of `slots`
+    boundaries and each river card a river subgame whose tree is check /
+    check, built once with every unblocked combo in both ranges.  A refresh
+    replaces their initial weights by the boundary reaches (river-card
+    combos dropped) and runs one cfvalue pass over all of them; hand i's
+    root cfvalue is (half pot / num_combinations) (win - lose mass), so
+    v_i(c) = cfv_i num_combinations / (half pot R_i(c)).  Same interface as
+    CheckdownLeaves (per_river_rows, values_for, values)."""
+
+    def __init__(self, board4: Sequence[int], slots: int, pot: int = 100, stack: int = 100):
+        from .range_solver import BoardState, CardConfig, Range
+        from .workloads import _pair_cards as _pc
+
+        self.board4 = list(board4)
+        self.rivers = [c for c in range(52) if c not in board4]
+        self.slots = slots
+        self.half = 0.5 * pot
+        tc = TreeConfig(initial_state=BoardState.River, starting_pot=pot, effective_stack=stack)
+        tree = ActionTree(tc)
+        c1, c2 = _pc()
+        self.games, self.idx, self.ncomb = [], [], []
+        for c in self.rivers:
+            board = self.board4 + [c]
+            w = np.where(blocked(board), 0.0, 1.0).astype(np.float32)
+            cc = CardConfig(range=(Range(w), Range(w)), flop=sorted(board[:3]), turn=board[3], river=c)
+            g = PostFlopGame.with_config(cc, tree)
+            self.games.append(g)
+            self.idx.append(hand_index(g, 0))   # both ranges hold the same combos
+            self.ncomb.append(g.num_combinations())
+        self.held = np.stack([(c1 == c) | (c2 == c) for c in self.rivers])
+        self.solver = GpuBatchSolver(0)
+        # game (slot k, river j) at k * K + j; the check / check tree needs no iterations
+        self.solver.prepare_batch([g for _ in range(slots) for g in self.games], 1)
+
+    def per_river_rows(self, r0: np.ndarray, r1: np.ndarray) -> np.ndarray:
+        """(2, K, B, 1326) check-down values of both players on every river
+        for B <= slots boundaries (reach rows r0, r1: (B, 1326))."""
+        r0 = np.atleast_2d(np.asarray(r0, np.float64))
+        r1 = np.atleast_2d(np.asarray(r1, np.float64))
+        B, K = r0.shape[0], len(self.rivers)
+        if B > self.slots:
+            raise ValueError(f"{B} boundaries, {self.slots} slots")
+        # weights: every slot gets a row (unused slots repeat row 0); scale free, so
+        # normalised per row to the [0, 1] a range weight must lie in
+        w, tops = [], []
+        for r in (r0, r1):
+            rows = np.concatenate([r, np.repeat(r[:1], self.slots - B, 0)]) if B < self.slots else r
+            top = rows.max(1, keepdims=True)
+            tops.append(top[:B, 0])
+            rows = np.where(top > 0, rows / np.where(top > 0, top, 1.0), 0.0)
+            w.append(np.concatenate([rows[k][self.idx[j]] for k in range(self.slots) for j in range(K)]))
+        self.solver.set_initial_weights(w[0].astype(np.float32), w[1].astype(np.float32))
+        self.solver.compute_root_cfvalues()
+        cf = [self.solver.root_cfvalues_all(p) for p in range(2)]
+        out = np.zeros((2, K, B, NUM_COMBOS))
+        n = len(self.idx[0])
+        for p, r_opp in ((0, r1), (1, r0)):
+            top = np.maximum(tops[p ^ 1], 1e-300)     # the opponent reach on the device is r_opp / top
+            for j in range(K):
+                ro = np.where(self.held[j], 0.0, r_opp)                         # (B, 1326)
+                R = unblocked_mass(ro)[:, self.idx[j]] / top[:, None]           # (B, H), device scale
+                for k in range(B):
+                    v = cf[p][(k * K + j) * n:(k * K + j + 1) * n].astype(np.float64)
+                    ok = R[k] > 0
+                    out[p, j, k, self.idx[j]] = np.where(
+                        ok, v * self.ncomb[j] / (self.half * np.where(ok, R[k], 1.0)), 0.0)
+        return out
+
+    def per_river(self, reach: List[np.ndarray]):
+        return self.per_river_rows(reach[0], reach[1])[:, :, 0]
+
+    def values_for(self, items):
+        r0 = np.stack([np.asarray(it[1][0], np.float64) for it in items])
+        r1 = np.stack([np.asarray(it[1][1], np.float64) for it in items])
+        v = self.per_river_rows(r0, r1)
+        m0 = river_mix(v[0], r1, self.rivers)
+        m1 = river_mix(v[1], r0, self.rivers)
+        return [(m0[k], m1[k]) for k in range(len(items))]
+
+    def __call__(self, board4, reach, pot, stack):
+        return self.values_for([(board4, reach, pot, stack)])[0]
+
+
 class ExactLeaves:
     """Leaf values from exact river solves: every (boundary, river card)
     subgame with the recorded reaches as ranges, solved in one batched GPU
@@ -420,7 +514,7 @@ class ExactLeaves:
                 where.append((k, c))
         vals = {}
         if subs:
-            games = [s.build() for s in subs]
+            games = build_all(subs)
             b = GpuBatchSolver(0)
             b.prepare_batch(games, self.iterations, [s.target for s in subs])
             b.solve(self.iterations, 5)

@@ -121,6 +121,9 @@ def load_library(path: Optional[str] = None):
         "psr_batch_exploitability": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "psr_batch_compute_root_cfvalues": (C.c_int, [vp]),
         "psr_batch_root_cfvalues": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "psr_batch_root_cfvalues_all": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
+        "psr_batch_set_initial_weights": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64,
+                                                    C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_strategy_sum": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_regrets": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_last_kernel_ms": (C.c_float, [vp]),
@@ -652,6 +655,21 @@ class GpuBatchSolver:
                 pair.append(v)
             out.append(tuple(pair))
         return out
+
+    def root_cfvalues_all(self, player: int) -> np.ndarray:
+        """Root cfvalues of `player` of every game, concatenated (as the last
+        solve() or cfvalue pass left them)."""
+        n = sum(g.num_private_hands(player) for g in self.games)
+        out = np.zeros(n, np.float32)
+        _check(load_library().psr_batch_root_cfvalues_all(self._h, player, _fptr(out), n))
+        return out
+
+    def set_initial_weights(self, w0, w1, game: int = -1):
+        """New initial weights (root reach) of one game or of every game
+        (game = -1: concatenated in batch order) on the same trees."""
+        a = np.ascontiguousarray(w0, np.float32)
+        b = np.ascontiguousarray(w1, np.float32)
+        _check(load_library().psr_batch_set_initial_weights(self._h, game, _fptr(a), len(a), _fptr(b), len(b)))
 
     def strategy_sum(self, i: int) -> np.ndarray:
         n = self._game(i).num_storage()

@@ -97,6 +97,19 @@ class Subgame:
         return 0.003 * self.pot
 
 
+def build_all(subgames: Sequence[Subgame], threads: int = 0) -> List[PostFlopGame]:
+    """PostFlopGame of every subgame, built on `threads` host threads (0: up
+    to 16); the C++ game build runs outside the interpreter lock."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    if threads == 1 or len(subgames) < 2:
+        return [s.build() for s in subgames]
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        return list(pool.map(lambda s: s.build(), subgames))
+
+
 def river_uniform(seed: int = 0) -> Subgame:
     rng = np.random.default_rng(seed)
     board = _board(rng, 5)
