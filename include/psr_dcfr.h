@@ -346,8 +346,10 @@ int psr_mlp_get_params(psr_mlp* net, float* out, int64_t n);
 int psr_mlp_set_params(psr_mlp* net, const float* in, int64_t n);
 /* Inference: n rows of dims[0] -> n rows of dims[last]. */
 int psr_mlp_forward(psr_mlp* net, const float* x, int n, float* y);
-/* One forward + backward on n <= max_batch rows: mean masked Huber loss over
- * all outputs and every parameter's gradient; no update. */
+/* One forward + backward on n <= max_batch rows: the masked Huber loss
+ * summed over the batch and divided by max(sum of the mask, 1) (the
+ * reference's masked_huber_loss, cfvnet/src/model/loss.rs:8-29) and every
+ * parameter's gradient; no update. */
 int psr_mlp_gradients(psr_mlp* net, const float* x, const float* y, const float* mask, int n, float delta,
                       double* loss, float* grads, int64_t n_grads);
 int psr_mlp_set_dataset(psr_mlp* net, const float* x, const float* y, const float* mask, int n);
@@ -355,6 +357,14 @@ int psr_mlp_set_dataset(psr_mlp* net, const float* x, const float* y, const floa
 int psr_mlp_train(psr_mlp* net, int epochs, int batch, float lr, float weight_decay, float delta, uint64_t seed,
                   double* epoch_loss);
 float psr_mlp_last_ms(psr_mlp* net);
+/* Gradient-norm clipping of psr_mlp_train's steps: every gradient is scaled
+ * by min(1, max_norm / (||g|| + 1e-6)), ||g|| the L2 norm over all
+ * parameters (torch.nn.utils.clip_grad_norm_, the reference trainer's clip,
+ * cfvnet/python/cfvnet/train.py:216; its configured norm is 1.0,
+ * cfvnet/src/config.rs:284-285).  0 (the default) turns it off. */
+int psr_mlp_set_grad_clip(psr_mlp* net, float max_norm);
+/* ||g|| of the last clipped training step (clipping on). */
+int psr_mlp_last_grad_norm(psr_mlp* net, float* norm);
 
 #ifdef __cplusplus
 }

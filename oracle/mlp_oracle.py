@@ -3,9 +3,9 @@ and never by the product path).
 
 A plain float64/float32 numpy statement of the network the device trains:
 dense layers y = x W + b (W stored in x out), GELU (exact erf form) on every
-hidden layer, a linear output layer, the masked Huber loss averaged over all
-outputs of the batch (the reference's loss, cfvnet/python/cfvnet/loss.py:6-90,
-uses a Huber term on masked per-combo values), and AdamW with decoupled weight
+hidden layer, a linear output layer, the masked Huber loss averaged over the
+batch's valid outputs (the reference's masked_huber_loss,
+cfvnet/src/model/loss.rs:8-29), and AdamW with decoupled weight
 decay (torch.optim.AdamW semantics).  Parameters are flattened as W_1, b_1,
 W_2, b_2, ... like psr_mlp_get_params.
 """
@@ -64,12 +64,12 @@ def huber_grad(d, delta):
 
 
 def loss_and_grads(params, dims, x, y, mask, delta):
-    """Mean masked Huber loss over all n * out outputs and its gradient with
-    respect to every parameter (flattened)."""
+    """Masked Huber loss averaged over the batch's valid outputs (sum over
+    the batch / max(sum of the mask, 1), cfvnet/src/model/loss.rs:8-29) and
+    its gradient with respect to every parameter (flattened)."""
     layers = unflatten(params, dims)
     pred, zs, hs = forward(params, dims, x, keep=True)
-    n, d_out = pred.shape
-    scale = 1.0 / (n * d_out)
+    scale = 1.0 / max(float(np.asarray(mask, np.float64).sum()), 1.0)
     d = pred - y
     loss = float((mask * huber(d, delta)).sum() * scale)
     g = mask * huber_grad(d, delta) * scale
@@ -83,6 +83,13 @@ def loss_and_grads(params, dims, x, y, mask, delta):
             g = (g @ W.T) * gelu_grad(zs[i - 1])
     flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return loss, flat
+
+
+def clip_grad_norm(grads, max_norm):
+    """torch.nn.utils.clip_grad_norm_ on the flattened gradient: scaled by
+    min(1, max_norm / (||g||_2 + 1e-6)); returns (clipped, ||g||_2)."""
+    norm = float(np.sqrt(np.sum(np.asarray(grads, np.float64) ** 2)))
+    return grads * min(1.0, max_norm / (norm + 1e-6)), norm
 
 
 class AdamW:

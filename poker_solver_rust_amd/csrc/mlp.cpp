@@ -87,7 +87,11 @@ struct psr_mlp {
     int32_t* drows = nullptr;        // the epoch's shuffled row order (n_data)
     float* epoch_rows = nullptr;     // per-row losses of an epoch's batches (n_data)
     int n_data = 0;
+    std::vector<double> row_mask;    // per training row: sum of its mask (the loss normaliser)
     int64_t step = 0;
+    float clip_norm = 0.0f;          // gradient-norm clip of the training steps (0: off)
+    double* clip_part = nullptr;     // CLIP_PARTS x segments partial sums of squares
+    float* clip_coef = nullptr;      // [coefficient, norm] of the last clipped step
     std::vector<void*> allocs;
 
     int alloc(void** p, size_t bytes) {
@@ -175,14 +179,14 @@ struct psr_mlp {
         return 0;
     }
 
-    // mean Huber loss over all outputs of n rows; dZ[L].  Targets and mask:
-    // rows `rows[r]` of y / mk when given (the resident training set), else
-    // the batch buffers yb / mbuf.  rows_out: where the per-row losses go
-    // (row_loss when null).
-    int loss(int n, float delta, double* out, float* rows_out = nullptr, const float* y = nullptr,
+    // masked Huber loss of n rows averaged over the batch's valid outputs
+    // (inv = 1 / max(sum of the mask, 1), loss.rs:8-29); dZ[L].  Targets and
+    // mask: rows `rows[r]` of y / mk when given (the resident training set),
+    // else the batch buffers yb / mbuf.  rows_out: where the per-row losses
+    // go (row_loss when null).
+    int loss(int n, float delta, float inv, double* out, float* rows_out = nullptr, const float* y = nullptr,
              const float* mk = nullptr, const int32_t* rows = nullptr) {
         const int D = dims[L];
-        const float inv = 1.0f / ((float)n * (float)D);
         HIP_OK(huber_launch(Z[L], y ? y : yb, y ? mk : mbuf, rows, n, D, ld[L], delta, inv, dZ[L],
                             rows_out ? rows_out : row_loss, stream));
         if (out) {
@@ -201,7 +205,9 @@ struct psr_mlp {
         const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
         const float bc1 = 1.0f - (float)std::pow(0.9, (double)step);
         const float bc2 = 1.0f - (float)std::pow(0.999, (double)step);
-        HIP_OK(adamw_launch(segs, 2 * L, max_seg, lr, b1, b2, eps, wd, bc1, bc2, stream));
+        if (clip_norm > 0.0f) HIP_OK(clip_coef_launch(segs, 2 * L, clip_part, clip_norm, clip_coef, stream));
+        HIP_OK(adamw_launch(segs, 2 * L, max_seg, lr, b1, b2, eps, wd, bc1, bc2,
+                            clip_norm > 0.0f ? clip_coef : nullptr, stream));
         return 0;
     }
 
@@ -310,7 +316,9 @@ int psr_mlp_create(int device, const int* dims, int n_dims, int max_batch, uint6
     if (bad(m->alloc((void**)&m->ws, (size_t)wsf * 4))) return -1;
     if (bad(m->alloc((void**)&m->row_loss, (size_t)max_batch * 4)) ||
         bad(m->alloc((void**)&m->yb, (size_t)max_batch * dims[L] * 4)) ||
-        bad(m->alloc((void**)&m->mbuf, (size_t)max_batch * dims[L] * 4)))
+        bad(m->alloc((void**)&m->mbuf, (size_t)max_batch * dims[L] * 4)) ||
+        bad(m->alloc((void**)&m->clip_part, (size_t)CLIP_PARTS * 2 * L * 8)) ||
+        bad(m->alloc_zero((void**)&m->clip_coef, 2 * 4)))
         return -1;
     if (hipStreamSynchronize(m->stream) != hipSuccess) return bad(fail("HIP error during init"));
     *out = m;
@@ -396,7 +404,10 @@ int psr_mlp_gradients(psr_mlp* m, const float* x, const float* y, const float* m
     if (m->upload_input(x, n)) return -1;
     HIP_OK(hipMemcpyAsync(m->yb, y, (size_t)n * DL * 4, hipMemcpyHostToDevice, m->stream));
     HIP_OK(hipMemcpyAsync(m->mbuf, mask, (size_t)n * DL * 4, hipMemcpyHostToDevice, m->stream));
-    if (m->forward(n) || m->loss(n, delta, loss) || m->backward(n)) return -1;
+    double msum = 0.0;
+    for (int64_t k = 0; k < (int64_t)n * DL; ++k) msum += mask[k];
+    const float inv = (float)(1.0 / std::max(msum, 1.0));
+    if (m->forward(n) || m->loss(n, delta, inv, loss) || m->backward(n)) return -1;
     HIP_OK(hipStreamSynchronize(m->stream));
     if (grads) {
         int64_t off = 0;
@@ -439,6 +450,9 @@ int psr_mlp_set_dataset(psr_mlp* m, const float* x, const float* y, const float*
     }
     HIP_OK(hipMemcpy(m->dy, y, (size_t)n * DL * 4, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(m->dmask, mask, (size_t)n * DL * 4, hipMemcpyHostToDevice));
+    m->row_mask.assign(n, 0.0);
+    for (int r = 0; r < n; ++r)
+        for (int c = 0; c < DL; ++c) m->row_mask[r] += mask[(size_t)r * DL + c];
     m->n_data = n;
     return 0;
 }
@@ -451,7 +465,6 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
     if (!m || m->n_data <= 0) return fail("no training set");
     if (batch <= 0 || batch > m->max_batch || epochs <= 0) return fail("invalid batch or epochs");
     HIP_OK(hipSetDevice(m->device));
-    const int DL = m->dims[m->L];
     std::vector<int32_t> order(m->n_data);
     std::iota(order.begin(), order.end(), 0);
     SplitMix rng{seed};
@@ -459,15 +472,18 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
     // The epoch's row order goes up once and the batches' losses stay on the
     // device until the epoch ends: no host round trip between the steps.
     std::vector<float> rows_loss(m->n_data);
-    const float inv = 1.0f / ((float)batch * (float)DL);
+    std::vector<float> inv((size_t)m->n_data / batch + 1);   // per batch: 1 / max(sum of its mask, 1)
     for (int e = 0; e < epochs; ++e) {
         for (int i = m->n_data - 1; i > 0; --i) std::swap(order[i], order[rng.next() % (uint64_t)(i + 1)]);
         HIP_OK(hipMemcpyAsync(m->drows, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice, m->stream));
         int nb = 0;
         for (int r0 = 0; r0 + batch <= m->n_data; r0 += batch) {
+            double msum = 0.0;
+            for (int i = 0; i < batch; ++i) msum += m->row_mask[order[r0 + i]];
+            inv[nb] = (float)(1.0 / std::max(msum, 1.0));
             HIP_OK(gather_launch(m->dx, m->drows + r0, batch, m->ld[0], m->H[0], m->stream));
             if (m->forward(batch) ||
-                m->loss(batch, delta, nullptr, m->epoch_rows + r0, m->dy, m->dmask, m->drows + r0) ||
+                m->loss(batch, delta, inv[nb], nullptr, m->epoch_rows + r0, m->dy, m->dmask, m->drows + r0) ||
                 m->backward(batch) || m->adamw(lr, weight_decay))
                 return -1;
             ++nb;
@@ -483,7 +499,7 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
             for (int b = 0; b < nb; ++b) {
                 double s = 0.0;
                 for (int i = 0; i < batch; ++i) s += rows_loss[(size_t)b * batch + i];
-                sum += s * (double)inv;
+                sum += s * (double)inv[b];
             }
             epoch_loss[e] = nb ? sum / nb : 0.0;
         }
@@ -495,5 +511,22 @@ int psr_mlp_train(psr_mlp* m, int epochs, int batch, float lr, float weight_deca
 }
 
 float psr_mlp_last_ms(psr_mlp* m) { return m ? m->last_ms : -1.0f; }
+
+int psr_mlp_set_grad_clip(psr_mlp* m, float max_norm) {
+    if (!m || !(max_norm >= 0.0f) || std::isinf(max_norm)) return fail("invalid argument");
+    m->clip_norm = max_norm;
+    return 0;
+}
+
+int psr_mlp_last_grad_norm(psr_mlp* m, float* norm) {
+    if (!m || !norm) return fail("invalid argument");
+    if (m->clip_norm <= 0.0f) return fail("gradient clipping is off");
+    HIP_OK(hipSetDevice(m->device));
+    float c[2];
+    HIP_OK(hipMemcpyAsync(c, m->clip_coef, sizeof c, hipMemcpyDeviceToHost, m->stream));
+    HIP_OK(hipStreamSynchronize(m->stream));
+    *norm = c[1];
+    return 0;
+}
 
 }  // extern "C"

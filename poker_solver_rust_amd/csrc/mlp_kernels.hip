@@ -274,12 +274,14 @@ __global__ void huber_kernel(const float* pred, const float* target, const float
 // BF16 working copy.  bc1 / bc2 are the bias corrections 1 - beta^t.
 // All parameter segments in one launch: blockIdx.y picks the segment, the
 // x blocks stride over its elements.
+// gscale (null: 1): the gradient-norm clip coefficient, read on the device.
 __global__ void adamw_kernel(const AdamSegment* segs, float lr, float beta1, float beta2, float eps, float wd, float bc1,
-                             float bc2) {
+                             float bc2, const float* gscale) {
     const AdamSegment sg = segs[blockIdx.y];
     const float d = sg.decay ? wd : 0.0f;
+    const float gs = gscale ? *gscale : 1.0f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < sg.n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float gi = sg.g[i];
+        const float gi = sg.g[i] * gs;
         const float mi = beta1 * sg.m[i] + (1.0f - beta1) * gi;
         const float vi = beta2 * sg.v[i] + (1.0f - beta2) * gi * gi;
         sg.m[i] = mi;
@@ -288,6 +290,50 @@ __global__ void adamw_kernel(const AdamSegment* segs, float lr, float beta1, flo
         const float pi = sg.p[i] - lr * (mh / (sqrtf(vh) + eps) + d * sg.p[i]);
         sg.p[i] = pi;
         if (sg.pb) sg.pb[(i / sg.cols) * sg.ldb + i % sg.cols] = f32_to_bf16_bits(pi);
+    }
+}
+
+constexpr int NORM_BLOCK = 256;
+
+// Sum of squares of every gradient element, in f64: block (x, y) sums its
+// stride of segment y into part[y * gridDim.x + x] (a fixed order, so the
+// clip coefficient does not depend on scheduling).
+__global__ void __launch_bounds__(NORM_BLOCK) grad_sumsq_kernel(const AdamSegment* segs, double* part) {
+    const AdamSegment sg = segs[blockIdx.y];
+    __shared__ double red[NORM_BLOCK];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * NORM_BLOCK + threadIdx.x; i < sg.n; i += (int64_t)gridDim.x * NORM_BLOCK) {
+        const double g = sg.g[i];
+        s += g * g;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = NORM_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// One workgroup: the global gradient norm from the n partial sums and the
+// clip coefficient min(1, max_norm / (norm + 1e-6)) (torch's
+// clip_grad_norm_).
+__global__ void __launch_bounds__(NORM_BLOCK) clip_coef_kernel(const double* part, int n, float max_norm, float* coef) {
+    __shared__ double red[NORM_BLOCK];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += NORM_BLOCK) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+#pragma unroll
+    for (int w = NORM_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const float norm = (float)sqrt(red[0]);
+        coef[0] = fminf(1.0f, max_norm / (norm + 1e-6f));
+        coef[1] = norm;
     }
 }
 
@@ -376,12 +422,22 @@ hipError_t huber_launch(const float* pred, const float* target, const float* mas
 }
 
 hipError_t adamw_launch(const AdamSegment* segs, int num_segs, int64_t max_n, float lr, float beta1, float beta2,
-                        float eps, float wd, float bc1, float bc2, hipStream_t stream) {
+                        float eps, float wd, float bc1, float bc2, const float* gscale, hipStream_t stream) {
     if (num_segs <= 0 || max_n <= 0) return hipErrorInvalidValue;
     int64_t blocks = (max_n + 255) / 256;
     if (blocks > 2048) blocks = 2048;
     hipLaunchKernelGGL(adamw_kernel, dim3((unsigned)blocks, (unsigned)num_segs), dim3(256), 0, stream, segs, lr, beta1,
-                       beta2, eps, wd, bc1, bc2);
+                       beta2, eps, wd, bc1, bc2, gscale);
+    return hipGetLastError();
+}
+
+hipError_t clip_coef_launch(const AdamSegment* segs, int num_segs, double* part, float max_norm, float* coef,
+                            hipStream_t stream) {
+    if (num_segs <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(grad_sumsq_kernel, dim3(CLIP_PARTS, (unsigned)num_segs), dim3(NORM_BLOCK), 0, stream, segs,
+                       part);
+    hipLaunchKernelGGL(clip_coef_kernel, dim3(1), dim3(NORM_BLOCK), 0, stream, part, CLIP_PARTS * num_segs, max_norm,
+                       coef);
     return hipGetLastError();
 }
 

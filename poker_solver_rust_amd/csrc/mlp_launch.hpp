@@ -69,7 +69,13 @@ struct AdamSegment {
 // AdamW over num_segs segments (device array segs) in one launch; max_n =
 // the largest segment
 hipError_t adamw_launch(const AdamSegment* segs, int num_segs, int64_t max_n, float lr, float beta1, float beta2,
-                        float eps, float wd, float bc1, float bc2, hipStream_t stream);
+                        float eps, float wd, float bc1, float bc2, const float* gscale, hipStream_t stream);
+// Gradient-norm clipping: coef[0] = min(1, max_norm / (||g|| + 1e-6)) over
+// every segment's gradient, coef[1] = ||g||; part holds CLIP_PARTS x
+// num_segs partial sums.
+constexpr int CLIP_PARTS = 128;
+hipError_t clip_coef_launch(const AdamSegment* segs, int num_segs, double* part, float max_norm, float* coef,
+                            hipStream_t stream);
 // x [rows x cols] FP32 -> y [rows x ldy] BF16
 hipError_t to_bf16_launch(const float* x, uint16_t* y, int64_t rows, int cols, int64_t ldy, hipStream_t stream);
 // batch row r <- training row rows[r]: rows of ldx BF16, copied whole

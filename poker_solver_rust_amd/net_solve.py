@@ -205,12 +205,15 @@ def river_training_set(turn: Subgame, tree_pots: Sequence[int], n: int, seed: in
 
 
 def train_value_net(ts: TrainingSet, epochs: int = 30, batch: int = 512, lr: float = 1e-3, seed: int = 0,
-                    weight_decay: float = 1e-4, holdout: float = 0.05):
-    """Trains the default 2706 -> 4 x 512 GELU -> 2652 net; returns the net,
-    the per-epoch training loss and the hold-out mean absolute error."""
+                    weight_decay: float = 1e-4, holdout: float = 0.05, grad_clip_norm: float = 1.0):
+    """Trains the default 2706 -> 4 x 512 GELU -> 2652 net, its gradients
+    clipped to grad_clip_norm (the reference trainer's default 1.0,
+    cfvnet/src/config.rs:284-285; 0: off); returns the net, the per-epoch
+    training loss and the hold-out mean absolute error."""
     n = len(ts.x)
     nh = max(1, int(n * holdout))
     net = ValueNet(max_batch=max(batch, 1024), seed=seed)
+    net.set_grad_clip(grad_clip_norm)
     net.set_dataset(ts.x[nh:], ts.y[nh:], ts.mask[nh:])
     losses = net.train(epochs, batch, lr=lr, weight_decay=weight_decay, delta=1.0, seed=seed)
     net.train_device_ms = net.last_ms()

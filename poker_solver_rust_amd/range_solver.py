@@ -160,6 +160,8 @@ def load_library(path: Optional[str] = None):
         "psr_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_uint64,
                                     C.POINTER(C.c_double)]),
         "psr_mlp_last_ms": (C.c_float, [vp]),
+        "psr_mlp_set_grad_clip": (C.c_int, [vp, C.c_float]),
+        "psr_mlp_last_grad_norm": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "psr_batch_expected_values_detail": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64]),
         "psr_batch_expected_values": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int]),
         "psr_layout_validate": (C.c_int, [vp, C.POINTER(C.c_int64)]),

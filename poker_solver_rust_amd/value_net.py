@@ -86,5 +86,17 @@ class ValueNet:
                                             out.ctypes.data_as(C.POINTER(C.c_double))))
         return out
 
+    def set_grad_clip(self, max_norm: float):
+        """Clip every training step's gradient to L2 norm max_norm over all
+        parameters (torch.nn.utils.clip_grad_norm_; the reference trainer
+        uses 1.0, cfvnet/src/config.rs:284-285).  0 turns it off."""
+        _check(load_library().psr_mlp_set_grad_clip(self._h, float(max_norm)))
+
+    def last_grad_norm(self) -> float:
+        """The gradient norm of the last clipped training step."""
+        v = C.c_float()
+        _check(load_library().psr_mlp_last_grad_norm(self._h, C.byref(v)))
+        return v.value
+
     def last_ms(self) -> float:
         return float(load_library().psr_mlp_last_ms(self._h))

@@ -387,6 +387,31 @@ def run_mlp_adamw_step():
         raise AssertionError(f"AdamW step deviates by {err}")
 
 
+def run_mlp_adamw_clipped():
+    """One training step with gradient-norm clipping: max_norm puts the
+    median clipped gradient at AdamW's eps, so the step is lr * g_c / (|g_c|
+    + eps) with a coefficient-dependent size.  The device step matches the
+    oracle's clipped step, misses its unclipped step, and the reported norm
+    is the oracle's gradient norm."""
+    dims = (70, 40, 33, 19)
+    net, M, p, x, y, mask = mlp_case(dims, 32, 6)
+    p = p.astype(np.float64)
+    _, go = M.loss_and_grads(p, dims, x.astype(np.float64), y.astype(np.float64), mask.astype(np.float64), 1.0)
+    gc, norm = M.clip_grad_norm(go, 1e-8 * np.linalg.norm(go) / np.median(np.abs(go)))
+    net.set_dataset(x, y, mask)
+    net.set_grad_clip(1e-8 * np.linalg.norm(go) / np.median(np.abs(go)))
+    net.train(1, 32, lr=1e-2, weight_decay=0.01, delta=1.0, seed=0)
+    d = net.get_params().astype(np.float64)
+    big = np.abs(go) > 1e-2 * np.abs(go).max()
+    err = np.abs(d - M.AdamW(dims, 1e-2, 0.01).step(p, gc))[big].max()
+    err_plain = np.abs(d - M.AdamW(dims, 1e-2, 0.01).step(p, go))[big].max()
+    dn = net.last_grad_norm()
+    print(f"clipped AdamW step: max deviation {err:.3g} from the clipped oracle step, {err_plain:.3g} from the "
+          f"unclipped one; norm {dn:.6g} vs {norm:.6g}", flush=True)
+    if err > 1e-3 or err_plain < 3e-3 or abs(dn - norm) > 2e-2 * norm:
+        raise AssertionError(f"clipped AdamW step: {err}, {err_plain}, {dn} vs {norm}")
+
+
 CHECKS = {name[4:]: fn for name, fn in globals().items() if name.startswith("run_")}
 
 if __name__ == "__main__":

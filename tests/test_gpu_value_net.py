@@ -137,6 +137,31 @@ def test_adamw_step_matches_oracle():
     assert np.abs(net.get_params() - ref)[big].max() <= 1e-3
 
 
+def test_clipped_adamw_step_matches_oracle():
+    """Gradient-norm clipping: max_norm puts the median clipped gradient at
+    AdamW's eps, so the step's size depends on the clip coefficient.  The
+    device step matches the oracle's clipped step and misses its unclipped
+    one; the reported norm is the oracle's gradient norm (BF16 tolerance)."""
+    dims = (70, 40, 33, 19)
+    net = ValueNet(dims, max_batch=64, seed=6)
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(6)
+    x = rng.normal(0, 1, (32, dims[0])).astype(np.float32)
+    y = rng.normal(0, 1, (32, dims[-1])).astype(np.float32)
+    mask = (rng.random((32, dims[-1])) < 0.8).astype(np.float32)
+    _, go = M.loss_and_grads(p, dims, x.astype(np.float64), y.astype(np.float64), mask.astype(np.float64), 1.0)
+    max_norm = 1e-8 * np.linalg.norm(go) / np.median(np.abs(go))
+    gc, norm = M.clip_grad_norm(go, max_norm)
+    net.set_dataset(x, y, mask)
+    net.set_grad_clip(max_norm)
+    net.train(1, 32, lr=1e-2, weight_decay=0.01, delta=1.0, seed=0)
+    d = net.get_params().astype(np.float64)
+    big = np.abs(go) > 1e-2 * np.abs(go).max()
+    assert np.abs(d - M.AdamW(dims, 1e-2, 0.01).step(p, gc))[big].max() <= 1e-3
+    assert np.abs(d - M.AdamW(dims, 1e-2, 0.01).step(p, go))[big].max() >= 3e-3
+    assert abs(net.last_grad_norm() - norm) <= 2e-2 * norm
+
+
 def test_training_reduces_loss():
     """A learnable target (a fixed random two-layer function of the input):
     the loss after training is a small fraction of the initial loss."""

@@ -33,7 +33,8 @@ def emu_lib():
 CHECKS = [("std_stepwise", True), ("std_raises_stepwise", True), ("std_solve", True), ("batch", True),
           ("compressed_stepwise", True), ("fp64_exploitability", True), ("half_storage", True),
           ("turn_stepwise", True), ("mixed_batch", False), ("boundary_stepwise", True),
-          ("expected_values", False), ("raked", True), ("reach_at_nodes", False), ("flop_stepwise", False), ("mlp_forward", True), ("mlp_gradients", True), ("mlp_gradients_split_k", True), ("mlp_gradients_one_row", True), ("mlp_adamw_step", True)]
+          ("expected_values", False), ("raked", True), ("reach_at_nodes", False), ("flop_stepwise", False), ("mlp_forward", True), ("mlp_gradients", True), ("mlp_gradients_split_k", True), ("mlp_gradients_one_row", True), ("mlp_adamw_step", True),
+          ("mlp_adamw_clipped", True)]
 
 
 def _run_check(check, uniform):

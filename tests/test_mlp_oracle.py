@@ -49,7 +49,10 @@ def test_matches_torch_autograd_and_adamw():
     for i, (W, b) in enumerate(tw):
         z = h @ W + b
         h = torch.nn.functional.gelu(z) if i + 1 < len(tw) else z
-    tl = (torch.tensor(mask) * torch.nn.functional.huber_loss(h, torch.tensor(y), reduction="none", delta=1.0)).mean()
+    # the reference's masked_huber_loss: masked sum / max(mask sum, 1)
+    tm = torch.tensor(mask)
+    tl = (tm * torch.nn.functional.huber_loss(h, torch.tensor(y), reduction="none", delta=1.0)).sum() \
+        / tm.sum().clamp_min(1.0)
     tl.backward()
     tg = np.concatenate([np.concatenate([W.grad.numpy().ravel(), b.grad.numpy()]) for W, b in tw])
     assert abs(float(tl.detach()) - loss) < 1e-12 and np.allclose(tg, g, rtol=1e-10, atol=1e-14)
@@ -60,3 +63,40 @@ def test_matches_torch_autograd_and_adamw():
     ours = M.AdamW(DIMS, 1e-2, 0.01).step(p, g)
     tp = np.concatenate([np.concatenate([W.detach().numpy().ravel(), b.detach().numpy()]) for W, b in tw])
     assert np.allclose(tp, ours, rtol=1e-12, atol=1e-14)
+
+
+def test_empty_mask_gives_zero_loss_and_gradient():
+    """All outputs masked out: the normaliser clamps at 1 and nothing flows."""
+    p, x, y, mask = setup(2)
+    loss, g = M.loss_and_grads(p, DIMS, x, y, np.zeros_like(mask), 1.0)
+    assert loss == 0.0 and not np.any(g)
+
+
+def test_loss_is_the_mean_over_valid_outputs():
+    """Appending fully masked rows leaves the loss and gradient unchanged:
+    the loss is a mean over the valid outputs, not over all rows."""
+    p, x, y, mask = setup(3)
+    _, xo, yo, _ = setup(4)
+    l1, g1 = M.loss_and_grads(p, DIMS, x, y, mask, 1.0)
+    l2, g2 = M.loss_and_grads(p, DIMS, np.vstack([x, xo]), np.vstack([y, yo]),
+                              np.vstack([mask, np.zeros_like(mask)]), 1.0)
+    assert abs(l1 - l2) < 1e-12 and np.allclose(g1, g2, rtol=1e-10, atol=1e-14)
+
+
+@pytest.mark.parametrize("max_norm", [1e-3, 0.05, 100.0])
+def test_clip_grad_norm_matches_torch(max_norm):
+    """Global-norm clipping against torch.nn.utils.clip_grad_norm_ on the
+    network's per-layer gradients (clipping and non-clipping cases)."""
+    torch = pytest.importorskip("torch")
+    p, x, y, mask = setup(5)
+    _, g = M.loss_and_grads(p, DIMS, x, y, mask, 1.0)
+    ours, norm = M.clip_grad_norm(g, max_norm)
+    ts = [torch.zeros(a.shape, dtype=torch.float64, requires_grad=True) for a in
+          [t for W, b in M.unflatten(g, DIMS) for t in (W, b)]]
+    for t, a in zip(ts, [t for W, b in M.unflatten(g, DIMS) for t in (W, b)]):
+        t.grad = torch.tensor(a)
+    tn = torch.nn.utils.clip_grad_norm_(ts, max_norm)
+    tg = np.concatenate([t.grad.numpy().ravel() for t in ts])
+    assert abs(float(tn) - norm) < 1e-12 * norm
+    assert np.allclose(ours, tg, rtol=1e-12, atol=0)
+    assert (norm > max_norm) == (not np.allclose(ours, g))
