@@ -23,6 +23,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "dcfr_launch.hpp"
@@ -531,8 +532,7 @@ __device__ __forceinline__ void seg_dpp_step(int& f, double& s) {
 // (segment-started, sum) pairs and a carry across the four waves.  Ends with
 // the scan output visible to the workgroup (LDS barriers only).
 template <typename V>
-__device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len) {
-    const V* rs = static_cast<const V*>(c.rs);
+__device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len, const V* rs) {
     const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x / WAVE;
     const int C = (len + BLOCK - 1) / BLOCK;
     const int e0 = threadIdx.x * C;
@@ -641,26 +641,41 @@ __device__ __forceinline__ void load_runout(const Ctx& c, int table, int p, Runo
 // Fold / showdown values of one terminal for traversing player p
 // (game/evaluation.rs:293-537), and depth-boundary leaves
 // (evaluation.rs:87-120, 544-630), whole workgroup.  rv holds the opponent
-// reach row of the terminal (slot k = hand hslot(k)); it is staged in LDS,
-// scanned, and the results are written to the terminal's cfv row.  Only LDS
-// barriers inside: the caller publishes the cfv rows with __syncthreads().
-template <typename V>
+// reach row of the terminal (slot k = hand hslot(k)).  On the level-by-level
+// path (LDS_ROW false) it is staged in LDS, scanned, and the results are
+// written to the terminal's cfv row; only LDS barriers inside, the caller
+// publishes the cfv rows with __syncthreads().  On the depth-first path
+// (LDS_ROW true) the reach row is already the LDS row `row` (rv unused) and
+// each hand's value is written to, or with `add` added to, its slot of the
+// LDS row `out`.
+template <typename V, bool LDS_ROW = false>
 __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p, const RunoutRegs& R,
-                                              const V (&rv)[KH], bool rescan) {
+                                              const V (&rv)[KH], bool rescan, const V* row = nullptr,
+                                              V* out = nullptr, bool add = false) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p), Ho = num_hands(c.G, p ^ 1);
-    V* out = cfv_rows<V>(c) + (size_t)n * c.stride;
-    V* rs = static_cast<V*>(c.rs);
+    if (!LDS_ROW) out = cfv_rows<V>(c) + (size_t)n * c.stride;
+    const V* rs = LDS_ROW ? row : static_cast<const V*>(c.rs);
     const double win = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]);
     const double lose = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]);
     const double tie = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]);
     if (rescan) {   // otherwise rs and the scan already hold this reach row
-        lds_barrier();   // the previous terminal's reads of rs and scan are done
-        FOR_K if (hslot(k) < Ho) rs[hslot(k)] = rv[k];
-        lds_barrier();
-        segmented_scan<V>(c, R.u, R.T.scan_len);
+        lds_barrier();   // the previous terminal's reads of rs and scan are done (depth-first:
+                         // and every thread's slots of the reach row are written)
+        if (!LDS_ROW) {
+            FOR_K if (hslot(k) < Ho) static_cast<V*>(c.rs)[hslot(k)] = rv[k];
+            lds_barrier();
+        }
+        segmented_scan<V>(c, R.u, R.T.scan_len, rs);
     }
     const HandScan (&qv)[KH] = R.q;
+    // the hand's result into its slot of the output row
+    auto put = [&](int k, V res) {
+        if constexpr (LDS_ROW) {
+            if (add) res = __fadd_rn(out[hslot(k)], res);
+        }
+        out[hslot(k)] = res;
+    };
 
     const double total = c.scan[R.T.total_pos];
     if (nd.kind == KIND_BOUNDARY) {
@@ -668,7 +683,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
         // every hand; win holds half_pot / num_combinations
         const int b = nd.state_off;
         float* rec = A.bnd_reach + c.G.bnd_base + (size_t)(2 * b + (p ^ 1)) * c.stride;
-        FOR_K if (hslot(k) < Ho) rec[hslot(k)] = (float)rv[k];
+        FOR_K if (hslot(k) < Ho) rec[hslot(k)] = (float)(LDS_ROW ? rs[hslot(k)] : rv[k]);
         const int mode = uni(A.bnd_mode[c.G.bnd_index + b]);
         const float* bv = A.bnd_cfv + c.G.bnd_base + (size_t)(2 * b + p) * c.stride;
         float val[KH];
@@ -685,7 +700,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
                 res = (V)(((double)val[k] * win) * cr);
             }
-            out[hslot(k)] = res;
+            put(k, res);
             }
             slot_fence();
         }
@@ -700,7 +715,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 const double cr = ((total + sm) - c.scan[q.w & 0xFFFF]) - c.scan[q.w >> 16];
                 res = (V)(payoff * cr);
             }
-            out[hslot(k)] = res;
+            put(k, res);
             }
             slot_fence();
         }
@@ -720,7 +735,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 const double ct = (S[q.x >> 16] - S[q.y >> 16]) - S[q.z >> 16];
                 res = (V)((win * cw + tie * ((ct - cw) + sm)) + lose * (tw - ct));
             }
-            out[hslot(k)] = res;
+            put(k, res);
             }
             slot_fence();
         }
@@ -737,7 +752,7 @@ __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, 
                 if constexpr (sizeof(V) == 8) res = win * cw + lose * cl;
                 else res = __fadd_rn((float)(win * cw), (float)(lose * cl));
             }
-            out[hslot(k)] = res;
+            put(k, res);
             }
             slot_fence();
         }
@@ -1312,6 +1327,195 @@ __device__ __forceinline__ void tile_traverse(Ctx& c, const TileDesc& T, int p, 
     tile_backward<MODE, SM, V>(c, T, p, disc);
 }
 
+// ---------------------------------------------------------- depth-first --
+// One traversal of a single-tile game from its depth-first program (DfsOp
+// list built by the host in the order of solver.rs:221-383).  Reach and
+// cfvalue rows are LDS rows of c.stride floats in the block after the scan
+// buffer (row 0: the root reach).  A thread touches only its own hand slots
+// of a row, except in a terminal's scan, which reads the whole reach row
+// after an LDS barrier; the host frees a row read that way only after the
+// next such barrier.  No scratch rows in HBM: global traffic is the state.
+// f32 storage only.
+
+__device__ __forceinline__ NodeState<SM_F32> dfs_state(const Ctx& c, const DfsOp& op) {
+    const size_t off = (size_t)c.G.state_base + op.arg;
+    return NodeState<SM_F32>{c.A.regret + off, c.A.strategy + off, nullptr, nullptr, 0.0f, 0.0f};
+}
+
+// The opponent's node: sigma from its regrets (regret matching, DCFR) or its
+// strategy sums (average strategy), child reach rows sigma_a * reach --
+// solver.rs:600-626, utility.rs:229-245, as opponent_rows.
+template <int MODE>
+__device__ __forceinline__ void dfs_opp(const Ctx& c, const DfsOp& op, float* rows, int Ho) {
+    const NodeState<SM_F32> st = dfs_state(c, op);
+    const float* rn = rows + (size_t)op.src * c.stride;
+    const int na = op.na;
+    float r[KH], v[DFS_MAXA][KH];
+    FOR_K r[k] = hslot(k) < Ho ? rn[hslot(k)] : 0.0f;
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            const NodeState<SM_F32> sa = action_state(st, a, Ho);
+            FOR_K v[a][k] = hslot(k) < Ho ? node_value<MODE>(sa, uslot(k)) : 0.0f;
+        }
+    float den[KH];
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) FOR_K {
+            const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
+            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+        }
+    const float def = __fdiv_rn(1.0f, (float)na);
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            float* ra = rows + (size_t)op.slot[a] * c.stride;
+            FOR_K if (hslot(k) < Ho) ra[hslot(k)] = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), r[k]);
+        }
+}
+
+// The traverser's node from its children's cfvalue rows: the max (best
+// response) or sum_a sigma_a * cfv_a with, in DCFR mode, the updates
+// S = S * gamma + sigma, R = (R * (alpha | beta) + cfv_a) - value
+// (solver.rs:334-383, 556-589), as traverser_rows / children_rows.
+template <int MODE>
+__device__ __forceinline__ void dfs_trav(const Ctx& c, const DfsOp& op, float* rows, int Hp, const float4 disc) {
+    const NodeState<SM_F32> st = dfs_state(c, op);
+    const int na = op.na;
+    float cv[DFS_MAXA][KH], v[DFS_MAXA][KH], sv[DFS_MAXA][KH];
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            const float* ca = rows + (size_t)op.slot[a] * c.stride;
+            const NodeState<SM_F32> sa = action_state(st, a, Hp);
+            FOR_K {
+                const bool live = hslot(k) < Hp;
+                cv[a][k] = live ? ca[hslot(k)] : 0.0f;
+                if (MODE != MODE_BR) v[a][k] = live ? node_value<MODE>(sa, uslot(k)) : 0.0f;
+                if (MODE == MODE_DCFR) sv[a][k] = live ? sa.S[uslot(k)] : 0.0f;
+            }
+        }
+    float res[KH];
+    if (MODE == MODE_BR) {
+        FOR_K res[k] = cv[0][k];
+#pragma unroll
+        for (int a = 1; a < DFS_MAXA; ++a)
+            if (a < na) FOR_K res[k] = fmaxr(res[k], cv[a][k]);
+    } else {
+        const float def = __fdiv_rn(1.0f, (float)na);
+        float den[KH];
+#pragma unroll
+        for (int a = 0; a < DFS_MAXA; ++a)
+            if (a < na) FOR_K {
+                const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
+                den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+            }
+#pragma unroll
+        for (int a = 0; a < DFS_MAXA; ++a)
+            if (a < na) FOR_K {
+                const float t = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), cv[a][k]);
+                res[k] = a == 0 ? t : __fadd_rn(res[k], t);
+            }
+        if (MODE == MODE_DCFR) {
+#pragma unroll
+            for (int a = 0; a < DFS_MAXA; ++a)
+                if (a < na) {
+                    const NodeState<SM_F32> sa = action_state(st, a, Hp);
+                    FOR_K if (hslot(k) < Hp) {
+                        const unsigned h = uslot(k);
+                        const float sig = strat_of<MODE>(v[a][k], den[k], def);
+                        sa.S[h] = __fadd_rn(__fmul_rn(sv[a][k], disc.z), sig);
+                        const float coef = signbit(v[a][k]) ? disc.y : disc.x;
+                        sa.R[h] = __fsub_rn(__fadd_rn(__fmul_rn(v[a][k], coef), cv[a][k]), res[k]);
+                    }
+                }
+        }
+    }
+    float* out = rows + (size_t)op.dst * c.stride;
+    if (op.flags & DFS_ADD) {
+        FOR_K if (hslot(k) < Hp) out[hslot(k)] = __fadd_rn(out[hslot(k)], res[k]);
+    } else {
+        FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+    }
+}
+
+// row dst (+)= row src: the value of an opponent's node whose parent sums it
+// with its siblings
+__device__ __forceinline__ void dfs_acc(const Ctx& c, const DfsOp& op, float* rows, int Hp) {
+    const float* src = rows + (size_t)op.src * c.stride;
+    float* out = rows + (size_t)op.dst * c.stride;
+    FOR_K if (hslot(k) < Hp) out[hslot(k)] = (op.flags & DFS_ADD) ? __fadd_rn(out[hslot(k)], src[hslot(k)]) : src[hslot(k)];
+}
+
+template <int MODE>
+__device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const DfsProg& P, int p, const float4 disc) {
+    const KernelArgs& A = c.A;
+    const int o = p ^ 1, Hp = num_hands(c.G, p), Ho = num_hands(c.G, o);
+    float* rows = static_cast<float*>(c.rs);
+    {   // row 0: the root reach, the opponent's initial weights
+        const float* w = A.hand_weight + hand_base(c.G, o);
+        float wv[KH];
+        FOR_K wv[k] = hslot(k) < Ho ? w[hslot(k)] : 0.0f;
+        FOR_K if (hslot(k) < Ho) rows[hslot(k)] = wv[k];
+    }
+    RunoutRegs R;
+    load_runout(c, T.runout, p, R);
+    const DfsOp* ops = A.dfs_ops + (p ? P.first[1] : P.first[0]);
+    const int n = p ? P.num_ops[1] : P.num_ops[0];
+    const float none[KH] = {};
+    for (int i = 0; i < n; ++i) {
+        const DfsOp op = uni_load(ops + i);
+        if (op.op == DFS_OPP) {
+            dfs_opp<MODE>(c, op, rows, Ho);
+        } else if (op.op == DFS_TRAV) {
+            dfs_trav<MODE>(c, op, rows, Hp, disc);
+        } else if (op.op == DFS_TERM) {
+            const NodeRec nd = uni_load(c.nodes + op.arg);
+            eval_terminal<float, true>(c, nd, op.arg, p, R, none, !(op.flags & DFS_SAME_SCAN),
+                                       rows + (size_t)op.src * c.stride, rows + (size_t)op.dst * c.stride,
+                                       (op.flags & DFS_ADD) != 0);
+        } else {
+            dfs_acc(c, op, rows, Hp);
+        }
+    }
+    // every thread is done with the rows (the last terminal's reads of its
+    // reach row included) before the next traversal writes them
+    lds_barrier();
+}
+
+// f64 weighted sum of the root cfvalues of p after a depth-first traversal,
+// as root_value
+__device__ __forceinline__ double dfs_root_value(Ctx& c, const DfsProg& P, int p) {
+    const float* w = c.A.hand_weight + hand_base(c.G, p);
+    const float* root = static_cast<const float*>(c.rs) + (size_t)(p ? P.root_slot[1] : P.root_slot[0]) * c.stride;
+    double s = 0.0;
+    for (int h = threadIdx.x; h < num_hands(c.G, p); h += BLOCK) s += (double)root[h] * (double)w[h];
+    return block_sum(s, c.sh);
+}
+
+// exploitability() with depth-first traversals
+__device__ __forceinline__ float dfs_exploitability(Ctx& c, const TileDesc& T, const DfsProg& P) {
+    float ev0 = 0.0f, ev1 = 0.0f;
+    const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+        dfs_traverse<MODE_BR>(c, T, P, p, none);
+        const float v = (float)dfs_root_value(c, P, p);
+        if (p == 0) ev0 = v;
+        else ev1 = v;
+    }
+    if (c.G.raked) {
+#pragma unroll 1
+        for (int p = 0; p < 2; ++p) {
+            dfs_traverse<MODE_CFV>(c, T, P, p, none);
+            const float v = (float)dfs_root_value(c, P, p);
+            if (p == 0) ev0 = __fsub_rn(ev0, v);
+            else ev1 = __fsub_rn(ev1, v);
+        }
+    }
+    return __fmul_rn(__fadd_rn(ev0, ev1), 0.5f);
+}
+
 __device__ __forceinline__ Ctx make_ctx(const KernelArgs& A, int gi, double* scan, void* rs, Shared* sh) {
     Ctx c{A, uni_load(A.games + gi), nullptr, nullptr, nullptr, nullptr, nullptr, 0, scan, rs, sh};
     c.nodes = A.nodes + c.G.node_base;
@@ -1447,21 +1651,40 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
             __syncthreads();
             continue;
         }
+        // the depth-first path (rows in LDS) when the game has a program
+        DfsProg P{};
+        if (SM == SM_F32 && A.dfs_progs) P = uni_load(A.dfs_progs + gi);
+        const bool dfs = SM == SM_F32 && P.num_ops[0] > 0;
+        // exploitability on the game's path (a macro: an inlined helper taking
+        // the context by reference left it in scratch memory)
+#define PSR_EXPL() (dfs ? dfs_exploitability(c, T, P) : exploitability<SM>(c, T))
         float expl = __int_as_float(0x7f800000);   // +inf: no check done yet
-        if (A.flags & JOB_INITIAL_CHECK) expl = exploitability<SM>(c, T);
+        if (A.flags & JOB_INITIAL_CHECK) expl = PSR_EXPL();
         uint32_t t = A.iter_begin;
         if (A.flags & JOB_ITERATE) {
             for (; t < A.iter_end; ++t) {
                 if (expl <= c.G.target) break;
                 const float4 d = A.discount[t];
 #pragma unroll 1
-                for (int p = 0; p < 2; ++p) tile_traverse<MODE_DCFR, SM>(c, T, p, d);
+                for (int p = 0; p < 2; ++p) {
+                    if constexpr (SM == SM_F32) {
+                        if (dfs) {
+                            dfs_traverse<MODE_DCFR>(c, T, P, p, d);
+                            continue;
+                        }
+                    }
+                    tile_traverse<MODE_DCFR, SM>(c, T, p, d);
+                }
                 if (A.check_interval > 0 && ((t + 1) % A.check_interval == 0 || t + 1 == A.max_iterations))
-                    expl = exploitability<SM>(c, T);
+                    expl = PSR_EXPL();
             }
         }
-        if (A.flags & JOB_FINAL_CHECK) expl = exploitability<SM>(c, T);
+        if (A.flags & JOB_FINAL_CHECK) expl = PSR_EXPL();
+#undef PSR_EXPL
+        // the level-by-level jobs below reuse the LDS rows
+        if (dfs) __syncthreads();
         if (A.flags & JOB_REACH) tile_forward<MODE_CFV, SM>(c, T, A.player);
+        if (A.flags & JOB_REACH_DCFR) tile_forward<MODE_DCFR, SM>(c, T, A.player);
         if (A.flags & JOB_ROOT_CFV) {
             const float4 none = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll 1
@@ -1587,7 +1810,16 @@ __global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs) {
 // (each launch takes games from a freshly reset work queue).
 template <int SM>
 hipError_t dcfr_launch_sm(const KernelArgs& args, int grid, hipStream_t stream) {
-    const size_t lds = dcfr_lds_bytes(args.max_hands);
+    const size_t lds = std::max<size_t>(dcfr_lds_bytes(args.max_hands), args.lds_bytes);
+    if (lds > 65536) {
+        // dynamic LDS beyond 64 KiB (depth-first rows) must be allowed per kernel
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dcfr_solve_kernel<SM, false>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+            return e;
+        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dcfr_solve_kernel<SM, true>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+            return e;
+    }
     if (args.flags & ~JOB_EXPL64) {
         if (hipError_t e = reset_counter(args.queue_counter, stream)) return e;
         hipLaunchKernelGGL((dcfr_solve_kernel<SM, false>), dim3(grid), dim3(BLOCK), lds, stream, args);

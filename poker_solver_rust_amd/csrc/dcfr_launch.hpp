@@ -16,6 +16,8 @@ enum JobFlags : int {
     JOB_EXPL64 = 16,         // FP64 best-response exploitability
     JOB_REACH = 32,          // forward pass of traversal `player` under the average strategy:
                              // every node's reach row then holds the opponent's reach
+    JOB_REACH_DCFR = 64,     // the same under the current regret-matching strategy (single-tile
+                             // games: the reach rows a DCFR traversal of `player` uses)
 };
 
 enum Phase : int {
@@ -105,6 +107,10 @@ struct KernelArgs {
     // A_root x H rows at root_act_off[game] (-1: root is not a decision node)
     float* root_action_cfv;
     const int64_t* root_act_off;
+    // depth-first traversal programs of single-tile games (dcfr_layout.hpp)
+    const DfsProg* dfs_progs;   // per game
+    const DfsOp* dfs_ops;
+    uint32_t lds_bytes;         // dynamic LDS of the whole-game launch (0: dcfr_lds_bytes(max_hands))
 };
 
 size_t dcfr_lds_bytes(int max_hands);

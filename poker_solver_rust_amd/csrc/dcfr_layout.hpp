@@ -138,6 +138,46 @@ struct GameDesc {
     int32_t pad;
 };
 
+// Depth-first traversal programs.  A single-tile game whose rows fit in LDS
+// is traversed depth first, in the order of the reference's recursive solve
+// (solver.rs:221-383), from a host-built list of operations per traversing
+// player.  Its reach and cfvalue rows live in LDS row slots instead of HBM:
+// a row is allocated when the traversal first writes it and released after
+// its last reader, so a traversal needs only about depth x actions rows.
+// Every per-hand operation is the one the level-by-level path performs, in
+// the same order, so both paths give identical results.
+constexpr int DFS_MAXA = 6;    // largest action count of a node on the depth-first path
+constexpr int DFS_SLOTS = 8;   // child rows one operation names
+enum DfsOpKind : uint8_t {
+    DFS_OPP = 0,    // the opponent's node: strategy from its state, child reach rows
+                    // slot[a] = sigma_a * row src
+    DFS_TRAV = 1,   // the traverser's node: value from the child cfvalue rows slot[a]
+                    // (and the state update in DCFR mode) into row dst
+    DFS_TERM = 2,   // terminal or depth-boundary leaf `arg` with reach row src into row dst
+    DFS_ACC = 3,    // row dst (+)= row src
+};
+enum DfsOpFlags : uint8_t {
+    DFS_ADD = 1,         // the result is added to row dst (otherwise it overwrites it)
+    DFS_SAME_SCAN = 2,   // terminal: the previous operation was a terminal on the same reach
+                         // row, whose scan is reused
+};
+struct DfsOp {
+    uint8_t op, flags, src, dst;
+    uint8_t na;
+    uint8_t pad0[3];
+    int32_t arg;            // OPP / TRAV: state offset of the node in its game; TERM: game-local node id
+    int32_t pad1[3];
+    uint8_t slot[DFS_SLOTS];
+};
+static_assert(sizeof(DfsOp) == 32, "DfsOp is 32 bytes");
+struct DfsProg {            // per game; num_ops[0] == 0: the game takes the level-by-level path
+    int32_t first[2];       // first operation of traversal p
+    int32_t num_ops[2];
+    int32_t root_slot[2];   // row holding the root cfvalues after traversal p
+    int32_t slots;          // LDS rows the game needs (row 0 holds the root reach)
+    int32_t pad;
+};
+
 // How a depth-boundary leaf produces its values (single continuation).
 enum BoundaryMode : int32_t {
     BND_UNSET = 0,    // no values given: the leaf contributes 0

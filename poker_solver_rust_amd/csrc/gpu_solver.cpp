@@ -81,6 +81,10 @@ struct HostLayout {
         double num_combinations = 0.0;
     };
     std::vector<Root> roots;
+    // depth-first traversal programs of the single-tile games (DfsProg per game; zero when
+    // the game takes the level-by-level path)
+    std::vector<DfsProg> dfs_progs;
+    std::vector<DfsOp> dfs_ops;
 };
 
 int32_t push_u16(std::vector<uint16_t>& pool, const std::vector<uint16_t>& v, size_t align = 4) {
@@ -258,6 +262,132 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
     }
     if ((int)plan.order.size() != N) return fail("tile plan does not cover the game");
     return 0;
+}
+
+// Depth-first program of traversal p of one single-tile game
+// (dcfr_layout.hpp): operations in the order of the reference's recursive
+// solve (solver.rs:221-383), LDS rows assigned lowest free first.
+//   terminal            TERM: its value from the reach row into the result row
+//   opponent's node     OPP: one reach row per child; the children's values are
+//                       summed into the result row in child order (the first
+//                       overwrites it), through a row of its own and an ACC when
+//                       the result row is itself a sum the node adds to
+//   traverser's node    each child's value into a row of its own (reach passed
+//                       through), then TRAV
+//   single-action node  passes reach and result row through to its child
+// A reach row is read across threads by the terminals' scans, so once
+// released it is reused only after the next terminal that starts with a
+// barrier (one that rescans).
+struct DfsBuilder {
+    const NodeRec* nodes;
+    int p;
+    std::vector<DfsOp>& ops;
+    std::vector<int> free_rows, pending;
+    int top = 0;
+    bool ok = true;
+
+    int alloc() {
+        if (free_rows.empty()) return top++;
+        auto it = std::min_element(free_rows.begin(), free_rows.end());
+        const int r = *it;
+        free_rows.erase(it);
+        return r;
+    }
+    void release(int r) { free_rows.push_back(r); }
+    void release_reach(int r) { pending.push_back(r); }
+
+    void term(int n, int src, int dst, bool add) {
+        DfsOp op{};
+        op.op = DFS_TERM;
+        op.arg = n;
+        op.src = (uint8_t)src;
+        op.dst = (uint8_t)dst;
+        const bool same = !ops.empty() && ops.back().op == DFS_TERM && ops.back().src == src;
+        op.flags = (uint8_t)((add ? DFS_ADD : 0) | (same ? DFS_SAME_SCAN : 0));
+        ops.push_back(op);
+        if (!same) {   // its barrier ends every earlier terminal's reads
+            free_rows.insert(free_rows.end(), pending.begin(), pending.end());
+            pending.clear();
+        }
+    }
+
+    void visit(int n, int rs, int dst, bool add) {
+        if (!ok) return;
+        const NodeRec& nd = nodes[n];
+        if (nd.kind == KIND_FOLD || nd.kind == KIND_SHOWDOWN || nd.kind == KIND_BOUNDARY) {
+            term(n, rs, dst, add);
+            return;
+        }
+        if (nd.kind != KIND_DECISION || nd.num_children > DFS_MAXA || nd.num_children == 0) {
+            ok = false;
+            return;
+        }
+        const int na = nd.num_children;
+        if (na == 1) {
+            visit(nd.first_child, rs, dst, add);
+            return;
+        }
+        DfsOp op{};
+        op.na = (uint8_t)na;
+        op.arg = nd.state_off;
+        int kids[DFS_MAXA];
+        if (nd.player != p) {
+            op.op = DFS_OPP;
+            op.src = (uint8_t)rs;
+            for (int a = 0; a < na; ++a) op.slot[a] = (uint8_t)(kids[a] = alloc());
+            ops.push_back(op);
+            const int sum = add ? alloc() : dst;
+            for (int a = 0; a < na; ++a) {
+                visit(nd.first_child + a, kids[a], sum, a > 0);
+                release_reach(kids[a]);
+            }
+            if (add) {
+                DfsOp acc{};
+                acc.op = DFS_ACC;
+                acc.src = (uint8_t)sum;
+                acc.dst = (uint8_t)dst;
+                acc.flags = DFS_ADD;
+                ops.push_back(acc);
+                release(sum);
+            }
+            return;
+        }
+        op.op = DFS_TRAV;
+        op.dst = (uint8_t)dst;
+        op.flags = add ? DFS_ADD : 0;
+        for (int a = 0; a < na; ++a) {
+            kids[a] = alloc();
+            visit(nd.first_child + a, rs, kids[a], false);
+            op.slot[a] = (uint8_t)kids[a];
+        }
+        ops.push_back(op);
+        for (int a = 0; a < na; ++a) release(kids[a]);
+    }
+};
+
+// Both traversals' programs of single-tile game gi (root: local node 0),
+// appended to L.dfs_ops; a zero DfsProg when the game does not qualify (a
+// node with more than DFS_MAXA actions, terminals on more than one runout).
+DfsProg build_dfs(HostLayout& L, const GameDesc& d) {
+    DfsProg prog{};
+    if (L.tiles[d.tile_base].runout < 0) return prog;
+    const size_t base = L.dfs_ops.size();
+    for (int p = 0; p < 2; ++p) {
+        const size_t first = L.dfs_ops.size();
+        DfsBuilder b{&L.nodes[d.node_base], p, L.dfs_ops};
+        const int reach = b.alloc(), root = b.alloc();   // rows 0 and 1
+        (void)reach;
+        b.visit(0, 0, root, false);
+        if (!b.ok || b.top > 255) {
+            L.dfs_ops.resize(base);
+            return DfsProg{};
+        }
+        prog.first[p] = (int32_t)first;
+        prog.num_ops[p] = (int32_t)(L.dfs_ops.size() - first);
+        prog.root_slot[p] = root;
+        prog.slots = std::max(prog.slots, b.top);
+    }
+    return prog;
 }
 
 int build_layout(const psr_game* const* games, int n, const float* targets, HostLayout& L) {
@@ -472,6 +602,7 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         } else {
             L.single_games.push_back(gi);
         }
+        L.dfs_progs.push_back(d.num_tiles == 1 ? build_dfs(L, d) : DfsProg{});
 
         {
             HostLayout::Root ri;
@@ -607,13 +738,26 @@ struct psr_batch {
     int32_t* d_bnd_mode = nullptr;
     float* d_root_action_cfv = nullptr;
     int64_t* d_root_act_off = nullptr;
+    // depth-first path: the device's LDS per workgroup, the programs of the games whose rows
+    // fit in it (zero entries take the level-by-level path) and the launch's LDS
+    int lds_max = 65536;
+    bool dfs_on = true;
+    std::vector<DfsProg> dfs_fit;
+    size_t dfs_lds = 0;
+    DfsProg* d_dfs_progs = nullptr;
+    DfsOp* d_dfs_ops = nullptr;
+    // the depth-first path writes no reach rows to HBM: after a job whose last traversal of a
+    // single-tile game ran on it, the rows download_reach reads are rebuilt by the forward
+    // pass of that traversal (JOB_REACH_DCFR after DCFR iterations, JOB_REACH after a best
+    // response)
+    int stale_reach = 0;
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_phase_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
-                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves, d_iter};
+                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves, d_iter, d_dfs_progs, d_dfs_ops};
         for (void* p : ptrs)
             if (p && !arena.base) (void)hipFree(p);
         arena.used = 0;
@@ -631,6 +775,8 @@ struct psr_batch {
         d_bnd_cfv = nullptr; d_bnd_reach = nullptr; d_bnd_mode = nullptr;
         d_root_action_cfv = nullptr; d_root_act_off = nullptr;
         d_iter = nullptr;
+        d_dfs_progs = nullptr;
+        d_dfs_ops = nullptr;
         drop_graphs();
         prepared = false;
     }
@@ -702,6 +848,9 @@ struct psr_batch {
         a.bnd_mode = d_bnd_mode;
         a.root_action_cfv = d_root_action_cfv;
         a.root_act_off = d_root_act_off;
+        a.dfs_progs = d_dfs_progs;
+        a.dfs_ops = d_dfs_ops;
+        a.lds_bytes = dfs_on ? (uint32_t)dfs_lds : 0u;
         return a;
     }
 
@@ -945,6 +1094,10 @@ struct psr_batch {
             a.flags = flags;
             a.player = reach_player;
             HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
+            if (flags & (JOB_ROOT_CFV | JOB_REACH | JOB_REACH_DCFR)) stale_reach = 0;   // level-by-level last
+            else if (flags & JOB_FINAL_CHECK) stale_reach = JOB_REACH;
+            else if ((flags & JOB_ITERATE) && end > begin) stale_reach = JOB_REACH_DCFR;
+            else if (flags & JOB_INITIAL_CHECK) stale_reach = JOB_REACH;
         }
         if (ev1) HIP_OK(hipEventRecord(ev1, stream));
         HIP_OK(hipStreamSynchronize(stream));
@@ -1031,6 +1184,7 @@ int psr_batch_create(int device, psr_batch** out) {
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&b->num_cus, hipDeviceAttributeMultiprocessorCount, device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&b->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
     if (e != hipSuccess) {
         delete b;
         return fail(std::string("HIP setup failed: ") + hipGetErrorString(e));
@@ -1057,6 +1211,21 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
     b->L = HostLayout{};
     if (build_layout(games, n, targets, b->L) != 0) return -1;
     HostLayout& L = b->L;
+    // depth-first programs whose LDS rows fit beside the scan buffer (f32 storage)
+    {
+        const size_t scan_bytes = sizeof(double) * (size_t)scan_slots(3 * L.max_hands + 53);
+        const size_t cap = (size_t)std::max(b->lds_max - 1024, 0);   // static LDS of the kernel
+        b->dfs_fit = L.dfs_progs;
+        b->dfs_lds = dcfr_lds_bytes(L.max_hands);
+        for (size_t gi = 0; gi < L.games.size(); ++gi) {
+            DfsProg& pr = b->dfs_fit[gi];
+            if (pr.num_ops[0] == 0) continue;
+            const size_t need = scan_bytes + (size_t)pr.slots * L.games[gi].row_stride * sizeof(float);
+            if (b->storage != SM_F32 || need > cap) pr = DfsProg{};
+            else b->dfs_lds = std::max(b->dfs_lds, need);
+        }
+    }
+    const std::vector<DfsProg> dfs_off(L.games.size());
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
     b->max_iterations = max_iterations;
@@ -1080,7 +1249,8 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         dev_buf_n(&b->d_root_cfv, L.cards.size()), dev_buf_n(&b->d_counter, 2), dev_buf_n(&b->d_ev, 2 * (size_t)n),
         dev_buf_n(&b->d_bnd_cfv, brows), dev_buf_n(&b->d_bnd_reach, brows), dev_buf_n(&b->d_bnd_mode, nbnd),
         dev_buf_n(&b->d_root_action_cfv, (size_t)std::max<int64_t>(L.root_act_total, 1)),
-        dev_buf_n(&b->d_iter, 1),
+        dev_buf_n(&b->d_iter, 1), dev_buf(&b->d_dfs_progs, b->dfs_on ? b->dfs_fit : dfs_off),
+        dev_buf(&b->d_dfs_ops, L.dfs_ops),
     };
     if (b->storage == SM_F32) {
         bufs.push_back(dev_buf_n(&b->d_regret, state));
@@ -1565,6 +1735,17 @@ int psr_batch_download_reach(psr_batch* b, float* out, int64_t cap, int32_t hand
     for (const GameDesc& d : L.games) need += (int64_t)d.num_nodes * hand_stride;
     if (!out || cap < need) return fail("out must hold sum over games of nodes x hand stride values");
     HIP_OK(hipSetDevice(b->device));
+    if (b->stale_reach && !L.single_games.empty()) {
+        // player 1's traversal, the last of an iteration or a best response, rebuilt on the
+        // single-tile games (multi-tile games always write their rows)
+        KernelArgs a = b->args();
+        a.flags = b->stale_reach;
+        a.player = 1;
+        a.max_iterations = b->max_iterations;
+        HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), b->grid_all()), b->stream));
+        HIP_OK(hipStreamSynchronize(b->stream));
+        b->stale_reach = 0;
+    }
     std::vector<float> rows(std::max<int64_t>(L.scratch_total, 1));
     HIP_OK(hipMemcpy(rows.data(), b->d_reach, (size_t)L.scratch_total * sizeof(float), hipMemcpyDeviceToHost));
     std::fill(out, out + need, 0.0f);
@@ -1691,6 +1872,26 @@ int psr_batch_autotune(psr_batch* b, uint32_t iterations, int32_t* chosen) {
     HIP_OK(hipStreamSynchronize(b->stream));
     if (chosen) std::copy(best.begin(), best.end(), chosen);
     return cached ? 1 : 0;
+}
+
+int psr_batch_set_depth_first(psr_batch* b, int32_t enable) {
+    if (!b) return fail("null batch");
+    b->dfs_on = enable != 0;
+    if (!b->prepared) return 0;
+    const std::vector<DfsProg> off(b->L.games.size());
+    const std::vector<DfsProg>& v = b->dfs_on ? b->dfs_fit : off;
+    HIP_OK(hipSetDevice(b->device));
+    HIP_OK(hipMemcpy(b->d_dfs_progs, v.data(), v.size() * sizeof(DfsProg), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int psr_batch_depth_first_info(psr_batch* b, int32_t* games, int64_t* lds_bytes) {
+    if (!b || !b->prepared) return fail("batch not prepared");
+    int32_t n = 0;
+    for (const DfsProg& pr : b->dfs_fit) n += pr.num_ops[0] > 0;
+    if (games) *games = b->dfs_on ? n : 0;
+    if (lds_bytes) *lds_bytes = b->dfs_on ? (int64_t)b->dfs_lds : (int64_t)dcfr_lds_bytes(b->L.max_hands);
+    return 0;
 }
 
 int psr_batch_launch_info(psr_batch* b, int* num_cus, int* grid, int* blocks_per_cu) {

@@ -132,6 +132,8 @@ def load_library(path: Optional[str] = None):
         "psr_batch_launch_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         "psr_batch_set_launch": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
         "psr_batch_autotune": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int32)]),
+        "psr_batch_set_depth_first": (C.c_int, [vp, C.c_int32]),
+        "psr_batch_depth_first_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
         "psr_game_num_boundaries": (C.c_int, [vp]),
         "psr_game_boundary_node": (C.c_int64, [vp, C.c_int]),
         "psr_game_boundary_pot": (C.c_int, [vp, C.c_int]),
@@ -809,3 +811,14 @@ class GpuBatchSolver:
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _check(load_library().psr_batch_launch_info(self._h, C.byref(a), C.byref(b), C.byref(c)))
         return {"num_cus": a.value, "grid": b.value, "blocks_per_cu": c.value}
+
+    def set_depth_first(self, enable: bool):
+        """Depth-first traversals with the rows in LDS for the single-tile
+        games that fit (default), or the level-by-level path for every game;
+        results are identical."""
+        _check(load_library().psr_batch_set_depth_first(self._h, 1 if enable else 0))
+
+    def depth_first_info(self) -> dict:
+        n, lds = C.c_int32(), C.c_int64()
+        _check(load_library().psr_batch_depth_first_info(self._h, C.byref(n), C.byref(lds)))
+        return {"games": n.value, "lds_bytes": lds.value}

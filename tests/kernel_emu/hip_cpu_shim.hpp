@@ -43,7 +43,8 @@ typedef void* hipStream_t;
 typedef void* hipEvent_t;
 enum hipMemcpyKind { hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice };
 enum { hipStreamNonBlocking = 1 };
-enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount };
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount, hipDeviceAttributeMaxSharedMemoryPerBlock };
+enum hipFuncAttribute { hipFuncAttributeMaxDynamicSharedMemorySize };
 
 #define __global__
 #define __device__
@@ -340,7 +341,12 @@ inline hipError_t hipGetDevice(int* d) { *d = 0; return hipSuccess; }
 template <class T>
 inline hipError_t hipGetSymbolAddress(void** p, T& symbol) { *p = (void*)&symbol; return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
-inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 2; return hipSuccess; }
+// two CUs, and gfx950's 160 KiB of LDS per workgroup
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
+    *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 163840 : 2;
+    return hipSuccess;
+}
+inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
 template <class T>
 inline hipError_t hipMalloc(T** p, size_t n) {
     // device memory is not cleared by hipMalloc: fill it with a pattern

@@ -286,6 +286,57 @@ def run_boundary_stepwise():
         raise AssertionError(f"exploitability {e} vs oracle {o.exploitability()}")
 
 
+def run_depth_first():
+    """The depth-first path (rows in LDS): seeded river subgames, a raked
+    river game and a depth-limited turn game with boundary leaves, all taking
+    that path, equal to the oracle after every iteration and to the
+    level-by-level path at the end (state, exploitability, solve() stop)."""
+    games = [s.build() for s in W.river_batch(2, seed=11, iterations=10)]
+    games += [std_game("33%, 75%, a", "2x, a", rake=(0.05, 30.0)), boundary_game()]
+    runs = []
+    for dfs in (True, False):
+        b = P.GpuBatchSolver(0)
+        b.set_depth_first(dfs)
+        b.prepare_batch(games, 12, [0.5] * len(games))
+        info = b.depth_first_info()
+        if info["games"] != (len(games) if dfs else 0):
+            raise AssertionError(f"depth-first games {info} (enabled: {dfs})")
+        set_boundaries(games[-1], [lambda k, q, v, raw: b.set_boundary_cfvs(len(games) - 1, k, q, v, raw)])
+        runs.append(b)
+    b, lv = runs
+    oracles = []
+    for g in games:
+        o = O.OracleGame(g.serialize())
+        if g.num_boundary_nodes():
+            set_boundaries(g, [lambda k, q, v, raw: o.set_boundary_cfvs(k, q, v, raw)])
+        oracles.append(o)
+    for t in range(3):
+        b.run_iterations(t, t + 1)
+        lv.run_iterations(t, t + 1)
+        for i, o in enumerate(oracles):
+            o.solve_step(t)
+            r, s = o.state()
+            same(b.regrets(i), r, f"game {i} iteration {t} regrets")
+            same(b.strategy_sum(i), s, f"game {i} iteration {t} strategy sums")
+            same(b.regrets(i), lv.regrets(i), f"game {i} iteration {t} regrets vs level-by-level")
+    same(b.exploitability(), lv.exploitability(), "exploitability vs level-by-level")
+    for i, o in enumerate(oracles):
+        e = float(b.exploitability()[i])
+        if abs(e - o.exploitability()) > 1e-6 * max(1.0, abs(o.exploitability())):
+            raise AssertionError(f"game {i} exploitability {e} vs oracle {o.exploitability()}")
+    for x in runs:
+        x.reset()
+    e1, i1 = b.solve(12, 4)
+    e2, i2 = lv.solve(12, 4)
+    same(e1, e2, "solve() exploitability vs level-by-level")
+    same(i1, i2, "solve() iterations vs level-by-level")
+    for i in range(len(games)):
+        same(b.strategy_sum(i), lv.strategy_sum(i), f"game {i} solve() strategy sums vs level-by-level")
+        for p in range(2):
+            same(b.root_cfvalues(compute=False)[i][p], lv.root_cfvalues(compute=False)[i][p],
+                 f"game {i} root cfvalues {p} vs level-by-level")
+
+
 def run_expected_values():
     """query.rs:498-760 at the root: normalized weights, per-action detail and
     expected values of both players, bit for bit with the oracle."""
