@@ -648,17 +648,15 @@ __device__ __forceinline__ void load_runout(const Ctx& c, int table, int p, Runo
 // (LDS_ROW true) the reach row is already the LDS row `row` (rv unused) and
 // each hand's value is written to, or with `add` added to, its slot of the
 // LDS row `out`.
+// win / lose / tie: the node's payoffs per combination (layout payoff[]).
 template <typename V, bool LDS_ROW = false>
 __device__ __forceinline__ void eval_terminal(Ctx& c, const NodeRec& nd, int n, int p, const RunoutRegs& R,
-                                              const V (&rv)[KH], bool rescan, const V* row = nullptr,
-                                              V* out = nullptr, bool add = false) {
+                                              const V (&rv)[KH], bool rescan, double win, double lose, double tie,
+                                              const V* row = nullptr, V* out = nullptr, bool add = false) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p), Ho = num_hands(c.G, p ^ 1);
     if (!LDS_ROW) out = cfv_rows<V>(c) + (size_t)n * c.stride;
     const V* rs = LDS_ROW ? row : static_cast<const V*>(c.rs);
-    const double win = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]);
-    const double lose = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]);
-    const double tie = uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]);
     if (rescan) {   // otherwise rs and the scan already hold this reach row
         lds_barrier();   // the previous terminal's reads of rs and scan are done (depth-first:
                          // and every thread's slots of the reach row are written)
@@ -1304,7 +1302,9 @@ __device__ __forceinline__ void tile_terminals(Ctx& c, const TileDesc& T, int p)
         const int nsrc = reach_src(c, nn, p);
         V rn[KH];
         FOR_K rn[k] = hslot(k) < Ho ? reach[(size_t)nsrc * c.stride + hslot(k)] : V(0);
-        eval_terminal<V>(c, nd, n, p, R, rv, rescan);
+        eval_terminal<V>(c, nd, n, p, R, rv, rescan, uni(c.A.payoff[(size_t)3 * (c.G.node_base + n) + 0]),
+                         uni(c.A.payoff[(size_t)3 * (c.G.node_base + n) + 1]),
+                         uni(c.A.payoff[(size_t)3 * (c.G.node_base + n) + 2]));
         // the next terminal reuses the scan when it reads the same reach row
         // on the same runout (e.g. fold and call below one of the traverser's nodes)
         rescan = nsrc != src || nnd.aux != table;
@@ -1335,7 +1335,23 @@ __device__ __forceinline__ void tile_traverse(Ctx& c, const TileDesc& T, int p, 
 // of a row, except in a terminal's scan, which reads the whole reach row
 // after an LDS barrier; the host frees a row read that way only after the
 // next such barrier.  No scratch rows in HBM: global traffic is the state.
-// f32 storage only.
+// The next operation's record is loaded while the current one runs, with a
+// vector load: waiting for it never waits for LDS, as a scalar load would
+// (both count on the same counter, which scalar loads decrement out of
+// order).  f32 storage only.
+
+// Lanes 8w..8w+7 of each wave hold word j = lane % 8 of an operation record.
+__device__ __forceinline__ uint32_t dfs_fetch(const DfsOp* op) {
+    return reinterpret_cast<const uint32_t*>(op)[threadIdx.x & 7];
+}
+__device__ __forceinline__ DfsOp dfs_unpack(uint32_t v) {
+    uint32_t w[sizeof(DfsOp) / 4];
+#pragma unroll
+    for (int j = 0; j < (int)(sizeof(DfsOp) / 4); ++j) w[j] = (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+    DfsOp op;
+    __builtin_memcpy(&op, w, sizeof(DfsOp));
+    return op;
+}
 
 __device__ __forceinline__ NodeState<SM_F32> dfs_state(const Ctx& c, const DfsOp& op) {
     const size_t off = (size_t)c.G.state_base + op.arg;
@@ -1452,6 +1468,9 @@ __device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const Df
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Hp = num_hands(c.G, p), Ho = num_hands(c.G, o);
     float* rows = static_cast<float*>(c.rs);
+    const DfsOp* ops = A.dfs_ops + (p ? P.first[1] : P.first[0]);
+    const int n = p ? P.num_ops[1] : P.num_ops[0];
+    uint32_t next = dfs_fetch(ops);
     {   // row 0: the root reach, the opponent's initial weights
         const float* w = A.hand_weight + hand_base(c.G, o);
         float wv[KH];
@@ -1460,18 +1479,21 @@ __device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const Df
     }
     RunoutRegs R;
     load_runout(c, T.runout, p, R);
-    const DfsOp* ops = A.dfs_ops + (p ? P.first[1] : P.first[0]);
-    const int n = p ? P.num_ops[1] : P.num_ops[0];
     const float none[KH] = {};
     for (int i = 0; i < n; ++i) {
-        const DfsOp op = uni_load(ops + i);
+        const DfsOp op = dfs_unpack(next);
+        next = dfs_fetch(ops + i + 1);   // ops[n] is the END record
         if (op.op == DFS_OPP) {
             dfs_opp<MODE>(c, op, rows, Ho);
         } else if (op.op == DFS_TRAV) {
             dfs_trav<MODE>(c, op, rows, Hp, disc);
         } else if (op.op == DFS_TERM) {
-            const NodeRec nd = uni_load(c.nodes + op.arg);
-            eval_terminal<float, true>(c, nd, op.arg, p, R, none, !(op.flags & DFS_SAME_SCAN),
+            NodeRec nd{};
+            nd.kind = op.kind;
+            nd.player = op.player;
+            nd.state_off = op.bnd;
+            const double tie = c.G.raked ? uni(A.payoff[(size_t)3 * (c.G.node_base + op.arg) + 2]) : 0.0;
+            eval_terminal<float, true>(c, nd, op.arg, p, R, none, !(op.flags & DFS_SAME_SCAN), op.win, op.lose, tie,
                                        rows + (size_t)op.src * c.stride, rows + (size_t)op.dst * c.stride,
                                        (op.flags & DFS_ADD) != 0);
         } else {
@@ -1758,7 +1780,9 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
                 load_runout(c, nd.aux, p, R);
                 V rv[KH];
                 FOR_K rv[k] = hslot(k) < Ho ? reach[hslot(k)] : V(0);
-                eval_terminal<V>(c, nd, n, p, R, rv, true);
+                eval_terminal<V>(c, nd, n, p, R, rv, true, uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 0]),
+                                 uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 1]),
+                                 uni(A.payoff[(size_t)3 * (c.G.node_base + n) + 2]));
             }
             __syncthreads();   // the next node reuses the scan buffers
         }

@@ -155,6 +155,7 @@ enum DfsOpKind : uint8_t {
                     // (and the state update in DCFR mode) into row dst
     DFS_TERM = 2,   // terminal or depth-boundary leaf `arg` with reach row src into row dst
     DFS_ACC = 3,    // row dst (+)= row src
+    DFS_END = 4,    // one past the last operation of a program (read, never executed)
 };
 enum DfsOpFlags : uint8_t {
     DFS_ADD = 1,         // the result is added to row dst (otherwise it overwrites it)
@@ -164,10 +165,15 @@ enum DfsOpFlags : uint8_t {
 struct DfsOp {
     uint8_t op, flags, src, dst;
     uint8_t na;
-    uint8_t pad0[3];
+    uint8_t kind, player;   // TERM: node kind, folded player (fold)
+    uint8_t pad;
     int32_t arg;            // OPP / TRAV: state offset of the node in its game; TERM: game-local node id
-    int32_t pad1[3];
-    uint8_t slot[DFS_SLOTS];
+    int32_t bnd;            // TERM depth-boundary leaf: its ordinal
+    union {
+        uint8_t slot[DFS_SLOTS];   // OPP / TRAV: child rows
+        double win;                // TERM: payoffs per combination (the tie payoff of a raked
+    };                             // showdown stays in the layout's payoff[])
+    double lose;
 };
 static_assert(sizeof(DfsOp) == 32, "DfsOp is 32 bytes");
 struct DfsProg {            // per game; num_ops[0] == 0: the game takes the level-by-level path

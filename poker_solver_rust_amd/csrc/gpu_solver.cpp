@@ -280,6 +280,7 @@ int plan_tiles(const PostFlopGame& g, TilePlan& plan) {
 // barrier (one that rescans).
 struct DfsBuilder {
     const NodeRec* nodes;
+    const double* payoff;   // the game's 3 x f64 per node
     int p;
     std::vector<DfsOp>& ops;
     std::vector<int> free_rows, pending;
@@ -300,6 +301,11 @@ struct DfsBuilder {
         DfsOp op{};
         op.op = DFS_TERM;
         op.arg = n;
+        op.kind = nodes[n].kind;
+        op.player = nodes[n].player;
+        op.bnd = nodes[n].state_off;
+        op.win = payoff[3 * n + 0];
+        op.lose = payoff[3 * n + 1];
         op.src = (uint8_t)src;
         op.dst = (uint8_t)dst;
         const bool same = !ops.empty() && ops.back().op == DFS_TERM && ops.back().src == src;
@@ -366,15 +372,16 @@ struct DfsBuilder {
 };
 
 // Both traversals' programs of single-tile game gi (root: local node 0),
-// appended to L.dfs_ops; a zero DfsProg when the game does not qualify (a
-// node with more than DFS_MAXA actions, terminals on more than one runout).
+// appended to L.dfs_ops, each closed by an END record; a zero DfsProg when
+// the game does not qualify (a node with more than DFS_MAXA actions,
+// terminals on more than one runout).
 DfsProg build_dfs(HostLayout& L, const GameDesc& d) {
     DfsProg prog{};
     if (L.tiles[d.tile_base].runout < 0) return prog;
     const size_t base = L.dfs_ops.size();
     for (int p = 0; p < 2; ++p) {
         const size_t first = L.dfs_ops.size();
-        DfsBuilder b{&L.nodes[d.node_base], p, L.dfs_ops};
+        DfsBuilder b{&L.nodes[d.node_base], &L.payoff[3 * (size_t)d.node_base], p, L.dfs_ops};
         const int reach = b.alloc(), root = b.alloc();   // rows 0 and 1
         (void)reach;
         b.visit(0, 0, root, false);
@@ -382,8 +389,12 @@ DfsProg build_dfs(HostLayout& L, const GameDesc& d) {
             L.dfs_ops.resize(base);
             return DfsProg{};
         }
+        const size_t end = L.dfs_ops.size();
+        DfsOp fin{};
+        fin.op = DFS_END;
+        L.dfs_ops.push_back(fin);
         prog.first[p] = (int32_t)first;
-        prog.num_ops[p] = (int32_t)(L.dfs_ops.size() - first);
+        prog.num_ops[p] = (int32_t)(end - first);
         prog.root_slot[p] = root;
         prog.slots = std::max(prog.slots, b.top);
     }

@@ -267,6 +267,8 @@ inline int __builtin_amdgcn_readfirstlane(int x) {
     }
     return first;
 }
+// readlane: lane `src` of the wave's value (every lane gets it)
+inline int __builtin_amdgcn_readlane(int x, int src) { return emu::exchange(x, src); }
 inline const void* __builtin_amdgcn_kernarg_segment_ptr() { return emu::cur_kernarg; }
 // DPP move (the patterns the kernels use): row_shr:n (0x111-0x11F) inside a
 // row of 16 lanes, row_bcast:15 (0x142, lane 15 of a row into the next row),
