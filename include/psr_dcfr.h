@@ -309,18 +309,18 @@ float psr_batch_last_kernel_ms(psr_batch* batch);
  * terminals, widest level, state floats, nodes.  No reference counterpart
  * (layout self-check of this library). */
 int psr_layout_validate(const psr_game* game, int64_t out[6]);
-/* Depth-first path of the whole-game kernel (no reference counterpart: the
- * reference's CPU solve is recursive, solver.rs:221-383, its GPU kernels walk
- * levels with every row in global memory).  Single-tile games with f32
- * storage whose reach and cfvalue rows fit in LDS beside the scan buffer are
- * traversed depth first from a host-built program, their rows in LDS instead
- * of HBM; results are identical to the level-by-level path.  enable = 0 sends
- * every game down the level-by-level path (may be called before or after
- * psr_batch_prepare).  On by default. */
+/* Depth-first path (no reference counterpart: the reference's CPU solve is
+ * recursive, solver.rs:221-383, its GPU kernels walk levels with every row in
+ * global memory).  Single-tile games and the river subtree tiles of turn and
+ * flop games, with f32 storage, whose reach and cfvalue rows fit in LDS
+ * beside the scan buffer are traversed depth first from a host-built
+ * program, their rows in LDS instead of HBM; results are identical to the
+ * level-by-level path.  enable = 0 sends every tile down the level-by-level
+ * path (may be called before or after psr_batch_prepare).  On by default. */
 int psr_batch_set_depth_first(psr_batch* batch, int32_t enable);
-/* Games of a prepared batch that take the depth-first path and the dynamic
- * LDS of its whole-game launch in bytes. */
-int psr_batch_depth_first_info(psr_batch* batch, int32_t* games, int64_t* lds_bytes);
+/* Tiles of a prepared batch that take the depth-first path and the dynamic
+ * LDS of its launches in bytes. */
+int psr_batch_depth_first_info(psr_batch* batch, int32_t* tiles, int64_t* lds_bytes);
 /* Device properties: number of CUs and the workgroups launched. */
 int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
 /* Launch shape of a prepared batch (no reference counterpart: the reference

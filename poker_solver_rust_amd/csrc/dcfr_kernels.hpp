@@ -1463,7 +1463,12 @@ __device__ __forceinline__ void dfs_acc(const Ctx& c, const DfsOp& op, float* ro
     FOR_K if (hslot(k) < Hp) out[hslot(k)] = (op.flags & DFS_ADD) ? __fadd_rn(out[hslot(k)], src[hslot(k)]) : src[hslot(k)];
 }
 
-template <int MODE>
+// SUBTREE: a subtree tile of a turn or flop game, whose root reach is the row
+// the chance node above wrote to HBM and whose root cfvalues go to the root's
+// HBM cfv row (the chance sums read them); otherwise a whole single-tile game,
+// its root reach the opponent's initial weights and its root cfvalues left in
+// LDS (dfs_root_value).
+template <int MODE, bool SUBTREE = false>
 __device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const DfsProg& P, int p, const float4 disc) {
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Hp = num_hands(c.G, p), Ho = num_hands(c.G, o);
@@ -1471,8 +1476,10 @@ __device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const Df
     const DfsOp* ops = A.dfs_ops + (p ? P.first[1] : P.first[0]);
     const int n = p ? P.num_ops[1] : P.num_ops[0];
     uint32_t next = dfs_fetch(ops);
-    {   // row 0: the root reach, the opponent's initial weights
-        const float* w = A.hand_weight + hand_base(c.G, o);
+    const int root = SUBTREE ? uni_load(A.level_ranges + T.level_base).start : 0;
+    {   // row 0: the root reach
+        const float* w = SUBTREE ? c.reach + (size_t)reach_src(c, root, p) * c.stride
+                                 : A.hand_weight + hand_base(c.G, o);
         float wv[KH];
         FOR_K wv[k] = hslot(k) < Ho ? w[hslot(k)] : 0.0f;
         FOR_K if (hslot(k) < Ho) rows[hslot(k)] = wv[k];
@@ -1499,6 +1506,11 @@ __device__ __forceinline__ void dfs_traverse(Ctx& c, const TileDesc& T, const Df
         } else {
             dfs_acc(c, op, rows, Hp);
         }
+    }
+    if (SUBTREE) {   // the root cfvalues (each thread its own hands) for the chance sums
+        const float* res = rows + (size_t)(p ? P.root_slot[1] : P.root_slot[0]) * c.stride;
+        float* out = c.cfv + (size_t)root * c.stride;
+        FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[hslot(k)];
     }
     // every thread is done with the rows (the last terminal's reads of its
     // reach row included) before the next traversal writes them
@@ -1675,7 +1687,7 @@ __global__ void __launch_bounds__(BLOCK) dcfr_solve_kernel(KernelArgs) {
         }
         // the depth-first path (rows in LDS) when the game has a program
         DfsProg P{};
-        if (SM == SM_F32 && A.dfs_progs) P = uni_load(A.dfs_progs + gi);
+        if (SM == SM_F32 && A.dfs_progs) P = uni_load(A.dfs_progs + c.G.tile_base);
         const bool dfs = SM == SM_F32 && P.num_ops[0] > 0;
         // exploitability on the game's path (a macro: an inlined helper taking
         // the context by reference left it in scratch memory)
@@ -1813,6 +1825,19 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         Ctx c = make_ctx(A, gi, scan, rs, &sh);
         const TileDesc T = uni_load(A.tiles + t);
         if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
+        if (A.fwd_only) {
+            tile_forward<MODE, SM, V>(c, T, p);
+            continue;
+        }
+        if constexpr (SM == SM_F32 && sizeof(V) == 4) {
+            // the depth-first path when the tile has a program
+            DfsProg P{};
+            if (A.dfs_progs) P = uni_load(A.dfs_progs + t);
+            if (P.num_ops[0] > 0) {
+                dfs_traverse<MODE, true>(c, T, P, p, disc);
+                continue;
+            }
+        }
         tile_traverse<MODE, SM, V>(c, T, p, disc);
     }
 }
@@ -1832,18 +1857,18 @@ __global__ void __launch_bounds__(BLOCK) dcfr_phase_kernel(KernelArgs) {
 
 // The FP64 best response runs as a kernel of its own after the other jobs
 // (each launch takes games from a freshly reset work queue).
+// Dynamic LDS beyond 64 KiB (depth-first rows) must be allowed per kernel.
+template <typename K>
+hipError_t allow_lds(K kernel, size_t lds) {
+    if (lds <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 template <int SM>
 hipError_t dcfr_launch_sm(const KernelArgs& args, int grid, hipStream_t stream) {
     const size_t lds = std::max<size_t>(dcfr_lds_bytes(args.max_hands), args.lds_bytes);
-    if (lds > 65536) {
-        // dynamic LDS beyond 64 KiB (depth-first rows) must be allowed per kernel
-        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dcfr_solve_kernel<SM, false>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
-            return e;
-        if (hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&dcfr_solve_kernel<SM, true>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
-            return e;
-    }
+    if (hipError_t e = allow_lds(dcfr_solve_kernel<SM, false>, lds)) return e;
+    if (hipError_t e = allow_lds(dcfr_solve_kernel<SM, true>, lds)) return e;
     if (args.flags & ~JOB_EXPL64) {
         if (hipError_t e = reset_counter(args.queue_counter, stream)) return e;
         hipLaunchKernelGGL((dcfr_solve_kernel<SM, false>), dim3(grid), dim3(BLOCK), lds, stream, args);
@@ -1857,12 +1882,16 @@ hipError_t dcfr_launch_sm(const KernelArgs& args, int grid, hipStream_t stream) 
 
 template <int SM>
 hipError_t dcfr_launch_phase_sm(const KernelArgs& args, int grid, hipStream_t stream) {
-    const size_t lds = dcfr_lds_bytes(args.max_hands);
+    // the subtree phase runs tiles depth first with their rows in LDS
+    const bool dfs = args.phase == PHASE_SUBTREES && !args.fwd_only;
+    const size_t lds = std::max<size_t>(dcfr_lds_bytes(args.max_hands), dfs ? args.lds_bytes : 0u);
     switch (args.mode) {
         case MODE_DCFR:
+            if (hipError_t e = allow_lds(dcfr_phase_kernel<SM, MODE_DCFR, float>, lds)) return e;
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_DCFR, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
             break;
         case MODE_BR:
+            if (hipError_t e = allow_lds(dcfr_phase_kernel<SM, MODE_BR, float>, lds)) return e;
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_BR, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
             break;
         case MODE_BR64:
@@ -1872,6 +1901,7 @@ hipError_t dcfr_launch_phase_sm(const KernelArgs& args, int grid, hipStream_t st
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_CFV, double>), dim3(grid), dim3(BLOCK), lds, stream, args);
             break;
         default:
+            if (hipError_t e = allow_lds(dcfr_phase_kernel<SM, MODE_CFV, float>, lds)) return e;
             hipLaunchKernelGGL((dcfr_phase_kernel<SM, MODE_CFV, float>), dim3(grid), dim3(BLOCK), lds, stream, args);
             break;
     }

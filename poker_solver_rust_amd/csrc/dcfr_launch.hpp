@@ -107,10 +107,12 @@ struct KernelArgs {
     // A_root x H rows at root_act_off[game] (-1: root is not a decision node)
     float* root_action_cfv;
     const int64_t* root_act_off;
-    // depth-first traversal programs of single-tile games (dcfr_layout.hpp)
-    const DfsProg* dfs_progs;   // per game
+    // depth-first traversal programs of single-tile games and subtree tiles (dcfr_layout.hpp)
+    const DfsProg* dfs_progs;   // per tile
     const DfsOp* dfs_ops;
-    uint32_t lds_bytes;         // dynamic LDS of the whole-game launch (0: dcfr_lds_bytes(max_hands))
+    uint32_t lds_bytes;         // dynamic LDS of the whole-game and subtree launches
+                                // (0: dcfr_lds_bytes(max_hands))
+    int fwd_only;               // PHASE_SUBTREES: only the forward pass, level by level (reach rows)
 };
 
 size_t dcfr_lds_bytes(int max_hands);

@@ -81,8 +81,8 @@ struct HostLayout {
         double num_combinations = 0.0;
     };
     std::vector<Root> roots;
-    // depth-first traversal programs of the single-tile games (DfsProg per game; zero when
-    // the game takes the level-by-level path)
+    // depth-first traversal programs of the single-tile games and the subtree tiles
+    // (DfsProg per tile; zero when the tile takes the level-by-level path)
     std::vector<DfsProg> dfs_progs;
     std::vector<DfsOp> dfs_ops;
 };
@@ -371,20 +371,23 @@ struct DfsBuilder {
     }
 };
 
-// Both traversals' programs of single-tile game gi (root: local node 0),
+// Both traversals' programs of tile t of game d -- a single-tile game, or a
+// subtree tile of a turn or flop game -- rooted at its level-0 node,
 // appended to L.dfs_ops, each closed by an END record; a zero DfsProg when
-// the game does not qualify (a node with more than DFS_MAXA actions,
+// the tile does not qualify (a node with more than DFS_MAXA actions,
 // terminals on more than one runout).
-DfsProg build_dfs(HostLayout& L, const GameDesc& d) {
+DfsProg build_dfs(HostLayout& L, const GameDesc& d, int t) {
     DfsProg prog{};
-    if (L.tiles[d.tile_base].runout < 0) return prog;
+    const TileDesc& td = L.tiles[t];
+    if (td.runout < 0) return prog;
+    const int root_node = L.level_ranges[td.level_base].start;
     const size_t base = L.dfs_ops.size();
     for (int p = 0; p < 2; ++p) {
         const size_t first = L.dfs_ops.size();
         DfsBuilder b{&L.nodes[d.node_base], &L.payoff[3 * (size_t)d.node_base], p, L.dfs_ops};
         const int reach = b.alloc(), root = b.alloc();   // rows 0 and 1
         (void)reach;
-        b.visit(0, 0, root, false);
+        b.visit(root_node, 0, root, false);
         if (!b.ok || b.top > 255) {
             L.dfs_ops.resize(base);
             return DfsProg{};
@@ -613,7 +616,12 @@ int build_layout(const psr_game* const* games, int n, const float* targets, Host
         } else {
             L.single_games.push_back(gi);
         }
-        L.dfs_progs.push_back(d.num_tiles == 1 ? build_dfs(L, d) : DfsProg{});
+        // programs of the tiles traversed whole: the game itself, or its subtree tiles
+        for (int t = 0; t < d.num_tiles; ++t) {
+            const TileDesc& td = L.tiles[d.tile_base + t];
+            const bool whole = d.num_tiles == 1 || (t > 0 && !(td.flags & TILE_HAS_CHANCE));
+            L.dfs_progs.push_back(whole ? build_dfs(L, d, d.tile_base + t) : DfsProg{});
+        }
 
         {
             HostLayout::Root ri;
@@ -760,8 +768,10 @@ struct psr_batch {
     // the depth-first path writes no reach rows to HBM: after a job whose last traversal of a
     // single-tile game ran on it, the rows download_reach reads are rebuilt by the forward
     // pass of that traversal (JOB_REACH_DCFR after DCFR iterations, JOB_REACH after a best
-    // response)
+    // response); for the multi-tile games, whose subtree tiles run depth first, stale_multi
+    // holds the traversal mode to rebuild with (-1: rows current)
     int stale_reach = 0;
+    int stale_multi = -1;
 
     void release() {
         void* ptrs[] = {d_games, d_tiles, d_levels, d_nodes, d_payoff, d_cards, d_same, d_weights, d_terms,
@@ -876,8 +886,9 @@ struct psr_batch {
 
     // one phase launch of a multi-tile traversal; dev_iter: the iteration
     // index is read from d_iter (graph capture) instead of t
-    int phase(int ph, int player, int mode, uint32_t t, bool dev_iter = false) {
+    int phase(int ph, int player, int mode, uint32_t t, bool dev_iter = false, bool fwd_only = false) {
         KernelArgs a = args();
+        a.fwd_only = fwd_only ? 1 : 0;
         a.phase = ph;
         a.player = player;
         a.mode = mode;
@@ -893,6 +904,22 @@ struct psr_batch {
         return 0;
     }
     int grid_all() const { return num_cus * blocks_per_cu; }
+
+    // The reach rows of traversal `player` of every multi-tile game under the regret-matching
+    // (MODE_DCFR) or average (MODE_CFV) strategy: the root and middle tile forwards and the
+    // subtree tiles' forwards, level by level (every game active).
+    int rebuild_reach(int player, int mode) {
+        std::vector<float> active(L.games.size(), INFINITY);
+        if (set_expl_all(active)) return -1;
+        if (phase(PHASE_ROOT_FORWARD, player, mode, 0)) return -1;
+        if (phase(PHASE_MID_FORWARD, player, mode, 0)) return -1;
+        return phase(PHASE_SUBTREES, player, mode, 0, false, true);
+    }
+    int set_expl_all(const std::vector<float>& v) {
+        HIP_OK(hipMemcpyAsync(d_phase_expl, v.data(), v.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_OK(hipStreamSynchronize(stream));
+        return 0;
+    }
 
     // f64 scratch rows for the FP64 best response, allocated on first use
     int ensure64() {
@@ -1077,12 +1104,17 @@ struct psr_batch {
                 for (int p = 0; p < 2; ++p)
                     if (traverse(p, MODE_CFV, 0)) return -1;
             }
+            if (flags & (JOB_ITERATE | JOB_INITIAL_CHECK | JOB_FINAL_CHECK | JOB_ROOT_CFV)) {
+                // the last traversal's subtree rows stayed in LDS
+                stale_multi = (flags & (JOB_FINAL_CHECK | JOB_ROOT_CFV)) ? MODE_CFV
+                              : (flags & JOB_ITERATE) && end > begin ? MODE_DCFR
+                                                                      : MODE_CFV;
+            }
             if (flags & JOB_REACH) {
-                // root and middle tile forwards, then every subtree tile (its forward fills the rows;
-                // its terminals and backward only touch cfv rows)
-                if (phase(PHASE_ROOT_FORWARD, reach_player, MODE_CFV, 0)) return -1;
-                if (phase(PHASE_MID_FORWARD, reach_player, MODE_CFV, 0)) return -1;
-                if (phase(PHASE_SUBTREES, reach_player, MODE_CFV, 0)) return -1;
+                // root and middle tile forwards, then the forward of every subtree tile (level
+                // by level: it writes the reach rows)
+                if (rebuild_reach(reach_player, MODE_CFV)) return -1;
+                stale_multi = -1;
             }
             // publish per-game results next to the single-tile ones (solve jobs only)
             const bool solve_job = flags & (JOB_INITIAL_CHECK | JOB_ITERATE | JOB_FINAL_CHECK);
@@ -1228,15 +1260,15 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         const size_t cap = (size_t)std::max(b->lds_max - 1024, 0);   // static LDS of the kernel
         b->dfs_fit = L.dfs_progs;
         b->dfs_lds = dcfr_lds_bytes(L.max_hands);
-        for (size_t gi = 0; gi < L.games.size(); ++gi) {
-            DfsProg& pr = b->dfs_fit[gi];
+        for (size_t t = 0; t < L.tiles.size(); ++t) {
+            DfsProg& pr = b->dfs_fit[t];
             if (pr.num_ops[0] == 0) continue;
-            const size_t need = scan_bytes + (size_t)pr.slots * L.games[gi].row_stride * sizeof(float);
+            const size_t need = scan_bytes + (size_t)pr.slots * L.games[L.tiles[t].game].row_stride * sizeof(float);
             if (b->storage != SM_F32 || need > cap) pr = DfsProg{};
             else b->dfs_lds = std::max(b->dfs_lds, need);
         }
     }
-    const std::vector<DfsProg> dfs_off(L.games.size());
+    const std::vector<DfsProg> dfs_off(L.tiles.size());
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
     b->max_iterations = max_iterations;
@@ -1746,6 +1778,11 @@ int psr_batch_download_reach(psr_batch* b, float* out, int64_t cap, int32_t hand
     for (const GameDesc& d : L.games) need += (int64_t)d.num_nodes * hand_stride;
     if (!out || cap < need) return fail("out must hold sum over games of nodes x hand stride values");
     HIP_OK(hipSetDevice(b->device));
+    if (b->stale_multi >= 0 && !L.multi_games.empty()) {
+        if (b->rebuild_reach(1, b->stale_multi)) return -1;
+        HIP_OK(hipStreamSynchronize(b->stream));
+        b->stale_multi = -1;
+    }
     if (b->stale_reach && !L.single_games.empty()) {
         // player 1's traversal, the last of an iteration or a best response, rebuilt on the
         // single-tile games (multi-tile games always write their rows)
@@ -1889,18 +1926,18 @@ int psr_batch_set_depth_first(psr_batch* b, int32_t enable) {
     if (!b) return fail("null batch");
     b->dfs_on = enable != 0;
     if (!b->prepared) return 0;
-    const std::vector<DfsProg> off(b->L.games.size());
+    const std::vector<DfsProg> off(b->L.tiles.size());
     const std::vector<DfsProg>& v = b->dfs_on ? b->dfs_fit : off;
     HIP_OK(hipSetDevice(b->device));
     HIP_OK(hipMemcpy(b->d_dfs_progs, v.data(), v.size() * sizeof(DfsProg), hipMemcpyHostToDevice));
     return 0;
 }
 
-int psr_batch_depth_first_info(psr_batch* b, int32_t* games, int64_t* lds_bytes) {
+int psr_batch_depth_first_info(psr_batch* b, int32_t* tiles, int64_t* lds_bytes) {
     if (!b || !b->prepared) return fail("batch not prepared");
     int32_t n = 0;
     for (const DfsProg& pr : b->dfs_fit) n += pr.num_ops[0] > 0;
-    if (games) *games = b->dfs_on ? n : 0;
+    if (tiles) *tiles = b->dfs_on ? n : 0;
     if (lds_bytes) *lds_bytes = b->dfs_on ? (int64_t)b->dfs_lds : (int64_t)dcfr_lds_bytes(b->L.max_hands);
     return 0;
 }

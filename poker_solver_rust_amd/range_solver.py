@@ -819,6 +819,8 @@ class GpuBatchSolver:
         _check(load_library().psr_batch_set_depth_first(self._h, 1 if enable else 0))
 
     def depth_first_info(self) -> dict:
+        """Tiles on the depth-first path (single-tile games and the river
+        subtree tiles of turn and flop games) and the LDS of its launches."""
         n, lds = C.c_int32(), C.c_int64()
         _check(load_library().psr_batch_depth_first_info(self._h, C.byref(n), C.byref(lds)))
-        return {"games": n.value, "lds_bytes": lds.value}
+        return {"tiles": n.value, "lds_bytes": lds.value}

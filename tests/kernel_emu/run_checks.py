@@ -299,7 +299,7 @@ def run_depth_first():
         b.set_depth_first(dfs)
         b.prepare_batch(games, 12, [0.5] * len(games))
         info = b.depth_first_info()
-        if info["games"] != (len(games) if dfs else 0):
+        if info["tiles"] != (len(games) if dfs else 0):
             raise AssertionError(f"depth-first games {info} (enabled: {dfs})")
         set_boundaries(games[-1], [lambda k, q, v, raw: b.set_boundary_cfvs(len(games) - 1, k, q, v, raw)])
         runs.append(b)

@@ -32,7 +32,7 @@ def test_river_batch_paths_identical():
     subs = W.river_batch(96, seed=20261)
     games = [s.build() for s in subs]
     (bd, ed, itd, info), (bl, el, itl, info_l) = solve_both(games, 300, [s.target for s in subs])
-    assert info["games"] == len(games) and info_l["games"] == 0
+    assert info["tiles"] == len(games) and info_l["tiles"] == 0
     assert info["lds_bytes"] > 65536
     assert np.array_equal(itd, itl)
     assert np.array_equal(ed, el)
@@ -51,7 +51,7 @@ def test_depth_first_stepwise_matches_oracle():
     games = [s.build() for s in W.river_batch(3, seed=77, iterations=8)]
     b = P.GpuBatchSolver(0)
     b.prepare_batch(games, 8)
-    assert b.depth_first_info()["games"] == len(games)
+    assert b.depth_first_info()["tiles"] == len(games)
     oracles = [O.OracleGame(g.serialize()) for g in games]
     for t in range(8):
         b.run_iterations(t, t + 1)
@@ -70,4 +70,47 @@ def test_depth_first_off_for_other_storage():
     g = W.river_batch(1, seed=3)[0].build()
     b = P.GpuBatchSolver(0, storage="fp16")
     b.prepare_batch([g], 4)
-    assert b.depth_first_info()["games"] == 0
+    assert b.depth_first_info()["tiles"] == 0
+
+
+def flop_game(oop, ip, board, bets="50%, a"):
+    cc = P.CardConfig(range=(P.Range.parse(oop), P.Range.parse(ip)), flop=P.flop_from_str(board))
+    s = [(bets, ""), (bets, "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s)
+    return P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+
+
+def test_subtree_tiles_paths_identical():
+    """Turn and flop games: their river subtree tiles run depth first in the
+    subtree phase (root reach from the chance node's HBM row, root cfvalues
+    back to HBM for the chance sums); solve(), download_reach after the
+    iterations and after a best response, and reach_at_nodes equal to the
+    level-by-level path."""
+    games = [W.turn_uniform01(20260).build(), flop_game("AA,KK,AKs", "QQ-JJ,AQs", "Qs Jh 2c"),
+             flop_game("AA,KK", "QQ,AK", "2s 7s Js", "a")]
+    runs = []
+    for dfs in (True, False):
+        b = P.GpuBatchSolver(0)
+        b.set_depth_first(dfs)
+        b.prepare_batch(games, 30, [0.001] * len(games))
+        info = b.depth_first_info()
+        print(f"depth-first {dfs}: {info}", flush=True)
+        assert (info["tiles"] > 0) == dfs
+        b.run_iterations(0, 7)
+        r_iter = b.download_reach(1326)
+        e = b.exploitability()
+        r_br = b.download_reach(1326)
+        reach = b.reach_at_nodes(1, 0, list(range(0, games[1].num_nodes(), 97)))
+        b.reset()
+        es, its = b.solve(30, 5)
+        runs.append((b, e, r_iter, r_br, reach, es, its))
+    (bd, *d), (bl, *l) = runs
+    for x, y, what in zip(d, l, ["exploitability", "download_reach", "download_reach after br", "reach_at_nodes",
+                                 "solve exploitability", "solve iterations"]):
+        assert np.array_equal(x, y), what
+    for i in range(len(games)):
+        assert np.array_equal(bd.strategy_sum(i), bl.strategy_sum(i)), i
+        assert np.array_equal(bd.regrets(i), bl.regrets(i)), i
+        for p in range(2):
+            assert np.array_equal(bd.root_cfvalues(compute=False)[i][p], bl.root_cfvalues(compute=False)[i][p]), (i, p)
