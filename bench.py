@@ -225,13 +225,23 @@ def net_solve_bench(args):
     ts = N.river_training_set(turn, pots, args.net_train_solves, seed=args.seed + 1, iterations=args.iterations,
                               log=log, situations=situations, checkdown=chk)
     t_data = time.time() - t0
+    # one net per training seed on the same data; the ensemble averages their corrections
+    seeds = [args.seed + 97 * k for k in range(max(1, args.net_seeds))]
+    nets, per_seed = [], []
     t0 = time.time()
-    net, losses, mae = N.train_value_net(ts, epochs=args.net_epochs, seed=args.seed)
+    for sd in seeds:
+        net, losses, mae = N.train_value_net(ts, epochs=args.net_epochs, seed=sd)
+        nets.append(net)
+        per_seed.append({"seed": sd, "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                         "holdout_mae_half_pots": mae})
+        log(f"net seed {sd}: loss {losses[0]:.4f} -> {losses[-1]:.4f}, hold-out MAE {mae:.4f} half-pots")
     t_train = time.time() - t0
-    log(f"trained on exact - check-down residuals: loss {losses[0]:.4f} -> {losses[-1]:.4f}, hold-out MAE "
-        f"{mae:.4f} half-pots, {t_train:.1f}s")
+    dev = N.DeviceCheckdownLeaves(turn.board[:4], dl_game.num_boundary_nodes())
+    for k, net in enumerate(nets):   # each net alone, to show the seed-to-seed spread
+        _, b_one, _, _ = N.solve_depth_limited(turn, N.NetLeaves(net, dev), iters, 10, lambda m: None)
+        per_seed[k]["b"] = b_one
     t0 = time.time()
-    g_net, b_net, leaf_net, dcfr_net = N.solve_depth_limited(turn, N.NetLeaves(net, chk), iters, 10, log)
+    g_net, b_net, leaf_net, dcfr_net = N.solve_depth_limited(turn, N.NetLeaves(nets, dev), iters, 10, log)
     t_net = time.time() - t0
     t0 = time.time()
     g_exl, b_exl, leaf_exl, dcfr_exl = N.solve_depth_limited(turn, N.ExactLeaves(turn, args.iterations), iters, 10,
@@ -245,6 +255,10 @@ def net_solve_bench(args):
     t_ex = time.time() - t0
     err = {name: N.root_ev_error(b_ex, b, g_ex, turn.pot)
            for name, b in (("net_backed", b_net), ("checkdown", b_chk), ("exact_leaves", b_exl))}
+    for ps in per_seed:
+        e = N.root_ev_error(b_ex, ps.pop("b"), g_ex, turn.pot)
+        ps["game_value_error"] = max(e["p0_value_diff_pot"], e["p1_value_diff_pot"])
+        log(f"net seed {ps['seed']} alone: game value error {ps['game_value_error'] * 100:.3f}% of the pot")
     # the net against the same method with exact leaves (isolates the net)
     err["net_vs_exact_leaves"] = N.root_ev_error(b_exl, b_net, g_exl, turn.pot)
     for k, v in err.items():
@@ -254,7 +268,7 @@ def net_solve_bench(args):
     # strategy) reaches, net and check-down against exact river solves
     items = N.boundary_items(turn, g_exl, b_exl)
     v_ex = N.ExactLeaves(turn, args.iterations).values_for(items)
-    leaf = {"net": N.leaf_error(items, N.NetLeaves(net, chk).values_for(items), v_ex),
+    leaf = {"net": N.leaf_error(items, N.NetLeaves(nets, dev).values_for(items), v_ex),
             "checkdown": N.leaf_error(items, chk.values_for(items), v_ex)}
     log(f"leaf error (fraction of pot, reach-weighted): {leaf}")
     out = {"metric": "net-backed turn solve: game value error vs the exact turn solve (fraction of pot)",
@@ -266,10 +280,10 @@ def net_solve_bench(args):
            "config": {"workload": "turn_uniform01 seed %d, depth_limit 0 at the river deal" % args.seed,
                       "turn_iterations": iters, "leaf_refresh": 10, "train_solves": ts.solves,
                       "train_solve_iterations_cap": args.iterations, "epochs": args.net_epochs,
-                      "net": "2706-512x4(GELU)-2652 bf16"},
+                      "net": "2706-512x4(GELU)-2652 bf16", "ensemble_seeds": seeds},
+           "per_seed": per_seed,
            **err,
-           "training": {"loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
-                        "holdout_mae_half_pots": mae, "data_seconds": t_data, "solve_seconds": ts.solve_seconds,
+           "training": {"data_seconds": t_data, "solve_seconds": ts.solve_seconds,
                         "river_solves_per_s": ts.solves / max(ts.solve_seconds, 1e-9), "train_seconds": t_train},
            "seconds": {"net_backed_turn": t_net, "net_leaf_eval": leaf_net, "net_dcfr": dcfr_net,
                        "checkdown_turn": t_chk, "exact_leaves_turn": t_exl, "exact_turn": t_ex}}
@@ -511,6 +525,7 @@ def main():
                     help="run the net-backed depth-limited turn solve instead (river solves -> value net -> turn)")
     ap.add_argument("--net-train-solves", type=int, default=32768, help="river solves in the value-net training set")
     ap.add_argument("--net-epochs", type=int, default=40)
+    ap.add_argument("--net-seeds", type=int, default=3, help="value nets trained (ensemble of their corrections)")
     ap.add_argument("--turn-solve-iters", type=int, default=500)
     ap.add_argument("--half-storage", action="store_true",
                     help="reduced-precision turn solve (FP16 / BF16 storage, FP64 oracle) instead")

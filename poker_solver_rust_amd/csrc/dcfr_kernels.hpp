@@ -566,11 +566,16 @@ __device__ __forceinline__ void segmented_scan(Ctx& c, const ScanRun& u, int len
         seg_dpp_step<0x143, 0xC>(fi, s);   // row_bcast:31 into rows 2 and 3
         f = fi;
     }
-    bool fe = __shfl_up((int)f, 1, WAVE);
-    double se = __shfl_up(s, 1, WAVE);
-    if (lane == 0) {
-        fe = false;
-        se = 0.0;
+    // exclusive values: the inclusive pair of the lane below (DPP wave_shr:1,
+    // lane 0 receives (0, 0))
+    bool fe;
+    double se;
+    {
+        fe = __builtin_amdgcn_update_dpp(0, (int)f, 0x138, 0xF, 0xF, true) != 0;
+        const uint64_t sb = (uint64_t)__double_as_longlong(s);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)sb, 0x138, 0xF, 0xF, true);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(sb >> 32), 0x138, 0xF, 0xF, true);
+        se = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
     }
     if (lane == WAVE - 1) {
         c.sh->wave_f[wave] = f;
