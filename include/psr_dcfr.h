@@ -318,9 +318,12 @@ int psr_layout_validate(const psr_game* game, int64_t out[6]);
  * level-by-level path.  enable = 0 sends every tile down the level-by-level
  * path (may be called before or after psr_batch_prepare).  On by default. */
 int psr_batch_set_depth_first(psr_batch* batch, int32_t enable);
-/* Tiles of a prepared batch that take the depth-first path and the dynamic
- * LDS of its launches in bytes. */
-int psr_batch_depth_first_info(psr_batch* batch, int32_t* tiles, int64_t* lds_bytes);
+/* Tiles of a prepared batch that take the depth-first path, how many of them
+ * are subtree tiles of games with at most 256 hands per player that run one
+ * tile per wave (eight at once per workgroup, each wave with its own LDS
+ * region), and the dynamic LDS of the whole-workgroup launches in bytes.
+ * Any output pointer may be null. */
+int psr_batch_depth_first_info(psr_batch* batch, int32_t* tiles, int32_t* wave_tiles, int64_t* lds_bytes);
 /* Device properties: number of CUs and the workgroups launched. */
 int psr_batch_launch_info(psr_batch* batch, int* num_cus, int* grid, int* blocks_per_cu);
 /* Launch shape of a prepared batch (no reference counterpart: the reference

@@ -1755,7 +1755,10 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
     const uint32_t it = A.iteration_dev ? uni((int)*A.iteration_dev) : A.iteration;
     const float4 disc = MODE == MODE_DCFR ? A.discount[it] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (A.phase == PHASE_ROOT_FORWARD || A.phase == PHASE_ROOT_BACKWARD) {
-        if (A.phase == PHASE_ROOT_FORWARD && blockIdx.x == 0 && threadIdx.x == 0) A.queue_counter[1] = 0;
+        if (A.phase == PHASE_ROOT_FORWARD && blockIdx.x == 0 && threadIdx.x == 0) {
+            A.queue_counter[1] = 0;
+            A.queue_counter[2] = 0;
+        }
         for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {
             const int gi = uni(A.multi_games[k]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
@@ -1847,6 +1850,320 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
     }
 }
 
+// ------------------------------------------------------------- wave path --
+// Subtree tiles of games with at most WAVE_HANDS hands per player (a flop
+// game's river tiles): one wave runs a tile's depth-first program alone,
+// WAVE_KH hands per lane (h = lane + 64 k), so the eight waves of a workgroup
+// traverse eight tiles at once and nothing waits at a workgroup barrier.
+// Each wave owns an LDS region: the one-wave scan output, then the tile's
+// rows.  Lanes exchange data only through their wave's region, and a wave's
+// LDS operations complete in order, so ordering them needs only a fence the
+// compiler cannot move accesses across.  The per-hand arithmetic is
+// eval_terminal's, the operation semantics dfs_opp / dfs_trav / dfs_acc's.
+#define FOR_W _Pragma("unroll") for (int k = 0; k < WAVE_KH; ++k)
+__device__ __forceinline__ int wslot(int k) { return (int)(threadIdx.x & (WAVE - 1)) + WAVE * k; }
+
+__device__ __forceinline__ void wave_lds_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+template <int MODE>
+__device__ __forceinline__ void w_opp(const Ctx& c, const DfsOp& op, float* rows, int Ho) {
+    const NodeState<SM_F32> st = dfs_state(c, op);
+    const float* rn = rows + (size_t)op.src * c.stride;
+    const int na = op.na;
+    float r[WAVE_KH], v[DFS_MAXA][WAVE_KH];
+    FOR_W r[k] = wslot(k) < Ho ? rn[wslot(k)] : 0.0f;
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            const NodeState<SM_F32> sa = action_state(st, a, Ho);
+            FOR_W v[a][k] = wslot(k) < Ho ? node_value<MODE>(sa, (size_t)wslot(k)) : 0.0f;
+        }
+    float den[WAVE_KH];
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) FOR_W {
+            const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
+            den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+        }
+    const float def = __fdiv_rn(1.0f, (float)na);
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            float* ra = rows + (size_t)op.slot[a] * c.stride;
+            FOR_W if (wslot(k) < Ho) ra[wslot(k)] = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), r[k]);
+        }
+}
+
+template <int MODE>
+__device__ __forceinline__ void w_trav(const Ctx& c, const DfsOp& op, float* rows, int Hp, const float4 disc) {
+    const NodeState<SM_F32> st = dfs_state(c, op);
+    const int na = op.na;
+    float cv[DFS_MAXA][WAVE_KH], v[DFS_MAXA][WAVE_KH], sv[DFS_MAXA][WAVE_KH];
+#pragma unroll
+    for (int a = 0; a < DFS_MAXA; ++a)
+        if (a < na) {
+            const float* ca = rows + (size_t)op.slot[a] * c.stride;
+            const NodeState<SM_F32> sa = action_state(st, a, Hp);
+            FOR_W {
+                const bool live = wslot(k) < Hp;
+                cv[a][k] = live ? ca[wslot(k)] : 0.0f;
+                if (MODE != MODE_BR) v[a][k] = live ? node_value<MODE>(sa, (size_t)wslot(k)) : 0.0f;
+                if (MODE == MODE_DCFR) sv[a][k] = live ? sa.S[wslot(k)] : 0.0f;
+            }
+        }
+    float res[WAVE_KH];
+    if (MODE == MODE_BR) {
+        FOR_W res[k] = cv[0][k];
+#pragma unroll
+        for (int a = 1; a < DFS_MAXA; ++a)
+            if (a < na) FOR_W res[k] = fmaxr(res[k], cv[a][k]);
+    } else {
+        const float def = __fdiv_rn(1.0f, (float)na);
+        float den[WAVE_KH];
+#pragma unroll
+        for (int a = 0; a < DFS_MAXA; ++a)
+            if (a < na) FOR_W {
+                const float x = MODE == MODE_DCFR ? fmax0(v[a][k]) : v[a][k];
+                den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+            }
+#pragma unroll
+        for (int a = 0; a < DFS_MAXA; ++a)
+            if (a < na) FOR_W {
+                const float t = __fmul_rn(strat_of<MODE>(v[a][k], den[k], def), cv[a][k]);
+                res[k] = a == 0 ? t : __fadd_rn(res[k], t);
+            }
+        if (MODE == MODE_DCFR) {
+#pragma unroll
+            for (int a = 0; a < DFS_MAXA; ++a)
+                if (a < na) {
+                    const NodeState<SM_F32> sa = action_state(st, a, Hp);
+                    FOR_W if (wslot(k) < Hp) {
+                        const int h = wslot(k);
+                        const float sig = strat_of<MODE>(v[a][k], den[k], def);
+                        sa.S[h] = __fadd_rn(__fmul_rn(sv[a][k], disc.z), sig);
+                        const float coef = signbit(v[a][k]) ? disc.y : disc.x;
+                        sa.R[h] = __fsub_rn(__fadd_rn(__fmul_rn(v[a][k], coef), cv[a][k]), res[k]);
+                    }
+                }
+        }
+    }
+    float* out = rows + (size_t)op.dst * c.stride;
+    if (op.flags & DFS_ADD) {
+        FOR_W if (wslot(k) < Hp) out[wslot(k)] = __fadd_rn(out[wslot(k)], res[k]);
+    } else {
+        FOR_W if (wslot(k) < Hp) out[wslot(k)] = res[k];
+    }
+}
+
+__device__ __forceinline__ void w_acc(const Ctx& c, const DfsOp& op, float* rows, int Hp) {
+    const float* src = rows + (size_t)op.src * c.stride;
+    float* out = rows + (size_t)op.dst * c.stride;
+    FOR_W if (wslot(k) < Hp) out[wslot(k)] = (op.flags & DFS_ADD) ? __fadd_rn(out[wslot(k)], src[wslot(k)]) : src[wslot(k)];
+}
+
+// A lane's run of the one-wave scan layout (RunoutTab::scan_runs64).
+struct WaveRun {
+    uint32_t w[WAVE_RUN / 2];
+};
+__device__ __forceinline__ uint16_t wrun_entry(const WaveRun& u, int j) {
+    return (uint16_t)(u.w[j >> 1] >> (16 * (j & 1)));
+}
+
+// One wave's exclusive segmented scan in f64 of the reach row rs over the
+// index list, as segmented_scan with the whole list on one wave: lane l owns
+// entries [l * C, l * C + C), the lanes' (segment-started, sum) pairs are
+// combined by the DPP wave scan, and the list starts with a segment, so the
+// lane below's inclusive sum is the whole carry.
+__device__ __forceinline__ void w_scan(double* scan, const WaveRun& u, int len, const float* rs) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int C = (len + WAVE - 1) / WAVE;
+    const int e0 = lane * C;
+    float x[WAVE_RUN];
+#pragma unroll
+    for (int j = 0; j < WAVE_RUN; ++j) {
+        const uint16_t k = wrun_entry(u, j) & SCAN_ZERO;
+        x[j] = k == SCAN_ZERO ? 0.0f : rs[k];
+    }
+    bool f = false;
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < WAVE_RUN; ++j) {
+        if (wrun_entry(u, j) & SCAN_HEAD) {
+            f = true;
+            s = (double)x[j];
+        } else {
+            s += (double)x[j];
+        }
+    }
+    {
+        int fi = f;
+        seg_dpp_step<0x111, 0xF>(fi, s);   // row_shr:1
+        seg_dpp_step<0x112, 0xF>(fi, s);   // row_shr:2
+        seg_dpp_step<0x114, 0xF>(fi, s);   // row_shr:4
+        seg_dpp_step<0x118, 0xF>(fi, s);   // row_shr:8
+        seg_dpp_step<0x142, 0xA>(fi, s);   // row_bcast:15 into rows 1 and 3
+        seg_dpp_step<0x143, 0xC>(fi, s);   // row_bcast:31 into rows 2 and 3
+    }
+    double run;
+    {
+        const uint64_t sb = (uint64_t)__double_as_longlong(s);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)sb, 0x138, 0xF, 0xF, true);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(sb >> 32), 0x138, 0xF, 0xF, true);
+        run = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    }
+#pragma unroll
+    for (int j = 0; j < WAVE_RUN; ++j) {
+        if (j < C && e0 + j < len) {
+            if (wrun_entry(u, j) & SCAN_HEAD) run = 0.0;
+            scan[j * WAVE + lane] = run;   // wave_scan_slot
+            run += (double)x[j];
+        }
+    }
+    wave_lds_fence();
+}
+
+// The runout tables of a tile on the wave path, per lane.
+struct WaveRunout {
+    RunoutTab T;
+    WaveRun u;
+    HandScan q[WAVE_KH];
+    uint16_t si[WAVE_KH];
+};
+
+__device__ __forceinline__ void w_load_runout(const Ctx& c, int table, int p, WaveRunout& R) {
+    const KernelArgs& A = c.A;
+    const int Hp = num_hands(c.G, p);
+    R.T = uni_load(A.runouts + table * 2 + p);
+    const uint4* src = reinterpret_cast<const uint4*>(A.u16pool + R.T.scan_runs64) + (WAVE_RUN / 8) * (threadIdx.x & (WAVE - 1));
+#pragma unroll
+    for (int i = 0; i < WAVE_RUN / 8; ++i) {
+        const uint4 x = src[i];
+        R.u.w[4 * i + 0] = x.x;
+        R.u.w[4 * i + 1] = x.y;
+        R.u.w[4 * i + 2] = x.z;
+        R.u.w[4 * i + 3] = x.w;
+    }
+    const HandScan* hs = A.hand_scan + R.T.hand_scan64;
+    const uint16_t* same = A.hand_same + hand_base(c.G, p);
+    FOR_W {
+        R.q[k] = wslot(k) < Hp ? hs[wslot(k)] : HandScan{0xFFFFFFFFu, 0, 0, 0};
+        R.si[k] = wslot(k) < Hp ? same[wslot(k)] : (uint16_t)0xFFFF;
+    }
+}
+
+// Fold / showdown values of a terminal (eval_terminal's arithmetic) into row
+// out, or added to it.
+__device__ __forceinline__ void w_term(const Ctx& c, const DfsOp& op, const float* rs, float* out, int p,
+                                       const WaveRunout& R, double tie) {
+    const int Hp = num_hands(c.G, p);
+    const bool add = (op.flags & DFS_ADD) != 0;
+    if (!(op.flags & DFS_SAME_SCAN)) w_scan(c.scan, R.u, R.T.scan_len, rs);
+    const double* S = c.scan;
+    const double total = S[wave_scan_slot(R.T.num_valid_opp, R.T.scan_len)];
+    const double win = op.win, lose = op.lose;
+    FOR_W {
+        if (wslot(k) < Hp) {
+            float res = 0.0f;
+            const HandScan& q = R.q[k];
+            if ((q.x & 0xFFFF) != 0xFFFF) {
+                if (op.kind == KIND_FOLD) {
+                    if (total != 0.0) {
+                        const double payoff = (int)op.player != p ? win : lose;
+                        const double sm = R.si[k] == 0xFFFF ? 0.0 : (double)rs[R.si[k]];
+                        const double cr = ((total + sm) - S[q.w & 0xFFFF]) - S[q.w >> 16];
+                        res = (float)(payoff * cr);
+                    }
+                } else if (tie != 0.0) {
+                    if (total != 0.0) {
+                        const double sm = R.si[k] == 0xFFFF ? 0.0 : (double)rs[R.si[k]];
+                        const double tw = (total - S[q.w & 0xFFFF]) - S[q.w >> 16];
+                        const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
+                        const double ct = (S[q.x >> 16] - S[q.y >> 16]) - S[q.z >> 16];
+                        res = (float)((win * cw + tie * ((ct - cw) + sm)) + lose * (tw - ct));
+                    }
+                } else {
+                    const double cw = (S[q.x & 0xFFFF] - S[q.y & 0xFFFF]) - S[q.z & 0xFFFF];
+                    const double cl = ((total - S[q.x >> 16]) - (S[q.w & 0xFFFF] - S[q.y >> 16])) -
+                                      (S[q.w >> 16] - S[q.z >> 16]);
+                    res = __fadd_rn((float)(win * cw), (float)(lose * cl));
+                }
+            }
+            if (add) res = __fadd_rn(out[wslot(k)], res);
+            out[wslot(k)] = res;
+        }
+    }
+}
+
+// One traversal of subtree tile T by this wave: root reach from the chance
+// node's HBM row, root cfvalues back to HBM.
+template <int MODE>
+__device__ __forceinline__ void w_traverse(Ctx& c, const TileDesc& T, const DfsProg& P, int p, const float4 disc) {
+    const KernelArgs& A = c.A;
+    const int o = p ^ 1, Hp = num_hands(c.G, p), Ho = num_hands(c.G, o);
+    float* rows = static_cast<float*>(c.rs);
+    const DfsOp* ops = A.dfs_ops + (p ? P.first[1] : P.first[0]);
+    const int n = p ? P.num_ops[1] : P.num_ops[0];
+    uint32_t next = dfs_fetch(ops);
+    const int root = uni_load(A.level_ranges + T.level_base).start;
+    {
+        const float* w = c.reach + (size_t)reach_src(c, root, p) * c.stride;
+        float wv[WAVE_KH];
+        FOR_W wv[k] = wslot(k) < Ho ? w[wslot(k)] : 0.0f;
+        FOR_W if (wslot(k) < Ho) rows[wslot(k)] = wv[k];
+    }
+    WaveRunout R;
+    w_load_runout(c, T.runout, p, R);
+    for (int i = 0; i < n; ++i) {
+        const DfsOp op = dfs_unpack(next);
+        next = dfs_fetch(ops + i + 1);   // ops[n] is the END record
+        if (op.op == DFS_OPP) {
+            w_opp<MODE>(c, op, rows, Ho);
+        } else if (op.op == DFS_TRAV) {
+            w_trav<MODE>(c, op, rows, Hp, disc);
+        } else if (op.op == DFS_TERM) {
+            const double tie = c.G.raked ? uni(A.payoff[(size_t)3 * (c.G.node_base + op.arg) + 2]) : 0.0;
+            wave_lds_fence();   // every lane's slots of the reach row are written
+            w_term(c, op, rows + (size_t)op.src * c.stride, rows + (size_t)op.dst * c.stride, p, R, tie);
+        } else {
+            w_acc(c, op, rows, Hp);
+        }
+    }
+    const float* res = rows + (size_t)(p ? P.root_slot[1] : P.root_slot[0]) * c.stride;
+    float* out = c.cfv + (size_t)root * c.stride;
+    FOR_W if (wslot(k) < Hp) out[wslot(k)] = res[wslot(k)];
+    wave_lds_fence();   // the next tile's writes of the rows come after every read of these
+}
+
+// The wave path's work queue: every wave takes tiles on its own.
+template <int MODE>
+__global__ void __launch_bounds__(BLOCK) dcfr_wave_kernel(KernelArgs) {
+    const KernelArgs& A = *(const KernelArgs*)__builtin_amdgcn_kernarg_segment_ptr();
+    PSR_DYNAMIC_LDS(double, lds);
+    const int wave = uni((int)(threadIdx.x / WAVE));
+    char* region = reinterpret_cast<char*>(lds) + (size_t)wave * A.wave_region;
+    double* scan = reinterpret_cast<double*>(region);
+    void* rows = region + (size_t)A.wave_scan_slots * sizeof(double);
+    const int p = A.player;
+    const uint32_t it = A.iteration_dev ? uni((int)*A.iteration_dev) : A.iteration;
+    const float4 disc = MODE == MODE_DCFR ? A.discount[it] : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (;;) {
+        int job = 0;
+        if ((threadIdx.x & (WAVE - 1)) == 0) job = atomicAdd(A.queue_counter + 2, 1);
+        const int q = __builtin_amdgcn_readlane(job, 0);
+        if (q >= A.num_wave_tiles) break;
+        const int t = uni(A.wave_tiles[2 * q]), gi = uni(A.wave_tiles[2 * q + 1]);
+        Ctx c = make_ctx(A, gi, scan, rows, nullptr);
+        if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
+        const TileDesc T = uni_load(A.tiles + t);
+        const DfsProg P = uni_load(A.dfs_progs + t);
+        w_traverse<MODE>(c, T, P, p, disc);
+    }
+}
+
 // One kernel per (storage, traversal mode), so that each body is inlined whole
 // and the kernel arguments stay in scalar registers.
 template <int SM, int MODE, typename V>
@@ -1887,6 +2204,29 @@ hipError_t dcfr_launch_sm(const KernelArgs& args, int grid, hipStream_t stream) 
 
 template <int SM>
 hipError_t dcfr_launch_phase_sm(const KernelArgs& args, int grid, hipStream_t stream) {
+    if (args.phase == PHASE_WAVE_SUBTREES) {
+        if constexpr (SM == SM_F32) {
+            const size_t lds = (size_t)WAVE_TILES * args.wave_region;
+            switch (args.mode) {
+                case MODE_DCFR:
+                    if (hipError_t e = allow_lds(dcfr_wave_kernel<MODE_DCFR>, lds)) return e;
+                    hipLaunchKernelGGL((dcfr_wave_kernel<MODE_DCFR>), dim3(grid), dim3(BLOCK), lds, stream, args);
+                    break;
+                case MODE_BR:
+                    if (hipError_t e = allow_lds(dcfr_wave_kernel<MODE_BR>, lds)) return e;
+                    hipLaunchKernelGGL((dcfr_wave_kernel<MODE_BR>), dim3(grid), dim3(BLOCK), lds, stream, args);
+                    break;
+                case MODE_CFV:
+                    if (hipError_t e = allow_lds(dcfr_wave_kernel<MODE_CFV>, lds)) return e;
+                    hipLaunchKernelGGL((dcfr_wave_kernel<MODE_CFV>), dim3(grid), dim3(BLOCK), lds, stream, args);
+                    break;
+                default:
+                    return hipErrorInvalidValue;   // FP64 passes take the level-by-level path
+            }
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
+    }
     // the subtree phase runs tiles depth first with their rows in LDS
     const bool dfs = args.phase == PHASE_SUBTREES && !args.fwd_only;
     const size_t lds = std::max<size_t>(dcfr_lds_bytes(args.max_hands), dfs ? args.lds_bytes : 0u);

@@ -29,6 +29,7 @@ enum Phase : int {
     PHASE_MID_FORWARD = 4,    // middle tiles (a flop game's turn tiles) forward, one block per tile
     PHASE_MID_LEAVES = 5,     // their chance nodes and terminals, one block per node
     PHASE_MID_BACKWARD = 6,   // their decision nodes, one block per tile
+    PHASE_WAVE_SUBTREES = 7,  // subtree tiles of small games on the wave path (one tile per wave)
 };
 
 enum TraverseMode : int { MODE_DCFR = 0, MODE_BR = 1, MODE_CFV = 2, MODE_BR64 = 3, MODE_CFV64 = 4 };
@@ -113,6 +114,12 @@ struct KernelArgs {
     uint32_t lds_bytes;         // dynamic LDS of the whole-game and subtree launches
                                 // (0: dcfr_lds_bytes(max_hands))
     int fwd_only;               // PHASE_SUBTREES: only the forward pass, level by level (reach rows)
+    // the wave path (dcfr_layout.hpp): its work queue of (tile, game) pairs and the LDS region
+    // of each wave (the one-wave scan output, then the tile's rows)
+    const int* wave_tiles;
+    int num_wave_tiles;
+    uint32_t wave_region;       // bytes per wave
+    int wave_scan_slots;        // f64 scan slots at the start of a wave's region
 };
 
 size_t dcfr_lds_bytes(int max_hands);

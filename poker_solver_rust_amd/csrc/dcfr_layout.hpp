@@ -79,7 +79,11 @@ struct RunoutTab {
                             // at [t * SCAN_RUN], padded with SCAN_ZERO (one aligned 16-byte
                             // vector per 8 entries instead of scattered loads)
     int32_t total_pos;      // scan-output slot of segment 0's closing entry (the total)
-    int32_t pad[2];
+    // one-wave layout (tables of games with at most WAVE_HANDS hands per player; -1 otherwise):
+    // lane l scans entries [l * C, l * C + C), C = ceil(scan_len / 64), its run of WAVE_RUN
+    // entries at scan_runs64 + l * WAVE_RUN, output slot (pos % C) * 64 + pos / C
+    int32_t scan_runs64;
+    int32_t hand_scan64;    // HandScan[H_player] in the one-wave slots; total at wave_slot(num_valid_opp)
 };
 // The scan output is stored transposed: entry j of thread t's run (list
 // position t * C + j, C = ceil(scan_len / SCAN_THREADS)) at slot
@@ -91,6 +95,20 @@ constexpr int scan_slot(int pos, int scan_len) {
 }
 // Slots the scan output of a list of up to max_len entries can occupy.
 constexpr int scan_slots(int max_len) { return (max_len + SCAN_THREADS - 1) / SCAN_THREADS * SCAN_THREADS; }
+
+// The wave path: a tile of a game with at most WAVE_HANDS hands per player is
+// traversed by one wave (WAVE_KH hands per lane, h = lane + 64 k), eight tiles
+// per workgroup at once, each wave with its own LDS region; its scan is one
+// wave's (no workgroup barrier anywhere).
+constexpr int WAVE_HANDS = 256;
+constexpr int WAVE_KH = WAVE_HANDS / 64;
+constexpr int WAVE_TILES = DCFR_BLOCK / 64;   // tiles a workgroup of the wave path runs at once
+constexpr int WAVE_RUN = ((3 * WAVE_HANDS + 53 + 63) / 64 + 7) / 8 * 8;   // 16
+constexpr int wave_scan_slot(int pos, int scan_len) {
+    const int C = (scan_len + 63) / 64;
+    return (pos % C) * 64 + pos / C;
+}
+constexpr int wave_scan_slots(int max_len) { return (max_len + 63) / 64 * 64; }
 
 struct ChanceTab {          // chance node: reach scale and isomorphic-card swaps
     float scale;            // 1 / chance factor (f32, as the reference computes it)

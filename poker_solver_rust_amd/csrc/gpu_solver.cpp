@@ -53,6 +53,7 @@ struct HostLayout {
     // per game: decision nodes in arena order as (state offset, A*H, scale index)
     std::vector<std::vector<std::array<int64_t, 3>>> dec_blocks;
     int max_hands = 0;
+    int max_wave_scan_slots = 0;     // one-wave scan outputs (wave path tables)
     bool any_raked = false;
     std::vector<int> single_games;   // whole-game work queue, most expensive first
     std::vector<int> multi_games;    // games solved by phase launches (turn)
@@ -180,6 +181,36 @@ RunoutTab build_runout(const PostFlopGame& g, const Runout& ro, int p, HostLayou
     }
     t.num_valid_opp = (int32_t)sorted.size();
     t.total_pos = scan_slot(t.num_valid_opp, t.scan_len);
+    t.scan_runs64 = t.hand_scan64 = -1;
+    if (Hp <= WAVE_HANDS && (int)oc.size() <= WAVE_HANDS) {
+        // the one-wave layout of the same list (wave path)
+        const int C = (t.scan_len + 63) / 64;
+        std::vector<uint16_t> runs((size_t)64 * WAVE_RUN, SCAN_ZERO);
+        for (int l = 0; l < 64; ++l)
+            for (int j = 0; j < C && l * C + j < t.scan_len; ++j) runs[(size_t)l * WAVE_RUN + j] = idx[l * C + j];
+        t.scan_runs64 = push_u16(L.u16, runs, 8);
+        t.hand_scan64 = (int32_t)L.hand_scan.size();
+        auto remap = [&](uint32_t packed) {
+            // list positions are recovered from the block layout's slots
+            auto pos_of = [&](int slot) {
+                const int Cb = (t.scan_len + SCAN_THREADS - 1) / SCAN_THREADS;
+                return (slot % SCAN_THREADS) * Cb + slot / SCAN_THREADS;
+            };
+            const int lo = pos_of((int)(packed & 0xFFFF)), hi = pos_of((int)(packed >> 16));
+            return (uint32_t)wave_scan_slot(lo, t.scan_len) | ((uint32_t)wave_scan_slot(hi, t.scan_len) << 16);
+        };
+        for (int i = 0; i < Hp; ++i) {
+            HandScan h = L.hand_scan[t.hand_scan + i];
+            if ((h.x & 0xFFFF) != 0xFFFF) {
+                h.x = remap(h.x);
+                h.y = remap(h.y);
+                h.z = remap(h.z);
+                h.w = remap(h.w);
+            }
+            L.hand_scan.push_back(h);
+        }
+        L.max_wave_scan_slots = std::max(L.max_wave_scan_slots, wave_scan_slots(t.scan_len));
+    }
     return t;
 }
 
@@ -765,6 +796,11 @@ struct psr_batch {
     size_t dfs_lds = 0;
     DfsProg* d_dfs_progs = nullptr;
     DfsOp* d_dfs_ops = nullptr;
+    // the wave path: subtree tiles of small games, one per wave (block_queue holds the other
+    // subtree tiles); both as (tile, game) pairs in the cost order of the full queue
+    std::vector<int> block_queue, wave_queue;
+    uint32_t wave_region = 0;
+    int *d_block_queue = nullptr, *d_wave_queue = nullptr;
     // the depth-first path writes no reach rows to HBM: after a job whose last traversal of a
     // single-tile game ran on it, the rows download_reach reads are rebuilt by the forward
     // pass of that traversal (JOB_REACH_DCFR after DCFR iterations, JOB_REACH after a best
@@ -778,7 +814,8 @@ struct psr_batch {
                         d_runouts, d_chance, d_iso_ref, d_iso_perm, d_u16, d_hand_scan, d_regret, d_strategy, d_regret16, d_strategy16, d_rscale, d_sscale, d_reach64, d_cfv64, d_expl64,
                         d_reach, d_cfv, d_discount, d_expl, d_phase_expl, d_iters, d_root_cfv, d_counter, d_single, d_multi,
                         d_subtrees, d_ev, d_bnd_cfv, d_bnd_reach, d_bnd_mode, d_root_action_cfv, d_root_act_off,
-                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves, d_iter, d_dfs_progs, d_dfs_ops};
+                        d_reach_src, d_root_leaves, d_mid_tiles, d_mid_leaves, d_iter, d_dfs_progs, d_dfs_ops,
+                        d_block_queue, d_wave_queue};
         for (void* p : ptrs)
             if (p && !arena.base) (void)hipFree(p);
         arena.used = 0;
@@ -798,6 +835,8 @@ struct psr_batch {
         d_iter = nullptr;
         d_dfs_progs = nullptr;
         d_dfs_ops = nullptr;
+        d_block_queue = nullptr;
+        d_wave_queue = nullptr;
         drop_graphs();
         prepared = false;
     }
@@ -872,10 +911,12 @@ struct psr_batch {
         a.dfs_progs = d_dfs_progs;
         a.dfs_ops = d_dfs_ops;
         a.lds_bytes = dfs_on ? (uint32_t)dfs_lds : 0u;
+        a.wave_region = wave_region;
+        a.wave_scan_slots = L.max_wave_scan_slots;
         return a;
     }
 
-    // one phase launch of a multi-tile traversal
+    // captured phase launches (turn and flop traversals), rebuilt on their next use
     void drop_graphs() {
         for (int k = 0; k < 2; ++k) {
             if (graph_exec[k]) (void)hipGraphExecDestroy(graph_exec[k]);
@@ -889,6 +930,17 @@ struct psr_batch {
     int phase(int ph, int player, int mode, uint32_t t, bool dev_iter = false, bool fwd_only = false) {
         KernelArgs a = args();
         a.fwd_only = fwd_only ? 1 : 0;
+        // the subtree phase: tiles of small games on the wave path, the rest on the block
+        // path (all of them when the depth-first path is off, or for the level-by-level
+        // forward and the FP64 passes)
+        const bool waves = ph == PHASE_SUBTREES && dfs_on && !fwd_only && !wave_queue.empty() &&
+                           (mode == MODE_DCFR || mode == MODE_BR || mode == MODE_CFV);
+        if (waves) {
+            a.subtree_tiles = d_block_queue;
+            a.num_subtree_tiles = (int)block_queue.size() / 2;
+            a.wave_tiles = d_wave_queue;
+            a.num_wave_tiles = (int)wave_queue.size() / 2;
+        }
         a.phase = ph;
         a.player = player;
         a.mode = mode;
@@ -899,8 +951,12 @@ struct psr_batch {
                 : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves, grid_all())
                 : ph == PHASE_MID_FORWARD || ph == PHASE_MID_BACKWARD ? std::min(a.num_mid_tiles, grid_all())
                                           : a.num_multi_games;
-        if (g <= 0) return 0;
-        HIP_OK(dcfr_launch_phase(a, g, stream));
+        if (g > 0) HIP_OK(dcfr_launch_phase(a, g, stream));
+        if (waves) {
+            // one workgroup per CU (the waves' regions fill its LDS), eight tiles at a time
+            a.phase = PHASE_WAVE_SUBTREES;
+            HIP_OK(dcfr_launch_phase(a, std::min((a.num_wave_tiles + WAVE_TILES - 1) / WAVE_TILES, num_cus), stream));
+        }
         return 0;
     }
     int grid_all() const { return num_cus * blocks_per_cu; }
@@ -1269,6 +1325,34 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         }
     }
     const std::vector<DfsProg> dfs_off(L.tiles.size());
+    // the wave path: subtree tiles of games with at most WAVE_HANDS hands, depth-first
+    // programs without depth-boundary leaves, whose scan and rows fit an eighth of the LDS
+    {
+        b->block_queue.clear();
+        b->wave_queue.clear();
+        const size_t scan_b = sizeof(double) * (size_t)L.max_wave_scan_slots;
+        const size_t cap = (size_t)std::max(b->lds_max - 1024, 0) / WAVE_TILES / 16 * 16;
+        size_t region = 0;
+        for (size_t k = 0; k < L.subtree_tiles.size(); k += 2) {
+            const int t = L.subtree_tiles[k], gi = L.subtree_tiles[k + 1];
+            const GameDesc& d = L.games[gi];
+            const DfsProg& pr = b->dfs_fit[t];
+            bool ok = b->storage == SM_F32 && pr.num_ops[0] > 0 && d.H[0] <= WAVE_HANDS && d.H[1] <= WAVE_HANDS &&
+                      L.runouts[2 * L.tiles[t].runout].scan_runs64 >= 0;
+            const size_t need = (scan_b + (size_t)pr.slots * d.row_stride * sizeof(float) + 15) / 16 * 16;
+            ok = ok && need <= cap;
+            for (int p = 0; ok && p < 2; ++p)
+                for (int i = 0; i < pr.num_ops[p]; ++i) {
+                    const DfsOp& op = L.dfs_ops[pr.first[p] + i];
+                    if (op.op == DFS_TERM && op.kind == KIND_BOUNDARY) ok = false;
+                }
+            std::vector<int>& q = ok ? b->wave_queue : b->block_queue;
+            q.push_back(t);
+            q.push_back(gi);
+            if (ok) region = std::max(region, need);
+        }
+        b->wave_region = (uint32_t)region;
+    }
     std::vector<float4> disc(std::max<uint32_t>(max_iterations, 1));
     for (uint32_t t = 0; t < disc.size(); ++t) disc[t] = discount_params(t);
     b->max_iterations = max_iterations;
@@ -1289,11 +1373,12 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         dev_buf(&b->d_root_act_off, L.root_act_off),
         dev_buf_n(&b->d_reach, rows), dev_buf_n(&b->d_cfv, rows), dev_buf_n(&b->d_expl, (size_t)n),
         dev_buf_n(&b->d_phase_expl, (size_t)n), dev_buf_n(&b->d_iters, (size_t)n),
-        dev_buf_n(&b->d_root_cfv, L.cards.size()), dev_buf_n(&b->d_counter, 2), dev_buf_n(&b->d_ev, 2 * (size_t)n),
+        dev_buf_n(&b->d_root_cfv, L.cards.size()), dev_buf_n(&b->d_counter, 3), dev_buf_n(&b->d_ev, 2 * (size_t)n),
         dev_buf_n(&b->d_bnd_cfv, brows), dev_buf_n(&b->d_bnd_reach, brows), dev_buf_n(&b->d_bnd_mode, nbnd),
         dev_buf_n(&b->d_root_action_cfv, (size_t)std::max<int64_t>(L.root_act_total, 1)),
         dev_buf_n(&b->d_iter, 1), dev_buf(&b->d_dfs_progs, b->dfs_on ? b->dfs_fit : dfs_off),
-        dev_buf(&b->d_dfs_ops, L.dfs_ops),
+        dev_buf(&b->d_dfs_ops, L.dfs_ops), dev_buf(&b->d_block_queue, b->block_queue),
+        dev_buf(&b->d_wave_queue, b->wave_queue),
     };
     if (b->storage == SM_F32) {
         bufs.push_back(dev_buf_n(&b->d_regret, state));
@@ -1316,7 +1401,7 @@ int psr_batch_prepare(psr_batch* b, const psr_game* const* games, int n, uint32_
         if (d.host_bytes)
             HIP_OK(hipMemcpyAsync(*d.slot, d.host, d.host_bytes, hipMemcpyHostToDevice, b->stream));
     }
-    HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_counter), 2, b->stream));
+    HIP_OK(fill_zero(reinterpret_cast<float*>(b->d_counter), 3, b->stream));
     b->h_bnd_cfv.assign(brows, 0.0f);
     b->h_bnd_mode.assign(nbnd, BND_UNSET);
     HIP_OK(fill_zero(b->d_bnd_cfv, brows, b->stream));
@@ -1929,15 +2014,17 @@ int psr_batch_set_depth_first(psr_batch* b, int32_t enable) {
     const std::vector<DfsProg> off(b->L.tiles.size());
     const std::vector<DfsProg>& v = b->dfs_on ? b->dfs_fit : off;
     HIP_OK(hipSetDevice(b->device));
+    b->drop_graphs();   // captured launches hold the subtree queues of the old setting
     HIP_OK(hipMemcpy(b->d_dfs_progs, v.data(), v.size() * sizeof(DfsProg), hipMemcpyHostToDevice));
     return 0;
 }
 
-int psr_batch_depth_first_info(psr_batch* b, int32_t* tiles, int64_t* lds_bytes) {
+int psr_batch_depth_first_info(psr_batch* b, int32_t* tiles, int32_t* wave_tiles, int64_t* lds_bytes) {
     if (!b || !b->prepared) return fail("batch not prepared");
     int32_t n = 0;
     for (const DfsProg& pr : b->dfs_fit) n += pr.num_ops[0] > 0;
     if (tiles) *tiles = b->dfs_on ? n : 0;
+    if (wave_tiles) *wave_tiles = b->dfs_on ? (int32_t)b->wave_queue.size() / 2 : 0;
     if (lds_bytes) *lds_bytes = b->dfs_on ? (int64_t)b->dfs_lds : (int64_t)dcfr_lds_bytes(b->L.max_hands);
     return 0;
 }

@@ -133,7 +133,8 @@ def load_library(path: Optional[str] = None):
         "psr_batch_set_launch": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
         "psr_batch_autotune": (C.c_int, [vp, C.c_uint32, C.POINTER(C.c_int32)]),
         "psr_batch_set_depth_first": (C.c_int, [vp, C.c_int32]),
-        "psr_batch_depth_first_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+        "psr_batch_depth_first_info": (C.c_int, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                                 C.POINTER(C.c_int64)]),
         "psr_game_num_boundaries": (C.c_int, [vp]),
         "psr_game_boundary_node": (C.c_int64, [vp, C.c_int]),
         "psr_game_boundary_pot": (C.c_int, [vp, C.c_int]),
@@ -820,7 +821,9 @@ class GpuBatchSolver:
 
     def depth_first_info(self) -> dict:
         """Tiles on the depth-first path (single-tile games and the river
-        subtree tiles of turn and flop games) and the LDS of its launches."""
-        n, lds = C.c_int32(), C.c_int64()
-        _check(load_library().psr_batch_depth_first_info(self._h, C.byref(n), C.byref(lds)))
-        return {"tiles": n.value, "lds_bytes": lds.value}
+        subtree tiles of turn and flop games), those of them that run one per
+        wave (subtree tiles of games with at most 256 hands per player), and
+        the LDS of the whole-workgroup launches."""
+        n, nw, lds = C.c_int32(), C.c_int32(), C.c_int64()
+        _check(load_library().psr_batch_depth_first_info(self._h, C.byref(n), C.byref(nw), C.byref(lds)))
+        return {"tiles": n.value, "wave_tiles": nw.value, "lds_bytes": lds.value}

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <typeinfo>
 #include <memory>
 #include <csetjmp>
 #include <ucontext.h>
@@ -129,12 +130,16 @@ inline void lane_entry() {
     _longjmp(g_group->sched_jb, 1);
 }
 
-[[noreturn]] inline void deadlock(const Group& G, unsigned block) {
+[[noreturn]] inline void deadlock(const Group& G, unsigned block, const char* kernel) {
     int n[4] = {0, 0, 0, 0};
     for (const Lane& l : G.lanes) n[l.wait]++;
     std::fprintf(stderr,
                  "emulated workgroup %u cannot proceed: %d lanes at a barrier, %d at a wave exchange, %d finished\n",
                  block, n[AT_BARRIER], n[AT_WAVE], n[DONE]);
+    std::fprintf(stderr, "kernel %s; finished lanes:", kernel);
+    for (size_t t = 0; t < G.lanes.size(); ++t)
+        if (G.lanes[t].wait == DONE) std::fprintf(stderr, " %zu", t);
+    std::fprintf(stderr, "\n");
     std::abort();
 }
 
@@ -208,7 +213,7 @@ void launch(K kernel, dim3 grid, dim3 block, size_t lds_bytes, Args... args) {
                     released = true;
                 }
             }
-            if (!released) deadlock(G, bx);
+            if (!released) deadlock(G, bx, typeid(K).name());
         }
     }
     g_group = saved;
@@ -269,6 +274,9 @@ inline int __builtin_amdgcn_readfirstlane(int x) {
 }
 // readlane: lane `src` of the wave's value (every lane gets it)
 inline int __builtin_amdgcn_readlane(int x, int src) { return emu::exchange(x, src); }
+// wave_barrier: every lane of the wave reaches it before any goes on (an exchange
+// whose value is unused)
+inline void __builtin_amdgcn_wave_barrier() { (void)emu::exchange(0, 0); }
 inline const void* __builtin_amdgcn_kernarg_segment_ptr() { return emu::cur_kernarg; }
 // DPP move (the patterns the kernels use): row_shr:n (0x111-0x11F) inside a
 // row of 16 lanes, row_bcast:15 (0x142, lane 15 of a row into the next row),

@@ -212,6 +212,48 @@ def run_flop_stepwise():
     check_stepwise(flop_game("AA,KK", "QQ,AK", "2s 7s Js"), 1)
 
 
+def run_wave_path():
+    """Subtree tiles of games with at most 256 hands per player run one per
+    wave (eight tiles per workgroup, each wave with its own LDS region): a
+    turn game and a raked turn game with every subtree tile on that path
+    (flop_stepwise and turn_stepwise take it too), state after every
+    iteration against the oracle and against the level-by-level path, then
+    exploitability, root cfvalues and the reach rows download_reach
+    rebuilds."""
+    games = [turn_game("AA,AKs", "KK,QQ"), turn_game("KK,AQs", "AA,JJ", rake=(0.05, 30.0))]
+    runs = []
+    for dfs in (True, False):
+        b = P.GpuBatchSolver(0)
+        b.set_depth_first(dfs)
+        b.prepare_batch(games, 4)
+        info = b.depth_first_info()
+        if dfs and (info["wave_tiles"] == 0 or info["wave_tiles"] != info["tiles"]):
+            raise AssertionError(f"wave path tiles {info}")
+        if not dfs and info["wave_tiles"] != 0:
+            raise AssertionError(f"wave path tiles with the depth-first path off: {info}")
+        runs.append(b)
+    b, lv = runs
+    oracles = [O.OracleGame(g.serialize()) for g in games]
+    for t in range(2):
+        b.run_iterations(t, t + 1)
+        lv.run_iterations(t, t + 1)
+        for i, o in enumerate(oracles):
+            o.solve_step(t)
+            r, s = o.state()
+            same(b.regrets(i), r, f"game {i} iteration {t} regrets")
+            same(b.strategy_sum(i), s, f"game {i} iteration {t} strategy sums")
+            same(b.regrets(i), lv.regrets(i), f"game {i} iteration {t} regrets vs level-by-level")
+    same(b.download_reach(64), lv.download_reach(64), "download_reach vs level-by-level")
+    e = b.exploitability()
+    same(e, lv.exploitability(), "exploitability vs level-by-level")
+    for i, o in enumerate(oracles):
+        if abs(float(e[i]) - o.exploitability()) > 1e-6 * max(1.0, abs(o.exploitability())):
+            raise AssertionError(f"game {i} exploitability {e[i]} vs oracle {o.exploitability()}")
+        cf = b.root_cfvalues()[i]
+        for p in range(2):
+            same(cf[p], o.root_cfvalues(p), f"game {i} root cfvalues player {p}")
+
+
 def run_half_storage():
     """FP16 / BF16 words with per-node scales: the solve converges like the
     FP64 oracle on the same tree (no reference counterpart)."""

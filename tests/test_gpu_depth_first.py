@@ -84,7 +84,8 @@ def flop_game(oop, ip, board, bets="50%, a"):
 def test_subtree_tiles_paths_identical():
     """Turn and flop games: their river subtree tiles run depth first in the
     subtree phase (root reach from the chance node's HBM row, root cfvalues
-    back to HBM for the chance sums); solve(), download_reach after the
+    back to HBM for the chance sums), the flop games' one tile per wave and
+    the turn game's one per workgroup, in the same launch sequence; solve(), download_reach after the
     iterations and after a best response, and reach_at_nodes equal to the
     level-by-level path."""
     games = [W.turn_uniform01(20260).build(), flop_game("AA,KK,AKs", "QQ-JJ,AQs", "Qs Jh 2c"),
@@ -97,6 +98,8 @@ def test_subtree_tiles_paths_identical():
         info = b.depth_first_info()
         print(f"depth-first {dfs}: {info}", flush=True)
         assert (info["tiles"] > 0) == dfs
+        # the flop games' subtree tiles run one per wave, the U[0,1] turn game's per workgroup
+        assert (0 < info["wave_tiles"] < info["tiles"]) if dfs else info["wave_tiles"] == 0
         b.run_iterations(0, 7)
         r_iter = b.download_reach(1326)
         e = b.exploitability()

@@ -33,7 +33,7 @@ def emu_lib():
 CHECKS = [("std_stepwise", True), ("std_raises_stepwise", True), ("std_solve", True), ("batch", True),
           ("compressed_stepwise", True), ("fp64_exploitability", True), ("half_storage", True),
           ("turn_stepwise", True), ("mixed_batch", False), ("boundary_stepwise", True),
-          ("expected_values", False), ("raked", True), ("depth_first", True), ("reach_at_nodes", False), ("flop_stepwise", False), ("mlp_forward", True), ("mlp_gradients", True), ("mlp_gradients_split_k", True), ("mlp_gradients_one_row", True), ("mlp_adamw_step", True),
+          ("expected_values", False), ("raked", True), ("depth_first", True), ("wave_path", False), ("reach_at_nodes", False), ("flop_stepwise", False), ("mlp_forward", True), ("mlp_gradients", True), ("mlp_gradients_split_k", True), ("mlp_gradients_one_row", True), ("mlp_adamw_step", True),
           ("mlp_adamw_clipped", True)]
 
 
@@ -49,7 +49,7 @@ def _run_check(check, uniform):
 def emu_runs(emu_lib):
     """Every check started at once on a pool of one process per core, the
     longest first; {check: future}."""
-    order = ["raked", "depth_first", "turn_stepwise", "flop_stepwise", "mixed_batch", "expected_values"]
+    order = ["raked", "depth_first", "wave_path", "turn_stepwise", "flop_stepwise", "mixed_batch", "expected_values"]
     todo = sorted(CHECKS, key=lambda c: order.index(c[0]) if c[0] in order else len(order))
     pool = ThreadPoolExecutor(max_workers=max(1, min(8, os.cpu_count() or 1)))
     futures = {check: pool.submit(_run_check, check, uniform) for check, uniform in todo}
