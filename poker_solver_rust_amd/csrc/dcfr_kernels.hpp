@@ -131,6 +131,38 @@ __device__ __forceinline__ int reach_src(const Ctx& c, int n, int p) {
     return uni(c.A.reach_src[2 * ((size_t)c.G.node_base + n) + p]);
 }
 
+// A level's node records (and, for the forward pass, reach sources), lane l
+// of every wave holding those of node start + l: one vector load for the
+// level instead of a dependent load per node that the level loop waits on.
+// Nodes past the first WAVE of a level are loaded one at a time.
+struct LevelRecs {
+    uint4 rec;
+    int src;
+    int start;
+};
+template <bool SRC>
+__device__ __forceinline__ LevelRecs level_recs(const Ctx& c, int start, int end, int p) {
+    LevelRecs L;
+    L.start = start;
+    const int n = start + (int)(threadIdx.x & (WAVE - 1));
+    L.rec = n < end ? reinterpret_cast<const uint4*>(c.nodes)[n] : make_uint4(0u, 0u, 0u, 0u);
+    L.src = SRC && n < end ? c.A.reach_src[2 * ((size_t)c.G.node_base + n) + p] : 0;
+    return L;
+}
+__device__ __forceinline__ NodeRec level_node(const Ctx& c, const LevelRecs& L, int n) {
+    const int i = n - L.start;
+    if (i >= WAVE) return uni_load(c.nodes + n);
+    uint32_t w[4] = {(uint32_t)__builtin_amdgcn_readlane((int)L.rec.x, i), (uint32_t)__builtin_amdgcn_readlane((int)L.rec.y, i),
+                     (uint32_t)__builtin_amdgcn_readlane((int)L.rec.z, i), (uint32_t)__builtin_amdgcn_readlane((int)L.rec.w, i)};
+    NodeRec nd;
+    __builtin_memcpy(&nd, w, sizeof(NodeRec));
+    return nd;
+}
+__device__ __forceinline__ int level_src(const Ctx& c, const LevelRecs& L, int n, int p) {
+    const int i = n - L.start;
+    return i < WAVE ? __builtin_amdgcn_readlane(L.src, i) : reach_src(c, n, p);
+}
+
 // Sum over the workgroup of one f64 per thread; every thread gets the same
 // value (same association order on every thread).
 __device__ __forceinline__ double block_sum(double v, Shared* sh) {
@@ -374,15 +406,16 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
     }
     for (int L = 0; L < T.num_levels; ++L) {
         const LevelRange lr = uni_load(lv + L);
+        const LevelRecs LR = level_recs<true>(c, lr.start, lr.end, p);
         bool any = false;
         for (int n = lr.start; n < lr.end; ++n) {
-            const NodeRec nd = uni_load(c.nodes + n);
+            const NodeRec nd = level_node(c, LR, n);
             const int na = nd.num_children;
             // a node of the traverser or a single-action node passes the reach
             // through: its children use the same row (KernelArgs::reach_src)
             if (!(nd.kind == KIND_CHANCE || (nd.kind == KIND_DECISION && nd.player == o && na > 1))) continue;
             any = true;
-            const V* rn = reach + (size_t)reach_src(c, n, p) * c.stride;
+            const V* rn = reach + (size_t)level_src(c, LR, n, p) * c.stride;
             V* rc = reach + (size_t)nd.first_child * c.stride;
             if constexpr (sizeof(V) == 4) {
                 if (nd.kind == KIND_DECISION && na <= NA_REG) {
@@ -398,11 +431,11 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     for (int slot = 1; slot < FWD_GROUP; ++slot) {   // compile-time slots: fn stays in registers
                         bool found = false;
                         for (; PAIR_NODES && m < lr.end; ++m) {
-                            const NodeRec nd2 = uni_load(c.nodes + m);
+                            const NodeRec nd2 = level_node(c, LR, m);
                             const bool opp = nd2.kind == KIND_DECISION && nd2.player == o && nd2.num_children > 1;
                             if (!(nd2.kind == KIND_CHANCE || opp)) continue;   // passes the reach through
                             if (opp && nd2.num_children <= NA_REG) {
-                                fn[slot] = {node_state<SM>(A, c.G, nd2), reach + (size_t)reach_src(c, m, p) * c.stride,
+                                fn[slot] = {node_state<SM>(A, c.G, nd2), reach + (size_t)level_src(c, LR, m, p) * c.stride,
                                             reach + (size_t)nd2.first_child * c.stride, (int)nd2.num_children,
                                             __fdiv_rn(1.0f, (float)nd2.num_children)};
                                 found = true;
@@ -1093,9 +1126,10 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
     V* cfv = cfv_rows<V>(c);
     for (int L = T.num_levels - 1; L >= 0; --L) {
         const LevelRange lr = uni_load(lv + L);
+        const LevelRecs LR = level_recs<false>(c, lr.start, lr.end, p);
         bool any = false;
         for (int n = lr.start; n < lr.end; ++n) {
-            const NodeRec nd = uni_load(c.nodes + n);
+            const NodeRec nd = level_node(c, LR, n);
             if (nd.kind != KIND_DECISION && nd.kind != KIND_CHANCE) continue;
             any = true;
             const int na = nd.num_children;
@@ -1108,7 +1142,7 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     int n2 = n + 1;
                     NodeRec nd2 = nd;
                     for (; n2 < lr.end; ++n2) {
-                        nd2 = uni_load(c.nodes + n2);
+                        nd2 = level_node(c, LR, n2);
                         if (nd2.kind == KIND_DECISION || nd2.kind == KIND_CHANCE) break;
                     }
                     const bool trav = nd.player == p && na > 1;
