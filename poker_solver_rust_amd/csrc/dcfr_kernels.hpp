@@ -47,9 +47,11 @@ constexpr bool PAIR_NODES = true;   // node passes take two such nodes of a leve
 // h = threadIdx.x + k * BLOCK.  The per-hand loops are unrolled over k and
 // split into a load phase and a compute/store phase, so every thread has up
 // to KH independent loads in flight per row instead of one.
-constexpr int KH = (1326 + BLOCK - 1) / BLOCK;
+constexpr int KH = DCFR_SLOTS;
 static_assert(KH * BLOCK >= 1326, "KH slots must cover the largest range");
 #define FOR_K _Pragma("unroll") for (int k = 0; k < KH; ++k)
+// the slot range [K0, K1) of a function templated on it
+#define FOR_KR _Pragma("unroll") for (int k = K0; k < K1; ++k)
 __device__ __forceinline__ int hslot(int k) { return (int)threadIdx.x + k * BLOCK; }
 
 __device__ __forceinline__ float fmax0(float x) { return x > 0.0f ? x : 0.0f; }
@@ -367,6 +369,50 @@ __device__ __forceinline__ void opponent_rows(const FwdNode<SM> (&nd)[NN], int s
     }
 }
 
+// Up to NN opponent nodes (cnt of them, wave-uniform) of one hand slot range
+// [K0, K1): the rows of every node in flight together, as opponent_rows.
+// With one slot per thread a level's nodes fit the registers of four nodes
+// on all three slots.
+template <int MODE, int SM, int NN, int K0, int K1>
+__device__ __forceinline__ void opponent_rows_slots(const FwdNode<SM> (&nd)[NN], int cnt, int stride, int Ho) {
+    float r[NN][KH], v[NN][NA_REG][KH];
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+        if (i < cnt) {
+#pragma unroll
+            for (int k = K0; k < K1; ++k) r[i][k] = hslot(k) < Ho ? nd[i].rn[uslot(k)] : 0.0f;
+#pragma unroll
+            for (int a = 0; a < NA_REG; ++a)
+                if (a < nd[i].na) {
+                    const NodeState<SM> sa = action_state(nd[i].st, a, Ho);
+#pragma unroll
+                    for (int k = K0; k < K1; ++k) v[i][a][k] = hslot(k) < Ho ? node_value<MODE>(sa, uslot(k)) : 0.0f;
+                }
+        }
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+        if (i < cnt) {
+            float den[KH];
+#pragma unroll
+            for (int a = 0; a < NA_REG; ++a)
+                if (a < nd[i].na) {
+#pragma unroll
+                    for (int k = K0; k < K1; ++k) {
+                        const float x = MODE == MODE_DCFR ? fmax0(v[i][a][k]) : v[i][a][k];
+                        den[k] = a == 0 ? x : __fadd_rn(den[k], x);
+                    }
+                }
+#pragma unroll
+            for (int a = 0; a < NA_REG; ++a)
+                if (a < nd[i].na) {
+                    float* ra = nd[i].rc + (size_t)a * stride;
+#pragma unroll
+                    for (int k = K0; k < K1; ++k)
+                        if (hslot(k) < Ho) ra[uslot(k)] = __fmul_rn(strat_of<MODE>(v[i][a][k], den[k], nd[i].def), r[i][k]);
+                }
+        }
+}
+
 // Opponent nodes of a level on the register path, `cnt` (1..FWD_GROUP,
 // wave-uniform) at a time.
 constexpr int FWD_GROUP = 4;
@@ -390,8 +436,15 @@ __device__ __forceinline__ void fwd_group(const FwdNode<SM> (&fn)[FWD_GROUP], in
 // The game-root tile seeds its root row with the opponent's initial weights;
 // a subtree tile finds its root row already written by the chance node above.
 // V = double is the FP64 best response: average strategy and reach in f64.
-template <int MODE, int SM, typename V = float>
+// K1 - K0 < KH: only hand slots [K0, K1) of every thread (f32; the root tile of
+// a turn or flop game split over KH workgroups, each node of a level taken
+// with up to SLOT_GROUP others).
+constexpr int SLOT_GROUP = 8;
+template <int MODE, int SM, typename V = float, int K0 = 0, int K1 = KH>
 __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
+    constexpr bool SPLIT = K1 - K0 < KH;
+    static_assert(!SPLIT || sizeof(V) == 4, "slot ranges on f32 rows only");
+    constexpr int GROUP = SPLIT ? SLOT_GROUP : FWD_GROUP;
     const KernelArgs& A = c.A;
     const int o = p ^ 1, Ho = num_hands(c.G, o);
     const LevelRange* lv = A.level_ranges + T.level_base;
@@ -400,8 +453,8 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
         const float* w = A.hand_weight + hand_base(c.G, o);
         V* r0 = reach + (size_t)uni_load(lv).start * c.stride;
         float wv[KH];
-        FOR_K wv[k] = hslot(k) < Ho ? w[hslot(k)] : 0.0f;
-        FOR_K if (hslot(k) < Ho) r0[hslot(k)] = (V)wv[k];
+        FOR_KR wv[k] = hslot(k) < Ho ? w[hslot(k)] : 0.0f;
+        FOR_KR if (hslot(k) < Ho) r0[hslot(k)] = (V)wv[k];
         __syncthreads();
     }
     for (int L = 0; L < T.num_levels; ++L) {
@@ -424,11 +477,11 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     // their rows in flight together); the group stops at a
                     // chance node or a node off the register path, which the
                     // loop then takes on its own
-                    FwdNode<SM> fn[FWD_GROUP];
+                    FwdNode<SM> fn[GROUP];
                     fn[0] = {node_state<SM>(A, c.G, nd), rn, rc, na, __fdiv_rn(1.0f, (float)na)};
                     int cnt = 1, m = n + 1;
 #pragma unroll
-                    for (int slot = 1; slot < FWD_GROUP; ++slot) {   // compile-time slots: fn stays in registers
+                    for (int slot = 1; slot < GROUP; ++slot) {   // compile-time slots: fn stays in registers
                         bool found = false;
                         for (; PAIR_NODES && m < lr.end; ++m) {
                             const NodeRec nd2 = level_node(c, LR, m);
@@ -446,12 +499,13 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                         cnt = slot + 1;
                         n = m++;
                     }
-                    fwd_group<MODE, SM>(fn, cnt, c.stride, Ho);
+                    if constexpr (SPLIT) opponent_rows_slots<MODE, SM, GROUP, K0, K1>(fn, cnt, c.stride, Ho);
+                    else fwd_group<MODE, SM>(fn, cnt, c.stride, Ho);
                     continue;
                 }
             }
             V r[KH];
-            FOR_K r[k] = hslot(k) < Ho ? rn[hslot(k)] : V(0);
+            FOR_KR r[k] = hslot(k) < Ho ? rn[hslot(k)] : V(0);
             if (nd.kind == KIND_DECISION && nd.player == o && na > 1) {
                 const NodeState<SM> st = node_state<SM>(A, c.G, nd);
                 if constexpr (sizeof(V) == 8) {
@@ -459,13 +513,13 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     double den[KH];
                     for (int a = 0; a < na; ++a) {
                         double v[KH];
-                        FOR_K v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
-                        FOR_K den[k] = a == 0 ? v[k] : den[k] + v[k];
+                        FOR_KR v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
+                        FOR_KR den[k] = a == 0 ? v[k] : den[k] + v[k];
                     }
                     for (int a = 0; a < na; ++a) {
                         double v[KH];
-                        FOR_K v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
-                        FOR_K if (hslot(k) < Ho) {
+                        FOR_KR v[k] = hslot(k) < Ho ? (double)st.str((size_t)a * Ho + hslot(k)) : 0.0;
+                        FOR_KR if (hslot(k) < Ho) {
                             const double sig = den[k] == 0.0 ? 1.0 / (double)na : v[k] / den[k];
                             rc[(size_t)a * c.stride + hslot(k)] = sig * r[k];
                         }
@@ -477,16 +531,16 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                     float den[KH];
                     for (int a = 0; a < na; ++a) {
                         float v[KH];
-                        FOR_K v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
-                        FOR_K {
+                        FOR_KR v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
+                        FOR_KR {
                             const float x = MODE == MODE_DCFR ? fmax0(v[k]) : v[k];
                             den[k] = a == 0 ? x : __fadd_rn(den[k], x);
                         }
                     }
                     for (int a = 0; a < na; ++a) {
                         float v[KH];
-                        FOR_K v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
-                        FOR_K if (hslot(k) < Ho)
+                        FOR_KR v[k] = hslot(k) < Ho ? node_value<MODE>(st, (size_t)a * Ho + hslot(k)) : 0.0f;
+                        FOR_KR if (hslot(k) < Ho)
                             rc[(size_t)a * c.stride + hslot(k)] = __fmul_rn(strat_of<MODE>(v[k], den[k], def), r[k]);
                     }
                 }
@@ -496,11 +550,11 @@ __device__ __forceinline__ void tile_forward(Ctx& c, const TileDesc& T, int p) {
                 const ChanceTab ct = uni_load(A.chance + nd.aux);
                 if constexpr (sizeof(V) == 8) {
                     const double sc = 1.0 / (double)ct.factor;
-                    FOR_K r[k] = r[k] * sc;
+                    FOR_KR r[k] = r[k] * sc;
                 } else {
-                    FOR_K r[k] = __fmul_rn(r[k], ct.scale);
+                    FOR_KR r[k] = __fmul_rn(r[k], ct.scale);
                 }
-                FOR_K if (hslot(k) < Ho) rc[hslot(k)] = r[k];
+                FOR_KR if (hslot(k) < Ho) rc[hslot(k)] = r[k];
             }
         }
         if (any) __syncthreads();
@@ -1045,7 +1099,7 @@ __device__ __forceinline__ void traverser_rows16(Ctx& c, const NodeRec& nd, cons
 // NN nodes whose value is a function of their children's rows only, at most
 // NA_REG children each: the sum (opponent and single-action nodes) or, for
 // the best response, the max (traverser nodes).  All rows in flight together.
-template <bool MAX, int NN, typename V>
+template <bool MAX, int NN, typename V, int K0 = 0, int K1 = KH>
 __device__ __forceinline__ void children_rows(const V* const (&cc)[NN], V* const (&out)[NN], const int (&na)[NN],
                                               int stride, int Hp) {
     V cv[NN][NA_REG][KH];
@@ -1055,24 +1109,24 @@ __device__ __forceinline__ void children_rows(const V* const (&cc)[NN], V* const
         for (int a = 0; a < NA_REG; ++a)
             if (a < na[i]) {
                 const V* ca = cc[i] + (size_t)a * stride;
-                FOR_K cv[i][a][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+                FOR_KR cv[i][a][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
             }
 #pragma unroll
     for (int i = 0; i < NN; ++i) {
         V res[KH];
-        FOR_K res[k] = cv[i][0][k];
+        FOR_KR res[k] = cv[i][0][k];
 #pragma unroll
         for (int a = 1; a < NA_REG; ++a)
             if (a < na[i]) {
                 if constexpr (MAX) {
-                    FOR_K res[k] = fmaxr(res[k], cv[i][a][k]);
+                    FOR_KR res[k] = fmaxr(res[k], cv[i][a][k]);
                 } else if constexpr (sizeof(V) == 8) {
-                    FOR_K res[k] += cv[i][a][k];
+                    FOR_KR res[k] += cv[i][a][k];
                 } else {
-                    FOR_K res[k] = __fadd_rn(res[k], cv[i][a][k]);
+                    FOR_KR res[k] = __fadd_rn(res[k], cv[i][a][k]);
                 }
             }
-        FOR_K if (hslot(k) < Hp) out[i][uslot(k)] = res[k];
+        FOR_KR if (hslot(k) < Hp) out[i][uslot(k)] = res[k];
     }
 }
 
@@ -1118,8 +1172,10 @@ __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int
 // A tile's chance nodes are not summed here: the root and middle tiles'
 // chance sums run in PHASE_ROOT_LEAVES / PHASE_MID_LEAVES, and river games
 // and subtree tiles hold none (build_layout checks it).
-template <int MODE, int SM, typename V = float>
+template <int MODE, int SM, typename V = float, int K0 = 0, int K1 = KH>
 __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, const float4 disc) {
+    static_assert(K1 - K0 == KH || (SM == SM_F32 && sizeof(V) == 4),
+                  "slot ranges need f32 storage (16-bit storage takes a node-wide scale)");
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p);
     const LevelRange* lv = A.level_ranges + T.level_base;
@@ -1154,8 +1210,8 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                             const V* ccs[2] = {cc, cc2};
                             V* outs[2] = {out, out2};
                             const int nas[2] = {na, na2};
-                            if (trav) children_rows<true, 2, V>(ccs, outs, nas, c.stride, Hp);
-                            else children_rows<false, 2, V>(ccs, outs, nas, c.stride, Hp);
+                            if (trav) children_rows<true, 2, V, K0, K1>(ccs, outs, nas, c.stride, Hp);
+                            else children_rows<false, 2, V, K0, K1>(ccs, outs, nas, c.stride, Hp);
                             n = n2;
                             continue;
                         }
@@ -1163,7 +1219,7 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                             const BwdNode<SM> bn[2] = {{node_state<SM>(A, c.G, nd), cc, out, na, __fdiv_rn(1.0f, (float)na)},
                                                        {node_state<SM>(A, c.G, nd2), cc2, out2, na2,
                                                         __fdiv_rn(1.0f, (float)na2)}};
-                            traverser_rows<MODE, SM, 2, 0, KH>(bn, c.stride, Hp, disc);
+                            traverser_rows<MODE, SM, 2, K0, K1>(bn, c.stride, Hp, disc);
                             n = n2;
                             continue;
                         }
@@ -1180,58 +1236,58 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     double den[KH], res[KH];
                     for (int a = 0; a < na; ++a) {
                         double v[KH];
-                        FOR_K v[k] = hslot(k) < Hp ? (double)st.str((size_t)a * Hp + hslot(k)) : 0.0;
-                        FOR_K den[k] = a == 0 ? v[k] : den[k] + v[k];
+                        FOR_KR v[k] = hslot(k) < Hp ? (double)st.str((size_t)a * Hp + hslot(k)) : 0.0;
+                        FOR_KR den[k] = a == 0 ? v[k] : den[k] + v[k];
                     }
                     for (int a = 0; a < na; ++a) {
                         double v[KH], cv[KH];
-                        FOR_K {
+                        FOR_KR {
                             v[k] = hslot(k) < Hp ? (double)st.str((size_t)a * Hp + hslot(k)) : 0.0;
                             cv[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0;
                         }
-                        FOR_K {
+                        FOR_KR {
                             const double sig = den[k] == 0.0 ? 1.0 / (double)na : v[k] / den[k];
                             res[k] = a == 0 ? sig * cv[k] : res[k] + sig * cv[k];
                         }
                     }
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                    FOR_KR if (hslot(k) < Hp) out[hslot(k)] = res[k];
                 } else if constexpr (sizeof(V) == 8) {
                     // FP64 best response: max over actions
                     double res[KH];
-                    FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0;
+                    FOR_KR res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0;
                     for (int a = 1; a < na; ++a) {
                         double v[KH];
-                        FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0;
-                        FOR_K res[k] = v[k] > res[k] ? v[k] : res[k];
+                        FOR_KR v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0;
+                        FOR_KR res[k] = v[k] > res[k] ? v[k] : res[k];
                     }
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                    FOR_KR if (hslot(k) < Hp) out[hslot(k)] = res[k];
                 } else if (MODE == MODE_BR && na <= NA_REG) {
                     // best response: max over the children, all rows in flight together
                     const V* ccs[1] = {cc};
                     V* outs[1] = {out};
                     const int nas[1] = {na};
-                    children_rows<true, 1, V>(ccs, outs, nas, c.stride, Hp);
+                    children_rows<true, 1, V, K0, K1>(ccs, outs, nas, c.stride, Hp);
                 } else if (MODE == MODE_DCFR && SM != SM_F32 && na <= NA_REG) {
                     traverser_rows16<SM>(c, nd, node_state<SM>(A, c.G, nd), cc, out, Hp, disc);
                 } else if (MODE != MODE_BR && na <= NA_REG && !(MODE == MODE_DCFR && SM != SM_F32)) {
                     // every row of the node loaded once, all in flight
                     // together (registers)
                     const BwdNode<SM> bn[1] = {{node_state<SM>(A, c.G, nd), cc, out, na, __fdiv_rn(1.0f, (float)na)}};
-                    if constexpr (KH > 3) {
-                        traverser_rows<MODE, SM, 1, 0, KH / 2>(bn, c.stride, Hp, disc);
-                        traverser_rows<MODE, SM, 1, KH / 2, KH>(bn, c.stride, Hp, disc);
+                    if constexpr (K1 - K0 > 3) {
+                        traverser_rows<MODE, SM, 1, K0, K0 + (K1 - K0) / 2>(bn, c.stride, Hp, disc);
+                        traverser_rows<MODE, SM, 1, K0 + (K1 - K0) / 2, K1>(bn, c.stride, Hp, disc);
                     } else {
-                        traverser_rows<MODE, SM, 1, 0, KH>(bn, c.stride, Hp, disc);
+                        traverser_rows<MODE, SM, 1, K0, K1>(bn, c.stride, Hp, disc);
                     }
                 } else if (MODE == MODE_BR) {
                     float res[KH];
-                    FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0f;
+                    FOR_KR res[k] = hslot(k) < Hp ? cc[hslot(k)] : 0.0f;
                     for (int a = 1; a < na; ++a) {
                         float v[KH];
-                        FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0f;
-                        FOR_K res[k] = fmaxr(res[k], v[k]);
+                        FOR_KR v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0f;
+                        FOR_KR res[k] = fmaxr(res[k], v[k]);
                     }
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                    FOR_KR if (hslot(k) < Hp) out[hslot(k)] = res[k];
                 } else {
                     const NodeState<SM> st = node_state<SM>(A, c.G, nd);
                     const float def = __fdiv_rn(1.0f, (float)na);
@@ -1240,8 +1296,8 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     float den[KH];
                     for (int a = 0; a < na; ++a) {
                         float v[KH];
-                        FOR_K v[k] = hslot(k) < Hp ? node_value<MODE>(st, (size_t)a * Hp + hslot(k)) : 0.0f;
-                        FOR_K {
+                        FOR_KR v[k] = hslot(k) < Hp ? node_value<MODE>(st, (size_t)a * Hp + hslot(k)) : 0.0f;
+                        FOR_KR {
                             const float x = MODE == MODE_DCFR ? fmax0(v[k]) : v[k];
                             den[k] = a == 0 ? x : __fadd_rn(den[k], x);
                         }
@@ -1249,30 +1305,30 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                     float res[KH];
                     for (int a = 0; a < na; ++a) {
                         float v[KH], cv[KH];
-                        FOR_K {
+                        FOR_KR {
                             v[k] = hslot(k) < Hp ? node_value<MODE>(st, (size_t)a * Hp + hslot(k)) : 0.0f;
                             cv[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : 0.0f;
                         }
-                        FOR_K {
+                        FOR_KR {
                             const float t = __fmul_rn(strat_of<MODE>(v[k], den[k], def), cv[k]);
                             res[k] = a == 0 ? t : __fadd_rn(res[k], t);
                         }
                     }
                     if (MODE == MODE_DCFR && SM != SM_F32) {
-                        FOR_K if (hslot(k) < Hp) static_cast<float*>(c.rs)[hslot(k)] = res[k];   // for update_16bit
+                        FOR_KR if (hslot(k) < Hp) static_cast<float*>(c.rs)[hslot(k)] = res[k];   // for update_16bit
                     } else if (MODE == MODE_DCFR) {
                         // S = S * gamma + sigma; R = (R * (alpha | beta) + cfv_a) - value
                         // (solver.rs:556-589)
                         for (int a = 0; a < na; ++a) {
                             float rv[KH], sv[KH], cv[KH];
-                            FOR_K {
+                            FOR_KR {
                                 const int h = hslot(k);
                                 const size_t q = (size_t)a * Hp + h;
                                 rv[k] = h < Hp ? st.R[q] : 0.0f;
                                 sv[k] = h < Hp ? st.S[q] : 0.0f;
                                 cv[k] = h < Hp ? cc[(size_t)a * c.stride + h] : 0.0f;
                             }
-                            FOR_K if (hslot(k) < Hp) {
+                            FOR_KR if (hslot(k) < Hp) {
                                 const size_t q = (size_t)a * Hp + hslot(k);
                                 const float sig = strat_of<MODE>(rv[k], den[k], def);
                                 st.S[q] = __fadd_rn(__fmul_rn(sv[k], disc.z), sig);
@@ -1281,7 +1337,7 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                             }
                         }
                     }
-                    FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                    FOR_KR if (hslot(k) < Hp) out[hslot(k)] = res[k];
                     if (MODE == MODE_DCFR && SM != SM_F32) {
                         __syncthreads();
                         update_16bit<SM>(c, nd, st, Hp, disc);
@@ -1293,21 +1349,21 @@ __device__ __forceinline__ void tile_backward(Ctx& c, const TileDesc& T, int p, 
                 const V* ccs[1] = {cc};
                 V* outs[1] = {out};
                 const int nas[1] = {na};
-                children_rows<false, 1, V>(ccs, outs, nas, c.stride, Hp);
+                children_rows<false, 1, V, K0, K1>(ccs, outs, nas, c.stride, Hp);
             } else {
                 // opponent (or single-action) node: sum of the children
                 V res[KH];
-                FOR_K res[k] = hslot(k) < Hp ? cc[hslot(k)] : V(0);
+                FOR_KR res[k] = hslot(k) < Hp ? cc[hslot(k)] : V(0);
                 for (int a = 1; a < na; ++a) {
                     V v[KH];
-                    FOR_K v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : V(0);
+                    FOR_KR v[k] = hslot(k) < Hp ? cc[(size_t)a * c.stride + hslot(k)] : V(0);
                     if constexpr (sizeof(V) == 8) {
-                        FOR_K res[k] += v[k];
+                        FOR_KR res[k] += v[k];
                     } else {
-                        FOR_K res[k] = __fadd_rn(res[k], v[k]);
+                        FOR_KR res[k] = __fadd_rn(res[k], v[k]);
                     }
                 }
-                FOR_K if (hslot(k) < Hp) out[hslot(k)] = res[k];
+                FOR_KR if (hslot(k) < Hp) out[hslot(k)] = res[k];
             }
         }
         if (any) __syncthreads();
@@ -1793,13 +1849,35 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
             A.queue_counter[1] = 0;
             A.queue_counter[2] = 0;
         }
-        for (int k = blockIdx.x; k < A.num_multi_games; k += gridDim.x) {
+        // the forward pass and the DCFR backward pass are per hand: with
+        // root_parts = KH, workgroup j of a game takes hand slot j of every thread
+        const int parts = A.root_parts > 1 ? A.root_parts : 1;
+        for (int j = blockIdx.x; j < A.num_multi_games * parts; j += gridDim.x) {
+            const int k = parts > 1 ? j / parts : j;
+            const int part = parts > 1 ? j - k * parts : -1;
             const int gi = uni(A.multi_games[k]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
             if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
             const TileDesc T = uni_load(A.tiles + c.G.tile_base);
             if (A.phase == PHASE_ROOT_FORWARD) {
-                tile_forward<MODE, SM, V>(c, T, p);
+                if constexpr (sizeof(V) == 4 && KH == 3) {
+                    if (part == 0) tile_forward<MODE, SM, V, 0, 1>(c, T, p);
+                    else if (part == 1) tile_forward<MODE, SM, V, 1, 2>(c, T, p);
+                    else if (part == 2) tile_forward<MODE, SM, V, 2, 3>(c, T, p);
+                    else tile_forward<MODE, SM, V>(c, T, p);
+                } else {
+                    tile_forward<MODE, SM, V>(c, T, p);
+                }
+            } else if (part >= 0) {
+                // DCFR on f32 storage: the update is per hand (the host splits no
+                // other backward pass)
+                if constexpr (MODE == MODE_DCFR && SM == SM_F32 && sizeof(V) == 4 && KH == 3) {
+                    if (part == 0) tile_backward<MODE, SM, V, 0, 1>(c, T, p, disc);
+                    else if (part == 1) tile_backward<MODE, SM, V, 1, 2>(c, T, p, disc);
+                    else tile_backward<MODE, SM, V, 2, 3>(c, T, p, disc);
+                } else if (part == 0) {
+                    tile_backward<MODE, SM, V>(c, T, p, disc);
+                }
             } else {
                 // terminal and chance rows: PHASE_ROOT_LEAVES
                 tile_backward<MODE, SM, V>(c, T, p, disc);

@@ -67,6 +67,7 @@ struct HandScan {
 constexpr int DCFR_BLOCK = 512;   // threads of a DCFR workgroup (one game or tile at a time)
 constexpr int DCFR_MAX_ACTIONS = 16;   // largest action count of a decision node on the device
 constexpr int SCAN_THREADS = DCFR_BLOCK;
+constexpr int DCFR_SLOTS = (1326 + DCFR_BLOCK - 1) / DCFR_BLOCK;   // hand slots per thread (h = t + 512 k)
 // (3 * 1326 + 53) / SCAN_THREADS rounded up to a multiple of 8 (16-byte loads)
 constexpr int SCAN_RUN = ((3 * 1326 + 53 + SCAN_THREADS - 1) / SCAN_THREADS + 7) / 8 * 8;
 

@@ -941,6 +941,11 @@ struct psr_batch {
             a.wave_tiles = d_wave_queue;
             a.num_wave_tiles = (int)wave_queue.size() / 2;
         }
+        // the root tiles' forward pass (f32 rows) and DCFR backward pass (f32 storage) split by
+        // hand slot over DCFR_SLOTS workgroups per game while the games leave CUs idle
+        const bool split = (ph == PHASE_ROOT_FORWARD && (mode == MODE_DCFR || mode == MODE_BR || mode == MODE_CFV)) ||
+                           (ph == PHASE_ROOT_BACKWARD && mode == MODE_DCFR && storage == SM_F32);
+        a.root_parts = split && (int64_t)a.num_multi_games * DCFR_SLOTS <= num_cus ? DCFR_SLOTS : 1;
         a.phase = ph;
         a.player = player;
         a.mode = mode;
@@ -950,7 +955,7 @@ struct psr_batch {
                 : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves, grid_all())
                 : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves, grid_all())
                 : ph == PHASE_MID_FORWARD || ph == PHASE_MID_BACKWARD ? std::min(a.num_mid_tiles, grid_all())
-                                          : a.num_multi_games;
+                                          : a.num_multi_games * a.root_parts;   // ROOT_FORWARD / ROOT_BACKWARD
         if (g > 0) HIP_OK(dcfr_launch_phase(a, g, stream));
         if (waves) {
             // one workgroup per CU (the waves' regions fill its LDS), eight tiles at a time

@@ -353,7 +353,9 @@ inline hipError_t hipGetSymbolAddress(void** p, T& symbol) { *p = (void*)&symbol
 inline hipError_t hipGetDeviceCount(int* n) { *n = 1; return hipSuccess; }
 // two CUs, and gfx950's 160 KiB of LDS per workgroup
 inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t a, int) {
-    *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 163840 : 2;
+    // two CUs unless EMU_NUM_CUS says otherwise (launch shapes that depend on the CU count)
+    const char* cus = std::getenv("EMU_NUM_CUS");
+    *v = a == hipDeviceAttributeMaxSharedMemoryPerBlock ? 163840 : cus ? std::atoi(cus) : 2;
     return hipSuccess;
 }
 inline hipError_t hipFuncSetAttribute(const void*, hipFuncAttribute, int) { return hipSuccess; }
