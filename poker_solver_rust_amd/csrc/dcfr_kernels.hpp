@@ -1133,7 +1133,8 @@ __device__ __forceinline__ void children_rows(const V* const (&cc)[NN], V* const
 // Backward of chance node n (record nd) for traversal p: its cfv row = the
 // f64 sum over the dealt cards' rows, in child order, plus the isomorphic
 // cards' rows (solver.rs:256-331).
-template <typename V>
+// K1 - K0 < KH: only hand slots [K0, K1) of every thread (the sum is per hand).
+template <typename V, int K0 = 0, int K1 = KH>
 __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int p) {
     const KernelArgs& A = c.A;
     const int Hp = num_hands(c.G, p);
@@ -1142,7 +1143,7 @@ __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int
     const V* cc = cfv + (size_t)nd.first_child * c.stride;
     const ChanceTab ct = uni_load(A.chance + nd.aux);
     double acc[KH];
-    FOR_K acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
+    FOR_KR acc[k] = hslot(k) < Hp ? (double)cc[hslot(k)] : 0.0;
     // children in groups of CHANCE_GROUP rows in flight together, summed in
     // child order
     constexpr int CHANCE_GROUP = 8;
@@ -1152,21 +1153,21 @@ __device__ __forceinline__ void chance_sum(Ctx& c, const NodeRec& nd, int n, int
         for (int g = 0; g < CHANCE_GROUP; ++g)
             if (a0 + g < na) {
                 const V* ca = cc + (size_t)(a0 + g) * c.stride;
-                FOR_K v[g][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
+                FOR_KR v[g][k] = hslot(k) < Hp ? ca[uslot(k)] : V(0);
             }
 #pragma unroll
         for (int g = 0; g < CHANCE_GROUP; ++g)
-            if (a0 + g < na) FOR_K acc[k] += (double)v[g][k];
+            if (a0 + g < na) FOR_KR acc[k] += (double)v[g][k];
     }
     for (int e = ct.iso_base; e < ct.iso_base + ct.iso_count; ++e) {
         const uint16_t* perm = A.u16pool + A.iso_perm[2 * e + p];
         const V* src = cc + (size_t)A.iso_ref[e] * c.stride;
         V v[KH];
-        FOR_K v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
-        FOR_K acc[k] += (double)v[k];
+        FOR_KR v[k] = hslot(k) < Hp ? src[perm[hslot(k)]] : V(0);
+        FOR_KR acc[k] += (double)v[k];
     }
     V* out = cfv + (size_t)n * c.stride;
-    FOR_K if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
+    FOR_KR if (hslot(k) < Hp) out[hslot(k)] = (V)acc[k];
 }
 
 // A tile's chance nodes are not summed here: the root and middle tiles'
@@ -1898,13 +1899,27 @@ __device__ __forceinline__ void phase_body(const KernelArgs& A, double* scan, vo
         const bool root = A.phase == PHASE_ROOT_LEAVES;
         const int count = root ? A.num_root_leaves : A.num_mid_leaves;
         const int* list = root ? A.root_leaves : A.mid_leaves;
-        for (int k = blockIdx.x; k < count; k += gridDim.x) {
+        // leaf_parts = KH: a chance sum split over KH workgroups by hand slot (the
+        // sum is per hand); a terminal (its scan spans every hand) is part 0's
+        const int parts = A.leaf_parts > 1 ? A.leaf_parts : 1;
+        for (int j = blockIdx.x; j < count * parts; j += gridDim.x) {
+            const int k = parts > 1 ? j / parts : j;
+            const int part = parts > 1 ? j - k * parts : -1;
             const int gi = uni(list[2 * k]), n = uni(list[2 * k + 1]);
             Ctx c = make_ctx(A, gi, scan, rs, &sh);
             if (MODE == MODE_DCFR && uni(A.phase_expl[gi]) <= c.G.target) continue;
             const NodeRec nd = uni_load(c.nodes + n);
             if (nd.kind == KIND_CHANCE) {
-                chance_sum<V>(c, nd, n, p);
+                if constexpr (KH == 3) {
+                    if (part == 0) chance_sum<V, 0, 1>(c, nd, n, p);
+                    else if (part == 1) chance_sum<V, 1, 2>(c, nd, n, p);
+                    else if (part == 2) chance_sum<V, 2, 3>(c, nd, n, p);
+                    else chance_sum<V>(c, nd, n, p);
+                } else if (part <= 0) {
+                    chance_sum<V>(c, nd, n, p);
+                }
+            } else if (part > 0) {
+                continue;
             } else {
                 const int Ho = num_hands(c.G, p ^ 1);
                 const V* reach = reach_rows<V>(c) + (size_t)reach_src(c, n, p) * c.stride;

@@ -114,8 +114,10 @@ struct KernelArgs {
     uint32_t lds_bytes;         // dynamic LDS of the whole-game and subtree launches
                                 // (0: dcfr_lds_bytes(max_hands))
     int fwd_only;               // PHASE_SUBTREES: only the forward pass, level by level (reach rows)
-    int root_parts;             // PHASE_ROOT_FORWARD: workgroups per game (1, or KH: workgroup j
-                                // takes hand slot j of every thread, f32 rows)
+    int root_parts;             // PHASE_ROOT_FORWARD / _BACKWARD: workgroups per game (1, or KH:
+                                // workgroup j takes hand slot j of every thread, f32 rows)
+    int leaf_parts;             // PHASE_ROOT_LEAVES / PHASE_MID_LEAVES: workgroups per chance sum
+                                // (1, or KH, as root_parts)
     // the wave path (dcfr_layout.hpp): its work queue of (tile, game) pairs and the LDS region
     // of each wave (the one-wave scan output, then the tile's rows)
     const int* wave_tiles;

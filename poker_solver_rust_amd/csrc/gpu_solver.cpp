@@ -946,14 +946,19 @@ struct psr_batch {
         const bool split = (ph == PHASE_ROOT_FORWARD && (mode == MODE_DCFR || mode == MODE_BR || mode == MODE_CFV)) ||
                            (ph == PHASE_ROOT_BACKWARD && mode == MODE_DCFR && storage == SM_F32);
         a.root_parts = split && (int64_t)a.num_multi_games * DCFR_SLOTS <= num_cus ? DCFR_SLOTS : 1;
+        const int leaves = ph == PHASE_ROOT_LEAVES ? a.num_root_leaves : ph == PHASE_MID_LEAVES ? a.num_mid_leaves : 0;
+        a.leaf_parts = leaves > 0 && (mode == MODE_DCFR || mode == MODE_BR || mode == MODE_CFV) &&
+                               (int64_t)leaves * DCFR_SLOTS <= num_cus
+                           ? DCFR_SLOTS
+                           : 1;
         a.phase = ph;
         a.player = player;
         a.mode = mode;
         a.iteration = t;
         a.iteration_dev = dev_iter ? d_iter : nullptr;
         int g = ph == PHASE_SUBTREES       ? std::min(a.num_subtree_tiles, num_cus * std::max(subtree_per_cu, 1))
-                : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves, grid_all())
-                : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves, grid_all())
+                : ph == PHASE_ROOT_LEAVES ? std::min(a.num_root_leaves * a.leaf_parts, grid_all())
+                : ph == PHASE_MID_LEAVES  ? std::min(a.num_mid_leaves * a.leaf_parts, grid_all())
                 : ph == PHASE_MID_FORWARD || ph == PHASE_MID_BACKWARD ? std::min(a.num_mid_tiles, grid_all())
                                           : a.num_multi_games * a.root_parts;   // ROOT_FORWARD / ROOT_BACKWARD
         if (g > 0) HIP_OK(dcfr_launch_phase(a, g, stream));
