@@ -725,6 +725,10 @@ size_t arena_bytes(size_t bytes) { return (bytes + 255) / 256 * 256; }
 struct psr_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a batch of single-tile and multi-tile games runs the whole-game kernel on
+    // side_stream beside the multi-tile phase loop (created on first use)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // device memory given from outside (the allocation-free entry point):
     // buffers are carved from it instead of hipMalloc'd, and never freed
     struct Arena {
@@ -1105,13 +1109,48 @@ struct psr_batch {
         return 0;
     }
 
+    // The whole-game kernel over the single-tile games for one job, on stream s.
+    int launch_single(int flags, uint32_t begin, uint32_t end, int check_interval, uint32_t max_iters, hipStream_t s) {
+        KernelArgs a = args();
+        a.iter_begin = begin;
+        a.iter_end = end;
+        a.max_iterations = max_iters;
+        a.check_interval = check_interval;
+        a.flags = flags;
+        a.player = reach_player;
+        HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), s));
+        if (flags & (JOB_ROOT_CFV | JOB_REACH | JOB_REACH_DCFR)) stale_reach = 0;   // level-by-level last
+        else if (flags & JOB_FINAL_CHECK) stale_reach = JOB_REACH;
+        else if ((flags & JOB_ITERATE) && end > begin) stale_reach = JOB_REACH_DCFR;
+        else if (flags & JOB_INITIAL_CHECK) stale_reach = JOB_REACH;
+        return 0;
+    }
+
     // Runs a job on every game: the phase loop (reference solve() schedule
-    // driven from the host) for multi-tile games, then the whole-game kernel
-    // for single-tile games.  Timed with events on the batch stream.
+    // driven from the host) for multi-tile games and the whole-game kernel for
+    // single-tile games, side by side when the batch holds both.  Timed with
+    // events on the batch stream.
     int run(int flags, uint32_t begin, uint32_t end, int check_interval, uint32_t max_iters) {
         if (!prepared) return fail("psr_batch_prepare has not been called");
         HIP_OK(hipSetDevice(device));
         if (ev0) HIP_OK(hipEventRecord(ev0, stream));
+        // Both kinds of game: the whole-game kernel for the single-tile games runs on the side
+        // stream while this thread drives the multi-tile games' phase loop on the batch stream.
+        // The two touch disjoint state: per-game rows and results, queue counter 0 against
+        // counters 1 and 2, and the phase loop's skip flags (phase_expl), which the whole-game
+        // kernel does not read.  The batch stream waits for the side stream before ev1.
+        const bool overlap = !L.multi_games.empty() && !L.single_games.empty();
+        if (overlap) {
+            if (!side_stream) {
+                HIP_OK(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
+                HIP_OK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+                HIP_OK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+            }
+            HIP_OK(hipEventRecord(ev_fork, stream));
+            HIP_OK(hipStreamWaitEvent(side_stream, ev_fork, 0));
+            if (launch_single(flags, begin, end, check_interval, max_iters, side_stream)) return -1;
+            HIP_OK(hipEventRecord(ev_join, side_stream));
+        }
         if (!L.multi_games.empty()) {
             const float inf = INFINITY;
             std::vector<float> expl(L.games.size(), inf);
@@ -1191,23 +1230,10 @@ struct psr_batch {
                 HIP_OK(hipStreamSynchronize(stream));
             }
         }
-        // Single-tile games after the multi-tile loop: that loop rewrites the
-        // whole per-game exploitability array (its skip flags), which would
-        // clobber results the whole-game kernel had already written.
-        if (!L.single_games.empty()) {
-            KernelArgs a = args();
-            a.iter_begin = begin;
-            a.iter_end = end;
-            a.max_iterations = max_iters;
-            a.check_interval = check_interval;
-            a.flags = flags;
-            a.player = reach_player;
-            HIP_OK(dcfr_launch(a, std::min((int)L.single_games.size(), grid_all()), stream));
-            if (flags & (JOB_ROOT_CFV | JOB_REACH | JOB_REACH_DCFR)) stale_reach = 0;   // level-by-level last
-            else if (flags & JOB_FINAL_CHECK) stale_reach = JOB_REACH;
-            else if ((flags & JOB_ITERATE) && end > begin) stale_reach = JOB_REACH_DCFR;
-            else if (flags & JOB_INITIAL_CHECK) stale_reach = JOB_REACH;
+        if (!L.single_games.empty() && !overlap) {
+            if (launch_single(flags, begin, end, check_interval, max_iters, stream)) return -1;
         }
+        if (overlap) HIP_OK(hipStreamWaitEvent(stream, ev_join, 0));
         if (ev1) HIP_OK(hipEventRecord(ev1, stream));
         HIP_OK(hipStreamSynchronize(stream));
         if (ev0 && ev1) HIP_OK(hipEventElapsedTime(&last_ms, ev0, ev1));
@@ -1309,6 +1335,9 @@ void psr_batch_destroy(psr_batch* b) {
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+    if (b->side_stream) (void)hipStreamDestroy(b->side_stream);
     delete b;
 }
 

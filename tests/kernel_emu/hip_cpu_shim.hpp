@@ -407,6 +407,10 @@ inline hipError_t hipEventCreate(hipEvent_t* e) {
     *e = new std::chrono::steady_clock::time_point();
     return hipSuccess;
 }
+constexpr unsigned hipEventDisableTiming = 2;
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+// launches run at once in the emulation, so waits between streams are no-ops
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
     *static_cast<std::chrono::steady_clock::time_point*>(e) = std::chrono::steady_clock::now();
     return hipSuccess;

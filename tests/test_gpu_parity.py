@@ -299,6 +299,33 @@ def test_mixed_river_and_turn_batch():
         close(b.strategy_sum(i), o.state()[1])
 
 
+def test_mixed_batch_equals_separate_batches():
+    """The whole-game kernel of a mixed batch runs on a second stream beside
+    the turn game's phase loop: every result equals that of the same games
+    solved in batches of one kind (solve() iterations and exploitability,
+    strategy sums, regrets)."""
+    rivers = [s.build() for s in W.river_batch(24, seed=31, iterations=30)]
+    turn = turn_game()
+    targets = [0.005] * 25
+
+    def run(games, tg):
+        b = P.GpuBatchSolver(0)
+        b.prepare_batch(games, 30, tg)
+        e, it = b.solve(30, 5)
+        return b, e, it
+
+    bm, em, im = run(rivers + [turn], targets)
+    br, er, ir = run(rivers, targets[:24])
+    bt, et, it_ = run([turn], targets[:1])
+    assert np.array_equal(em[:24], er) and np.array_equal(im[:24], ir)
+    assert np.array_equal(em[24:], et) and np.array_equal(im[24:], it_)
+    for i in range(24):
+        assert np.array_equal(bm.strategy_sum(i), br.strategy_sum(i)), i
+        assert np.array_equal(bm.regrets(i), br.regrets(i)), i
+    assert np.array_equal(bm.strategy_sum(24), bt.strategy_sum(0))
+    assert np.array_equal(bm.regrets(24), bt.regrets(0))
+
+
 @pytest.mark.parametrize("make,iters", [(lambda: std_game("33%, 75%, a", "2x, a"), 40),
                                         (lambda: W.river_uniform(2).build(), 60), (turn_game, 10),
                                         (lambda: flop_game("AA,KK", "QQ,AKs", "Qs Jh 2c", "50%, a"), 3)])
