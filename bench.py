@@ -668,15 +668,23 @@ def main():
     n_single = min(args.single_sample, len(games))
     if n_single > 0:
         t = time.perf_counter()
+        single_kms = 0.0
         for i in range(n_single):
             one = P.GpuBatchSolver(0)
             one.prepare_batch([games[i]], args.iterations, targets[i:i + 1] if args.mode == "solve" else None)
             one.solve(args.iterations, interval)
+            single_kms += one.last_kernel_ms()
         single_rate = n_single / (time.perf_counter() - t)
+        # per CU: a single subgame runs on one workgroup of one CU, the batch on every CU;
+        # device time only on both sides
+        cus = solver.launch_info()["num_cus"]
+        per_cu = (value / cus) / (n_single / (single_kms * 1e-3))
         out["single_subgame"] = {"value": round(single_rate, 3), "unit": "solves/s", "sample": n_single,
-                                 "batched_over_single": round(value / single_rate, 2)}
+                                 "batched_over_single": round(value / single_rate, 2),
+                                 "device_ms_per_game": round(single_kms / n_single, 3),
+                                 "batched_per_cu_over_single_device": round(per_cu, 3)}
         log(f"single-subgame path: {single_rate:.1f} solves/s on {n_single} subgames; batched is "
-            f"{value / single_rate:.1f}x")
+            f"{value / single_rate:.1f}x; per CU, device time only: {per_cu:.2f}x")
     if args.turn_iters > 0:
         out["turn"] = turn_bench(P, W, args.turn_iters, args.seed, cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:

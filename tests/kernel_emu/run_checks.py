@@ -112,9 +112,10 @@ def run_batch():
 def run_turn_stepwise():
     """A turn game: root tile plus one subtree tile per river card, driven by
     the phase kernel; then the FP64 best response through the phase kernel.
-    On a device of 8 CUs, so the root tile's forward pass runs split over
-    three workgroups by hand slot."""
-    os.environ["EMU_NUM_CUS"] = "8"
+    On a device of 64 CUs, so the root tile's forward and backward passes
+    and the chance sums and terminals of its leaves run split over three
+    workgroups by hand slot."""
+    os.environ["EMU_NUM_CUS"] = "64"
     g = turn_game("AA,AKs", "KK,QQ")
     check_stepwise(g, 2)
     b = P.GpuBatchSolver(0)
@@ -211,9 +212,9 @@ def run_flop_stepwise():
     one tile per turn card holding its river deals (isomorphic rivers too)
     and river subtrees, so chance nodes inside a tile and terminals of many
     runouts in one tile; state, exploitability and root cfvalues against the
-    oracle.  On a device of 8 CUs (the flop tile's forward pass split by hand
-    slot)."""
-    os.environ["EMU_NUM_CUS"] = "8"
+    oracle.  On a device of 64 CUs (the flop tile's passes and leaves split
+    by hand slot)."""
+    os.environ["EMU_NUM_CUS"] = "64"
     check_stepwise(flop_game("AA,KK", "QQ,AK", "2s 7s Js"), 1)
 
 
