@@ -117,3 +117,37 @@ def test_subtree_tiles_paths_identical():
         assert np.array_equal(bd.regrets(i), bl.regrets(i)), i
         for p in range(2):
             assert np.array_equal(bd.root_cfvalues(compute=False)[i][p], bl.root_cfvalues(compute=False)[i][p]), (i, p)
+
+
+def test_wave_path_raked_flop():
+    """A raked flop game (rake 5%, cap 30): its river tiles run one per wave
+    through the raked showdown (win, tie and lose bands); state after every
+    iteration equal to the oracle's and to the level-by-level path, then the
+    exploitability (the best response and the average strategy's value)."""
+    cc = P.CardConfig(range=(P.Range.parse("AA,KK,QQ,AKs"), P.Range.parse("QQ-TT,AQs,KQs")),
+                      flop=P.flop_from_str("Qs Jh 2c"))
+    s = [("50%, a", ""), ("50%, a", "")]
+    tc = P.TreeConfig(initial_state=P.BoardState.Flop, starting_pot=100, effective_stack=100, flop_bet_sizes=s,
+                      turn_bet_sizes=s, river_bet_sizes=s, rake_rate=0.05, rake_cap=30.0)
+    g = P.PostFlopGame.with_config(cc, P.ActionTree(tc))
+    runs = []
+    for dfs in (True, False):
+        b = P.GpuBatchSolver(0)
+        b.set_depth_first(dfs)
+        b.prepare_batch([g], 4)
+        info = b.depth_first_info()
+        assert (info["wave_tiles"] > 0) == dfs, info
+        runs.append(b)
+    b, lv = runs
+    o = O.OracleGame(g.serialize())
+    for t in range(3):
+        b.run_iterations(t, t + 1)
+        lv.run_iterations(t, t + 1)
+        o.solve_step(t)
+        r, st = o.state()
+        assert np.array_equal(b.regrets(0), r), t
+        assert np.array_equal(b.strategy_sum(0), st), t
+        assert np.array_equal(b.regrets(0), lv.regrets(0)), t
+    e, el = b.exploitability(), lv.exploitability()
+    assert np.array_equal(e, el)
+    assert abs(float(e[0]) - o.exploitability()) <= 1e-6 * max(1.0, o.exploitability())
